@@ -1,0 +1,382 @@
+"""bola_amd -- the reference crate's dense-solver path on MI355X (gfx950).
+
+The functions here mirror the crate's public functions one for one (same names, same
+argument order and meaning, column-major f64, 1-based int32 pivots, in-place results,
+and the same error texts), on top of the C ABI of libbola_hip.so (include/bola_hip.h).
+
+Operands may be numpy arrays (host memory: the call copies them to HBM and back, like a
+drop-in replacement of the CPU function) or torch CUDA tensors (device memory: the call
+runs on torch's current stream and copies nothing).  A matrix operand is a flat array
+that starts at the operand's first element, exactly like the crate's raw pointers.
+
+Functions that return ``Result<(), String>`` in the crate raise :class:`BolaError` with
+the crate's message instead of returning ``Err``.
+
+Crate sources mirrored (file:line): gemm.rs:88, getrf.rs:37, getrf2.rs:39, potrf.rs:101,
+potrf2.rs:25, trsm.rs:33, syrk.rs:17, laswp.rs:22, idamax.rs:23, ger.rs:28, scal.rs:45,
+utilities.rs:22 (dot_product), utilities.rs:107 (daxpy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+__all__ = [
+    "BolaError", "Handle", "build", "lib", "library_path",
+    "gemm", "getrf", "getrf2", "potrf", "potrf2", "trsm", "syrk", "laswp", "idamax",
+    "ger", "scal", "daxpy", "dot_product", "getrs", "potrs",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libbola_hip.so")
+_lib = None
+
+ERR_DEVICE = -1000
+
+
+class BolaError(Exception):
+    """The crate's ``Err(String)``; ``code`` is the C ABI status (negative = illegal
+    argument, positive = 1-based column of a zero pivot / failing leading minor)."""
+
+    def __init__(self, message: str, code: int = 0):
+        super().__init__(message)
+        self.message = message
+        self.code = code
+
+
+def library_path() -> str:
+    return _SO
+
+
+def build() -> None:
+    """Compile libbola_hip.so for gfx950 (bola_amd/csrc/Makefile)."""
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_sz = C.c_size_t
+_vp = C.c_void_p
+_ch = C.c_char
+_d = C.c_double
+_i32 = C.c_int32
+
+# (name, restype, argtypes)
+_SIGS = [
+    ("bola_hip_create", C.c_int, [C.POINTER(_vp)]),
+    ("bola_hip_destroy", C.c_int, [_vp]),
+    ("bola_hip_set_stream", C.c_int, [_vp, _vp]),
+    ("bola_hip_get_stream", _vp, [_vp]),
+    ("bola_hip_synchronize", C.c_int, [_vp]),
+    ("bola_hip_error_string", C.c_char_p, [_vp]),
+    ("bola_hip_last_error", C.c_char_p, []),
+    ("bola_hip_profile_enable", C.c_int, [_vp, C.c_int]),
+    ("bola_hip_profile_reset", C.c_int, [_vp]),
+    ("bola_hip_profile_get", C.c_int, [_vp, C.c_char_p, _dp, C.POINTER(C.c_long), _dp]),
+    ("bola_hip_profile_report", C.c_int, [_vp, _d, C.c_char_p, _sz]),
+    ("bola_hip_idamax", _sz, [_sz, _vp]),
+    ("bola_hip_idamax_dev", C.c_int, [_vp, _sz, _vp, _vp]),
+    ("bola_hip_dlaswp", C.c_int, [_sz, _vp, _sz, _i32, _i32, _vp, _i32]),
+    ("bola_hip_dlaswp_dev", C.c_int, [_vp, _sz, _vp, _sz, _i32, _i32, _vp, _i32]),
+    ("bola_hip_dger", C.c_int, [_sz, _sz, _d, _vp, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dger_dev", C.c_int, [_vp, _sz, _sz, _d, _vp, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dscal", C.c_int, [_sz, _d, _vp, _sz]),
+    ("bola_hip_dscal_dev", C.c_int, [_vp, _sz, _d, _vp, _sz]),
+    ("bola_hip_daxpy", C.c_int, [_sz, _d, _vp, _sz, _vp, _sz]),
+    ("bola_hip_daxpy_dev", C.c_int, [_vp, _sz, _d, _vp, _sz, _vp, _sz]),
+    ("bola_hip_ddot", _d, [_sz, _vp, _sz, _vp, _sz]),
+    ("bola_hip_ddot_dev", C.c_int, [_vp, _sz, _vp, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dgemm", C.c_int, [_ch, _ch, _sz, _sz, _sz, _d, _vp, _sz, _vp, _sz, _d, _vp, _sz]),
+    ("bola_hip_dgemm_dev", C.c_int, [_vp, _ch, _ch, _sz, _sz, _sz, _d, _vp, _sz, _vp, _sz, _d, _vp, _sz]),
+    ("bola_hip_dtrsm", C.c_int, [_ch, _ch, _ch, _ch, _sz, _sz, _d, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dtrsm_dev", C.c_int, [_vp, _ch, _ch, _ch, _ch, _sz, _sz, _d, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dsyrk", C.c_int, [_ch, _ch, _sz, _sz, _d, _vp, _sz, _d, _vp, _sz]),
+    ("bola_hip_dsyrk_dev", C.c_int, [_vp, _ch, _ch, _sz, _sz, _d, _vp, _sz, _d, _vp, _sz]),
+    ("bola_hip_dgetrf", C.c_int, [_sz, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dgetrf_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp]),
+    ("bola_hip_dgetrf2", C.c_int, [_sz, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dgetrf2_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp]),
+    ("bola_hip_dpotrf", C.c_int, [_ch, _sz, _vp, _sz]),
+    ("bola_hip_dpotrf_dev", C.c_int, [_vp, _ch, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dpotrf2", C.c_int, [_ch, _sz, _vp, _sz]),
+    ("bola_hip_dpotrf2_dev", C.c_int, [_vp, _ch, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dgetrs", C.c_int, [_sz, _sz, _vp, _sz, _vp, _vp, _sz]),
+    ("bola_hip_dgetrs_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _sz]),
+    ("bola_hip_dpotrs", C.c_int, [_ch, _sz, _sz, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dpotrs_dev", C.c_int, [_vp, _ch, _sz, _sz, _vp, _sz, _vp, _sz]),
+]
+
+
+def lib():
+    """Load libbola_hip.so.  There is no fallback: without the built library every call
+    raises."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            raise RuntimeError(f"{_SO} is not built; run bola_amd.build() (hipcc, gfx950)")
+        L = C.CDLL(_SO)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return [s[0] for s in _SIGS]
+
+
+# ------------------------------------------------------------------ operand plumbing
+
+def _is_dev(x) -> bool:
+    return hasattr(x, "is_cuda") and x.is_cuda
+
+
+def _ptr(x, dtype):
+    if x is None:
+        return None
+    if _is_dev(x):
+        import torch
+        want = torch.float64 if dtype == np.float64 else torch.int32
+        if x.dtype != want or not x.is_contiguous():
+            raise TypeError(f"expected a contiguous {want} CUDA tensor")
+        return C.c_void_p(x.data_ptr())
+    if not isinstance(x, np.ndarray) or x.dtype != dtype or not x.flags.c_contiguous:
+        raise TypeError(f"expected a contiguous numpy {np.dtype(dtype).name} array")
+    return C.c_void_p(x.ctypes.data)
+
+
+def _c(ch: str) -> bytes:
+    return ch.encode("ascii")[:1]
+
+
+class Handle:
+    """A library handle (stream + device workspace).  By default it follows torch's
+    current stream at every call."""
+
+    def __init__(self, stream=None):
+        h = C.c_void_p()
+        rc = lib().bola_hip_create(C.byref(h))
+        if rc != 0:
+            raise BolaError(lib().bola_hip_last_error().decode(), rc)
+        self._h = h
+        self._fixed_stream = stream is not None
+        if stream is not None:
+            lib().bola_hip_set_stream(self._h, C.c_void_p(stream))
+
+    @property
+    def ptr(self):
+        if not self._fixed_stream:
+            import torch
+            s = torch.cuda.current_stream().cuda_stream
+            if lib().bola_hip_get_stream(self._h) != s:
+                lib().bola_hip_set_stream(self._h, C.c_void_p(s))
+        return self._h
+
+    def error(self) -> str:
+        return lib().bola_hip_error_string(self._h).decode()
+
+    def synchronize(self):
+        lib().bola_hip_synchronize(self.ptr)
+
+    def profile(self, on: bool = True):
+        lib().bola_hip_profile_enable(self._h, 1 if on else 0)
+
+    def profile_reset(self):
+        lib().bola_hip_profile_reset(self._h)
+
+    def profile_get(self, name: str):
+        ms = C.c_double()
+        calls = C.c_long()
+        fl = C.c_double()
+        lib().bola_hip_profile_get(self._h, name.encode(), C.byref(ms), C.byref(calls), C.byref(fl))
+        return ms.value, calls.value, fl.value
+
+    def profile_report(self, iterations: float = 1.0) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        lib().bola_hip_profile_report(self._h, iterations, buf, len(buf))
+        return buf.value.decode()
+
+    def close(self):
+        if self._h:
+            lib().bola_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_dev_handle = None
+
+
+def default_handle() -> Handle:
+    global _default_dev_handle
+    if _default_dev_handle is None:
+        _default_dev_handle = Handle()
+    return _default_dev_handle
+
+
+def _check(rc, dev):
+    if rc != 0:
+        msg = default_handle().error() if dev else lib().bola_hip_last_error().decode()
+        raise BolaError(msg, rc)
+
+
+# ------------------------------------------------------------------ BLAS-1/2
+
+def idamax(n, x) -> int:
+    """idamax.rs:23 -- 0-based index of the largest |x_i| (the crate's tie/NaN rule)."""
+    if _is_dev(x):
+        import torch
+        out = torch.zeros(1, dtype=torch.int64, device=x.device)
+        rc = lib().bola_hip_idamax_dev(default_handle().ptr, n, _ptr(x, np.float64), C.c_void_p(out.data_ptr()))
+        _check(rc, True)
+        return int(out.item())
+    r = lib().bola_hip_idamax(n, _ptr(x, np.float64))
+    if r == C.c_size_t(-1).value:
+        raise BolaError(lib().bola_hip_last_error().decode(), ERR_DEVICE)
+    return int(r)
+
+
+def laswp(n, a, lda, k1, k2, ipiv, incx) -> None:
+    """laswp.rs:22 -- row interchanges k1..k2 (1-based) applied to n columns."""
+    if _is_dev(a):
+        rc = lib().bola_hip_dlaswp_dev(default_handle().ptr, n, _ptr(a, np.float64), lda, k1, k2,
+                                       _ptr(ipiv, np.int32), incx)
+        _check(rc, True)
+    else:
+        _check(lib().bola_hip_dlaswp(n, _ptr(a, np.float64), lda, k1, k2, _ptr(ipiv, np.int32), incx), False)
+
+
+def ger(m, n, alpha, x, y, ldy, a, lda) -> None:
+    """ger.rs:28 -- A += alpha * x * y^T."""
+    if _is_dev(a):
+        _check(lib().bola_hip_dger_dev(default_handle().ptr, m, n, alpha, _ptr(x, np.float64), _ptr(y, np.float64),
+                                       ldy, _ptr(a, np.float64), lda), True)
+    else:
+        _check(lib().bola_hip_dger(m, n, alpha, _ptr(x, np.float64), _ptr(y, np.float64), ldy,
+                                   _ptr(a, np.float64), lda), False)
+
+
+def scal(n, alpha, x, incx) -> None:
+    """scal.rs:45 -- x *= alpha."""
+    if _is_dev(x):
+        _check(lib().bola_hip_dscal_dev(default_handle().ptr, n, alpha, _ptr(x, np.float64), incx), True)
+    else:
+        _check(lib().bola_hip_dscal(n, alpha, _ptr(x, np.float64), incx), False)
+
+
+def daxpy(n, alpha, x, incx, y, incy) -> None:
+    """utilities.rs:107 -- y += alpha * x."""
+    if _is_dev(y):
+        _check(lib().bola_hip_daxpy_dev(default_handle().ptr, n, alpha, _ptr(x, np.float64), incx,
+                                        _ptr(y, np.float64), incy), True)
+    else:
+        _check(lib().bola_hip_daxpy(n, alpha, _ptr(x, np.float64), incx, _ptr(y, np.float64), incy), False)
+
+
+def dot_product(n, x, incx, y, incy) -> float:
+    """utilities.rs:22 -- sum_i x_i * y_i."""
+    if _is_dev(x):
+        import torch
+        out = torch.zeros(1, dtype=torch.float64, device=x.device)
+        _check(lib().bola_hip_ddot_dev(default_handle().ptr, n, _ptr(x, np.float64), incx, _ptr(y, np.float64), incy,
+                                       C.c_void_p(out.data_ptr())), True)
+        return float(out.item())
+    return float(lib().bola_hip_ddot(n, _ptr(x, np.float64), incx, _ptr(y, np.float64), incy))
+
+
+# ------------------------------------------------------------------ BLAS-3
+
+def gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """gemm.rs:88 -- C = alpha*op(A)*op(B) + beta*C."""
+    if _is_dev(c):
+        _check(lib().bola_hip_dgemm_dev(default_handle().ptr, _c(transa), _c(transb), m, n, k, alpha,
+                                        _ptr(a, np.float64), lda, _ptr(b, np.float64), ldb, beta,
+                                        _ptr(c, np.float64), ldc), True)
+    else:
+        _check(lib().bola_hip_dgemm(_c(transa), _c(transb), m, n, k, alpha, _ptr(a, np.float64), lda,
+                                    _ptr(b, np.float64), ldb, beta, _ptr(c, np.float64), ldc), False)
+
+
+def trsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb) -> None:
+    """trsm.rs:33 -- op(A) X = alpha B ('L') or X op(A) = alpha B ('R'); X overwrites B."""
+    if _is_dev(b):
+        _check(lib().bola_hip_dtrsm_dev(default_handle().ptr, _c(side), _c(uplo), _c(transa), _c(diag), m, n, alpha,
+                                        _ptr(a, np.float64), lda, _ptr(b, np.float64), ldb), True)
+    else:
+        _check(lib().bola_hip_dtrsm(_c(side), _c(uplo), _c(transa), _c(diag), m, n, alpha, _ptr(a, np.float64), lda,
+                                    _ptr(b, np.float64), ldb), False)
+
+
+def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc) -> None:
+    """syrk.rs:17 -- C = alpha*A*A^T + beta*C ('N') or alpha*A^T*A + beta*C ('T'), one triangle."""
+    if _is_dev(c):
+        _check(lib().bola_hip_dsyrk_dev(default_handle().ptr, _c(uplo), _c(trans), n, k, alpha, _ptr(a, np.float64),
+                                        lda, beta, _ptr(c, np.float64), ldc), True)
+    else:
+        _check(lib().bola_hip_dsyrk(_c(uplo), _c(trans), n, k, alpha, _ptr(a, np.float64), lda, beta,
+                                    _ptr(c, np.float64), ldc), False)
+
+
+# ------------------------------------------------------------------ LAPACK
+
+def getrf(m, n, a, lda, ipiv) -> None:
+    """getrf.rs:37 -- P*A = L*U in place; raises BolaError like the crate's Err."""
+    if _is_dev(a):
+        _check(lib().bola_hip_dgetrf_dev(default_handle().ptr, m, n, _ptr(a, np.float64), lda,
+                                         _ptr(ipiv, np.int32), None), True)
+    else:
+        _check(lib().bola_hip_dgetrf(m, n, _ptr(a, np.float64), lda, _ptr(ipiv, np.int32)), False)
+
+
+def getrf2(m, n, a, lda, ipiv) -> None:
+    """getrf2.rs:39 -- the crate's recursive/unblocked LU, callable directly."""
+    if _is_dev(a):
+        _check(lib().bola_hip_dgetrf2_dev(default_handle().ptr, m, n, _ptr(a, np.float64), lda,
+                                          _ptr(ipiv, np.int32), None), True)
+    else:
+        _check(lib().bola_hip_dgetrf2(m, n, _ptr(a, np.float64), lda, _ptr(ipiv, np.int32)), False)
+
+
+def potrf(uplo, n, a, lda) -> None:
+    """potrf.rs:101 -- Cholesky in place ('L': A = L L^T, 'U': A = U^T U)."""
+    if _is_dev(a):
+        _check(lib().bola_hip_dpotrf_dev(default_handle().ptr, _c(uplo), n, _ptr(a, np.float64), lda, None), True)
+    else:
+        _check(lib().bola_hip_dpotrf(_c(uplo), n, _ptr(a, np.float64), lda), False)
+
+
+def potrf2(uplo, n, a, lda) -> None:
+    """potrf2.rs:25 -- the crate's recursive Cholesky, callable directly."""
+    if _is_dev(a):
+        _check(lib().bola_hip_dpotrf2_dev(default_handle().ptr, _c(uplo), n, _ptr(a, np.float64), lda, None), True)
+    else:
+        _check(lib().bola_hip_dpotrf2(_c(uplo), n, _ptr(a, np.float64), lda), False)
+
+
+def getrs(n, nrhs, lu, lda, ipiv, b, ldb) -> None:
+    """LU solve composed from the crate's parts: laswp, trsm(L,L,N,U), trsm(L,U,N,N)."""
+    if _is_dev(b):
+        _check(lib().bola_hip_dgetrs_dev(default_handle().ptr, n, nrhs, _ptr(lu, np.float64), lda,
+                                         _ptr(ipiv, np.int32), _ptr(b, np.float64), ldb), True)
+    else:
+        _check(lib().bola_hip_dgetrs(n, nrhs, _ptr(lu, np.float64), lda, _ptr(ipiv, np.int32),
+                                     _ptr(b, np.float64), ldb), False)
+
+
+def potrs(uplo, n, nrhs, f, lda, b, ldb) -> None:
+    """Cholesky solve composed from the crate's trsm: 'L' -> (L,L,N,N), (L,L,T,N)."""
+    if _is_dev(b):
+        _check(lib().bola_hip_dpotrs_dev(default_handle().ptr, _c(uplo), n, nrhs, _ptr(f, np.float64), lda,
+                                         _ptr(b, np.float64), ldb), True)
+    else:
+        _check(lib().bola_hip_dpotrs(_c(uplo), n, nrhs, _ptr(f, np.float64), lda, _ptr(b, np.float64), ldb), False)

@@ -1,0 +1,165 @@
+// Shared definitions for the gfx950 dense-solver library: the handle, error plumbing,
+// profiling phases and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "bola_hip.h"
+
+namespace bola {
+
+// Thrown inside the library only; every C-ABI entry point catches it.
+struct HipFailure {
+    hipError_t err;
+    const char* what;
+    int line;
+};
+
+#define BOLA_CHECK(expr)                                           \
+    do {                                                           \
+        hipError_t e_ = (expr);                                    \
+        if (e_ != hipSuccess) throw ::bola::HipFailure{e_, #expr, __LINE__}; \
+    } while (0)
+
+#define BOLA_LAUNCH_CHECK() BOLA_CHECK(hipGetLastError())
+
+// Growable device workspace owned by a handle.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need) {
+        if (need > bytes) {
+            // work already queued may still use the old buffer
+            if (ptr) {
+                BOLA_CHECK(hipDeviceSynchronize());
+                BOLA_CHECK(hipFree(ptr));
+            }
+            ptr = nullptr;
+            size_t nb = need + (need >> 2);
+            BOLA_CHECK(hipMalloc(&ptr, nb));
+            bytes = nb;
+        }
+        return ptr;
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// A composed row permutation: row `row[s]` receives the old contents of row `src[s]`.
+// Produced by the LASWP compose step (one map per group of up to 64 interchanges) and by
+// the LU panel kernel (one map per panel).
+constexpr int ROWMAP_MAX = 128;
+struct RowMap {
+    int32_t count;
+    int32_t pad;
+    int64_t row[ROWMAP_MAX];
+    int64_t src[ROWMAP_MAX];
+};
+
+struct PhaseStat {
+    long calls = 0;
+    double ms = 0.0;
+    double flops = 0.0;
+};
+
+struct PendingEvent {
+    std::string name;
+    hipEvent_t start, stop;
+    double flops;
+};
+
+}  // namespace bola
+
+struct bola_hip_handle_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cus = 0;
+    bola::Workspace ws;         // factorization scratch (panel flags/slots, row maps)
+    bola::Workspace ws_aux;     // LASWP group maps, IDAMAX / DOT partials
+    bola::Workspace stage;      // operands of host-pointer calls
+    int32_t* d_info = nullptr;  // device status word for factorizations
+    std::string last_error;
+    // profiling: per-phase GPU time measured with events on the handle's stream
+    bool profile = false;
+    std::vector<bola::PendingEvent> pending;
+    std::map<std::string, bola::PhaseStat> stats;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace bola {
+
+// RAII phase marker: records events around a phase when profiling is on.
+struct Phase {
+    bola_hip_handle_s* h;
+    const char* name;
+    double flops;
+    hipEvent_t e0 = nullptr;
+    Phase(bola_hip_handle_s* h_, const char* n, double fl = 0.0);
+    ~Phase();
+};
+
+void set_error(bola_hip_handle_s* h, const std::string& msg);
+bola_hip_handle_s* default_handle();
+void collect_profile(bola_hip_handle_s* h);
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- launchers
+// (defined in the .hip translation units; all enqueue on `st`)
+
+// idamax.hip
+void launch_idamax(hipStream_t st, size_t n, const double* x, int64_t* out, void* ws);
+size_t idamax_ws_bytes(size_t n);
+
+// blas12.hip
+void launch_dscal(hipStream_t st, size_t n, double alpha, double* x, size_t incx);
+void launch_daxpy(hipStream_t st, size_t n, double alpha, const double* x, size_t incx, double* y, size_t incy);
+void launch_ddot(hipStream_t st, size_t n, const double* x, size_t incx, const double* y, size_t incy, double* out,
+                 void* ws);
+size_t ddot_ws_bytes(size_t n);
+void launch_dger(hipStream_t st, size_t m, size_t n, double alpha, const double* x, const double* y, size_t ldy,
+                 double* a, size_t lda);
+void launch_fill(hipStream_t st, double* a, size_t lda, size_t m, size_t n, double v);
+void launch_scale(hipStream_t st, double* a, size_t lda, size_t m, size_t n, double beta);
+void launch_scale_tri(hipStream_t st, char uplo, double* a, size_t lda, size_t n, double beta);
+
+// laswp.hip
+size_t laswp_ws_bytes(size_t npiv);
+void launch_laswp(hipStream_t st, size_t n, double* a, size_t lda, int32_t k1, int32_t k2, const int32_t* ipiv,
+                  int32_t incx, void* ws, const int32_t* abort = nullptr);
+// Applies `nmaps` row maps in order to the columns [0, left) and [skip_to, skip_to + right)
+// of a.  When fix_ipiv is non-null, block 0 also adds fix_off to fix_ipiv[0..fix_cnt) --
+// both only when *abort is zero.
+void launch_rowmap_apply(hipStream_t st, double* a, size_t lda, size_t left, size_t skip_to, size_t right,
+                         const RowMap* maps, int nmaps, const int32_t* abort, int32_t* fix_ipiv, int fix_cnt,
+                         int fix_off);
+
+// gemm.hip
+// `abort`: optional device status word; when it is nonzero at kernel start the launch does nothing
+// (used inside factorizations so that work queued behind a failed pivot is skipped).
+void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
+                  const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
+                  const int32_t* abort = nullptr);
+// slab: depth of one accumulate-then-fold step (64 = gemm.rs KC; 32 = the syrk.rs block
+// loop); mask: 0 all of C, 1 only rows in a later 32-block than the column, 2 earlier.
+void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
+                     const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
+                     const int32_t* abort, int slab, int mask);
+void set_num_cus(int n);
+
+// trsm.hip
+void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
+                  const double* a, size_t lda, double* b, size_t ldb, const int32_t* abort = nullptr);
+
+}  // namespace bola

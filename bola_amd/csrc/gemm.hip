@@ -1,0 +1,306 @@
+// DGEMM on gfx950 MFMA (v_mfma_f64_16x16x4_f64), gemm.rs:88-211.
+//
+// C = alpha*op(A)*op(B) + beta*C, column-major.  The kernel keeps the reference's
+// numerical structure so that every element sees the same sequence of roundings:
+//   * beta: C*beta (one multiply) or 0 when beta == 0, applied as C is loaded
+//     (gemm.rs:110-122);
+//   * k is consumed in slabs of 64 (KC, gemm.rs:15): a slab's products accumulate from
+//     zero in increasing k (the MFMA chain), then C = fma(alpha, acc, C)
+//     (gemm.rs:245-268); the slab partials are never summed among themselves;
+//   * v_mfma_f64_16x16x4_f64 rounds exactly as four chained correctly rounded FMAs in
+//     increasing k (measured on gfx950), so the slab accumulator equals the reference's
+//     _mm256_fmadd_pd chain bit for bit;
+//   * the last m % 8 rows and the last n % 4 columns are the reference's partial 8x4
+//     micro-tiles (gemm.rs:177-190), which it computes unfused as C += (alpha*b)*a in
+//     increasing k; a separate kernel does exactly that for those elements.
+//
+// Tiling: 256 threads = 4 wavefronts in a 2x2 grid; each wavefront owns a
+// (BM/2)x(BN/2) block of 16x16 MFMA tiles.  K advances 16 at a time through a
+// double-buffered LDS stage.  LDS layouts put each MFMA operand read on 32 distinct
+// banks: A as [k][BM] (or [m][BK+1] when op(A) = A^T), B as [n][BK+1] (or [k][BN] when
+// op(B) = B^T).  Block ids are remapped so each XCD's L2 serves a compact C region.
+#include "common.hpp"
+
+namespace bola {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BK = 16;
+constexpr int KC = 64;  // reference slab depth
+
+template <int BM, int BN, bool TA, bool TB>
+struct GemmCfg {
+    static constexpr int WM = BM / 2, WN = BN / 2;  // wavefront tile
+    static constexpr int TM = WM / 16, TN = WN / 16;  // MFMA tiles per wavefront
+    // LDS strides (doubles)
+    static constexpr int A_STRIDE = TA ? (BK + 1) : BM;   // TA: [m][BK+1]; else [k][BM]
+    static constexpr int A_SIZE = TA ? BM * (BK + 1) : BK * BM;
+    static constexpr int B_STRIDE = TB ? BN : (BK + 1);   // TB: [k][BN]; else [n][BK+1]
+    static constexpr int B_SIZE = TB ? BK * BN : BN * (BK + 1);
+    static constexpr int A_LOADS = BM * BK / 256;
+    static constexpr int B_LOADS = BN * BK / 256;
+};
+
+// Element (i, kk) of the BMxBK tile of op(A) handled by thread t for load slot s.
+template <int BM, bool TA>
+__device__ __forceinline__ void a_coord(int t, int s, int& i, int& kk) {
+    int e = t + 256 * s;
+    if (!TA) { i = e % BM; kk = e / BM; }       // column-major A: m fastest
+    else { kk = e % BK; i = e / BK; }           // A^T: k fastest
+}
+template <int BN, bool TB>
+__device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
+    int e = t + 256 * s;
+    if (!TB) { kk = e % BK; j = e / BK; }       // column-major B: k fastest
+    else { j = e % BN; kk = e / BN; }           // B^T: n fastest
+}
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
+                                                  size_t lda, const double* __restrict__ B, size_t ldb, double beta,
+                                                  double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
+                                                  int edge_m, int edge_n, const int32_t* __restrict__ abort,
+                                                  int slab, int mask) {
+    using Cfg = GemmCfg<BM, BN, TA, TB>;
+    if (abort && *abort != 0) return;
+    __shared__ double sA[2][Cfg::A_SIZE];
+    __shared__ double sB[2][Cfg::B_SIZE];
+
+    // XCD-aware block remap: consecutive tile indices land on the same XCD.
+    const int ntiles = tiles_m * tiles_n;
+    int bid = blockIdx.x;
+    if ((ntiles & 7) == 0) bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
+    // grouped ordering: 8 tile-rows at a time, column-major inside a group
+    const int GROUP = 8;
+    int group = bid / (GROUP * tiles_n);
+    int first_m = group * GROUP;
+    int gsz = min(tiles_m - first_m, GROUP);
+    int tm = first_m + (bid % (GROUP * tiles_n)) % gsz;
+    int tn = (bid % (GROUP * tiles_n)) / gsz;
+    const int row0 = tm * BM, col0 = tn * BN;
+    // triangle mask (SYRK off-diagonal blocks): 1 = rows in a later 32-block than the
+    // column, 2 = rows in an earlier 32-block; skip tiles with no such element
+    if (mask == 1 && (row0 + BM - 1) / 32 <= col0 / 32) return;
+    if (mask == 2 && row0 / 32 >= (col0 + BN - 1) / 32) return;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * Cfg::WM, wn = (w >> 1) * Cfg::WN;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    // running C (reference: C after beta, updated once per 64-slab) and slab accumulator
+    d4 crun[Cfg::TM][Cfg::TN];
+    d4 acc[Cfg::TM][Cfg::TN];
+#pragma unroll
+    for (int a = 0; a < Cfg::TM; a++)
+#pragma unroll
+        for (int b = 0; b < Cfg::TN; b++) {
+            int gc = col0 + wn + 16 * b + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int gr = row0 + wm + 16 * a + lk + 4 * r;
+                double v = 0.0;
+                if (gr < m && gc < n && beta != 0.0) {
+                    v = C[gr + (size_t)gc * ldc];
+                    if (beta != 1.0) v = v * beta;
+                }
+                crun[a][b][r] = v;
+                acc[a][b][r] = 0.0;
+            }
+        }
+
+    double ra[Cfg::A_LOADS], rb[Cfg::B_LOADS];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < Cfg::A_LOADS; s++) {
+            int i, kk;
+            a_coord<BM, TA>(t, s, i, kk);
+            int gi = row0 + i, gk = k0 + kk;
+            double v = 0.0;
+            if (gi < m && gk < k) v = TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda];
+            ra[s] = v;
+        }
+#pragma unroll
+        for (int s = 0; s < Cfg::B_LOADS; s++) {
+            int kk, j;
+            b_coord<BN, TB>(t, s, kk, j);
+            int gj = col0 + j, gk = k0 + kk;
+            double v = 0.0;
+            if (gj < n && gk < k) v = TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb];
+            rb[s] = v;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < Cfg::A_LOADS; s++) {
+            int i, kk;
+            a_coord<BM, TA>(t, s, i, kk);
+            if (!TA) sA[buf][kk * BM + i] = ra[s];
+            else sA[buf][i * (BK + 1) + kk] = ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < Cfg::B_LOADS; s++) {
+            int kk, j;
+            b_coord<BN, TB>(t, s, kk, j);
+            if (!TB) sB[buf][j * (BK + 1) + kk] = rb[s];
+            else sB[buf][kk * BN + j] = rb[s];
+        }
+    };
+
+    const int nsteps = (k + BK - 1) / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int step = 0; step < nsteps; step++) {
+        const int buf = step & 1;
+        if (step + 1 < nsteps) gload((step + 1) * BK);
+#pragma unroll
+        for (int k4 = 0; k4 < BK; k4 += 4) {
+            double fa[Cfg::TM], fb[Cfg::TN];
+#pragma unroll
+            for (int a = 0; a < Cfg::TM; a++) {
+                int i = wm + 16 * a + lr, kk = k4 + lk;
+                fa[a] = TA ? sA[buf][i * (BK + 1) + kk] : sA[buf][kk * BM + i];
+            }
+#pragma unroll
+            for (int b = 0; b < Cfg::TN; b++) {
+                int j = wn + 16 * b + lr, kk = k4 + lk;
+                fb[b] = TB ? sB[buf][kk * BN + j] : sB[buf][j * (BK + 1) + kk];
+            }
+#pragma unroll
+            for (int a = 0; a < Cfg::TM; a++)
+#pragma unroll
+                for (int b = 0; b < Cfg::TN; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+        // end of a 64-deep slab (or of k): fold the slab into C, restart the accumulator
+        const int kdone = (step + 1) * BK;
+        if ((kdone % slab) == 0 || step + 1 == nsteps) {
+#pragma unroll
+            for (int a = 0; a < Cfg::TM; a++)
+#pragma unroll
+                for (int b = 0; b < Cfg::TN; b++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        crun[a][b][r] = fma(alpha, acc[a][b][r], crun[a][b][r]);
+                        acc[a][b][r] = 0.0;
+                    }
+        }
+        if (step + 1 < nsteps) {
+            lstore(buf ^ 1);
+            __syncthreads();
+        }
+    }
+
+#pragma unroll
+    for (int a = 0; a < Cfg::TM; a++)
+#pragma unroll
+        for (int b = 0; b < Cfg::TN; b++) {
+            int gc = col0 + wn + 16 * b + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int gr = row0 + wm + 16 * a + lk + 4 * r;
+                bool keep = gr < m && gc < n && gr < edge_m && gc < edge_n;
+                if (mask == 1) keep = keep && (gr / 32 > gc / 32);
+                if (mask == 2) keep = keep && (gr / 32 < gc / 32);
+                if (keep) C[gr + (size_t)gc * ldc] = crun[a][b][r];
+            }
+        }
+}
+
+// The reference's partial micro-tiles: rows >= edge_m or columns >= edge_n.
+// C = C*beta (or 0), then C += (alpha*b_lj)*a_il unfused for l = 0..k-1.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double alpha, const double* __restrict__ A,
+                                                  size_t lda, const double* __restrict__ B, size_t ldb, double beta,
+                                                  double* __restrict__ C, size_t ldc, int edge_m, int edge_n,
+                                                  const int32_t* __restrict__ abort, int mask) {
+    if (abort && *abort != 0) return;
+    // enumerate edge elements: first the edge rows for all columns, then the edge columns
+    // for the interior rows
+    const int er = m - edge_m, ec = n - edge_n;
+    const long n1 = (long)er * n;
+    const long total = n1 + (long)edge_m * ec;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        if (e < n1) { i = edge_m + (int)(e % er); j = (int)(e / er); }
+        else { long f = e - n1; i = (int)(f % edge_m); j = edge_n + (int)(f / edge_m); }
+        if (mask == 1 && !(i / 32 > j / 32)) continue;
+        if (mask == 2 && !(i / 32 < j / 32)) continue;
+        double c = 0.0;
+        if (beta != 0.0) {
+            c = C[i + (size_t)j * ldc];
+            if (beta != 1.0) c = c * beta;
+        }
+        if (alpha != 0.0) {
+            for (int l = 0; l < k; l++) {
+                double bv = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+                double av = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
+                double temp = alpha * bv;
+                c = c + temp * av;
+            }
+        }
+        C[i + (size_t)j * ldc] = c;
+    }
+}
+
+template <int BM, int BN, bool TA, bool TB>
+void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
+                 size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, const int32_t* abort,
+                 int slab, int mask) {
+    int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+    BOLA_LAUNCH_CHECK();
+}
+
+template <bool TA, bool TB>
+void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
+              size_t ldb, double beta, double* C, size_t ldc, int num_cus, const int32_t* abort, int slab,
+              int mask) {
+    const int edge_m = m - (m % 8), edge_n = n - (n % 4);
+    if (edge_m > 0 && edge_n > 0) {
+        long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
+        if (t128 >= 2L * num_cus)
+            launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab, mask);
+        else
+            launch_tile<64, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
+                                        mask);
+    }
+    long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
+    if (nedge > 0) {
+        unsigned blocks = (unsigned)std::min<long>((nedge + 255) / 256, 4096);
+        hipLaunchKernelGGL((dgemm_edge<TA, TB>), dim3(blocks), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, edge_m, edge_n, abort, mask);
+        BOLA_LAUNCH_CHECK();
+    }
+}
+
+int g_num_cus = 256;
+
+}  // namespace
+
+void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
+
+// Requires m, n, k < 2^31.  alpha == 0 must be handled by the caller (beta-only pass).
+void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
+                  const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
+                  const int32_t* abort) {
+    launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0);
+}
+
+void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
+                     const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
+                     const int32_t* abort, int slab, int mask) {
+    bool ta = !(transa == 'N' || transa == 'n');
+    bool tb = !(transb == 'N' || transb == 'n');
+    int M = (int)m, N = (int)n, K = (int)k;
+    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
+    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
+    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
+    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
+}
+
+}  // namespace bola

@@ -1,0 +1,575 @@
+// LU with partial pivoting on gfx950: getrf.rs:37-101, getrf2.rs:39-185.
+//
+// The factorization keeps the reference's structure -- 32-column panels factored by
+// the unblocked iterative_getrf2, the panel's interchanges applied to the columns on
+// either side, TRSM(L,L,N,U) of the U12 block row and a rank-32 GEMM update -- because
+// that structure fixes the rounding of every element.  Each step runs on the GPU:
+//
+//   * getf2_panel: the unblocked panel as one persistent multi-workgroup kernel.  Rows
+//     of the m x w panel (w <= 32) live in registers, 256*R rows per workgroup.  Rows are
+//     never moved during the panel: each thread tracks the logical position of its rows,
+//     which is what the reference's physical swaps change.  Per column: every workgroup
+//     reduces its candidates under the reference's IDAMAX rule (idamax.rs:23-133, keys
+//     relative to the column's diagonal row), publishes its best vector-phase and tail
+//     candidates with their row data, and raises its own flag; wavefront 0 of every
+//     workgroup waits for all flags and picks the same pivot.  Then scale by the
+//     reciprocal (getrf2.rs:156-164) and the GER with its pair/fused/tail structure
+//     (ger.rs:28-90), all in registers.  At the end rows are stored at their final
+//     positions and the composed permutation is emitted as a row map.
+//   * the row map is applied to the left and right column ranges (laswp.rs semantics,
+//     pure moves), the U12 TRSM uses the reference's base-case arithmetic, and the
+//     trailing update is the slab-exact MFMA DGEMM.
+//   * the m <= 32 < n case of iterative_getrf2 (one short, wide panel) runs as three
+//     small kernels per column.
+//
+// Errors: a zero pivot stops the panel; every later kernel of the factorization sees
+// the status word and does nothing, so the matrix is left as the reference leaves it
+// (partially factored, panel pivots panel-local).
+#include "common.hpp"
+
+namespace bola {
+
+namespace {
+
+constexpr int PW = 32;     // panel width (getrf.rs:11 NB, getrf2.rs:11)
+constexpr int PT = 256;    // threads per panel workgroup
+constexpr uint64_t NOKEY = ~0ull;
+constexpr uint64_t KEYMASK = (1ull << 60) - 1;
+
+struct Cand {
+    double v;
+    uint64_t k;
+};
+
+struct PanelSlot {
+    double v;
+    uint64_t key;
+    double row[PW];
+};
+
+__device__ __forceinline__ bool better(double v, uint64_t k, double cv, uint64_t ck) {
+    return v > cv || (v == cv && k < ck);
+}
+__device__ __forceinline__ uint64_t vec_key(uint64_t i) {
+    return ((i & 3ull) << 62) | (((i >> 2) & 3ull) << 60) | i;
+}
+
+__device__ __forceinline__ double getc(const double (&row)[PW], int j) {
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < PW; c++)
+        if (c == j) v = row[c];
+    return v;
+}
+__device__ __forceinline__ void setc(double (&row)[PW], int j, double v) {
+#pragma unroll
+    for (int c = 0; c < PW; c++)
+        if (c == j) row[c] = v;
+}
+
+// reduce (v, k) pairs over the workgroup; result broadcast through LDS to every thread
+__device__ void wg_reduce(Cand& c, Cand* sh, Cand* out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(c.v, off, 64);
+        uint64_t ok = (uint64_t)__shfl_xor((long long)c.k, off, 64);
+        if (better(ov, ok, c.v, c.k)) { c.v = ov; c.k = ok; }
+    }
+    if (lane == 0) sh[w] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand b = sh[0];
+        for (int q = 1; q < PT / 64; q++)
+            if (better(sh[q].v, sh[q].k, b.v, b.k)) b = sh[q];
+        *out = b;
+    }
+    __syncthreads();
+    c = *out;
+}
+
+// Registers hold the not-yet-retired columns of each row as a window that shifts left
+// by one column per step, so the current column is always a[.][0] and every register
+// index is a compile-time constant; a retired column goes to LDS (sret).
+template <int R>
+__global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restrict__ A, size_t lda,
+                                                  int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
+                                                  long row_off, int* __restrict__ flags, PanelSlot* __restrict__ slots,
+                                                  PanelSlot* __restrict__ cur, RowMap* __restrict__ map, int nwg,
+                                                  long spin_limit) {
+    __shared__ Cand red[PT / 64];
+    __shared__ Cand rv, rt;
+    __shared__ double prow[PW + 1];
+    __shared__ int ctl[2];
+    __shared__ double sret[PW * PT * R];  // sret[c * PT*R + s*PT + tid]
+    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double a[R][PW];
+    int pos[R];
+    long rowid[R];
+    bool valid[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        long r = (long)g * PT * R + (long)s * PT + tid;
+        rowid[s] = r;
+        valid[s] = r < m;
+        pos[s] = (int)r;
+#pragma unroll
+        for (int c = 0; c < PW; c++) a[s][c] = (valid[s] && c < w) ? A[r + (size_t)c * lda] : 0.0;
+    }
+    if (tid == 0) prow[PW] = 0.0;
+
+    const int min_mn = min(m, w);
+    int nret = 0;  // retired columns
+    for (int j = 0; j < min_mn; j++) {
+        const int len = m - j;
+        const int T = len < 16 ? 0 : (len & ~15);
+        Cand vb{-1.0, NOKEY}, tb{-1.0, NOKEY};
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s] && pos[s] >= j) {
+                double v = fabs(a[s][0]);
+                if (v == v) {
+                    uint64_t rel = (uint64_t)(pos[s] - j);
+                    if ((long)rel < T) {
+                        uint64_t k = vec_key(rel);
+                        if (better(v, k, vb.v, vb.k)) { vb.v = v; vb.k = k; }
+                    } else if (better(v, rel, tb.v, tb.k)) {
+                        tb.v = v;
+                        tb.k = rel;
+                    }
+                }
+            }
+        }
+        wg_reduce(vb, red, &rv);
+        wg_reduce(tb, red, &rt);
+        const int par = j & 1;
+        PanelSlot* mine = slots + ((size_t)par * nwg + g) * 2;
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s] && pos[s] >= j) {
+                uint64_t rel = (uint64_t)(pos[s] - j);
+                PanelSlot* dst0 = nullptr;
+                PanelSlot* dst1 = nullptr;
+                if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) dst0 = mine; }
+                else if (tb.v >= 0.0 && rel == tb.k) dst0 = mine + 1;
+                if (rel == 0) dst1 = cur + par;
+                // columns j..w-1 of the row (the only ones the step reads)
+#pragma unroll
+                for (int c = 0; c < PW; c++) {
+                    if (c < w - j) {
+                        if (dst0) dst0->row[j + c] = a[s][c];
+                        if (dst1) dst1->row[j + c] = a[s][c];
+                    }
+                }
+            }
+        }
+        if (tid == 0) {
+            mine[0].v = vb.v; mine[0].key = vb.k;
+            mine[1].v = tb.v; mine[1].key = tb.k;
+        }
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+
+        if (wv == 0) {
+            int ok = 1;
+            for (int q = lane; q < nwg; q += 64) {
+                long sp = 0;
+                while (__hip_atomic_load(flags + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
+                    if (++sp > spin_limit ||
+                        __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            ok = __ballot(!ok) == 0ull;
+            Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+            int gvs = -1, gts = -1;
+            if (ok) {
+                for (int q = lane; q < nwg; q += 64) {
+                    const PanelSlot* S = slots + ((size_t)par * nwg + q) * 2;
+                    double v0 = S[0].v, v1 = S[1].v;
+                    uint64_t k0 = S[0].key, k1 = S[1].key;
+                    if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
+                    if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
+                }
+            }
+            for (int off = 32; off >= 1; off >>= 1) {
+                double ov = __shfl_xor(gv.v, off, 64);
+                uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
+                int os = __shfl_xor(gvs, off, 64);
+                if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
+                ov = __shfl_xor(gt.v, off, 64);
+                okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
+                os = __shfl_xor(gts, off, 64);
+                if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+            }
+            // the reference's final rule (idamax.rs:27-38, 91-130)
+            long rel;
+            int src;  // slot index, or -1 for the row currently at position j
+            if (len < 16) {
+                rel = gt.v >= 0.0 ? (long)gt.k : 0;
+                src = gts;
+            } else {
+                double V = 0.0;
+                long vi = 0;
+                if (gv.v > 0.0) { V = gv.v; vi = (long)(gv.k & KEYMASK); }
+                if (gt.v > V) { rel = (long)gt.k; src = gts; }
+                else { rel = vi; src = gvs; }
+            }
+            if (rel == 0) src = -1;
+            const PanelSlot* P = src < 0 ? cur + par : slots + (size_t)par * nwg * 2 + src;
+            if (ok && lane >= j && lane < w) prow[lane] = P->row[lane];
+            if (lane == 0) { ctl[0] = (int)rel; ctl[1] = ok; }
+        }
+        __syncthreads();
+        if (!ctl[1]) {
+            // a workgroup never arrived (or another one failed): give up without writing
+            if (g == 0 && tid == 0)
+                __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const int jp = j + ctl[0];
+        const double piv = prow[j];
+        if (g == 0 && tid == 0) ipiv[j] = jp + 1;
+        if (!(piv != 0.0)) {
+            if (g == 0 && tid == 0) {
+                info[1] = j + 1;
+                __hip_atomic_store(info, col_off + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        if (jp != j) {
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (pos[s] == j) pos[s] = jp;
+                else if (pos[s] == jp) pos[s] = j;
+            }
+        }
+        if (j < m - 1) {
+            const double inv = 1.0 / piv;
+#pragma unroll
+            for (int s = 0; s < R; s++)
+                if (valid[s] && pos[s] > j) a[s][0] = a[s][0] * inv;
+        }
+        if (j < min_mn - 1) {
+            const int np = w - j - 1;
+            const int nu = np - (np & 1);
+            const int mf = (m - j - 1) & ~3;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (valid[s] && pos[s] > j) {
+                    const double x = a[s][0];
+                    const bool frow = (pos[s] - (j + 1)) < mf;
+#pragma unroll
+                    for (int c = 1; c < PW; c++) {
+                        if (c <= np) {
+                            const int rc = c - 1;  // column index relative to j+1
+                            const double tc = -1.0 * prow[j + c];
+                            const bool paired = rc < nu;
+                            bool skip;
+                            if (paired) {
+                                const double tp = -1.0 * ((rc & 1) ? prow[j + c - 1] : prow[j + c + 1]);
+                                skip = (tc == 0.0 && tp == 0.0);
+                            } else {
+                                skip = (tc == 0.0);
+                            }
+                            if (!skip) a[s][c] = (paired && frow) ? fma(x, tc, a[s][c]) : a[s][c] + tc * x;
+                        }
+                    }
+                }
+            }
+        }
+        // retire column j and shift the window
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            sret[(size_t)j * (PT * R) + s * PT + tid] = a[s][0];
+#pragma unroll
+            for (int c = 0; c < PW - 1; c++) a[s][c] = a[s][c + 1];
+            a[s][PW - 1] = 0.0;
+        }
+        nret = j + 1;
+    }
+    // rows to their final positions; moved rows form the panel's row map
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        if (valid[s]) {
+            for (int c = 0; c < nret; c++)
+                A[(size_t)pos[s] + (size_t)c * lda] = sret[(size_t)c * (PT * R) + s * PT + tid];
+#pragma unroll
+            for (int c = 0; c < PW; c++)
+                if (nret + c < w) A[(size_t)pos[s] + (size_t)(nret + c) * lda] = a[s][c];
+            if (pos[s] != rowid[s]) {
+                int idx = atomicAdd(&map->count, 1);
+                if (idx < ROWMAP_MAX) {
+                    map->row[idx] = row_off + pos[s];
+                    map->src[idx] = row_off + rowid[s];
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_zero_i32(int32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---- the m <= 32 < n panel (one thread per column; three launches per pivot column)
+__global__ void wide_pivot(int m, double* __restrict__ A, size_t lda, int j, int32_t* __restrict__ ipiv,
+                           int32_t* __restrict__ info, int col_off, int32_t* __restrict__ jp_out) {
+    if (*info != 0) return;
+    const int lane = threadIdx.x;  // 64 threads, m - j <= 32
+    const int len = m - j;
+    const int T = len < 16 ? 0 : (len & ~15);
+    Cand vb{-1.0, NOKEY}, tb{-1.0, NOKEY};
+    if (lane < len) {
+        double v = fabs(A[(size_t)(j + lane) + (size_t)j * lda]);
+        if (v == v) {
+            if (lane < T) { vb.v = v; vb.k = vec_key((uint64_t)lane); }
+            else { tb.v = v; tb.k = (uint64_t)lane; }
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(vb.v, off, 64);
+        uint64_t ok = (uint64_t)__shfl_xor((long long)vb.k, off, 64);
+        if (better(ov, ok, vb.v, vb.k)) { vb.v = ov; vb.k = ok; }
+        ov = __shfl_xor(tb.v, off, 64);
+        ok = (uint64_t)__shfl_xor((long long)tb.k, off, 64);
+        if (better(ov, ok, tb.v, tb.k)) { tb.v = ov; tb.k = ok; }
+    }
+    if (lane == 0) {
+        long rel;
+        if (len < 16) rel = tb.v >= 0.0 ? (long)tb.k : 0;
+        else {
+            double V = 0.0;
+            long vi = 0;
+            if (vb.v > 0.0) { V = vb.v; vi = (long)(vb.k & KEYMASK); }
+            rel = (tb.v > V) ? (long)tb.k : vi;
+        }
+        const int jp = j + (int)rel;
+        ipiv[j] = jp + 1;
+        *jp_out = jp;
+        if (!(A[(size_t)jp + (size_t)j * lda] != 0.0)) {
+            info[1] = j + 1;
+            info[0] = col_off + j + 1;
+        }
+    }
+}
+
+__global__ void wide_swap_scale(int m, int n, double* __restrict__ A, size_t lda, int j,
+                                const int32_t* __restrict__ jp_in, const int32_t* __restrict__ info) {
+    if (*info != 0) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int jp = *jp_in;
+    double* col = A + (size_t)c * lda;
+    if (jp != j) {
+        double t = col[j];
+        col[j] = col[jp];
+        col[jp] = t;
+    }
+    if (c == j && j < m - 1) {
+        const double inv = 1.0 / col[j];
+        for (int i = j + 1; i < m; i++) col[i] *= inv;
+    }
+}
+
+__global__ void wide_ger(int m, int n, double* __restrict__ A, size_t lda, int j, const int32_t* __restrict__ info) {
+    if (*info != 0) return;
+    const int c = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int np = n - j - 1, nu = np - (np & 1), rc = c - (j + 1);
+    const int mf = (m - j - 1) & ~3;
+    const double tc = -1.0 * A[(size_t)j + (size_t)c * lda];
+    const bool paired = rc < nu;
+    bool skip;
+    if (paired) {
+        const int pc = (rc & 1) ? c - 1 : c + 1;
+        const double tp = -1.0 * A[(size_t)j + (size_t)pc * lda];
+        skip = tc == 0.0 && tp == 0.0;
+    } else {
+        skip = tc == 0.0;
+    }
+    if (skip) return;
+    const double* x = A + (size_t)j * lda;
+    double* col = A + (size_t)c * lda;
+    for (int i = j + 1; i < m; i++) {
+        const bool fr = (i - (j + 1)) < mf;
+        col[i] = (paired && fr) ? fma(x[i], tc, col[i]) : col[i] + tc * x[i];
+    }
+}
+
+__global__ void k_ipiv_add(int32_t* __restrict__ ipiv, int cnt, int off, const int32_t* __restrict__ info) {
+    if (*info != 0) return;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) ipiv[i] += off;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host drivers
+
+namespace {
+
+struct LuWs {
+    int* flags;         // [npanels][nwg_max]
+    PanelSlot* slots;   // [2][nwg_max][2]
+    PanelSlot* cur;     // [2]
+    RowMap* maps;       // [npanels]
+    int32_t* jp;        // wide-panel pivot scratch
+    int nwg_max;
+    int npanels;
+};
+
+int panel_rows_per_thread(int m) { return m <= PT ? 1 : 2; }
+int panel_nwg(int m) {
+    int r = panel_rows_per_thread(m);
+    return (m + PT * r - 1) / (PT * r);
+}
+
+LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
+    LuWs w;
+    w.nwg_max = std::max(1, panel_nwg(m));
+    w.npanels = std::max(1, npanels);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    size_t o_flags = take(sizeof(int) * (size_t)w.npanels * w.nwg_max);
+    size_t o_maps = take(sizeof(RowMap) * (size_t)w.npanels);
+    size_t o_jp = take(sizeof(int32_t) * 4);
+    size_t zero_bytes = off;
+    size_t o_slots = take(sizeof(PanelSlot) * 4 * (size_t)w.nwg_max);
+    size_t o_cur = take(sizeof(PanelSlot) * 2);
+    char* base = (char*)h->ws.get(off);
+    w.flags = (int*)(base + o_flags);
+    w.maps = (RowMap*)(base + o_maps);
+    w.jp = (int32_t*)(base + o_jp);
+    w.slots = (PanelSlot*)(base + o_slots);
+    w.cur = (PanelSlot*)(base + o_cur);
+    size_t nzero = zero_bytes / 4;
+    hipLaunchKernelGGL(k_zero_i32, dim3(std::min<size_t>((nzero + 255) / 256, 1024)), dim3(256), 0, h->stream,
+                       (int32_t*)base, nzero);
+    BOLA_LAUNCH_CHECK();
+    return w;
+}
+
+const long kSpinLimit = 1L << 22;
+
+// iterative_getrf2 (getrf2.rs:131-185) on an m x n block, n <= 32 or m <= 32.
+void getf2(bola_hip_handle_s* h, int m, int n, double* a, size_t lda, int32_t* ipiv, int32_t* info, int col_off,
+           long row_off, const LuWs& w, int panel_idx) {
+    hipStream_t st = h->stream;
+    if (n <= PW) {
+        const int r = panel_rows_per_thread(m);
+        const int nwg = panel_nwg(m);
+        int* flags = w.flags + (size_t)panel_idx * w.nwg_max;
+        RowMap* map = w.maps + panel_idx;
+        if (r == 1)
+            hipLaunchKernelGGL(getf2_panel<1>, dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off, row_off,
+                               flags, w.slots, w.cur, map, nwg, kSpinLimit);
+        else
+            hipLaunchKernelGGL(getf2_panel<2>, dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off, row_off,
+                               flags, w.slots, w.cur, map, nwg, kSpinLimit);
+        BOLA_LAUNCH_CHECK();
+        return;
+    }
+    // m <= 32 < n
+    const int min_mn = std::min(m, n);
+    for (int j = 0; j < min_mn; j++) {
+        hipLaunchKernelGGL(wide_pivot, dim3(1), dim3(64), 0, st, m, a, lda, j, ipiv, info, col_off, w.jp);
+        BOLA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(wide_swap_scale, dim3(cdiv(n, 256)), dim3(256), 0, st, m, n, a, lda, j, w.jp, info);
+        BOLA_LAUNCH_CHECK();
+        if (j < min_mn - 1 && j + 1 < n) {
+            hipLaunchKernelGGL(wide_ger, dim3(cdiv(n - j - 1, 256)), dim3(256), 0, st, m, n, a, lda, j, info);
+            BOLA_LAUNCH_CHECK();
+        }
+    }
+}
+
+// getrf2, getrf2.rs:39-97 (recursive above 32 in both dimensions)
+void getrf2_rec(bola_hip_handle_s* h, int m, int n, double* a, size_t lda, int32_t* ipiv, int32_t* info, int col_off,
+                const LuWs& w, void* laswp_ws) {
+    if (m == 0 || n == 0) return;
+    if (m <= PW || n <= PW) {
+        // fresh flags / row map for this leaf panel
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, h->stream, (int32_t*)w.flags, (size_t)w.nwg_max);
+        BOLA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, h->stream, (int32_t*)w.maps, (size_t)1);
+        BOLA_LAUNCH_CHECK();
+        getf2(h, m, n, a, lda, ipiv, info, col_off, 0, w, 0);
+        // the panel kernel's own row map is not needed here: the recursion applies
+        // interchanges through laswp with the pivots, as the reference does
+        return;
+    }
+    hipStream_t st = h->stream;
+    const int n1 = std::min(m, n) / 2, n2 = n - n1;
+    getrf2_rec(h, m, n1, a, lda, ipiv, info, col_off, w, laswp_ws);
+    launch_laswp(st, (size_t)n2, a + (size_t)n1 * lda, lda, 1, n1, ipiv, 1, laswp_ws, info);
+    launch_dtrsm(st, 'L', 'L', 'N', 'U', (size_t)n1, (size_t)n2, 1.0, a, lda, a + (size_t)n1 * lda, lda, info);
+    if (n1 < m)
+        launch_dgemm(st, 'N', 'N', (size_t)(m - n1), (size_t)n2, (size_t)n1, -1.0, a + n1, lda, a + (size_t)n1 * lda,
+                     lda, 1.0, a + n1 + (size_t)n1 * lda, lda, info);
+    if (n1 < m) getrf2_rec(h, m - n1, n2, a + n1 + (size_t)n1 * lda, lda, ipiv + n1, info, col_off + n1, w, laswp_ws);
+    const int mn = std::min(m, n);
+    hipLaunchKernelGGL(k_ipiv_add, dim3(1), dim3(256), 0, st, ipiv + n1, mn - n1, n1, info);
+    BOLA_LAUNCH_CHECK();
+    launch_laswp(st, (size_t)n1, a, lda, n1 + 1, mn, ipiv, 1, laswp_ws, info);
+}
+
+}  // namespace
+
+// Status words: info[0] = 0 or global 1-based failing column (or BOLA_HIP_ERR_DEVICE);
+// info[1] = the column number the reference prints (local to its panel).
+void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
+               bool recursive_entry) {
+    hipStream_t st = h->stream;
+    const int m = (int)m_, n = (int)n_;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (size_t)2);
+    BOLA_LAUNCH_CHECK();
+    if (m == 0 || n == 0) return;
+    const int min_mn = std::min(m, n);
+    if (recursive_entry || PW >= min_mn) {
+        Phase ph(h, recursive_entry ? "GETRF2" : "GETRF");
+        void* lws = h->ws_aux.get(laswp_ws_bytes((size_t)min_mn));
+        LuWs w = lu_workspace(h, m, 1);
+        getrf2_rec(h, m, n, a, lda, ipiv, info, 0, w, lws);
+        return;
+    }
+    Phase ph(h, "GETRF", 2.0 * (double)min_mn * min_mn * (3.0 * std::max(m, n) - min_mn) / 6.0);
+    const int npanels = (min_mn + PW - 1) / PW;
+    LuWs w = lu_workspace(h, m, npanels);
+    for (int p = 0; p < npanels; p++) {
+        const int j = p * PW;
+        const int jb = std::min(PW, min_mn - j);
+        {
+            Phase pp(h, "GETRF2_IT");
+            getf2(h, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
+        }
+        {
+            Phase pl(h, "LASWP");
+            const size_t right = (j + jb < n) ? (size_t)(n - j - jb) : 0;
+            launch_rowmap_apply(st, a, lda, (size_t)j, (size_t)(j + jb), right, w.maps + p, 1, info, ipiv + j, jb, j);
+        }
+        if (j + jb < n) {
+            double* l11 = a + j + (size_t)j * lda;
+            double* u12 = a + j + (size_t)(j + jb) * lda;
+            {
+                Phase pt(h, "TRSM", (double)jb * jb * (n - j - jb));
+                launch_dtrsm(st, 'L', 'L', 'N', 'U', (size_t)jb, (size_t)(n - j - jb), 1.0, l11, lda, u12, lda, info);
+            }
+            if (j + jb < m) {
+                Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * (n - j - jb) * jb);
+                launch_dgemm(st, 'N', 'N', (size_t)(m - j - jb), (size_t)(n - j - jb), (size_t)jb, -1.0,
+                             a + (j + jb) + (size_t)j * lda, lda, u12, lda, 1.0, a + (j + jb) + (size_t)(j + jb) * lda,
+                             lda, info);
+            }
+        }
+    }
+}
+
+}  // namespace bola

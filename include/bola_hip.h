@@ -1,0 +1,165 @@
+/*
+ * bola_hip.h -- C ABI of libbola_hip.so, the gfx950 (MI355X) dense-solver path.
+ *
+ * Every entry point below replaces one public function of the Rust crate bola 0.1.2
+ * (cited as file:line of that crate) or one composition the crate's callers build from
+ * those functions.  Conventions follow the crate exactly:
+ *   - column-major f64 matrices, element (i,j) at a[i + j*lda];
+ *   - pivots are int32 and 1-based (ipiv[i] = row swapped with row i+1);
+ *   - the character arguments are ASCII ('N','T','L','U','R'); a Rust `char` is narrowed
+ *     to one byte by the binding.
+ *
+ * Two flavours per routine:
+ *   bola_hip_X(...)            host pointers, synchronous: copies the operands to HBM,
+ *                              runs on the calling thread's default handle, copies the
+ *                              results back.  Same in-place semantics as the crate.
+ *   bola_hip_X_dev(h, ...)     device pointers on handle h's stream; never copies.
+ *
+ * Return codes (int):
+ *   0                    Ok
+ *   -i  (i = 1..)        argument i had an illegal value (the crate's "Argument i to ..."
+ *                        errors; getrf lda -> -4, potrf uplo -> -1, potrf lda -> -4)
+ *   +k                   getrf: pivot k (1-based, global column) is exactly zero;
+ *                        potrf: leading minor k is not positive definite
+ *   BOLA_HIP_ERR_DEVICE  a HIP runtime call failed (no counterpart in the crate)
+ * For every nonzero code, bola_hip_last_error() (host flavour) or
+ * bola_hip_error_string(h) (device flavour) holds the text the crate's Err(String)
+ * would carry, e.g. "Matrix is singular. The pivot in column 9 is zero." -- the crate
+ * numbers that column within its 32-column panel, and so does the text.
+ */
+#ifndef BOLA_HIP_H
+#define BOLA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BOLA_HIP_ERR_DEVICE (-1000)
+
+typedef struct bola_hip_handle_s* bola_hip_handle_t;
+
+/* ------------------------------------------------------------------ handle */
+
+/* Creates a handle on the current HIP device with its own stream.  No crate
+ * counterpart (the crate has no state); it owns the stream and the device workspace. */
+int bola_hip_create(bola_hip_handle_t* out);
+int bola_hip_destroy(bola_hip_handle_t h);
+/* Runs subsequent _dev calls on `stream` (a hipStream_t; NULL = the null stream). */
+int bola_hip_set_stream(bola_hip_handle_t h, void* stream);
+void* bola_hip_get_stream(bola_hip_handle_t h);
+/* Waits for all work enqueued on the handle's stream. */
+int bola_hip_synchronize(bola_hip_handle_t h);
+/* Text of the last failing call on h / on this thread's host-flavour calls. */
+const char* bola_hip_error_string(bola_hip_handle_t h);
+const char* bola_hip_last_error(void);
+
+/* ------------------------------------------------------------------ profiling
+ * Replaces the crate's `profiling` feature (profiling.rs:10-140): per-phase call counts
+ * and GPU time under the same phase names (GEMM, GETRF, GETRF2, LASWP, TRSM, SYRK,
+ * POTRF, POTRF2, IDAMAX, ...), measured with HIP events on the handle's stream. */
+int bola_hip_profile_enable(bola_hip_handle_t h, int on);
+/* reset_counters, profiling.rs:60-64 */
+int bola_hip_profile_reset(bola_hip_handle_t h);
+/* One phase: total GPU ms, number of calls, algorithmic flops attributed to it. */
+int bola_hip_profile_get(bola_hip_handle_t h, const char* name, double* ms, long* calls, double* flops);
+/* print_profile, profiling.rs:70-115: the same table, written into buf. */
+int bola_hip_profile_report(bola_hip_handle_t h, double iterations, char* buf, size_t len);
+
+/* ------------------------------------------------------------------ BLAS-1/2 */
+
+/* idamax, idamax.rs:23 -- 0-based index of the largest |x_i| under the crate's x86_64
+ * tie and NaN rule.  Host flavour returns it; device flavour writes *result_dev. */
+size_t bola_hip_idamax(size_t n, const double* x);
+int bola_hip_idamax_dev(bola_hip_handle_t h, size_t n, const double* x, int64_t* result_dev);
+
+/* laswp, laswp.rs:22 -- row interchanges k1..k2 (1-based) of ipiv, applied to n columns. */
+int bola_hip_dlaswp(size_t n, double* a, size_t lda, int32_t k1, int32_t k2, const int32_t* ipiv, int32_t incx);
+int bola_hip_dlaswp_dev(bola_hip_handle_t h, size_t n, double* a, size_t lda, int32_t k1, int32_t k2,
+                        const int32_t* ipiv, int32_t incx);
+
+/* ger, ger.rs:28 -- A += alpha * x * y^T, y with stride ldy. */
+int bola_hip_dger(size_t m, size_t n, double alpha, const double* x, const double* y, size_t ldy, double* a,
+                  size_t lda);
+int bola_hip_dger_dev(bola_hip_handle_t h, size_t m, size_t n, double alpha, const double* x, const double* y,
+                      size_t ldy, double* a, size_t lda);
+
+/* scal, scal.rs:45 -- x *= alpha. */
+int bola_hip_dscal(size_t n, double alpha, double* x, size_t incx);
+int bola_hip_dscal_dev(bola_hip_handle_t h, size_t n, double alpha, double* x, size_t incx);
+
+/* daxpy, utilities.rs:107 -- y += alpha * x. */
+int bola_hip_daxpy(size_t n, double alpha, const double* x, size_t incx, double* y, size_t incy);
+int bola_hip_daxpy_dev(bola_hip_handle_t h, size_t n, double alpha, const double* x, size_t incx, double* y,
+                       size_t incy);
+
+/* dot_product, utilities.rs:22 -- sum x_i * y_i. */
+double bola_hip_ddot(size_t n, const double* x, size_t incx, const double* y, size_t incy);
+int bola_hip_ddot_dev(bola_hip_handle_t h, size_t n, const double* x, size_t incx, const double* y, size_t incy,
+                      double* result_dev);
+
+/* ------------------------------------------------------------------ BLAS-3 */
+
+/* gemm, gemm.rs:88-102 -- C = alpha*op(A)*op(B) + beta*C. */
+int bola_hip_dgemm(char transa, char transb, size_t m, size_t n, size_t k, double alpha, const double* a,
+                   size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc);
+int bola_hip_dgemm_dev(bola_hip_handle_t h, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
+                       const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c,
+                       size_t ldc);
+
+/* trsm, trsm.rs:33-45 -- op(A) X = alpha B (side 'L') or X op(A) = alpha B ('R'). */
+int bola_hip_dtrsm(char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha, const double* a,
+                   size_t lda, double* b, size_t ldb);
+int bola_hip_dtrsm_dev(bola_hip_handle_t h, char side, char uplo, char transa, char diag, size_t m, size_t n,
+                       double alpha, const double* a, size_t lda, double* b, size_t ldb);
+
+/* syrk, syrk.rs:17-28 -- C = alpha*A*A^T + beta*C ('N') or alpha*A^T*A + beta*C ('T'),
+ * one triangle. */
+int bola_hip_dsyrk(char uplo, char trans, size_t n, size_t k, double alpha, const double* a, size_t lda,
+                   double beta, double* c, size_t ldc);
+int bola_hip_dsyrk_dev(bola_hip_handle_t h, char uplo, char trans, size_t n, size_t k, double alpha,
+                       const double* a, size_t lda, double beta, double* c, size_t ldc);
+
+/* ------------------------------------------------------------------ LAPACK */
+
+/* getrf, getrf.rs:37 -- P*A = L*U with partial pivoting. */
+int bola_hip_dgetrf(size_t m, size_t n, double* a, size_t lda, int32_t* ipiv);
+/* Device flavour: if info_dev is NULL the call waits and returns the status; otherwise
+ * it returns at once and the status (0 / +k) is written to *info_dev. */
+int bola_hip_dgetrf_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv,
+                        int32_t* info_dev);
+
+/* getrf2, getrf2.rs:39 -- the crate's unblocked/recursive panel LU, callable directly. */
+int bola_hip_dgetrf2(size_t m, size_t n, double* a, size_t lda, int32_t* ipiv);
+int bola_hip_dgetrf2_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv,
+                         int32_t* info_dev);
+
+/* potrf, potrf.rs:101 -- A = L*L^T ('L') or U^T*U ('U'). */
+int bola_hip_dpotrf(char uplo, size_t n, double* a, size_t lda);
+int bola_hip_dpotrf_dev(bola_hip_handle_t h, char uplo, size_t n, double* a, size_t lda, int32_t* info_dev);
+
+/* potrf2, potrf2.rs:25 -- the crate's recursive Cholesky, callable directly. */
+int bola_hip_dpotrf2(char uplo, size_t n, double* a, size_t lda);
+int bola_hip_dpotrf2_dev(bola_hip_handle_t h, char uplo, size_t n, double* a, size_t lda, int32_t* info_dev);
+
+/* LU solve.  The crate has no getrs; its callers compose laswp(nrhs, B, 1, n, ipiv, 1),
+ * trsm('L','L','N','U') and trsm('L','U','N','N') (laswp.rs:22, trsm.rs:33).  Same
+ * composition, same results. */
+int bola_hip_dgetrs(size_t n, size_t nrhs, const double* lu, size_t lda, const int32_t* ipiv, double* b,
+                    size_t ldb);
+int bola_hip_dgetrs_dev(bola_hip_handle_t h, size_t n, size_t nrhs, const double* lu, size_t lda,
+                        const int32_t* ipiv, double* b, size_t ldb);
+
+/* Cholesky solve: 'L' -> trsm('L','L','N','N'), trsm('L','L','T','N');
+ *                 'U' -> trsm('L','U','T','N'), trsm('L','U','N','N'). */
+int bola_hip_dpotrs(char uplo, size_t n, size_t nrhs, const double* f, size_t lda, double* b, size_t ldb);
+int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, const double* f, size_t lda,
+                        double* b, size_t ldb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOLA_HIP_H */

@@ -1,0 +1,476 @@
+"""Parity of the gfx950 path with the oracle (the CPU restatement of the reference's
+algorithm), through the C ABI.  The GPU kernels keep the reference's rounding structure,
+so the bar here is bit-for-bit equality; where a routine is a tree reduction (dot) or
+takes the BLAS meaning the reference does not compute (syrk (U,N)/(L,T) beyond 32), the
+tolerance is stated in the test."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+HERE = os.path.dirname(os.path.abspath(__file__))
+KATS = json.load(open(os.path.join(HERE, "golden", "kats.json")))
+
+
+@pytest.fixture(scope="module")
+def B():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import bola_amd
+    bola_amd.lib()
+    return bola_amd
+
+
+@pytest.fixture(scope="module")
+def O():
+    import oracle
+    oracle.lib()
+    return oracle
+
+
+def same(got, want, what=""):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    if np.array_equal(got, want, equal_nan=True):
+        return
+    bad = np.flatnonzero(~((got == want) | (np.isnan(got) & np.isnan(want))))
+    i = bad[0]
+    raise AssertionError(f"{what}: {len(bad)} of {got.size} differ; first at {i}: {got.flat[i]!r} vs {want.flat[i]!r}")
+
+
+def arr(x):
+    return np.array(x, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ reference KATs on the GPU
+
+@pytest.mark.parametrize("name", ["gemm_nn", "gemm_nn_dup", "gemm_nt"])
+def test_kat_gemm(B, name):
+    t = KATS[name]
+    c = arr(t["c"])
+    B.gemm(t["transa"], t["transb"], t["m"], t["n"], t["k"], t["alpha"], arr(t["a"]), t["lda"], arr(t["b"]),
+           t["ldb"], t["beta"], c, t["ldc"])
+    assert np.all(np.abs(c - arr(t["expected"])) < t["tol"])
+
+
+def test_kat_laswp_scal(B):
+    t = KATS["laswp"]
+    a = arr(t["a"])
+    B.laswp(t["n"], a, t["lda"], t["k1"], t["k2"], np.array(t["ipiv"], np.int32), t["incx"])
+    same(a, arr(t["expected"]), "laswp KAT")
+    t = KATS["scal"]
+    x = arr(t["x"])
+    B.scal(len(x), t["alpha"], x, 1)
+    same(x, arr(t["expected"]), "scal KAT")
+
+
+@pytest.mark.parametrize("name", ["syrk_ln", "syrk_ut"])
+def test_kat_syrk(B, name):
+    t = KATS[name]
+    c = arr(t["c"])
+    n = t["n"]
+    B.syrk(t["uplo"], t["trans"], n, t["k"], t["alpha"], arr(t["a"]), t["lda"], t["beta"], c, t["ldc"])
+    exp = arr(t["expected_triangle"])
+    for j in range(n):
+        for i in (range(j, n) if t["uplo"] == "L" else range(0, j + 1)):
+            assert abs(c[i + j * t["ldc"]] - exp[i + j * t["ldc"]]) < t["tol"]
+
+
+@pytest.mark.parametrize("name", ["trsm_llnn", "trsm_rutn", "trsm_rltn"])
+def test_kat_trsm(B, name):
+    t = KATS[name]
+    b = arr(t["b"])
+    B.trsm(t["side"], t["uplo"], t["transa"], t["diag"], t["m"], t["n"], t["alpha"], arr(t["a"]), t["lda"], b,
+           t["ldb"])
+    assert np.all(np.abs(b - arr(t["expected"])) < t["tol"])
+
+
+@pytest.mark.parametrize("name", ["potrf_u", "potrf_l"])
+def test_kat_potrf(B, name):
+    t = KATS[name]
+    a = arr(t["a"])
+    B.potrf(t["uplo"], t["n"], a, t["lda"])
+    exp = arr(t["expected_triangle"])
+    n = t["n"]
+    for j in range(n):
+        for i in (range(j, n) if t["uplo"] == "L" else range(0, j + 1)):
+            assert abs(a[i + j * n] - exp[i + j * n]) < t["tol"]
+
+
+def test_kat_getrf(B):
+    t = KATS["getrf"]
+    a = arr(t["a"])
+    ipiv = np.zeros(3, np.int32)
+    B.getrf(3, 3, a, 3, ipiv)
+    assert np.all(np.abs(a - arr(t["expected"])) < t["tol"])
+    assert list(ipiv) == t["ipiv_expected"]
+
+
+# ------------------------------------------------------------------ IDAMAX / LASWP (bit-exact)
+
+def test_idamax_random_and_ties(B, O):
+    rng = np.random.default_rng(1)
+    for n in [0, 1, 5, 15, 16, 17, 31, 32, 33, 100, 1000, 4099, 100000, 1 << 20]:
+        x = rng.uniform(-1, 1, max(n, 1))
+        assert B.idamax(n, x) == O.idamax(n, x), n
+    for trial in range(400):
+        n = int(rng.integers(1, 300))
+        x = rng.integers(0, 4, n).astype(np.float64) * rng.choice([-1.0, 1.0], n)
+        if trial % 3 == 0:
+            x[rng.random(n) < 0.2] = np.nan
+        if trial % 7 == 0:
+            x[:] = 0.0
+        if trial % 11 == 0:
+            x[:] = np.nan
+        assert B.idamax(n, x) == O.idamax(n, x), (trial, n)
+
+
+@pytest.mark.parametrize("ncols", [1, 5, 64, 300])
+@pytest.mark.parametrize("incx", [1, -1, 2])
+def test_laswp_bitexact(B, O, ncols, incx):
+    rng = np.random.default_rng(ncols * 7 + incx)
+    m, lda = 200, 203
+    k1, k2 = 5, 150
+    span = (k1 - 1) + (k2 - k1) * abs(incx) + 1
+    ipiv = rng.integers(1, m + 1, span).astype(np.int32)
+    a = rng.uniform(-1, 1, lda * ncols)
+    ref = a.copy()
+    O.laswp(ncols, ref, lda, k1, k2, ipiv, incx)
+    B.laswp(ncols, a, lda, k1, k2, ipiv, incx)
+    same(a, ref, "laswp")
+
+
+# ------------------------------------------------------------------ BLAS-1/2
+
+def test_ger_scal_daxpy_bitexact(B, O):
+    rng = np.random.default_rng(3)
+    for m, n in [(1, 1), (7, 5), (130, 33), (1000, 64)]:
+        x = rng.uniform(-1, 1, m)
+        y = rng.uniform(-1, 1, 3 * n)
+        y[::7] = 0.0
+        a = rng.uniform(-1, 1, (m + 2) * n)
+        ref = a.copy()
+        O.ger(m, n, -1.3, x, y, 3, ref, m + 2)
+        B.ger(m, n, -1.3, x, y, 3, a, m + 2)
+        same(a, ref, f"ger {m}x{n}")
+    for n in [1, 3, 4, 1001]:
+        for inc in [1, 3]:
+            x = rng.uniform(-1, 1, (n - 1) * inc + 1)
+            y = rng.uniform(-1, 1, (n - 1) * inc + 1)
+            rx, ry = x.copy(), y.copy()
+            O.scal(n, 0.37, rx, inc)
+            B.scal(n, 0.37, x, inc)
+            same(x, rx, "scal")
+            O.daxpy(n, 1.7, x, inc, ry, inc)
+            B.daxpy(n, 1.7, x, inc, y, inc)
+            same(y, ry, "daxpy")
+
+
+def test_dot_tolerance(B, O):
+    rng = np.random.default_rng(4)
+    for n in [0, 1, 5, 1000, 1 << 20]:
+        x = rng.uniform(-1, 1, max(n, 1))
+        y = rng.uniform(-1, 1, max(n, 1))
+        got = B.dot_product(n, x, 1, y, 1)
+        want = O.dot_product(n, x, 1, y, 1)
+        bound = n * EPS * float(np.sum(np.abs(x[:n] * y[:n]))) + 1e-300
+        assert abs(got - want) <= 2 * bound, (n, got, want)
+
+
+# ------------------------------------------------------------------ GEMM (bit-exact)
+
+def _rand_op(rng, trans, rows, cols, ld_extra=0):
+    """storage for op(X) of shape rows x cols, column-major with padding"""
+    sr, sc = (rows, cols) if trans == "N" else (cols, rows)
+    ld = sr + ld_extra
+    x = rng.uniform(-1, 1, ld * sc)
+    return x, ld
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (7, 5, 3), (64, 64, 64), (100, 37, 130), (257, 129, 200),
+                                   (300, 600, 65)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (-1.0, 1.0), (0.7, 0.0), (2.5, -0.3)])
+def test_gemm_bitexact(B, O, ta, tb, m, n, k, alpha, beta):
+    rng = np.random.default_rng(m * 131 + n * 17 + k)
+    a, lda = _rand_op(rng, ta, m, k, 3)
+    b, ldb = _rand_op(rng, tb, k, n, 1)
+    ldc = m + 2
+    c = rng.uniform(-1, 1, ldc * n)
+    ref = c.copy()
+    O.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, ref, ldc)
+    B.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+    same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
+
+
+def test_gemm_alpha_zero_beta_rules(B, O):
+    c = np.array([np.nan, 1.0, 2.0, 3.0])
+    B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
+    same(c, np.zeros(4), "beta=0")
+    c = np.array([1.0, 2.0, 3.0, 4.0])
+    B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 3.0, c, 2)
+    same(c, arr([3, 6, 9, 12]), "alpha=0")
+
+
+def test_gemm_config1_1024(B, O):
+    """DGEMM 'N','N' 1024^3, U[0,1), alpha = beta = 1: bit-exact with the oracle, and
+    elementwise relative error <= k*eps against a float128-free float64 reference."""
+    n = 1024
+    rng = np.random.default_rng(1)
+    a = rng.random(n * n)
+    b = rng.random(n * n)
+    c = rng.random(n * n)
+    ref = c.copy()
+    O.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 1.0, ref, n)
+    got = c.copy()
+    B.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 1.0, got, n)
+    same(got, ref, "gemm 1024")
+    exact = (a.reshape(n, n).T @ b.reshape(n, n).T + c.reshape(n, n).T).T.ravel()
+    assert np.max(np.abs(got - exact) / exact) <= n * EPS
+
+
+# ------------------------------------------------------------------ TRSM (bit-exact)
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("size", [5, 64, 65, 200])
+def test_trsm_bitexact(B, O, side, uplo, trans, diag, size):
+    rng = np.random.default_rng(size * 16 + "LR".index(side) * 8 + "LU".index(uplo) * 4 + "NT".index(trans) * 2
+                                + "NU".index(diag))
+    m, n = (size, 37) if side == "L" else (37, size)
+    na = m if side == "L" else n
+    lda = na + 1
+    A = rng.uniform(-1, 1, (na, na)) / na
+    np.fill_diagonal(A, rng.uniform(1, 2, na))
+    a = np.zeros(lda * na)
+    a.reshape(na, lda).T[:na] = A
+    b = rng.uniform(-1, 1, (m + 3) * n)
+    b[5] = 0.0
+    ref = b.copy()
+    O.trsm(side, uplo, trans, diag, m, n, 1.5, a, lda, ref, m + 3)
+    B.trsm(side, uplo, trans, diag, m, n, 1.5, a, lda, b, m + 3)
+    same(b, ref, f"trsm {side}{uplo}{trans}{diag} {m}x{n}")
+
+
+# ------------------------------------------------------------------ SYRK
+
+@pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T")])
+@pytest.mark.parametrize("n,k", [(3, 2), (32, 32), (33, 70), (100, 64), (130, 129)])
+@pytest.mark.parametrize("beta", [1.0, 0.0, 0.5])
+def test_syrk_bitexact(B, O, uplo, trans, n, k, beta):
+    rng = np.random.default_rng(n * 1000 + k)
+    ar, ac = (n, k) if trans == "N" else (k, n)
+    a = rng.uniform(-1, 1, ar * ac)
+    c = rng.uniform(-1, 1, (n + 1) * n)
+    ref = c.copy()
+    O.syrk(uplo, trans, n, k, -1.0, a, ar, beta, ref, n + 1)
+    B.syrk(uplo, trans, n, k, -1.0, a, ar, beta, c, n + 1)
+    same(c, ref, f"syrk {uplo}{trans} {n}x{k}")
+
+
+@pytest.mark.parametrize("uplo,trans", [("U", "N"), ("L", "T")])
+def test_syrk_blas_meaning_other_combos(B, O, uplo, trans):
+    rng = np.random.default_rng(9)
+    # n, k <= 32: the reference computes the BLAS meaning; equal bit for bit
+    n, k = 20, 30
+    ar, ac = (n, k) if trans == "N" else (k, n)
+    a = rng.uniform(-1, 1, ar * ac)
+    c = rng.uniform(-1, 1, n * n)
+    ref = c.copy()
+    O.syrk(uplo, trans, n, k, 1.0, a, ar, 1.0, ref, n)
+    B.syrk(uplo, trans, n, k, 1.0, a, ar, 1.0, c, n)
+    same(c, ref, "syrk small")
+    # larger: BLAS meaning, tolerance k*eps*|A||A|
+    n, k = 90, 70
+    ar, ac = (n, k) if trans == "N" else (k, n)
+    A = rng.uniform(-1, 1, (ar, ac))
+    c0 = rng.uniform(-1, 1, (n, n))
+    c = c0.ravel(order="F").copy()
+    B.syrk(uplo, trans, n, k, 1.0, A.ravel(order="F").copy(), ar, 1.0, c, n)
+    full = (A @ A.T if trans == "N" else A.T @ A) + c0
+    got = c.reshape(n, n).T
+    mask = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    assert np.max(np.abs(got[mask] - full[mask])) <= 4 * k * EPS * k
+    assert np.array_equal(got[~mask], c0[~mask])
+
+
+# ------------------------------------------------------------------ GETRF / GETRS (bit-exact)
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 3), (31, 31), (32, 32), (33, 33), (100, 100), (257, 257),
+                                 (1000, 1000), (300, 100), (100, 300), (20, 90), (90, 20), (513, 64)])
+def test_getrf_bitexact(B, O, m, n):
+    rng = np.random.default_rng(m * 7919 + n)
+    lda = m + 1
+    a = rng.uniform(-1, 1, lda * n)
+    ipiv = np.zeros(min(m, n), np.int32)
+    ref, rp = a.copy(), ipiv.copy()
+    code, msg = O.getrf(m, n, ref, lda, rp)
+    assert code == 0, msg
+    B.getrf(m, n, a, lda, ipiv)
+    same(ipiv, rp, "ipiv")
+    same(a, ref, f"getrf {m}x{n}")
+
+
+@pytest.mark.parametrize("m,n", [(80, 80), (70, 100), (100, 70), (65, 65)])
+def test_getrf2_recursive_bitexact(B, O, m, n):
+    rng = np.random.default_rng(m * 31 + n)
+    a = rng.uniform(-1, 1, m * n)
+    ipiv = np.zeros(min(m, n), np.int32)
+    ref, rp = a.copy(), ipiv.copy()
+    assert O.getrf2(m, n, ref, m, rp)[0] == 0
+    B.getrf2(m, n, a, m, ipiv)
+    same(ipiv, rp, "ipiv")
+    same(a, ref, "getrf2")
+
+
+def test_getrf_ties_and_zero_rows(B, O):
+    # integer-valued matrix with many equal-magnitude candidates: pivot choice follows
+    # the reference's IDAMAX tie rule, not "first index"
+    rng = np.random.default_rng(12)
+    n = 96
+    a = rng.integers(-2, 3, n * n).astype(np.float64)
+    a[: n * 4] = np.where(a[: n * 4] == 0, 1.0, a[: n * 4])
+    ref, rp = a.copy(), np.zeros(n, np.int32)
+    code, msg = O.getrf(n, n, ref, n, rp)
+    ipiv = np.zeros(n, np.int32)
+    if code == 0:
+        B.getrf(n, n, a, n, ipiv)
+        same(ipiv, rp, "ipiv ties")
+        same(a, ref, "getrf ties")
+    else:
+        with pytest.raises(B.BolaError) as e:
+            B.getrf(n, n, a, n, ipiv)
+        assert e.value.message == msg
+
+
+def test_getrf_errors(B, O):
+    with pytest.raises(B.BolaError) as e:
+        B.getrf(4, 4, np.zeros(16), 3, np.zeros(4, np.int32))
+    assert str(e.value) == "Argument 4 to getrf had an illegal value of 3" and e.value.code == -4
+    n = 96
+    rng = np.random.default_rng(13)
+    A = rng.uniform(-1, 1, (n, n))
+    A[:, 70] = 0.0
+    a = A.ravel(order="F").copy()
+    ref = a.copy()
+    code, msg = O.getrf(n, n, ref, n, np.zeros(n, np.int32))
+    assert code == 7
+    with pytest.raises(B.BolaError) as e:
+        B.getrf(n, n, a, n, np.zeros(n, np.int32))
+    assert e.value.message == msg and e.value.code == 71
+    same(a, ref, "partially factored state")
+
+
+def test_getrs_bitexact(B, O):
+    rng = np.random.default_rng(14)
+    n, nrhs = 300, 64
+    a = rng.uniform(-1, 1, n * n)
+    ipiv = np.zeros(n, np.int32)
+    assert O.getrf(n, n, a, n, ipiv)[0] == 0
+    b = rng.uniform(-1, 1, n * nrhs)
+    ref = b.copy()
+    O.getrs(n, nrhs, a, n, ipiv, ref, n)
+    B.getrs(n, nrhs, a, n, ipiv, b, n)
+    same(b, ref, "getrs")
+
+
+def test_getrf_getrs_2048_bitexact(B, O):
+    n, nrhs = 2048, 16
+    rng = np.random.default_rng(15)
+    a = rng.uniform(-1, 1, n * n)
+    b = rng.uniform(-1, 1, n * nrhs)
+    ra, rp, rb = a.copy(), np.zeros(n, np.int32), b.copy()
+    assert O.getrf(n, n, ra, n, rp)[0] == 0
+    O.getrs(n, nrhs, ra, n, rp, rb, n)
+    ipiv = np.zeros(n, np.int32)
+    B.getrf(n, n, a, n, ipiv)
+    B.getrs(n, nrhs, a, n, ipiv, b, n)
+    same(ipiv, rp, "ipiv 2048")
+    same(a, ra, "lu 2048")
+    same(b, rb, "x 2048")
+
+
+# ------------------------------------------------------------------ POTRF / POTRS (bit-exact)
+
+def _spd(rng, n):
+    G = rng.uniform(-1, 1, (n, n))
+    return (G @ G.T + n * np.eye(n)).ravel(order="F").copy()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n", [1, 5, 32, 33, 100, 300])
+def test_potrf_bitexact(B, O, uplo, n):
+    rng = np.random.default_rng(n * 3 + (uplo == "U"))
+    a = _spd(rng, n)
+    ref = a.copy()
+    assert O.potrf(uplo, n, ref, n)[0] == 0
+    B.potrf(uplo, n, a, n)
+    same(a, ref, f"potrf {uplo} {n}")
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n", [20, 50, 77])
+def test_potrf2_bitexact(B, O, uplo, n):
+    rng = np.random.default_rng(n)
+    a = _spd(rng, n)
+    ref = a.copy()
+    assert O.potrf2(uplo, n, ref, n)[0] == 0
+    B.potrf2(uplo, n, a, n)
+    same(a, ref, f"potrf2 {uplo} {n}")
+
+
+def test_potrf_errors(B, O):
+    with pytest.raises(B.BolaError) as e:
+        B.potrf("X", 3, np.eye(3).ravel(), 3)
+    assert str(e.value) == "Argument 1 to potrf had an illegal value"
+    with pytest.raises(B.BolaError) as e:
+        B.potrf("L", 3, np.eye(3).ravel(), 2)
+    assert str(e.value) == "Argument 4 to potrf had an illegal value"
+    n = 64
+    for uplo in "LU":
+        A = np.eye(n)
+        A[36, 36] = -1.0
+        ref = A.ravel(order="F").copy()
+        code, msg = O.potrf(uplo, n, ref, n)
+        with pytest.raises(B.BolaError) as e:
+            B.potrf(uplo, n, A.ravel(order="F").copy(), n)
+        assert e.value.message == msg and e.value.code == 37
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrs_bitexact(B, O, uplo):
+    rng = np.random.default_rng(16)
+    n, nrhs = 200, 9
+    f = _spd(rng, n)
+    assert O.potrf(uplo, n, f, n)[0] == 0
+    b = rng.uniform(-1, 1, n * nrhs)
+    ref = b.copy()
+    O.potrs(uplo, n, nrhs, f, n, ref, n)
+    B.potrs(uplo, n, nrhs, f, n, b, n)
+    same(b, ref, "potrs")
+
+
+# ------------------------------------------------------------------ device-pointer flavour
+
+def test_device_flavour_matches_host(B, O):
+    import torch
+    rng = np.random.default_rng(17)
+    n = 500
+    a = rng.uniform(-1, 1, n * n)
+    ref, rp = a.copy(), np.zeros(n, np.int32)
+    O.getrf(n, n, ref, n, rp)
+    da = torch.from_numpy(a.copy()).cuda()
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    B.getrf(n, n, da, n, dp)
+    torch.cuda.synchronize()
+    same(dp.cpu().numpy(), rp, "dev ipiv")
+    same(da.cpu().numpy(), ref, "dev lu")
+    x = torch.from_numpy(rng.uniform(-1, 1, 1000)).cuda()
+    assert B.idamax(1000, x) == O.idamax(1000, x.cpu().numpy())
