@@ -90,17 +90,57 @@ __device__ void wg_reduce(Cand& c, Cand* sh, Cand* out) {
 // Registers hold the not-yet-retired columns of each row as a window that shifts left
 // by one column per step, so the current column is always a[.][0] and every register
 // index is a compile-time constant; a retired column goes to LDS (sret).
+// reduce two candidate streams at once; every thread gets both results
+__device__ __forceinline__ void wg_reduce2(Cand& a, Cand& b, Cand (*sh)[2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(a.v, off, 64);
+        uint64_t ok = (uint64_t)__shfl_xor((long long)a.k, off, 64);
+        if (better(ov, ok, a.v, a.k)) { a.v = ov; a.k = ok; }
+        ov = __shfl_xor(b.v, off, 64);
+        ok = (uint64_t)__shfl_xor((long long)b.k, off, 64);
+        if (better(ov, ok, b.v, b.k)) { b.v = ov; b.k = ok; }
+    }
+    if (lane == 0) { sh[w][0] = a; sh[w][1] = b; }
+    __syncthreads();
+    a = sh[0][0];
+    b = sh[0][1];
+#pragma unroll
+    for (int q = 1; q < PT / 64; q++) {
+        if (better(sh[q][0].v, sh[q][0].k, a.v, a.k)) a = sh[q][0];
+        if (better(sh[q][1].v, sh[q][1].k, b.v, b.k)) b = sh[q][1];
+    }
+}
+
+// the reference's final IDAMAX rule (idamax.rs:27-38, 91-130) on the best vector-phase
+// and tail candidates; returns the winning offset from the diagonal row
+__device__ __forceinline__ long idamax_rule(int len, const Cand& gv, const Cand& gt, bool& from_tail) {
+    from_tail = false;
+    if (len < 16) {
+        from_tail = gt.v >= 0.0;
+        return from_tail ? (long)gt.k : 0;
+    }
+    double V = 0.0;
+    long vi = 0;
+    if (gv.v > 0.0) { V = gv.v; vi = (long)(gv.k & KEYMASK); }
+    if (gt.v > V) { from_tail = true; return (long)gt.k; }
+    return vi;
+}
+
+constexpr int MAXWG_FAST = 32;  // up to this many workgroups, one read round per column
+
 template <int R>
 __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restrict__ A, size_t lda,
                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
                                                   long row_off, int* __restrict__ flags, PanelSlot* __restrict__ slots,
                                                   PanelSlot* __restrict__ cur, RowMap* __restrict__ map, int nwg,
                                                   long spin_limit) {
-    __shared__ Cand red[PT / 64];
-    __shared__ Cand rv, rt;
+    __shared__ Cand red[PT / 64][2];
     __shared__ double prow[PW + 1];
     __shared__ int ctl[2];
     __shared__ double sret[PW * PT * R];  // sret[c * PT*R + s*PT + tid]
+    __shared__ double lrows[2 * MAXWG_FAST + 1][PW];  // candidate rows, then the row at position j
+    __shared__ Cand lcand[2 * MAXWG_FAST];
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -120,6 +160,7 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
     if (tid == 0) prow[PW] = 0.0;
 
     const int min_mn = min(m, w);
+    const bool fast = nwg <= MAXWG_FAST;
     int nret = 0;  // retired columns
     for (int j = 0; j < min_mn; j++) {
         const int len = m - j;
@@ -141,39 +182,62 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                 }
             }
         }
-        wg_reduce(vb, red, &rv);
-        wg_reduce(tb, red, &rt);
+        wg_reduce2(vb, tb, red);
         const int par = j & 1;
-        PanelSlot* mine = slots + ((size_t)par * nwg + g) * 2;
+        const int CUR = 2 * MAXWG_FAST;  // lrows index of the row at position j
+
+        if (nwg == 1) {
+            // single workgroup: exchange through LDS only
 #pragma unroll
-        for (int s = 0; s < R; s++) {
-            if (valid[s] && pos[s] >= j) {
-                uint64_t rel = (uint64_t)(pos[s] - j);
-                PanelSlot* dst0 = nullptr;
-                PanelSlot* dst1 = nullptr;
-                if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) dst0 = mine; }
-                else if (tb.v >= 0.0 && rel == tb.k) dst0 = mine + 1;
-                if (rel == 0) dst1 = cur + par;
-                // columns j..w-1 of the row (the only ones the step reads)
+            for (int s = 0; s < R; s++) {
+                if (valid[s] && pos[s] >= j) {
+                    uint64_t rel = (uint64_t)(pos[s] - j);
+                    int d0 = -1;
+                    if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) d0 = 0; }
+                    else if (tb.v >= 0.0 && rel == tb.k) d0 = 1;
 #pragma unroll
-                for (int c = 0; c < PW; c++) {
-                    if (c < w - j) {
-                        if (dst0) dst0->row[j + c] = a[s][c];
-                        if (dst1) dst1->row[j + c] = a[s][c];
+                    for (int c = 0; c < PW; c++) {
+                        if (c < w - j) {
+                            if (d0 >= 0) lrows[d0][j + c] = a[s][c];
+                            if (rel == 0) lrows[CUR][j + c] = a[s][c];
+                        }
                     }
                 }
             }
-        }
-        if (tid == 0) {
-            mine[0].v = vb.v; mine[0].key = vb.k;
-            mine[1].v = tb.v; mine[1].key = tb.k;
-        }
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-
-        if (wv == 0) {
+            __syncthreads();
+            bool ft;
+            long rel = idamax_rule(len, vb, tb, ft);
+            const int src = rel == 0 ? CUR : (ft ? 1 : 0);
+            if (tid >= j && tid < w) prow[tid] = lrows[src][tid];
+            if (tid == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
+        } else {
+            PanelSlot* mine = slots + ((size_t)par * nwg + g) * 2;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (valid[s] && pos[s] >= j) {
+                    uint64_t rel = (uint64_t)(pos[s] - j);
+                    PanelSlot* dst0 = nullptr;
+                    if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) dst0 = mine; }
+                    else if (tb.v >= 0.0 && rel == tb.k) dst0 = mine + 1;
+                    PanelSlot* dst1 = (rel == 0) ? cur + par : nullptr;
+#pragma unroll
+                    for (int c = 0; c < PW; c++) {
+                        if (c < w - j) {
+                            if (dst0) dst0->row[j + c] = a[s][c];
+                            if (dst1) dst1->row[j + c] = a[s][c];
+                        }
+                    }
+                }
+            }
+            if (tid == 0) {
+                mine[0].v = vb.v; mine[0].key = vb.k;
+                mine[1].v = tb.v; mine[1].key = tb.k;
+            }
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // every workgroup's flag is acquired by one thread here
             int ok = 1;
-            for (int q = lane; q < nwg; q += 64) {
+            for (int q = tid; q < nwg; q += PT) {
                 long sp = 0;
                 while (__hip_atomic_load(flags + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
                     if (++sp > spin_limit ||
@@ -181,56 +245,86 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                         ok = 0;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
                 }
             }
-            ok = __ballot(!ok) == 0ull;
-            Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
-            int gvs = -1, gts = -1;
-            if (ok) {
-                for (int q = lane; q < nwg; q += 64) {
-                    const PanelSlot* S = slots + ((size_t)par * nwg + q) * 2;
-                    double v0 = S[0].v, v1 = S[1].v;
-                    uint64_t k0 = S[0].key, k1 = S[1].key;
-                    if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
-                    if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
+            ok = __syncthreads_and(ok);
+            if (!ok) {
+                if (g == 0 && tid == 0)
+                    __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            const PanelSlot* base = slots + (size_t)par * nwg * 2;
+            if (fast) {
+                // one read round: every candidate's key and row, and the row at position j
+                const int nrow = 2 * nwg;
+                for (int e = tid; e < nrow * (PW + 2) + PW; e += PT) {
+                    if (e < nrow * (PW + 2)) {
+                        const int q = e / (PW + 2), f = e % (PW + 2);
+                        if (f == 0) lcand[q].v = base[q].v;
+                        else if (f == 1) lcand[q].k = base[q].key;
+                        else if (f - 2 >= j && f - 2 < w) lrows[q][f - 2] = base[q].row[f - 2];
+                    } else {
+                        const int c = e - nrow * (PW + 2);
+                        if (c >= j && c < w) lrows[CUR][c] = cur[par].row[c];
+                    }
                 }
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                double ov = __shfl_xor(gv.v, off, 64);
-                uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
-                int os = __shfl_xor(gvs, off, 64);
-                if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
-                ov = __shfl_xor(gt.v, off, 64);
-                okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
-                os = __shfl_xor(gts, off, 64);
-                if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
-            }
-            // the reference's final rule (idamax.rs:27-38, 91-130)
-            long rel;
-            int src;  // slot index, or -1 for the row currently at position j
-            if (len < 16) {
-                rel = gt.v >= 0.0 ? (long)gt.k : 0;
-                src = gts;
+                __syncthreads();
+                if (wv == 0) {
+                    Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+                    int gvs = -1, gts = -1;
+                    for (int q = lane; q < nwg; q += 64) {
+                        if (better(lcand[2 * q].v, lcand[2 * q].k, gv.v, gv.k)) { gv = lcand[2 * q]; gvs = 2 * q; }
+                        if (better(lcand[2 * q + 1].v, lcand[2 * q + 1].k, gt.v, gt.k)) {
+                            gt = lcand[2 * q + 1];
+                            gts = 2 * q + 1;
+                        }
+                    }
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        double ov = __shfl_xor(gv.v, off, 64);
+                        uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
+                        int os = __shfl_xor(gvs, off, 64);
+                        if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
+                        ov = __shfl_xor(gt.v, off, 64);
+                        okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
+                        os = __shfl_xor(gts, off, 64);
+                        if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+                    }
+                    bool ft;
+                    long rel = idamax_rule(len, gv, gt, ft);
+                    const int src = rel == 0 ? CUR : (ft ? gts : gvs);
+                    if (lane >= j && lane < w) prow[lane] = lrows[src][lane];
+                    if (lane == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
+                }
             } else {
-                double V = 0.0;
-                long vi = 0;
-                if (gv.v > 0.0) { V = gv.v; vi = (long)(gv.k & KEYMASK); }
-                if (gt.v > V) { rel = (long)gt.k; src = gts; }
-                else { rel = vi; src = gvs; }
+                // many workgroups: keys first, then the winning row
+                if (wv == 0) {
+                    Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+                    int gvs = -1, gts = -1;
+                    for (int q = lane; q < nwg; q += 64) {
+                        double v0 = base[2 * q].v, v1 = base[2 * q + 1].v;
+                        uint64_t k0 = base[2 * q].key, k1 = base[2 * q + 1].key;
+                        if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
+                        if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
+                    }
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        double ov = __shfl_xor(gv.v, off, 64);
+                        uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
+                        int os = __shfl_xor(gvs, off, 64);
+                        if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
+                        ov = __shfl_xor(gt.v, off, 64);
+                        okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
+                        os = __shfl_xor(gts, off, 64);
+                        if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+                    }
+                    bool ft;
+                    long rel = idamax_rule(len, gv, gt, ft);
+                    const PanelSlot* P = rel == 0 ? cur + par : base + (ft ? gts : gvs);
+                    if (lane >= j && lane < w) prow[lane] = P->row[lane];
+                    if (lane == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
+                }
             }
-            if (rel == 0) src = -1;
-            const PanelSlot* P = src < 0 ? cur + par : slots + (size_t)par * nwg * 2 + src;
-            if (ok && lane >= j && lane < w) prow[lane] = P->row[lane];
-            if (lane == 0) { ctl[0] = (int)rel; ctl[1] = ok; }
         }
         __syncthreads();
-        if (!ctl[1]) {
-            // a workgroup never arrived (or another one failed): give up without writing
-            if (g == 0 && tid == 0)
-                __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
-        }
         const int jp = j + ctl[0];
         const double piv = prow[j];
         if (g == 0 && tid == 0) ipiv[j] = jp + 1;

@@ -8,149 +8,170 @@
 // skipped for a zero right-hand-side entry); left/transpose solves accumulate
 // `temp -= a*b` unfused then divide; right solves subtract unfused (skipping zero
 // coefficients) then multiply by the reciprocal of the diagonal.  The base-case kernels
-// below run those exact sequences, one thread per independent right-hand-side column
-// (side 'L') or row (side 'R'), with the triangle and the thread's vector in LDS.
+// below run those exact sequences: the daxpy-form solves with one lane per row, the
+// dot-form solves with one thread per independent vector held in registers.
 #include "common.hpp"
 
 namespace bola {
 
 namespace {
 
-constexpr int TB = 64;      // threads per base-case block (one wavefront)
 constexpr int MAXT = 64;    // base-case triangle size (trsm.rs:8)
 
 // variant ids
 enum { LNL = 0, LNU, LTL, LTU, RNL, RNU, RTL, RTU };
 
-template <int V>
-__global__ void __launch_bounds__(TB) trsm_base(int unit, int t, int nvec, const double* __restrict__ A, size_t lda,
-                                                double* __restrict__ B, size_t ldb, const int32_t* __restrict__ abort) {
+// ---- left, no-transpose (daxpy form): one wavefront per right-hand-side column (two
+// columns per wavefront when t <= 32), one lane per row.  Step k broadcasts the solved
+// entry from lane k; every lane below (LNL) / above (LNU) applies its own fused or
+// unfused update, the same operation the reference's daxpy applies to that element.
+template <bool LOWER>
+__global__ void __launch_bounds__(256) trsm_ln_lanes(int unit, int t, int nvec, const double* __restrict__ A,
+                                                     size_t lda, double* __restrict__ B, size_t ldb,
+                                                     const int32_t* __restrict__ abort) {
     if (abort && *abort != 0) return;
-    // t: triangle order (<= 64); nvec: number of independent vectors (columns for 'L',
-    // rows for 'R'); vector v of B: 'L' -> B[0..t, v], 'R' -> B[v, 0..t]
-    __shared__ double sA[MAXT * (MAXT + 1)];   // sA[r * (MAXT+1) + c] = A[r, c]
-    __shared__ double sV[MAXT * TB];           // sV[i * TB + lane] = element i of the lane's vector
+    const int lane = threadIdx.x & 63;
+    const bool two = t <= 32;
+    const int sub = two ? (lane >> 5) : 0;
+    const int i = two ? (lane & 31) : lane;
+    const int base = sub * 32;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long v = two ? wave * 2 + sub : wave;
+    const bool col_ok = v < nvec;
+    const bool act = col_ok && i < t;
+    double x = act ? B[i + (size_t)v * ldb] : 0.0;
+#pragma unroll
+    for (int kk = 0; kk < MAXT; kk++) {
+        const int k = LOWER ? kk : (MAXT - 1 - kk);
+        if (k < t) {
+            const double bk = __shfl(x, base + k, 64);
+            if (bk != 0.0) {
+                const double u = unit ? bk : bk / A[(size_t)k + (size_t)k * lda];
+                if (i == k) x = u;
+                const double al = -u;
+                if (al != 0.0) {
+                    if (LOWER) {
+                        const int nf = (t - k - 1) & ~3;
+                        if (act && i > k) {
+                            const double aik = A[(size_t)i + (size_t)k * lda];
+                            x = (i - k - 1 < nf) ? fma(al, aik, x) : x + al * aik;
+                        }
+                    } else {
+                        const int nf = k & ~3;
+                        if (act && i < k) {
+                            const double aik = A[(size_t)i + (size_t)k * lda];
+                            x = (i < nf) ? fma(al, aik, x) : x + al * aik;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (act) B[i + (size_t)v * ldb] = x;
+}
+
+// ---- dot-form variants (LTL, LTU and the four right-side ones): one thread per
+// independent vector, the vector in registers (compile-time indices), the triangle in
+// LDS (broadcast reads).  Sequences as in trsm.rs:174-241 (left) and :252-376 (right).
+template <int V>
+__global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, const double* __restrict__ A,
+                                                    size_t lda, double* __restrict__ B, size_t ldb,
+                                                    const int32_t* __restrict__ abort) {
+    if (abort && *abort != 0) return;
+    __shared__ double sA[MAXT * (MAXT + 1)];  // sA[r * (MAXT+1) + c] = A[r, c]
     const int lane = threadIdx.x;
-    for (int e = lane; e < t * t; e += TB) {
+    for (int e = lane; e < t * t; e += 64) {
         int r = e % t, c = e / t;
         sA[r * (MAXT + 1) + c] = A[r + (size_t)c * lda];
     }
-    const long v = (long)blockIdx.x * TB + lane;
-    const bool active = v < nvec;
-    constexpr bool LEFT = V <= LTU;
-    if (active) {
-        for (int i = 0; i < t; i++)
-            sV[i * TB + lane] = LEFT ? B[i + (size_t)v * ldb] : B[v + (size_t)i * ldb];
-    }
     __syncthreads();
-    if (active) {
+    const long v = (long)blockIdx.x * 64 + lane;
+    if (v >= nvec) return;
+    constexpr bool LEFT = V <= LTU;
 #define AT(r, c) sA[(r) * (MAXT + 1) + (c)]
-#define X(i) sV[(i) * TB + lane]
-        if (V == LNL) {
-            for (int k = 0; k < t; k++) {
-                double bk = X(k);
-                if (bk != 0.0) {
-                    double u = unit ? bk : bk / AT(k, k);
-                    X(k) = u;
-                    double al = -u;
-                    int len = t - k - 1;
-                    if (len > 0 && al != 0.0) {
-                        int nf = len & ~3;
-                        for (int i = 0; i < len; i++) {
-                            int r = k + 1 + i;
-                            double y = X(r), x = AT(r, k);
-                            X(r) = (i < nf) ? fma(al, x, y) : y + al * x;
-                        }
-                    }
-                }
-            }
-        } else if (V == LNU) {
-            for (int k = t - 1; k >= 0; k--) {
-                double bk = X(k);
-                if (bk != 0.0) {
-                    double u = unit ? bk : bk / AT(k, k);
-                    X(k) = u;
-                    double al = -u;
-                    int len = k;
-                    if (len > 0 && al != 0.0) {
-                        int nf = len & ~3;
-                        for (int i = 0; i < len; i++) {
-                            double y = X(i), x = AT(i, k);
-                            X(i) = (i < nf) ? fma(al, x, y) : y + al * x;
-                        }
-                    }
-                }
-            }
-        } else if (V == LTL) {
-            for (int i = t - 1; i >= 0; i--) {
-                double temp = X(i);
-                for (int k = i + 1; k < t; k++) temp = temp - AT(k, i) * X(k);
+    double x[MAXT];
+#pragma unroll
+    for (int q = 0; q < MAXT; q++)
+        x[q] = (q < t) ? (LEFT ? B[q + (size_t)v * ldb] : B[v + (size_t)q * ldb]) : 0.0;
+    if (V == LTL) {
+#pragma unroll
+        for (int ii = 0; ii < MAXT; ii++) {
+            const int i = MAXT - 1 - ii;
+            if (i < t) {
+                double temp = x[i];
+#pragma unroll
+                for (int k = i + 1; k < MAXT; k++)
+                    if (k < t) temp = temp - AT(k, i) * x[k];
                 if (!unit) temp = temp / AT(i, i);
-                X(i) = temp;
-            }
-        } else if (V == LTU) {
-            for (int i = 0; i < t; i++) {
-                double temp = X(i);
-                for (int k = 0; k < i; k++) temp = temp - AT(k, i) * X(k);
-                if (!unit) temp = temp / AT(i, i);
-                X(i) = temp;
-            }
-        } else if (V == RNL) {
-            for (int j = t - 1; j >= 0; j--) {
-                double bj = X(j);
-                for (int k = j + 1; k < t; k++) {
-                    double a = AT(k, j);
-                    if (a != 0.0) bj = bj - a * X(k);
-                }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
-                X(j) = bj;
-            }
-        } else if (V == RNU) {
-            for (int j = 0; j < t; j++) {
-                double bj = X(j);
-                for (int k = 0; k < j; k++) {
-                    double a = AT(k, j);
-                    if (a != 0.0) bj = bj - a * X(k);
-                }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
-                X(j) = bj;
-            }
-        } else if (V == RTL) {
-            for (int j = 0; j < t; j++) {
-                double bj = X(j);
-                for (int k = 0; k < j; k++) {
-                    double a = AT(j, k);
-                    if (a != 0.0) bj = bj - a * X(k);
-                }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
-                X(j) = bj;
-            }
-        } else {  // RTU
-            for (int j = t - 1; j >= 0; j--) {
-                double bj = X(j);
-                for (int k = j + 1; k < t; k++) {
-                    double a = AT(j, k);
-                    if (a != 0.0) bj = bj - a * X(k);
-                }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
-                X(j) = bj;
+                x[i] = temp;
             }
         }
-        for (int i = 0; i < t; i++) {
-            if (LEFT) B[i + (size_t)v * ldb] = X(i);
-            else B[v + (size_t)i * ldb] = X(i);
+    } else if (V == LTU) {
+#pragma unroll
+        for (int i = 0; i < MAXT; i++) {
+            if (i < t) {
+                double temp = x[i];
+#pragma unroll
+                for (int k = 0; k < i; k++) temp = temp - AT(k, i) * x[k];
+                if (!unit) temp = temp / AT(i, i);
+                x[i] = temp;
+            }
         }
-#undef AT
-#undef X
+    } else if (V == RNL || V == RTU) {
+#pragma unroll
+        for (int jj = 0; jj < MAXT; jj++) {
+            const int j = MAXT - 1 - jj;
+            if (j < t) {
+                double bj = x[j];
+#pragma unroll
+                for (int k = j + 1; k < MAXT; k++) {
+                    if (k < t) {
+                        const double a = (V == RNL) ? AT(k, j) : AT(j, k);
+                        if (a != 0.0) bj = bj - a * x[k];
+                    }
+                }
+                if (!unit) bj = bj * (1.0 / AT(j, j));
+                x[j] = bj;
+            }
+        }
+    } else {  // RNU, RTL
+#pragma unroll
+        for (int j = 0; j < MAXT; j++) {
+            if (j < t) {
+                double bj = x[j];
+#pragma unroll
+                for (int k = 0; k < j; k++) {
+                    const double a = (V == RNU) ? AT(k, j) : AT(j, k);
+                    if (a != 0.0) bj = bj - a * x[k];
+                }
+                if (!unit) bj = bj * (1.0 / AT(j, j));
+                x[j] = bj;
+            }
+        }
     }
+#undef AT
+#pragma unroll
+    for (int q = 0; q < MAXT; q++)
+        if (q < t) {
+            if (LEFT) B[q + (size_t)v * ldb] = x[q];
+            else B[v + (size_t)q * ldb] = x[q];
+        }
 }
 
 template <int V>
 void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size_t lda, double* b, size_t ldb,
           const int32_t* abort) {
     if (t == 0 || nvec == 0) return;
-    unsigned blocks = (unsigned)((nvec + TB - 1) / TB);
-    hipLaunchKernelGGL((trsm_base<V>), dim3(blocks), dim3(TB), 0, st, unit, (int)t, (int)nvec, a, lda, b, ldb, abort);
+    if (V == LNL || V == LNU) {
+        const size_t waves = t <= 32 ? (nvec + 1) / 2 : nvec;
+        const unsigned blocks = (unsigned)((waves + 3) / 4);
+        hipLaunchKernelGGL((trsm_ln_lanes<V == LNL>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec, a, lda,
+                           b, ldb, abort);
+    } else {
+        const unsigned blocks = (unsigned)((nvec + 63) / 64);
+        hipLaunchKernelGGL((trsm_dot_regs<V>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda, b,
+                           ldb, abort);
+    }
     BOLA_LAUNCH_CHECK();
 }
 
