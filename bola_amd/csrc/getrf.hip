@@ -41,9 +41,13 @@ struct Cand {
     uint64_t k;
 };
 
+// A published pivot candidate: its key, and its row as it stood before the previous
+// step's deferred rank-1 update (x, fr: that row's multiplier and fused-row flag).
 struct PanelSlot {
     double v;
     uint64_t key;
+    double x;
+    int32_t fr, upd;
     double row[PW];
 };
 
@@ -129,6 +133,41 @@ __device__ __forceinline__ long idamax_rule(int len, const Cand& gv, const Cand&
 
 constexpr int MAXWG_FAST = 32;  // up to this many workgroups, one read round per column
 
+// One element of the reference's GER (ger.rs:28-90) for step j: column offset rc from
+// j+1 (pair = rc/2), tc = -pivot_row[c], tp = -pivot_row[partner].  `np`, `nu`: columns
+// right of j and their even part; `frow`: row inside the 4-aligned fused prefix.
+__device__ __forceinline__ double ger_elem(double a, double x, double tc, double tp, int rc, int np, int nu,
+                                           bool frow, bool upd) {
+    const bool valid = upd && rc < np;
+    const bool paired = rc < nu;
+    const bool skip = !valid || (paired ? (tc == 0.0 && tp == 0.0) : (tc == 0.0));
+    const double f = fma(x, tc, a);
+    const double u = a + tc * x;
+    return skip ? a : ((paired && frow) ? f : u);
+}
+
+// Deferred part of step j's GER on a row window whose slot 0 is column j+1: columns
+// j+2.. (window slots 1..PW-1, rc = slot).  pr: step j's pivot row (absolute columns).
+template <int N>
+__device__ __forceinline__ void ger_rest(double (&row)[N], double x, const double* pr, int j, int np, int nu,
+                                         bool frow, bool upd) {
+#pragma unroll
+    for (int c = 1; c < N; c++) {
+        const int colabs = j + 1 + c;
+        const double tc = -1.0 * pr[colabs];
+        const double tp = -1.0 * ((c & 1) ? pr[colabs - 1] : pr[colabs + 1]);
+        row[c] = ger_elem(row[c], x, tc, tp, c, np, nu, frow, upd);
+    }
+}
+
+// Registers hold the not-yet-retired columns of each row as a window that shifts left
+// by one column per step, so the current column is always slot 0 and every register
+// index is a compile-time constant; a retired column goes to LDS (sret).  Step j's GER
+// is split: column j+1 is updated at once (it is the next pivot column); columns j+2..
+// are updated during step j+1 while the workgroups exchange candidates.  Candidate rows
+// are published before that deferred update together with their multiplier, and the
+// receiver applies it to the one winning row, so every element still sees the
+// reference's sequence of operations.
 template <int R>
 __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restrict__ A, size_t lda,
                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
@@ -136,11 +175,10 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                                                   PanelSlot* __restrict__ cur, RowMap* __restrict__ map, int nwg,
                                                   long spin_limit) {
     __shared__ Cand red[PT / 64][2];
-    __shared__ double prow[PW + 1];
+    __shared__ double prow[2][2 * PW + 2];  // pivot rows of consecutive steps (parity)
     __shared__ int ctl[2];
     __shared__ double sret[PW * PT * R];  // sret[c * PT*R + s*PT + tid]
-    __shared__ double lrows[2 * MAXWG_FAST + 1][PW];  // candidate rows, then the row at position j
-    __shared__ Cand lcand[2 * MAXWG_FAST];
+    __shared__ PanelSlot lslot[2 * MAXWG_FAST + 1];  // candidates, then the row at position j
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -157,14 +195,23 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
 #pragma unroll
         for (int c = 0; c < PW; c++) a[s][c] = (valid[s] && c < w) ? A[r + (size_t)c * lda] : 0.0;
     }
-    if (tid == 0) prow[PW] = 0.0;
+    for (int e = tid; e < 2 * (2 * PW + 2); e += PT) (&prow[0][0])[e] = 0.0;
+    __syncthreads();
 
     const int min_mn = min(m, w);
     const bool fast = nwg <= MAXWG_FAST;
-    int nret = 0;  // retired columns
+    const int CUR = 2 * MAXWG_FAST;
+    // deferred GER state of the previous step
+    double xprev[R];
+    bool fprev[R], uprev[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) { xprev[s] = 0.0; fprev[s] = false; uprev[s] = false; }
+    int dnp = 0, dnu = 0, dj = -1;  // previous step index and its column counts
+    int nret = 0;
     for (int j = 0; j < min_mn; j++) {
         const int len = m - j;
         const int T = len < 16 ? 0 : (len & ~15);
+        const int par = j & 1;
         Cand vb{-1.0, NOKEY}, tb{-1.0, NOKEY};
 #pragma unroll
         for (int s = 0; s < R; s++) {
@@ -183,59 +230,53 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             }
         }
         wg_reduce2(vb, tb, red);
-        const int par = j & 1;
-        const int CUR = 2 * MAXWG_FAST;  // lrows index of the row at position j
-
+        // publish candidate rows (pre-update form) and the row at position j
+        PanelSlot* mine = (nwg == 1) ? lslot : slots + ((size_t)par * nwg + g) * 2;
+        PanelSlot* curp = (nwg == 1) ? lslot + CUR : cur + par;
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s] && pos[s] >= j) {
+                uint64_t rel = (uint64_t)(pos[s] - j);
+                PanelSlot* d0 = nullptr;
+                if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) d0 = mine; }
+                else if (tb.v >= 0.0 && rel == tb.k) d0 = mine + 1;
+                PanelSlot* d1 = (rel == 0) ? curp : nullptr;
+                if (d0) { d0->x = xprev[s]; d0->fr = fprev[s]; d0->upd = uprev[s]; }
+                if (d1) { d1->x = xprev[s]; d1->fr = fprev[s]; d1->upd = uprev[s]; }
+#pragma unroll
+                for (int c = 0; c < PW; c++) {
+                    if (c < w - j) {
+                        if (d0) d0->row[j + c] = a[s][c];
+                        if (d1) d1->row[j + c] = a[s][c];
+                    }
+                }
+            }
+        }
+        if (tid == 0) {
+            mine[0].v = vb.v; mine[0].key = vb.k;
+            mine[1].v = tb.v; mine[1].key = tb.k;
+        }
+        __syncthreads();
+        if (nwg > 1 && tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // the deferred columns of the previous step, while the other workgroups catch up
+        if (dj >= 0) {
+#pragma unroll
+            for (int s = 0; s < R; s++)
+                ger_rest<PW>(a[s], xprev[s], prow[dj & 1], dj, dnp, dnu, fprev[s], uprev[s]);
+        }
+        int src = -1;
+        long rel = 0;
+        const PanelSlot* table;
         if (nwg == 1) {
-            // single workgroup: exchange through LDS only
-#pragma unroll
-            for (int s = 0; s < R; s++) {
-                if (valid[s] && pos[s] >= j) {
-                    uint64_t rel = (uint64_t)(pos[s] - j);
-                    int d0 = -1;
-                    if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) d0 = 0; }
-                    else if (tb.v >= 0.0 && rel == tb.k) d0 = 1;
-#pragma unroll
-                    for (int c = 0; c < PW; c++) {
-                        if (c < w - j) {
-                            if (d0 >= 0) lrows[d0][j + c] = a[s][c];
-                            if (rel == 0) lrows[CUR][j + c] = a[s][c];
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            bool ft;
-            long rel = idamax_rule(len, vb, tb, ft);
-            const int src = rel == 0 ? CUR : (ft ? 1 : 0);
-            if (tid >= j && tid < w) prow[tid] = lrows[src][tid];
-            if (tid == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
-        } else {
-            PanelSlot* mine = slots + ((size_t)par * nwg + g) * 2;
-#pragma unroll
-            for (int s = 0; s < R; s++) {
-                if (valid[s] && pos[s] >= j) {
-                    uint64_t rel = (uint64_t)(pos[s] - j);
-                    PanelSlot* dst0 = nullptr;
-                    if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) dst0 = mine; }
-                    else if (tb.v >= 0.0 && rel == tb.k) dst0 = mine + 1;
-                    PanelSlot* dst1 = (rel == 0) ? cur + par : nullptr;
-#pragma unroll
-                    for (int c = 0; c < PW; c++) {
-                        if (c < w - j) {
-                            if (dst0) dst0->row[j + c] = a[s][c];
-                            if (dst1) dst1->row[j + c] = a[s][c];
-                        }
-                    }
-                }
-            }
+            table = lslot;
             if (tid == 0) {
-                mine[0].v = vb.v; mine[0].key = vb.k;
-                mine[1].v = tb.v; mine[1].key = tb.k;
+                bool ft;
+                rel = idamax_rule(len, vb, tb, ft);
+                src = rel == 0 ? CUR : (ft ? 1 : 0);
+                ctl[0] = (int)rel;
+                ctl[1] = src;
             }
-            __syncthreads();
-            if (tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            // every workgroup's flag is acquired by one thread here
+        } else {
             int ok = 1;
             for (int q = tid; q < nwg; q += PT) {
                 long sp = 0;
@@ -255,78 +296,75 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             }
             const PanelSlot* base = slots + (size_t)par * nwg * 2;
             if (fast) {
-                // one read round: every candidate's key and row, and the row at position j
-                const int nrow = 2 * nwg;
-                for (int e = tid; e < nrow * (PW + 2) + PW; e += PT) {
-                    if (e < nrow * (PW + 2)) {
-                        const int q = e / (PW + 2), f = e % (PW + 2);
-                        if (f == 0) lcand[q].v = base[q].v;
-                        else if (f == 1) lcand[q].k = base[q].key;
-                        else if (f - 2 >= j && f - 2 < w) lrows[q][f - 2] = base[q].row[f - 2];
-                    } else {
-                        const int c = e - nrow * (PW + 2);
-                        if (c >= j && c < w) lrows[CUR][c] = cur[par].row[c];
-                    }
+                // one read round: every candidate (key, multiplier, row) and the row at position j
+                constexpr int NW = sizeof(PanelSlot) / 8;
+                const int nslot = 2 * nwg;
+                for (int e = tid; e < (nslot + 1) * NW; e += PT) {
+                    const int q = e / NW, f = e % NW;
+                    const double* srcp = (q < nslot) ? (const double*)(base + q) : (const double*)(cur + par);
+                    double* dstp = (double*)(lslot + (q < nslot ? q : CUR));
+                    dstp[f] = srcp[f];
                 }
-                __syncthreads();
-                if (wv == 0) {
-                    Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
-                    int gvs = -1, gts = -1;
-                    for (int q = lane; q < nwg; q += 64) {
-                        if (better(lcand[2 * q].v, lcand[2 * q].k, gv.v, gv.k)) { gv = lcand[2 * q]; gvs = 2 * q; }
-                        if (better(lcand[2 * q + 1].v, lcand[2 * q + 1].k, gt.v, gt.k)) {
-                            gt = lcand[2 * q + 1];
-                            gts = 2 * q + 1;
-                        }
-                    }
-                    for (int off = 32; off >= 1; off >>= 1) {
-                        double ov = __shfl_xor(gv.v, off, 64);
-                        uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
-                        int os = __shfl_xor(gvs, off, 64);
-                        if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
-                        ov = __shfl_xor(gt.v, off, 64);
-                        okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
-                        os = __shfl_xor(gts, off, 64);
-                        if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
-                    }
-                    bool ft;
-                    long rel = idamax_rule(len, gv, gt, ft);
-                    const int src = rel == 0 ? CUR : (ft ? gts : gvs);
-                    if (lane >= j && lane < w) prow[lane] = lrows[src][lane];
-                    if (lane == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
-                }
+                table = lslot;
             } else {
-                // many workgroups: keys first, then the winning row
-                if (wv == 0) {
-                    Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
-                    int gvs = -1, gts = -1;
-                    for (int q = lane; q < nwg; q += 64) {
-                        double v0 = base[2 * q].v, v1 = base[2 * q + 1].v;
-                        uint64_t k0 = base[2 * q].key, k1 = base[2 * q + 1].key;
-                        if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
-                        if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
-                    }
-                    for (int off = 32; off >= 1; off >>= 1) {
-                        double ov = __shfl_xor(gv.v, off, 64);
-                        uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
-                        int os = __shfl_xor(gvs, off, 64);
-                        if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
-                        ov = __shfl_xor(gt.v, off, 64);
-                        okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
-                        os = __shfl_xor(gts, off, 64);
-                        if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
-                    }
-                    bool ft;
-                    long rel = idamax_rule(len, gv, gt, ft);
-                    const PanelSlot* P = rel == 0 ? cur + par : base + (ft ? gts : gvs);
-                    if (lane >= j && lane < w) prow[lane] = P->row[lane];
-                    if (lane == 0) { ctl[0] = (int)rel; ctl[1] = 1; }
+                table = nullptr;
+            }
+            __syncthreads();
+            if (wv == 0) {
+                Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+                int gvs = -1, gts = -1;
+                const PanelSlot* T2 = fast ? lslot : base;
+                for (int q = lane; q < nwg; q += 64) {
+                    double v0 = T2[2 * q].v, v1 = T2[2 * q + 1].v;
+                    uint64_t k0 = T2[2 * q].key, k1 = T2[2 * q + 1].key;
+                    if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
+                    if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
                 }
+                for (int off = 32; off >= 1; off >>= 1) {
+                    double ov = __shfl_xor(gv.v, off, 64);
+                    uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
+                    int os = __shfl_xor(gvs, off, 64);
+                    if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
+                    ov = __shfl_xor(gt.v, off, 64);
+                    okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
+                    os = __shfl_xor(gts, off, 64);
+                    if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+                }
+                if (lane == 0) {
+                    bool ft;
+                    long r2 = idamax_rule(len, gv, gt, ft);
+                    ctl[0] = (int)r2;
+                    ctl[1] = r2 == 0 ? CUR : (ft ? gts : gvs);
+                }
+            }
+            (void)table;
+        }
+        __syncthreads();
+        // the winning row: apply the previous step's deferred GER to it, then it is the pivot row
+        {
+            const int sidx = ctl[1];
+            const PanelSlot* P;
+            if (nwg == 1 || fast) P = lslot + sidx;
+            else P = (sidx == CUR) ? cur + par : slots + (size_t)par * nwg * 2 + sidx;
+            if (tid < PW) {
+                const int c = tid;  // absolute column
+                double v = 0.0;
+                if (c >= j && c < w) {
+                    v = P->row[c];
+                    if (dj >= 0 && c > j) {
+                        const int rc = c - (dj + 1);
+                        const double* pr = prow[dj & 1];
+                        const double tc = -1.0 * pr[c];
+                        const double tp = -1.0 * ((rc & 1) ? pr[c - 1] : pr[c + 1]);
+                        v = ger_elem(v, P->x, tc, tp, rc, dnp, dnu, P->fr != 0, P->upd != 0);
+                    }
+                }
+                prow[par][c] = v;
             }
         }
         __syncthreads();
         const int jp = j + ctl[0];
-        const double piv = prow[j];
+        const double piv = prow[par][j];
         if (g == 0 && tid == 0) ipiv[j] = jp + 1;
         if (!(piv != 0.0)) {
             if (g == 0 && tid == 0) {
@@ -348,34 +386,28 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             for (int s = 0; s < R; s++)
                 if (valid[s] && pos[s] > j) a[s][0] = a[s][0] * inv;
         }
-        if (j < min_mn - 1) {
-            const int np = w - j - 1;
-            const int nu = np - (np & 1);
-            const int mf = (m - j - 1) & ~3;
+        // step j's GER: column j+1 now, the rest deferred to the next step
+        const int np = w - j - 1;
+        const int nu = np - (np & 1);
+        const int mf = (m - j - 1) & ~3;
+        const bool do_ger = j < min_mn - 1;
+        {
+            const double* pr = prow[par];
+            const double tc = -1.0 * pr[j + 1];
+            const double tp = -1.0 * pr[j + 2];
 #pragma unroll
             for (int s = 0; s < R; s++) {
-                if (valid[s] && pos[s] > j) {
-                    const double x = a[s][0];
-                    const bool frow = (pos[s] - (j + 1)) < mf;
-#pragma unroll
-                    for (int c = 1; c < PW; c++) {
-                        if (c <= np) {
-                            const int rc = c - 1;  // column index relative to j+1
-                            const double tc = -1.0 * prow[j + c];
-                            const bool paired = rc < nu;
-                            bool skip;
-                            if (paired) {
-                                const double tp = -1.0 * ((rc & 1) ? prow[j + c - 1] : prow[j + c + 1]);
-                                skip = (tc == 0.0 && tp == 0.0);
-                            } else {
-                                skip = (tc == 0.0);
-                            }
-                            if (!skip) a[s][c] = (paired && frow) ? fma(x, tc, a[s][c]) : a[s][c] + tc * x;
-                        }
-                    }
-                }
+                const bool upd = do_ger && valid[s] && pos[s] > j;
+                const bool frow = (pos[s] - (j + 1)) < mf;
+                a[s][1] = ger_elem(a[s][1], a[s][0], tc, tp, 0, np, nu, frow, upd);
+                xprev[s] = a[s][0];
+                fprev[s] = frow;
+                uprev[s] = upd;
             }
         }
+        dj = do_ger ? j : -1;
+        dnp = np;
+        dnu = nu;
         // retire column j and shift the window
 #pragma unroll
         for (int s = 0; s < R; s++) {
@@ -386,6 +418,8 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
         }
         nret = j + 1;
     }
+    // (no deferred update is pending here: the last step has no GER, and an error stops
+    // the loop after the previous step's deferred columns were applied)
     // rows to their final positions; moved rows form the panel's row map
 #pragma unroll
     for (int s = 0; s < R; s++) {
