@@ -39,10 +39,11 @@ hipEvent_t take_event(bola_hip_handle_s* h) {
 
 }  // namespace
 
-Phase::Phase(bola_hip_handle_s* h_, const char* n, double fl) : h(h_), name(n), flops(fl) {
+Phase::Phase(bola_hip_handle_s* h_, const char* n, double fl, hipStream_t s) : h(h_), name(n), flops(fl) {
+    st = s ? s : (h ? h->stream : nullptr);
     if (h && h->profile) {
         e0 = take_event(h);
-        BOLA_CHECK(hipEventRecord(e0, h->stream));
+        BOLA_CHECK(hipEventRecord(e0, st));
     }
 }
 
@@ -51,7 +52,7 @@ Phase::~Phase() {
         hipEvent_t e1 = nullptr;
         try {
             e1 = take_event(h);
-            hipEventRecord(e1, h->stream);
+            (void)hipEventRecord(e1, st);
             h->pending.push_back(PendingEvent{name, e0, e1, flops});
         } catch (...) {
         }
@@ -61,6 +62,7 @@ Phase::~Phase() {
 void collect_profile(bola_hip_handle_s* h) {
     if (h->pending.empty()) return;
     BOLA_CHECK(hipStreamSynchronize(h->stream));
+    if (h->stream2) BOLA_CHECK(hipStreamSynchronize(h->stream2));
     for (auto& p : h->pending) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, p.start, p.stop);
@@ -224,8 +226,13 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     h->ws.release();
     h->ws_aux.release();
     h->stage.release();
-    if (h->d_info) hipFree(h->d_info);
-    if (h->own_stream) hipStreamDestroy(h->stream);
+    if (h->d_info) (void)hipFree(h->d_info);
+    for (auto e : h->sync_events) (void)hipEventDestroy(e);
+    if (h->stream2) {
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamDestroy(h->stream2);
+    }
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
     if (t_default == h) t_default = nullptr;
     delete h;
     return 0;

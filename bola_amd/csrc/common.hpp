@@ -95,6 +95,15 @@ struct bola_hip_handle_s {
     std::vector<bola::PendingEvent> pending;
     std::map<std::string, bola::PhaseStat> stats;
     std::vector<hipEvent_t> event_pool;
+    // second stream for work that overlaps the critical path, and ordering events
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> sync_events;
+    hipStream_t aux_stream() {
+        if (!stream2) {
+            if (hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking) != hipSuccess) stream2 = nullptr;
+        }
+        return stream2 ? stream2 : stream;
+    }
 };
 
 namespace bola {
@@ -105,7 +114,8 @@ struct Phase {
     const char* name;
     double flops;
     hipEvent_t e0 = nullptr;
-    Phase(bola_hip_handle_s* h_, const char* n, double fl = 0.0);
+    hipStream_t st = nullptr;
+    Phase(bola_hip_handle_s* h_, const char* n, double fl = 0.0, hipStream_t s = nullptr);
     ~Phase();
 };
 
@@ -152,10 +162,12 @@ void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, 
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
                   const int32_t* abort = nullptr);
 // slab: depth of one accumulate-then-fold step (64 = gemm.rs KC; 32 = the syrk.rs block
-// loop); mask: 0 all of C, 1 only rows in a later 32-block than the column, 2 earlier.
+// loop, or one LU panel per step); mask: 0 all of C, 1 only rows in a later 32-block than
+// the column, 2 earlier; edge_m / edge_n: first row / column (relative to C) that takes
+// the reference's unfused edge path, -1 for this call's own m - m%8 / n - n%4.
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask);
+                     const int32_t* abort, int slab, int mask, long edge_m = -1, long edge_n = -1);
 void set_num_cus(int n);
 
 // trsm.hip

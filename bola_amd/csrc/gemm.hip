@@ -259,14 +259,18 @@ void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double
 template <bool TA, bool TB>
 void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
               size_t ldb, double beta, double* C, size_t ldc, int num_cus, const int32_t* abort, int slab,
-              int mask) {
-    const int edge_m = m - (m % 8), edge_n = n - (n % 4);
+              int mask, int edge_m, int edge_n) {
     if (edge_m > 0 && edge_n > 0) {
-        long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
+        const long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
+        const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
         if (t128 >= 2L * num_cus)
-            launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab, mask);
-        else
+            launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
+                                         slab, mask);
+        else if (t64 >= num_cus || m * (long)n <= 64L * 64)
             launch_tile<64, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
+                                        mask);
+        else
+            launch_tile<32, 32, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
                                         mask);
     }
     long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
@@ -288,19 +292,23 @@ void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
 void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
                   const int32_t* abort) {
-    launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0);
+    launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0, -1, -1);
 }
 
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask) {
+                     const int32_t* abort, int slab, int mask, long edge_m, long edge_n) {
     bool ta = !(transa == 'N' || transa == 'n');
     bool tb = !(transb == 'N' || transb == 'n');
     int M = (int)m, N = (int)n, K = (int)k;
-    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
-    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
-    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
-    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask);
+    // elements in rows >= em or columns >= en take the reference's unfused edge path;
+    // by default the last m % 8 rows and n % 4 columns of this call (gemm.rs:177-190)
+    const int em = edge_m < 0 ? M - (M % 8) : (int)std::min<long>(std::max<long>(edge_m, 0), M);
+    const int en = edge_n < 0 ? N - (N % 4) : (int)std::min<long>(std::max<long>(edge_n, 0), N);
+    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
+    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
+    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
+    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
 }
 
 }  // namespace bola

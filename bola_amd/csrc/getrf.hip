@@ -47,7 +47,7 @@ struct PanelSlot {
     double v;
     uint64_t key;
     double x;
-    int32_t fr, upd;
+    uint64_t flags;  // bit 0: fused row, bit 1: row updated
     double row[PW];
 };
 
@@ -91,9 +91,6 @@ __device__ void wg_reduce(Cand& c, Cand* sh, Cand* out) {
     c = *out;
 }
 
-// Registers hold the not-yet-retired columns of each row as a window that shifts left
-// by one column per step, so the current column is always a[.][0] and every register
-// index is a compile-time constant; a retired column goes to LDS (sret).
 // reduce two candidate streams at once; every thread gets both results
 __device__ __forceinline__ void wg_reduce2(Cand& a, Cand& b, Cand (*sh)[2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -146,18 +143,57 @@ __device__ __forceinline__ double ger_elem(double a, double x, double tc, double
     return skip ? a : ((paired && frow) ? f : u);
 }
 
-// Deferred part of step j's GER on a row window whose slot 0 is column j+1: columns
-// j+2.. (window slots 1..PW-1, rc = slot).  pr: step j's pivot row (absolute columns).
-template <int N>
-__device__ __forceinline__ void ger_rest(double (&row)[N], double x, const double* pr, int j, int np, int nu,
-                                         bool frow, bool upd) {
+// Deferred part of step j's GER on the row windows of one thread (slot 0 = column j+1):
+// columns j+2.. (window slots 1..PW-1, rc = slot).  pr: step j's pivot row (absolute
+// columns).  Whether a column is touched at all (pair skip) is the same for every row,
+// and in most wavefronts every row takes the fused form, so the common case is one FMA
+// per element.
+template <int R>
+__device__ __forceinline__ void ger_rest(double (&a)[R][PW], const double (&x)[R], const bool (&fr)[R],
+                                         const bool (&up)[R], const double* pr, int j, int np, int nu) {
+    bool allf = true;
 #pragma unroll
-    for (int c = 1; c < N; c++) {
-        const int colabs = j + 1 + c;
-        const double tc = -1.0 * pr[colabs];
-        const double tp = -1.0 * ((c & 1) ? pr[colabs - 1] : pr[colabs + 1]);
-        row[c] = ger_elem(row[c], x, tc, tp, c, np, nu, frow, upd);
+    for (int s = 0; s < R; s++) allf = allf && fr[s] && up[s];
+    const bool wave_fast = __all(allf);
+#pragma unroll
+    for (int c = 1; c < PW; c++) {
+        if (c < np) {
+            const int colabs = j + 1 + c;
+            const double tc = -1.0 * pr[colabs];
+            const double tp = -1.0 * ((c & 1) ? pr[colabs - 1] : pr[colabs + 1]);
+            const bool paired = c < nu;
+            const int skip = __builtin_amdgcn_readfirstlane(paired ? (tc == 0.0 && tp == 0.0) : (tc == 0.0));
+            if (!skip) {
+                if (paired && wave_fast) {
+#pragma unroll
+                    for (int s = 0; s < R; s++) a[s][c] = fma(x[s], tc, a[s][c]);
+                } else {
+#pragma unroll
+                    for (int s = 0; s < R; s++) {
+                        const double f = fma(x[s], tc, a[s][c]);
+                        const double u = a[s][c] + tc * x[s];
+                        a[s][c] = up[s] ? ((paired && fr[s]) ? f : u) : a[s][c];
+                    }
+                }
+            }
+        }
     }
+}
+
+// Exchange between workgroups uses relaxed device-scope atomic loads and stores, which
+// are coherent across XCDs without cache flushes; a writer waits for its stores to
+// complete before raising its flag, a reader loads the data after seeing the flag.
+__device__ __forceinline__ void xst(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double xld(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xstu(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t xldu(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Registers hold the not-yet-retired columns of each row as a window that shifts left
@@ -231,8 +267,9 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
         }
         wg_reduce2(vb, tb, red);
         // publish candidate rows (pre-update form) and the row at position j
-        PanelSlot* mine = (nwg == 1) ? lslot : slots + ((size_t)par * nwg + g) * 2;
-        PanelSlot* curp = (nwg == 1) ? lslot + CUR : cur + par;
+        const bool lds_only = (nwg == 1);
+        PanelSlot* mine = lds_only ? lslot : slots + ((size_t)par * nwg + g) * 2;
+        PanelSlot* curp = lds_only ? lslot + CUR : cur + par;
 #pragma unroll
         for (int s = 0; s < R; s++) {
             if (valid[s] && pos[s] >= j) {
@@ -241,46 +278,53 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                 if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) d0 = mine; }
                 else if (tb.v >= 0.0 && rel == tb.k) d0 = mine + 1;
                 PanelSlot* d1 = (rel == 0) ? curp : nullptr;
-                if (d0) { d0->x = xprev[s]; d0->fr = fprev[s]; d0->upd = uprev[s]; }
-                if (d1) { d1->x = xprev[s]; d1->fr = fprev[s]; d1->upd = uprev[s]; }
+                const uint64_t fl = (fprev[s] ? 1ull : 0ull) | (uprev[s] ? 2ull : 0ull);
+                for (int t = 0; t < 2; t++) {
+                    PanelSlot* d = t ? d1 : d0;
+                    if (!d) continue;
+                    if (lds_only) {
+                        d->x = xprev[s];
+                        d->flags = fl;
+                    } else {
+                        xst(&d->x, xprev[s]);
+                        xstu(&d->flags, fl);
+                    }
 #pragma unroll
-                for (int c = 0; c < PW; c++) {
-                    if (c < w - j) {
-                        if (d0) d0->row[j + c] = a[s][c];
-                        if (d1) d1->row[j + c] = a[s][c];
+                    for (int c = 0; c < PW; c++) {
+                        if (c < w - j) {
+                            if (lds_only) d->row[j + c] = a[s][c];
+                            else xst(&d->row[j + c], a[s][c]);
+                        }
                     }
                 }
             }
         }
         if (tid == 0) {
-            mine[0].v = vb.v; mine[0].key = vb.k;
-            mine[1].v = tb.v; mine[1].key = tb.k;
+            if (lds_only) {
+                mine[0].v = vb.v; mine[0].key = vb.k;
+                mine[1].v = tb.v; mine[1].key = tb.k;
+            } else {
+                xst(&mine[0].v, vb.v); xstu(&mine[0].key, vb.k);
+                xst(&mine[1].v, tb.v); xstu(&mine[1].key, tb.k);
+            }
         }
+        if (!lds_only) __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
         __syncthreads();
-        if (nwg > 1 && tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!lds_only && tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // the deferred columns of the previous step, while the other workgroups catch up
-        if (dj >= 0) {
-#pragma unroll
-            for (int s = 0; s < R; s++)
-                ger_rest<PW>(a[s], xprev[s], prow[dj & 1], dj, dnp, dnu, fprev[s], uprev[s]);
-        }
-        int src = -1;
-        long rel = 0;
-        const PanelSlot* table;
-        if (nwg == 1) {
-            table = lslot;
+        if (dj >= 0) ger_rest<R>(a, xprev, fprev, uprev, prow[dj & 1], dj, dnp, dnu);
+        if (lds_only) {
             if (tid == 0) {
                 bool ft;
-                rel = idamax_rule(len, vb, tb, ft);
-                src = rel == 0 ? CUR : (ft ? 1 : 0);
+                const long rel = idamax_rule(len, vb, tb, ft);
                 ctl[0] = (int)rel;
-                ctl[1] = src;
+                ctl[1] = rel == 0 ? CUR : (ft ? 1 : 0);
             }
         } else {
             int ok = 1;
             for (int q = tid; q < nwg; q += PT) {
                 long sp = 0;
-                while (__hip_atomic_load(flags + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
+                while (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
                     if (++sp > spin_limit ||
                         __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                         ok = 0;
@@ -301,22 +345,25 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                 const int nslot = 2 * nwg;
                 for (int e = tid; e < (nslot + 1) * NW; e += PT) {
                     const int q = e / NW, f = e % NW;
-                    const double* srcp = (q < nslot) ? (const double*)(base + q) : (const double*)(cur + par);
-                    double* dstp = (double*)(lslot + (q < nslot ? q : CUR));
-                    dstp[f] = srcp[f];
+                    const uint64_t* srcp = (q < nslot) ? (const uint64_t*)(base + q) : (const uint64_t*)(cur + par);
+                    uint64_t* dstp = (uint64_t*)(lslot + (q < nslot ? q : CUR));
+                    dstp[f] = xldu(srcp + f);
                 }
-                table = lslot;
-            } else {
-                table = nullptr;
+                __syncthreads();
             }
-            __syncthreads();
             if (wv == 0) {
                 Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
                 int gvs = -1, gts = -1;
-                const PanelSlot* T2 = fast ? lslot : base;
                 for (int q = lane; q < nwg; q += 64) {
-                    double v0 = T2[2 * q].v, v1 = T2[2 * q + 1].v;
-                    uint64_t k0 = T2[2 * q].key, k1 = T2[2 * q + 1].key;
+                    double v0, v1;
+                    uint64_t k0, k1;
+                    if (fast) {
+                        v0 = lslot[2 * q].v; v1 = lslot[2 * q + 1].v;
+                        k0 = lslot[2 * q].key; k1 = lslot[2 * q + 1].key;
+                    } else {
+                        v0 = xld(&base[2 * q].v); v1 = xld(&base[2 * q + 1].v);
+                        k0 = xldu(&base[2 * q].key); k1 = xldu(&base[2 * q + 1].key);
+                    }
                     if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
                     if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
                 }
@@ -332,31 +379,32 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                 }
                 if (lane == 0) {
                     bool ft;
-                    long r2 = idamax_rule(len, gv, gt, ft);
+                    const long r2 = idamax_rule(len, gv, gt, ft);
                     ctl[0] = (int)r2;
                     ctl[1] = r2 == 0 ? CUR : (ft ? gts : gvs);
                 }
             }
-            (void)table;
         }
         __syncthreads();
         // the winning row: apply the previous step's deferred GER to it, then it is the pivot row
         {
             const int sidx = ctl[1];
-            const PanelSlot* P;
-            if (nwg == 1 || fast) P = lslot + sidx;
-            else P = (sidx == CUR) ? cur + par : slots + (size_t)par * nwg * 2 + sidx;
+            const bool in_lds = lds_only || fast;
+            const PanelSlot* P = in_lds ? lslot + sidx
+                                        : ((sidx == CUR) ? cur + par : slots + (size_t)par * nwg * 2 + sidx);
             if (tid < PW) {
                 const int c = tid;  // absolute column
                 double v = 0.0;
                 if (c >= j && c < w) {
-                    v = P->row[c];
+                    v = in_lds ? P->row[c] : xld(&P->row[c]);
                     if (dj >= 0 && c > j) {
+                        const double px = in_lds ? P->x : xld(&P->x);
+                        const uint64_t pf = in_lds ? P->flags : xldu(&P->flags);
                         const int rc = c - (dj + 1);
                         const double* pr = prow[dj & 1];
                         const double tc = -1.0 * pr[c];
                         const double tp = -1.0 * ((rc & 1) ? pr[c - 1] : pr[c + 1]);
-                        v = ger_elem(v, P->x, tc, tp, rc, dnp, dnu, P->fr != 0, P->upd != 0);
+                        v = ger_elem(v, px, tc, tp, rc, dnp, dnu, (pf & 1) != 0, (pf & 2) != 0);
                     }
                 }
                 prow[par][c] = v;
@@ -588,9 +636,8 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
 const long kSpinLimit = 1L << 22;
 
 // iterative_getrf2 (getrf2.rs:131-185) on an m x n block, n <= 32 or m <= 32.
-void getf2(bola_hip_handle_s* h, int m, int n, double* a, size_t lda, int32_t* ipiv, int32_t* info, int col_off,
+void getf2(hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv, int32_t* info, int col_off,
            long row_off, const LuWs& w, int panel_idx) {
-    hipStream_t st = h->stream;
     if (n <= PW) {
         const int r = panel_rows_per_thread(m);
         const int nwg = panel_nwg(m);
@@ -629,7 +676,7 @@ void getrf2_rec(bola_hip_handle_s* h, int m, int n, double* a, size_t lda, int32
         BOLA_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, h->stream, (int32_t*)w.maps, (size_t)1);
         BOLA_LAUNCH_CHECK();
-        getf2(h, m, n, a, lda, ipiv, info, col_off, 0, w, 0);
+        getf2(h->stream, m, n, a, lda, ipiv, info, col_off, 0, w, 0);
         // the panel kernel's own row map is not needed here: the recursion applies
         // interchanges through laswp with the pivots, as the reference does
         return;
@@ -651,8 +698,129 @@ void getrf2_rec(bola_hip_handle_s* h, int m, int n, double* a, size_t lda, int32
 
 }  // namespace
 
+namespace {
+
+// Row/column where the reference's trailing GEMM of the panel at column j (width jb)
+// switches to its unfused edge path: rows >= m - (m-j-jb)%8, columns >= n - (n-j-jb)%4.
+struct Edge {
+    long row, col;
+};
+inline Edge panel_edge(int m, int n, int j, int jb) {
+    return Edge{(long)m - (long)((m - j - jb) % 8), (long)n - (long)((n - j - jb) % 4)};
+}
+
+// Update of the columns [c0, c1) right of the super-panel [s0, s0+sw) whose panels are
+// factored: the panels' interchanges, then for each panel its U rows (GEMM from the
+// earlier panels of the super-panel, TRSM with its L11), then one GEMM for all rows
+// below the super-panel with one 32-deep fold per panel.  Every element receives the
+// reference's operations in the reference's order (getrf.rs:53-98).
+void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, size_t lda, int32_t* info,
+                 const LuWs& w, int s0, int sw, int c0, int c1) {
+    if (c1 <= c0) return;
+    const int ncols = c1 - c0;
+    const int p0 = s0 / PW, G = (sw + PW - 1) / PW;
+    double* B = a + (size_t)c0 * lda;
+    {
+        Phase pl(h, "LASWP", 0.0, st);
+        launch_rowmap_apply(st, B, lda, (size_t)ncols, 0, 0, w.maps + p0, G, info, nullptr, 0, 0);
+    }
+    const bool uniform = (sw % PW) == 0;
+    if (uniform) {
+        const Edge e = panel_edge(m, n, s0, PW);  // the same for every full-width panel
+        for (int q = 0; q < G; q++) {
+            const int j = s0 + q * PW;
+            if (q > 0) {
+                Phase pg(h, "GEMM", 2.0 * PW * ncols * (q * PW), st);
+                launch_dgemm_ex(st, 'N', 'N', PW, ncols, (size_t)q * PW, -1.0, a + j + (size_t)s0 * lda, lda,
+                                a + s0 + (size_t)c0 * lda, lda, 1.0, a + j + (size_t)c0 * lda, lda, info, PW, 0,
+                                e.row - j, e.col - c0);
+            }
+            Phase pt(h, "TRSM", (double)PW * PW * ncols, st);
+            launch_dtrsm(st, 'L', 'L', 'N', 'U', PW, ncols, 1.0, a + j + (size_t)j * lda, lda, a + j + (size_t)c0 * lda,
+                         lda, info);
+        }
+        if (s0 + sw < m) {
+            Phase pg(h, "GEMM", 2.0 * (double)(m - s0 - sw) * ncols * sw, st);
+            launch_dgemm_ex(st, 'N', 'N', (size_t)(m - s0 - sw), ncols, (size_t)sw, -1.0, a + (s0 + sw) + (size_t)s0 * lda,
+                            lda, a + s0 + (size_t)c0 * lda, lda, 1.0, a + (s0 + sw) + (size_t)c0 * lda, lda, info, PW, 0,
+                            e.row - (s0 + sw), e.col - c0);
+        }
+    } else {
+        for (int q = 0; q < G; q++) {
+            const int j = s0 + q * PW;
+            const int jb = std::min(PW, s0 + sw - j);
+            {
+                Phase pt(h, "TRSM", (double)jb * jb * ncols, st);
+                launch_dtrsm(st, 'L', 'L', 'N', 'U', jb, ncols, 1.0, a + j + (size_t)j * lda, lda,
+                             a + j + (size_t)c0 * lda, lda, info);
+            }
+            if (j + jb < m) {
+                const Edge e = panel_edge(m, n, j, jb);
+                Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * ncols * jb, st);
+                launch_dgemm_ex(st, 'N', 'N', (size_t)(m - j - jb), ncols, (size_t)jb, -1.0,
+                                a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)c0 * lda, lda, 1.0,
+                                a + (j + jb) + (size_t)c0 * lda, lda, info, PW, 0, e.row - (j + jb), e.col - c0);
+            }
+        }
+    }
+}
+
+// The columns [s0, s0+sw) themselves, panel by panel: the reference's loop restricted
+// to these columns (their left/right interchanges, TRSM and GEMM stay inside them).
+void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv,
+                  int32_t* info, const LuWs& w, int s0, int sw) {
+    const int G = (sw + PW - 1) / PW;
+    for (int q = 0; q < G; q++) {
+        const int j = s0 + q * PW;
+        const int jb = std::min(PW, s0 + sw - j);
+        const int p = j / PW;
+        {
+            Phase pp(h, "GETRF2_IT", 0.0, st);
+            getf2(st, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
+        }
+        {
+            Phase pl(h, "LASWP", 0.0, st);
+            launch_rowmap_apply(st, a + (size_t)s0 * lda, lda, (size_t)(j - s0), (size_t)(j + jb - s0),
+                                (size_t)(s0 + sw - j - jb), w.maps + p, 1, info, ipiv + j, jb, j);
+        }
+        const int nr = s0 + sw - j - jb;
+        if (nr > 0) {
+            {
+                Phase pt(h, "TRSM", (double)jb * jb * nr, st);
+                launch_dtrsm(st, 'L', 'L', 'N', 'U', jb, nr, 1.0, a + j + (size_t)j * lda, lda,
+                             a + j + (size_t)(j + jb) * lda, lda, info);
+            }
+            if (j + jb < m) {
+                const Edge e = panel_edge(m, n, j, jb);
+                Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * nr * jb, st);
+                launch_dgemm_ex(st, 'N', 'N', (size_t)(m - j - jb), (size_t)nr, (size_t)jb, -1.0,
+                                a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, 1.0,
+                                a + (j + jb) + (size_t)(j + jb) * lda, lda, info, PW, 0, e.row - (j + jb),
+                                e.col - (j + jb));
+            }
+        }
+    }
+}
+
+hipEvent_t lu_event(bola_hip_handle_s* h, int i) {
+    while ((int)h->sync_events.size() <= i) {
+        hipEvent_t e;
+        BOLA_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->sync_events.push_back(e);
+    }
+    return h->sync_events[i];
+}
+
+}  // namespace
+
 // Status words: info[0] = 0 or global 1-based failing column (or BOLA_HIP_ERR_DEVICE);
 // info[1] = the column number the reference prints (local to its panel).
+//
+// Schedule: super-panels of W columns.  Stream 1 factors a super-panel and then brings
+// the next super-panel's columns up to date (look-ahead); stream 2 meanwhile applies the
+// super-panel to all columns further right (the bulk GEMM), so the next factorization
+// overlaps the bulk update.  Work on the two streams never touches the same columns at
+// the same time; events order the hand-offs.
 void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
                bool recursive_entry) {
     hipStream_t st = h->stream;
@@ -671,33 +839,34 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     Phase ph(h, "GETRF", 2.0 * (double)min_mn * min_mn * (3.0 * std::max(m, n) - min_mn) / 6.0);
     const int npanels = (min_mn + PW - 1) / PW;
     LuWs w = lu_workspace(h, m, npanels);
-    for (int p = 0; p < npanels; p++) {
-        const int j = p * PW;
-        const int jb = std::min(PW, min_mn - j);
-        {
-            Phase pp(h, "GETRF2_IT");
-            getf2(h, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
+    const int W = min_mn >= 2048 ? 256 : 128;
+    hipStream_t s2 = h->aux_stream();
+    int ev = 0;
+    hipEvent_t bulk_done = nullptr;  // the previous super-panel's bulk update
+    for (int s0 = 0; s0 < min_mn; s0 += W) {
+        const int sw = std::min(W, min_mn - s0);
+        factor_super(h, st, m, n, a, lda, ipiv, info, w, s0, sw);
+        hipEvent_t fact_done = lu_event(h, ev++);
+        BOLA_CHECK(hipEventRecord(fact_done, st));
+        // from here on stream 1 touches columns the previous bulk update wrote
+        if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
+        if (s0 > 0) {
+            Phase pl(h, "LASWP", 0.0, st);
+            launch_rowmap_apply(st, a, lda, (size_t)s0, 0, 0, w.maps + s0 / PW, (sw + PW - 1) / PW, info, nullptr, 0, 0);
         }
-        {
-            Phase pl(h, "LASWP");
-            const size_t right = (j + jb < n) ? (size_t)(n - j - jb) : 0;
-            launch_rowmap_apply(st, a, lda, (size_t)j, (size_t)(j + jb), right, w.maps + p, 1, info, ipiv + j, jb, j);
-        }
-        if (j + jb < n) {
-            double* l11 = a + j + (size_t)j * lda;
-            double* u12 = a + j + (size_t)(j + jb) * lda;
-            {
-                Phase pt(h, "TRSM", (double)jb * jb * (n - j - jb));
-                launch_dtrsm(st, 'L', 'L', 'N', 'U', (size_t)jb, (size_t)(n - j - jb), 1.0, l11, lda, u12, lda, info);
-            }
-            if (j + jb < m) {
-                Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * (n - j - jb) * jb);
-                launch_dgemm(st, 'N', 'N', (size_t)(m - j - jb), (size_t)(n - j - jb), (size_t)jb, -1.0,
-                             a + (j + jb) + (size_t)j * lda, lda, u12, lda, 1.0, a + (j + jb) + (size_t)(j + jb) * lda,
-                             lda, info);
-            }
+        const int c_lo = s0 + sw;
+        const int c_mid = std::min(n, c_lo + W);
+        update_cols(h, st, m, n, a, lda, info, w, s0, sw, c_lo, c_mid);
+        if (c_mid < n) {
+            BOLA_CHECK(hipStreamWaitEvent(s2, fact_done, 0));
+            update_cols(h, s2, m, n, a, lda, info, w, s0, sw, c_mid, n);
+            bulk_done = lu_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(bulk_done, s2));
+        } else {
+            bulk_done = nullptr;
         }
     }
+    if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
 }
 
 }  // namespace bola
