@@ -145,25 +145,31 @@ __device__ __forceinline__ double ger_elem(double a, double x, double tc, double
 
 // Deferred part of step j's GER on the row windows of one thread (slot 0 = column j+1):
 // columns j+2.. (window slots 1..PW-1, rc = slot).  pr: step j's pivot row (absolute
-// columns).  Whether a column is touched at all (pair skip) is the same for every row,
-// and in most wavefronts every row takes the fused form, so the common case is one FMA
-// per element.
+// columns), zmask: bit i set when pr[i] == 0.  Whether a column is touched at all (pair
+// skip) is the same for every row, and in most wavefronts every row takes the fused
+// form, so the common case is one FMA per element.
 template <int R>
 __device__ __forceinline__ void ger_rest(double (&a)[R][PW], const double (&x)[R], const bool (&fr)[R],
-                                         const bool (&up)[R], const double* pr, int j, int np, int nu) {
+                                         const bool (&up)[R], const double* pr, uint64_t zmask, int j, int np,
+                                         int nu) {
     bool allf = true;
 #pragma unroll
     for (int s = 0; s < R; s++) allf = allf && fr[s] && up[s];
     const bool wave_fast = __all(allf);
+    double tcv[PW];
+#pragma unroll
+    for (int c = 1; c < PW; c++) tcv[c] = -1.0 * pr[j + 1 + c];
 #pragma unroll
     for (int c = 1; c < PW; c++) {
         if (c < np) {
             const int colabs = j + 1 + c;
-            const double tc = -1.0 * pr[colabs];
-            const double tp = -1.0 * ((c & 1) ? pr[colabs - 1] : pr[colabs + 1]);
             const bool paired = c < nu;
-            const int skip = __builtin_amdgcn_readfirstlane(paired ? (tc == 0.0 && tp == 0.0) : (tc == 0.0));
+            const int partner = (c & 1) ? colabs - 1 : colabs + 1;
+            const bool zc = (zmask >> colabs) & 1ull;
+            const bool zp = (zmask >> partner) & 1ull;
+            const bool skip = paired ? (zc && zp) : zc;
             if (!skip) {
+                const double tc = tcv[c];
                 if (paired && wave_fast) {
 #pragma unroll
                     for (int s = 0; s < R; s++) a[s][c] = fma(x[s], tc, a[s][c]);
@@ -203,8 +209,9 @@ __device__ __forceinline__ uint64_t xldu(const uint64_t* p) {
 // are updated during step j+1 while the workgroups exchange candidates.  Candidate rows
 // are published before that deferred update together with their multiplier, and the
 // receiver applies it to the one winning row, so every element still sees the
-// reference's sequence of operations.
-template <int R>
+// reference's sequence of operations.  MULTI: more than one workgroup (exchange through
+// global memory); otherwise everything stays in LDS.
+template <int R, bool MULTI>
 __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restrict__ A, size_t lda,
                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
                                                   long row_off, int* __restrict__ flags, PanelSlot* __restrict__ slots,
@@ -212,9 +219,12 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                                                   long spin_limit) {
     __shared__ Cand red[PT / 64][2];
     __shared__ double prow[2][2 * PW + 2];  // pivot rows of consecutive steps (parity)
+    __shared__ uint64_t pzero[2];           // zero masks of those rows
     __shared__ int ctl[2];
     __shared__ double sret[PW * PT * R];  // sret[c * PT*R + s*PT + tid]
-    __shared__ PanelSlot lslot[2 * MAXWG_FAST + 1];  // candidates, then the row at position j
+    __shared__ PanelSlot lslot[MULTI ? 2 * MAXWG_FAST + 1 : 1];  // all workgroups' slots (exchange)
+    __shared__ PanelSlot stg[3];  // this workgroup's candidates and the row at position j
+    __shared__ int curmark;       // step whose row at position j this workgroup holds
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -232,12 +242,13 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
         for (int c = 0; c < PW; c++) a[s][c] = (valid[s] && c < w) ? A[r + (size_t)c * lda] : 0.0;
     }
     for (int e = tid; e < 2 * (2 * PW + 2); e += PT) (&prow[0][0])[e] = 0.0;
+    if (tid < 2) pzero[tid] = ~0ull;
+    if (tid == 0) curmark = -1;
     __syncthreads();
 
     const int min_mn = min(m, w);
     const bool fast = nwg <= MAXWG_FAST;
     const int CUR = 2 * MAXWG_FAST;
-    // deferred GER state of the previous step
     double xprev[R];
     bool fprev[R], uprev[R];
 #pragma unroll
@@ -266,61 +277,75 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             }
         }
         wg_reduce2(vb, tb, red);
-        // publish candidate rows (pre-update form) and the row at position j
-        const bool lds_only = (nwg == 1);
-        PanelSlot* mine = lds_only ? lslot : slots + ((size_t)par * nwg + g) * 2;
-        PanelSlot* curp = lds_only ? lslot + CUR : cur + par;
+        // stage this workgroup's candidate rows (pre-update form) and the row at position j
+        // in LDS: stg[0] vector-phase best, stg[1] tail best, stg[2] position j
+        {
+            int sv = -1, st = -1, sc = -1;
 #pragma unroll
-        for (int s = 0; s < R; s++) {
-            if (valid[s] && pos[s] >= j) {
-                uint64_t rel = (uint64_t)(pos[s] - j);
-                PanelSlot* d0 = nullptr;
-                if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) d0 = mine; }
-                else if (tb.v >= 0.0 && rel == tb.k) d0 = mine + 1;
-                PanelSlot* d1 = (rel == 0) ? curp : nullptr;
-                const uint64_t fl = (fprev[s] ? 1ull : 0ull) | (uprev[s] ? 2ull : 0ull);
-                for (int t = 0; t < 2; t++) {
-                    PanelSlot* d = t ? d1 : d0;
-                    if (!d) continue;
-                    if (lds_only) {
-                        d->x = xprev[s];
-                        d->flags = fl;
-                    } else {
-                        xst(&d->x, xprev[s]);
-                        xstu(&d->flags, fl);
-                    }
+            for (int s = 0; s < R; s++) {
+                if (valid[s] && pos[s] >= j) {
+                    const uint64_t rel = (uint64_t)(pos[s] - j);
+                    if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) sv = s; }
+                    else if (tb.v >= 0.0 && rel == tb.k) st = s;
+                    if (rel == 0) sc = s;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int sx = q == 0 ? sv : (q == 1 ? st : sc);
+                if (sx >= 0) {
+                    PanelSlot* d = stg + q;
+                    double xv = xprev[0];
+                    bool f0 = fprev[0], u0 = uprev[0];
+#pragma unroll
+                    for (int s = 1; s < R; s++)
+                        if (sx == s) { xv = xprev[s]; f0 = fprev[s]; u0 = uprev[s]; }
+                    d->x = xv;
+                    d->flags = (f0 ? 1ull : 0ull) | (u0 ? 2ull : 0ull);
 #pragma unroll
                     for (int c = 0; c < PW; c++) {
                         if (c < w - j) {
-                            if (lds_only) d->row[j + c] = a[s][c];
-                            else xst(&d->row[j + c], a[s][c]);
+                            double v = a[0][c];
+#pragma unroll
+                            for (int s = 1; s < R; s++)
+                                if (sx == s) v = a[s][c];
+                            d->row[j + c] = v;
                         }
                     }
+                    if (q == 2) curmark = j;
                 }
             }
         }
         if (tid == 0) {
-            if (lds_only) {
-                mine[0].v = vb.v; mine[0].key = vb.k;
-                mine[1].v = tb.v; mine[1].key = tb.k;
-            } else {
-                xst(&mine[0].v, vb.v); xstu(&mine[0].key, vb.k);
-                xst(&mine[1].v, tb.v); xstu(&mine[1].key, tb.k);
-            }
-        }
-        if (!lds_only) __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
-        __syncthreads();
-        if (!lds_only && tid == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // the deferred columns of the previous step, while the other workgroups catch up
-        if (dj >= 0) ger_rest<R>(a, xprev, fprev, uprev, prow[dj & 1], dj, dnp, dnu);
-        if (lds_only) {
-            if (tid == 0) {
+            stg[0].v = vb.v; stg[0].key = vb.k;
+            stg[1].v = tb.v; stg[1].key = tb.k;
+            if (!MULTI) {
                 bool ft;
                 const long rel = idamax_rule(len, vb, tb, ft);
                 ctl[0] = (int)rel;
-                ctl[1] = rel == 0 ? CUR : (ft ? 1 : 0);
+                ctl[1] = rel == 0 ? 2 : (ft ? 1 : 0);
             }
-        } else {
+        }
+        __syncthreads();
+        if (MULTI && wv == 0) {
+            // wavefront 0 copies the staged slots out and raises this workgroup's flag once
+            // its stores have completed
+            PanelSlot* mine = slots + ((size_t)par * nwg + g) * 2;
+            const bool hc = curmark == j;
+            constexpr int NW = sizeof(PanelSlot) / 8;
+            for (int e = lane; e < 3 * NW; e += 64) {
+                const int q = e / NW, f = e % NW;
+                if (q < 2 || hc) {
+                    uint64_t* dst = (uint64_t*)(q < 2 ? mine + q : cur + par);
+                    xstu(dst + f, ((const uint64_t*)(stg + q))[f]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            if (lane == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the deferred columns of the previous step, while the other workgroups catch up
+        if (dj >= 0) ger_rest<R>(a, xprev, fprev, uprev, prow[dj & 1], pzero[dj & 1], dj, dnp, dnu);
+        if (MULTI) {
             int ok = 1;
             for (int q = tid; q < nwg; q += PT) {
                 long sp = 0;
@@ -340,14 +365,30 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             }
             const PanelSlot* base = slots + (size_t)par * nwg * 2;
             if (fast) {
-                // one read round: every candidate (key, multiplier, row) and the row at position j
+                // one read round: every candidate (key, multiplier, row) and the row at position j;
+                // all loads of a thread are issued before the first is used
                 constexpr int NW = sizeof(PanelSlot) / 8;
+                constexpr int NLD = ((2 * MAXWG_FAST + 1) * NW + PT - 1) / PT;
                 const int nslot = 2 * nwg;
-                for (int e = tid; e < (nslot + 1) * NW; e += PT) {
-                    const int q = e / NW, f = e % NW;
-                    const uint64_t* srcp = (q < nslot) ? (const uint64_t*)(base + q) : (const uint64_t*)(cur + par);
-                    uint64_t* dstp = (uint64_t*)(lslot + (q < nslot ? q : CUR));
-                    dstp[f] = xldu(srcp + f);
+                const int total = (nslot + 1) * NW;
+                uint64_t tmp[NLD];
+#pragma unroll
+                for (int k = 0; k < NLD; k++) {
+                    const int e = tid + k * PT;
+                    if (e < total) {
+                        const int q = e / NW, f = e % NW;
+                        const uint64_t* srcp =
+                            (q < nslot) ? (const uint64_t*)(base + q) : (const uint64_t*)(cur + par);
+                        tmp[k] = xldu(srcp + f);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NLD; k++) {
+                    const int e = tid + k * PT;
+                    if (e < total) {
+                        const int q = e / NW, f = e % NW;
+                        ((uint64_t*)(lslot + (q < nslot ? q : CUR)))[f] = tmp[k];
+                    }
                 }
                 __syncthreads();
             }
@@ -387,19 +428,28 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
         }
         __syncthreads();
         // the winning row: apply the previous step's deferred GER to it, then it is the pivot row
-        {
+        if (wv == 0) {
             const int sidx = ctl[1];
-            const bool in_lds = lds_only || fast;
-            const PanelSlot* P = in_lds ? lslot + sidx
-                                        : ((sidx == CUR) ? cur + par : slots + (size_t)par * nwg * 2 + sidx);
-            if (tid < PW) {
-                const int c = tid;  // absolute column
-                double v = 0.0;
-                if (c >= j && c < w) {
-                    v = in_lds ? P->row[c] : xld(&P->row[c]);
+            const bool in_lds = !MULTI || fast;
+            const int c = lane;  // absolute column
+            double v = 0.0;
+            if (c >= j && c < w) {
+                if (in_lds) {
+                    const PanelSlot* P = MULTI ? lslot + sidx : stg + sidx;
+                    v = P->row[c];
                     if (dj >= 0 && c > j) {
-                        const double px = in_lds ? P->x : xld(&P->x);
-                        const uint64_t pf = in_lds ? P->flags : xldu(&P->flags);
+                        const int rc = c - (dj + 1);
+                        const double* pr = prow[dj & 1];
+                        const double tc = -1.0 * pr[c];
+                        const double tp = -1.0 * ((rc & 1) ? pr[c - 1] : pr[c + 1]);
+                        v = ger_elem(v, P->x, tc, tp, rc, dnp, dnu, (P->flags & 1) != 0, (P->flags & 2) != 0);
+                    }
+                } else {
+                    const PanelSlot* P = (sidx == CUR) ? cur + par : slots + (size_t)par * nwg * 2 + sidx;
+                    v = xld(&P->row[c]);
+                    if (dj >= 0 && c > j) {
+                        const double px = xld(&P->x);
+                        const uint64_t pf = xldu(&P->flags);
                         const int rc = c - (dj + 1);
                         const double* pr = prow[dj & 1];
                         const double tc = -1.0 * pr[c];
@@ -407,8 +457,10 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
                         v = ger_elem(v, px, tc, tp, rc, dnp, dnu, (pf & 1) != 0, (pf & 2) != 0);
                     }
                 }
-                prow[par][c] = v;
             }
+            if (c < PW) prow[par][c] = v;
+            const uint64_t zm = __ballot(v == 0.0);  // lanes >= w carry 0.0
+            if (lane == 0) pzero[par] = zm;
         }
         __syncthreads();
         const int jp = j + ctl[0];
@@ -643,12 +695,21 @@ void getf2(hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv, i
         const int nwg = panel_nwg(m);
         int* flags = w.flags + (size_t)panel_idx * w.nwg_max;
         RowMap* map = w.maps + panel_idx;
-        if (r == 1)
-            hipLaunchKernelGGL(getf2_panel<1>, dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off, row_off,
-                               flags, w.slots, w.cur, map, nwg, kSpinLimit);
-        else
-            hipLaunchKernelGGL(getf2_panel<2>, dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off, row_off,
-                               flags, w.slots, w.cur, map, nwg, kSpinLimit);
+        if (nwg == 1) {
+            if (r == 1)
+                hipLaunchKernelGGL((getf2_panel<1, false>), dim3(1), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, flags, w.slots, w.cur, map, nwg, kSpinLimit);
+            else
+                hipLaunchKernelGGL((getf2_panel<2, false>), dim3(1), dim3(PT), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, flags, w.slots, w.cur, map, nwg, kSpinLimit);
+        } else {
+            if (r == 1)
+                hipLaunchKernelGGL((getf2_panel<1, true>), dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info,
+                                   col_off, row_off, flags, w.slots, w.cur, map, nwg, kSpinLimit);
+            else
+                hipLaunchKernelGGL((getf2_panel<2, true>), dim3(nwg), dim3(PT), 0, st, m, n, a, lda, ipiv, info,
+                                   col_off, row_off, flags, w.slots, w.cur, map, nwg, kSpinLimit);
+        }
         BOLA_LAUNCH_CHECK();
         return;
     }
@@ -816,7 +877,11 @@ hipEvent_t lu_event(bola_hip_handle_s* h, int i) {
 // Status words: info[0] = 0 or global 1-based failing column (or BOLA_HIP_ERR_DEVICE);
 // info[1] = the column number the reference prints (local to its panel).
 //
-// Schedule: super-panels of W columns.  Stream 1 factors a super-panel and then brings
+// Schedule (m a multiple of 8): super-panels of W columns.  Deferring a super-panel's
+// interchanges on the columns to its right past the updates of its earlier panels is
+// exact only while no row takes the reference's unfused GEMM edge path, i.e. m % 8 == 0;
+// otherwise the reference's panel-by-panel order is kept.  Stream 1 factors a
+// super-panel and then brings
 // the next super-panel's columns up to date (look-ahead); stream 2 meanwhile applies the
 // super-panel to all columns further right (the bulk GEMM), so the next factorization
 // overlaps the bulk update.  Work on the two streams never touches the same columns at
@@ -839,6 +904,39 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     Phase ph(h, "GETRF", 2.0 * (double)min_mn * min_mn * (3.0 * std::max(m, n) - min_mn) / 6.0);
     const int npanels = (min_mn + PW - 1) / PW;
     LuWs w = lu_workspace(h, m, npanels);
+    if (m % 8 != 0) {
+        // The reference's GEMM computes the last (m-j-32) % 8 rows of each trailing update
+        // unfused; later interchanges move such rows, so interchanges cannot be deferred
+        // past an update.  Keep the reference's own order: panel, interchanges, TRSM, GEMM.
+        for (int p = 0; p < npanels; p++) {
+            const int j = p * PW;
+            const int jb = std::min(PW, min_mn - j);
+            {
+                Phase pp(h, "GETRF2_IT", 0.0, st);
+                getf2(st, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
+            }
+            {
+                Phase pl(h, "LASWP", 0.0, st);
+                const size_t right = (j + jb < n) ? (size_t)(n - j - jb) : 0;
+                launch_rowmap_apply(st, a, lda, (size_t)j, (size_t)(j + jb), right, w.maps + p, 1, info, ipiv + j, jb,
+                                    j);
+            }
+            if (j + jb < n) {
+                {
+                    Phase pt(h, "TRSM", (double)jb * jb * (n - j - jb), st);
+                    launch_dtrsm(st, 'L', 'L', 'N', 'U', (size_t)jb, (size_t)(n - j - jb), 1.0,
+                                 a + j + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, info);
+                }
+                if (j + jb < m) {
+                    Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * (n - j - jb) * jb, st);
+                    launch_dgemm(st, 'N', 'N', (size_t)(m - j - jb), (size_t)(n - j - jb), (size_t)jb, -1.0,
+                                 a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, 1.0,
+                                 a + (j + jb) + (size_t)(j + jb) * lda, lda, info);
+                }
+            }
+        }
+        return;
+    }
     const int W = min_mn >= 2048 ? 256 : 128;
     hipStream_t s2 = h->aux_stream();
     int ev = 0;
