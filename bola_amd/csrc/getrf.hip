@@ -50,6 +50,16 @@ struct PanelSlot {
     uint64_t flags;  // bit 0: fused row, bit 1: row updated
     double row[PW];
 };
+// The same for the wavefront panel, with the row window relative to the current column
+// (win[c] = column j + c), so a candidate row is staged without per-column tests.
+struct WaveSlot {
+    double v;
+    uint64_t key;
+    double x;
+    uint64_t flags;
+    double win[PW];
+};
+static_assert(sizeof(WaveSlot) == sizeof(PanelSlot), "slot layouts share the workspace");
 
 __device__ __forceinline__ bool better(double v, uint64_t k, double cv, uint64_t ck) {
     return v > cv || (v == cv && k < ck);
@@ -145,42 +155,44 @@ __device__ __forceinline__ double ger_elem(double a, double x, double tc, double
 
 // Deferred part of step j's GER on the row windows of one thread (slot 0 = column j+1):
 // columns j+2.. (window slots 1..PW-1, rc = slot).  pr: step j's pivot row (absolute
-// columns), zmask: bit i set when pr[i] == 0.  Whether a column is touched at all (pair
-// skip) is the same for every row, and in most wavefronts every row takes the fused
-// form, so the common case is one FMA per element.
+// columns), zmask: bit i set when pr[i] == 0.  Which columns are touched (the pair skip
+// of ger.rs) and which take the fused form are the same for every row, so they become
+// two scalar bit masks; a row inside the fused prefix then costs one FMA per touched
+// column, a row outside it the unfused multiply-add, a row that is not updated nothing.
 template <int R>
 __device__ __forceinline__ void ger_rest(double (&a)[R][PW], const double (&x)[R], const bool (&fr)[R],
                                          const bool (&up)[R], const double* pr, uint64_t zmask, int j, int np,
                                          int nu) {
-    bool allf = true;
-#pragma unroll
-    for (int s = 0; s < R; s++) allf = allf && fr[s] && up[s];
-    const bool wave_fast = __all(allf);
+    constexpr uint64_t EVEN = 0x5555555555555555ull;
+    const uint64_t z = zmask >> (j + 1);                       // bit c: pr[j+1+c] == 0
+    const uint64_t zpair = z & (((z & EVEN) << 1) | ((z >> 1) & EVEN));  // c and its partner c^1
+    const uint64_t paired = (1ull << nu) - 1;
+    const uint64_t skip = (zpair & paired) | (z & ~paired);
+    const uint64_t touch = ((1ull << np) - 1) & ~skip & ~1ull;
     double tcv[PW];
 #pragma unroll
     for (int c = 1; c < PW; c++) tcv[c] = -1.0 * pr[j + 1 + c];
 #pragma unroll
-    for (int c = 1; c < PW; c++) {
-        if (c < np) {
-            const int colabs = j + 1 + c;
-            const bool paired = c < nu;
-            const int partner = (c & 1) ? colabs - 1 : colabs + 1;
-            const bool zc = (zmask >> colabs) & 1ull;
-            const bool zp = (zmask >> partner) & 1ull;
-            const bool skip = paired ? (zc && zp) : zc;
-            if (!skip) {
-                const double tc = tcv[c];
-                if (paired && wave_fast) {
+    for (int s = 0; s < R; s++) {
+        if (up[s]) {
+            if (fr[s]) {
 #pragma unroll
-                    for (int s = 0; s < R; s++) a[s][c] = fma(x[s], tc, a[s][c]);
-                } else {
-#pragma unroll
-                    for (int s = 0; s < R; s++) {
-                        const double f = fma(x[s], tc, a[s][c]);
-                        const double u = a[s][c] + tc * x[s];
-                        a[s][c] = up[s] ? ((paired && fr[s]) ? f : u) : a[s][c];
+                for (int c = 1; c < PW; c++) {
+                    if ((touch >> c) & 1) {
+                        if ((paired >> c) & 1) {
+                            a[s][c] = fma(x[s], tcv[c], a[s][c]);
+                        } else {
+                            // at most one such column; the empty asm keeps this a branch
+                            // rather than a select between both forms in every column
+                            asm volatile("");
+                            a[s][c] = a[s][c] + tcv[c] * x[s];
+                        }
                     }
                 }
+            } else {
+#pragma unroll
+                for (int c = 1; c < PW; c++)
+                    if ((touch >> c) & 1) a[s][c] = a[s][c] + tcv[c] * x[s];
             }
         }
     }
@@ -540,6 +552,334 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
     }
 }
 
+// ---- the panel with one wavefront per workgroup and no workgroup barriers
+//
+// Every wavefront holds 64*R rows and takes part in the pivot search on its own: it
+// reduces its candidates with shuffles, stages the candidate rows in its LDS, copies them
+// to its global slots and raises its flag; after the deferred columns of the previous
+// step it waits for every flag, reads every wavefront's keys, picks the same pivot as
+// all the others and reads the winning row.  With one wavefront there is no exchange.
+//
+// Register layout of a row: a window of columns j .. w-2 that shifts left by one column
+// per step (slot 0 is the current column, every index is a compile-time constant) and
+// the panel's last column w-1 in a register of its own.  The last column is the only one
+// a GER can leave unpaired (ger.rs:52-59), so every window column takes the same form
+// and the deferred update is straight-line code.
+constexpr int WV = 64;
+constexpr int WW = PW - 1;  // window slots
+
+__device__ __forceinline__ void wave_reduce2(Cand& a, Cand& b) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(a.v, off, 64);
+        uint64_t ok = (uint64_t)__shfl_xor((long long)a.k, off, 64);
+        if (better(ov, ok, a.v, a.k)) { a.v = ov; a.k = ok; }
+        ov = __shfl_xor(b.v, off, 64);
+        ok = (uint64_t)__shfl_xor((long long)b.k, off, 64);
+        if (better(ov, ok, b.v, b.k)) { b.v = ov; b.k = ok; }
+    }
+}
+
+// Deferred part of step dj's GER (ger.rs:28-90): window slots 1.. (column dj+1+c, rc = c)
+// and the last column (rc = np-1, when np >= 2).  pr: step dj's pivot row (absolute
+// columns), zmask: bit i set when pr[i] == 0.  Slots past the window's valid part meet
+// coefficients of columns >= w-1 and turn into values that are never read.
+template <int R>
+__device__ __forceinline__ void wave_ger_rest(double (&a)[R][WW], double (&al)[R], const double (&x)[R],
+                                              const bool (&fr)[R], const bool (&up)[R], const double* pr,
+                                              uint64_t zmask, int dj, int np, int nu) {
+    constexpr uint64_t EVEN = 0x5555555555555555ull;
+    const uint64_t z = zmask >> (dj + 1);  // bit rc: pr[dj+1+rc] == 0
+    const uint64_t zpair = z & (((z & EVEN) << 1) | ((z >> 1) & EVEN));
+    const uint64_t paired = (1ull << nu) - 1;
+    const uint64_t skip = (zpair & paired) | (z & ~paired);
+    const uint64_t touch = ((1ull << np) - 1) & ~skip & ~1ull;
+    double tcv[WW];
+#pragma unroll
+    for (int c = 1; c < WW; c++) tcv[c] = -1.0 * pr[dj + 1 + c];
+    const uint64_t want = paired & ~1ull;
+    if ((touch & want) == want) {
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (up[s]) {
+                if (fr[s]) {
+#pragma unroll
+                    for (int c = 1; c < WW; c++) a[s][c] = fma(x[s], tcv[c], a[s][c]);
+                } else {
+#pragma unroll
+                    for (int c = 1; c < WW; c++) a[s][c] = a[s][c] + tcv[c] * x[s];
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (up[s]) {
+                if (fr[s]) {
+#pragma unroll
+                    for (int c = 1; c < WW; c++)
+                        if ((touch >> c) & 1) a[s][c] = fma(x[s], tcv[c], a[s][c]);
+                } else {
+#pragma unroll
+                    for (int c = 1; c < WW; c++)
+                        if ((touch >> c) & 1) a[s][c] = a[s][c] + tcv[c] * x[s];
+                }
+            }
+        }
+    }
+    if (np >= 2 && ((touch >> (np - 1)) & 1)) {
+        const double tl = -1.0 * pr[dj + np];  // column w-1
+        const bool lp = !(np & 1);             // the last column is paired
+#pragma unroll
+        for (int s = 0; s < R; s++)
+            if (up[s]) al[s] = (lp && fr[s]) ? fma(x[s], tl, al[s]) : al[s] + tl * x[s];
+    }
+}
+
+template <int R, bool MULTI>
+__global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restrict__ A, size_t lda,
+                                                 int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
+                                                 long row_off, int* __restrict__ flags, WaveSlot* __restrict__ slots,
+                                                 WaveSlot* __restrict__ cur, RowMap* __restrict__ map, int nwv,
+                                                 long spin_limit) {
+    __shared__ double sret[PW * WV * R];  // retired columns: sret[c * WV*R + s*WV + lane]
+    __shared__ WaveSlot stg[3];           // candidates of this wavefront, then the row at position j
+    __shared__ double prow[2][2 * PW + 2];
+    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const int wl = w - 1;  // the last column
+    double a[R][WW], al[R];
+    int pos[R];
+    long rowid[R];
+    bool valid[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        long r = (long)g * WV * R + (long)s * WV + lane;
+        rowid[s] = r;
+        valid[s] = r < m;
+        pos[s] = (int)r;
+#pragma unroll
+        for (int c = 0; c < WW; c++) a[s][c] = (valid[s] && c < wl) ? A[r + (size_t)c * lda] : 0.0;
+        al[s] = valid[s] ? A[r + (size_t)wl * lda] : 0.0;
+    }
+    for (int e = lane; e < 2 * (2 * PW + 2); e += WV) (&prow[0][0])[e] = 0.0;
+    uint64_t pz0 = ~0ull, pz1 = ~0ull;  // zero masks of prow[0], prow[1]
+    __syncthreads();
+
+    const int min_mn = min(m, w);
+    constexpr int NW = sizeof(WaveSlot) / 8;
+    double xprev[R];
+    bool fprev[R], uprev[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) { xprev[s] = 0.0; fprev[s] = false; uprev[s] = false; }
+    int dnp = 0, dnu = 0, dj = -1;
+    int nret = 0;
+    for (int j = 0; j < min_mn; j++) {
+        const int len = m - j;
+        const int T = len < 16 ? 0 : (len & ~15);
+        const int par = j & 1;
+        const bool lastcol = j == wl;  // the current column is the last one
+        Cand vb{-1.0, NOKEY}, tb{-1.0, NOKEY};
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s] && pos[s] >= j) {
+                double v = fabs(lastcol ? al[s] : a[s][0]);
+                if (v == v) {
+                    uint64_t rel = (uint64_t)(pos[s] - j);
+                    if ((long)rel < T) {
+                        uint64_t k = vec_key(rel);
+                        if (better(v, k, vb.v, vb.k)) { vb.v = v; vb.k = k; }
+                    } else if (better(v, rel, tb.v, tb.k)) {
+                        tb.v = v;
+                        tb.k = rel;
+                    }
+                }
+            }
+        }
+        wave_reduce2(vb, tb);
+        // stage the candidate rows (pre-update form) and the row at position j
+        int sv = -1, st = -1, sc = -1;
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s] && pos[s] >= j) {
+                const uint64_t rel = (uint64_t)(pos[s] - j);
+                if ((long)rel < T) { if (vb.v >= 0.0 && vec_key(rel) == vb.k) sv = s; }
+                else if (tb.v >= 0.0 && rel == tb.k) st = s;
+                if (rel == 0) sc = s;
+            }
+        }
+        const bool hascur = __ballot(sc >= 0) != 0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int sx = q == 0 ? sv : (q == 1 ? st : sc);
+            WaveSlot* d = stg + q;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (sx == s) {  // a branch per row, not a select, keeps the rows in registers
+                    d->x = xprev[s];
+                    d->flags = (fprev[s] ? 1ull : 0ull) | (uprev[s] ? 2ull : 0ull);
+#pragma unroll
+                    for (int c = 0; c < WW; c++) d->win[c] = a[s][c];  // slots past the window are never read
+                    d->win[wl - j] = al[s];
+                }
+            }
+        }
+        if (lane == 0) {
+            stg[0].v = vb.v; stg[0].key = vb.k;
+            stg[1].v = tb.v; stg[1].key = tb.k;
+        }
+        __syncthreads();  // one wavefront: orders the staging writes before the reads below
+        if (MULTI) {
+            WaveSlot* mine = slots + ((size_t)par * nwv + g) * 2;
+            for (int e = lane; e < 3 * NW; e += WV) {
+                const int q = e / NW, f = e % NW;
+                if (q < 2 || hascur) {
+                    uint64_t* dst = (uint64_t*)(q < 2 ? mine + q : cur + par);
+                    xstu(dst + f, ((const uint64_t*)(stg + q))[f]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // the slot stores have completed
+            if (lane == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the deferred columns of the previous step, while the other wavefronts catch up
+        if (dj >= 0) wave_ger_rest<R>(a, al, xprev, fprev, uprev, prow[dj & 1], (dj & 1) ? pz1 : pz0, dj, dnp, dnu);
+        // the pivot: every wavefront reaches the same decision
+        const int c = lane;  // absolute column held by this lane in the pivot row
+        double v = 0.0, px = 0.0;
+        uint64_t pf = 0;
+        long rel;
+        if (MULTI) {
+            int ok = 1;
+            for (int q = lane; q < nwv; q += WV) {
+                long sp = 0;
+                while (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
+                    if (++sp > spin_limit ||
+                        __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+            if (!__all(ok)) {
+                if (lane == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            const WaveSlot* base = slots + (size_t)par * nwv * 2;
+            Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+            int gvs = -1, gts = -1;
+            for (int q = lane; q < nwv; q += WV) {
+                const double v0 = xld(&base[2 * q].v), v1 = xld(&base[2 * q + 1].v);
+                const uint64_t k0 = xldu(&base[2 * q].key), k1 = xldu(&base[2 * q + 1].key);
+                if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
+                if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                double ov = __shfl_xor(gv.v, off, 64);
+                uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
+                int os = __shfl_xor(gvs, off, 64);
+                if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
+                ov = __shfl_xor(gt.v, off, 64);
+                okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
+                os = __shfl_xor(gts, off, 64);
+                if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+            }
+            bool ft;
+            rel = idamax_rule(len, gv, gt, ft);
+            const WaveSlot* P = rel == 0 ? cur + par : base + (ft ? gts : gvs);
+            if (c >= j && c < w) v = xld(&P->win[c - j]);
+            px = xld(&P->x);
+            pf = xldu(&P->flags);
+        } else {
+            bool ft;
+            rel = idamax_rule(len, vb, tb, ft);
+            const WaveSlot* P = stg + (rel == 0 ? 2 : (ft ? 1 : 0));
+            if (c >= j && c < w) v = P->win[c - j];
+            px = P->x;
+            pf = P->flags;
+        }
+        // the winning row receives the previous step's deferred update and becomes the pivot row
+        if (dj >= 0 && c > j && c < w) {
+            const int rc = c - (dj + 1);
+            const double* pr = prow[dj & 1];
+            const double tc = -1.0 * pr[c];
+            const double tp = -1.0 * ((rc & 1) ? pr[c - 1] : pr[c + 1]);
+            v = ger_elem(v, px, tc, tp, rc, dnp, dnu, (pf & 1) != 0, (pf & 2) != 0);
+        }
+        if (c < PW) prow[par][c] = v;
+        const uint64_t zm = __ballot(v == 0.0);  // lanes outside [j, w) carry 0.0
+        if (par) pz1 = zm; else pz0 = zm;
+        __syncthreads();
+        const int jp = j + (int)rel;
+        const double piv = prow[par][j];
+        if (g == 0 && lane == 0) ipiv[j] = jp + 1;
+        if (!(piv != 0.0)) {
+            if (g == 0 && lane == 0) {
+                info[1] = j + 1;
+                __hip_atomic_store(info, col_off + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        if (jp != j) {
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (pos[s] == j) pos[s] = jp;
+                else if (pos[s] == jp) pos[s] = j;
+            }
+        }
+        // scale (getrf2.rs:156-164) and step j's GER: column j+1 now, the rest deferred
+        const int np = w - j - 1;
+        const int nu = np - (np & 1);
+        const int mf = (m - j - 1) & ~3;
+        const bool do_ger = j < min_mn - 1;
+        const double inv = 1.0 / piv;
+        const double* pr = prow[par];
+        const double tc = -1.0 * pr[j + 1];
+        const double tp = -1.0 * pr[j + 2];
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            double xj = lastcol ? al[s] : a[s][0];
+            if (j < m - 1 && valid[s] && pos[s] > j) xj = xj * inv;
+            const bool upd = do_ger && valid[s] && pos[s] > j;
+            const bool frow = (pos[s] - (j + 1)) < mf;
+            if (j + 1 == wl) al[s] = ger_elem(al[s], xj, tc, tp, 0, np, nu, frow, upd);
+            else a[s][1] = ger_elem(a[s][1], xj, tc, tp, 0, np, nu, frow, upd);
+            xprev[s] = xj;
+            fprev[s] = frow;
+            uprev[s] = upd;
+            // retire column j and shift the window
+            sret[(size_t)j * (WV * R) + s * WV + lane] = xj;
+#pragma unroll
+            for (int c2 = 0; c2 < WW - 1; c2++) a[s][c2] = a[s][c2 + 1];
+            a[s][WW - 1] = 0.0;
+        }
+        dj = do_ger ? j : -1;
+        dnp = np;
+        dnu = nu;
+        nret = j + 1;
+    }
+    // rows to their final positions; moved rows form the panel's row map
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        if (valid[s]) {
+            for (int c = 0; c < nret; c++)
+                A[(size_t)pos[s] + (size_t)c * lda] = sret[(size_t)c * (WV * R) + s * WV + lane];
+#pragma unroll
+            for (int c = 0; c < WW; c++)
+                if (nret + c < wl) A[(size_t)pos[s] + (size_t)(nret + c) * lda] = a[s][c];
+            if (nret <= wl) A[(size_t)pos[s] + (size_t)wl * lda] = al[s];
+            if (pos[s] != rowid[s]) {
+                int idx = atomicAdd(&map->count, 1);
+                if (idx < ROWMAP_MAX) {
+                    map->row[idx] = row_off + pos[s];
+                    map->src[idx] = row_off + rowid[s];
+                }
+            }
+        }
+    }
+}
+
 __global__ void k_zero_i32(int32_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -655,10 +995,18 @@ int panel_nwg(int m) {
     int r = panel_rows_per_thread(m);
     return (m + PT * r - 1) / (PT * r);
 }
+// the wavefront panel: rows per lane and wavefronts; taller panels use the workgroup kernel
+constexpr int kWaveMax = 512;
+int wave_rows_per_lane(int m) { return m <= 4096 ? 1 : 2; }
+int wave_count(int m) {
+    int r = wave_rows_per_lane(m);
+    return (m + WV * r - 1) / (WV * r);
+}
+int panel_slots(int m) { return std::max(panel_nwg(m), wave_count(m)); }
 
 LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
     LuWs w;
-    w.nwg_max = std::max(1, panel_nwg(m));
+    w.nwg_max = std::max(1, panel_slots(m));
     w.npanels = std::max(1, npanels);
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -690,6 +1038,31 @@ const long kSpinLimit = 1L << 22;
 // iterative_getrf2 (getrf2.rs:131-185) on an m x n block, n <= 32 or m <= 32.
 void getf2(hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv, int32_t* info, int col_off,
            long row_off, const LuWs& w, int panel_idx) {
+    if (n <= PW && wave_count(m) <= kWaveMax) {
+        const int r = wave_rows_per_lane(m);
+        const int nwv = wave_count(m);
+        WaveSlot* wslots = reinterpret_cast<WaveSlot*>(w.slots);
+        WaveSlot* wcur = reinterpret_cast<WaveSlot*>(w.cur);
+        int* flags = w.flags + (size_t)panel_idx * w.nwg_max;
+        RowMap* map = w.maps + panel_idx;
+        if (nwv == 1) {
+            if (r == 1)
+                hipLaunchKernelGGL((getf2_wave<1, false>), dim3(1), dim3(WV), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+            else
+                hipLaunchKernelGGL((getf2_wave<2, false>), dim3(1), dim3(WV), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+        } else {
+            if (r == 1)
+                hipLaunchKernelGGL((getf2_wave<1, true>), dim3(nwv), dim3(WV), 0, st, m, n, a, lda, ipiv, info,
+                                   col_off, row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+            else
+                hipLaunchKernelGGL((getf2_wave<2, true>), dim3(nwv), dim3(WV), 0, st, m, n, a, lda, ipiv, info,
+                                   col_off, row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+        }
+        BOLA_LAUNCH_CHECK();
+        return;
+    }
     if (n <= PW) {
         const int r = panel_rows_per_thread(m);
         const int nwg = panel_nwg(m);
