@@ -22,46 +22,56 @@ constexpr int MAXT = 64;    // base-case triangle size (trsm.rs:8)
 enum { LNL = 0, LNU, LTL, LTU, RNL, RNU, RTL, RTU };
 
 // ---- left, no-transpose (daxpy form): one wavefront per right-hand-side column (two
-// columns per wavefront when t <= 32), one lane per row.  Step k broadcasts the solved
-// entry from lane k; every lane below (LNL) / above (LNU) applies its own fused or
-// unfused update, the same operation the reference's daxpy applies to that element.
-template <bool LOWER>
+// columns per wavefront when t <= 32), one lane per row.  Each lane loads its row of the
+// triangle once; step k reads the solved entry of lane k with a lane read (no LDS), and
+// every lane below (LNL) / above (LNU) applies its own fused or unfused update, the same
+// operation the reference's daxpy applies to that element.
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <bool LOWER, int TW>
 __global__ void __launch_bounds__(256) trsm_ln_lanes(int unit, int t, int nvec, const double* __restrict__ A,
                                                      size_t lda, double* __restrict__ B, size_t ldb,
                                                      const int32_t* __restrict__ abort) {
     if (abort && *abort != 0) return;
+    constexpr bool TWO = TW == 32;
     const int lane = threadIdx.x & 63;
-    const bool two = t <= 32;
-    const int sub = two ? (lane >> 5) : 0;
-    const int i = two ? (lane & 31) : lane;
-    const int base = sub * 32;
+    const int sub = TWO ? (lane >> 5) : 0;
+    const int i = TWO ? (lane & 31) : lane;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long v = two ? wave * 2 + sub : wave;
-    const bool col_ok = v < nvec;
-    const bool act = col_ok && i < t;
+    const long v = TWO ? wave * 2 + sub : wave;
+    const bool act = v < nvec && i < t;
+    double arow[TW];  // row i of the triangle
+#pragma unroll
+    for (int k = 0; k < TW; k++) {
+        const bool need = LOWER ? k <= i : k >= i;
+        arow[k] = (i < t && k < t && need) ? A[(size_t)i + (size_t)k * lda] : 0.0;
+    }
     double x = act ? B[i + (size_t)v * ldb] : 0.0;
 #pragma unroll
-    for (int kk = 0; kk < MAXT; kk++) {
-        const int k = LOWER ? kk : (MAXT - 1 - kk);
+    for (int kk = 0; kk < TW; kk++) {
+        const int k = LOWER ? kk : (TW - 1 - kk);
         if (k < t) {
-            const double bk = __shfl(x, base + k, 64);
+            double bk = rdlane(x, k);
+            if (TWO) {
+                const double b1 = rdlane(x, 32 + k);
+                bk = sub ? b1 : bk;
+            }
             if (bk != 0.0) {
-                const double u = unit ? bk : bk / A[(size_t)k + (size_t)k * lda];
+                const double u = unit ? bk : bk / rdlane(arow[k], k);
                 if (i == k) x = u;
                 const double al = -u;
                 if (al != 0.0) {
                     if (LOWER) {
                         const int nf = (t - k - 1) & ~3;
-                        if (act && i > k) {
-                            const double aik = A[(size_t)i + (size_t)k * lda];
-                            x = (i - k - 1 < nf) ? fma(al, aik, x) : x + al * aik;
-                        }
+                        if (act && i > k) x = (i - k - 1 < nf) ? fma(al, arow[k], x) : x + al * arow[k];
                     } else {
                         const int nf = k & ~3;
-                        if (act && i < k) {
-                            const double aik = A[(size_t)i + (size_t)k * lda];
-                            x = (i < nf) ? fma(al, aik, x) : x + al * aik;
-                        }
+                        if (act && i < k) x = (i < nf) ? fma(al, arow[k], x) : x + al * arow[k];
                     }
                 }
             }
@@ -165,8 +175,12 @@ void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size
     if (V == LNL || V == LNU) {
         const size_t waves = t <= 32 ? (nvec + 1) / 2 : nvec;
         const unsigned blocks = (unsigned)((waves + 3) / 4);
-        hipLaunchKernelGGL((trsm_ln_lanes<V == LNL>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec, a, lda,
-                           b, ldb, abort);
+        if (t <= 32)
+            hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 32>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec,
+                               a, lda, b, ldb, abort);
+        else
+            hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 64>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec,
+                               a, lda, b, ldb, abort);
     } else {
         const unsigned blocks = (unsigned)((nvec + 63) / 64);
         hipLaunchKernelGGL((trsm_dot_regs<V>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda, b,
