@@ -28,6 +28,7 @@ __all__ = [
     "BolaError", "Handle", "build", "lib", "library_path",
     "gemm", "getrf", "getrf2", "potrf", "potrf2", "trsm", "syrk", "laswp", "idamax",
     "ger", "scal", "daxpy", "dot_product", "getrs", "potrs",
+    "MixedStats", "PRECISIONS", "dgesv_mixed", "sgemm_lp",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -108,6 +109,9 @@ _SIGS = [
     ("bola_hip_dgetrs_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp, _sz]),
     ("bola_hip_dpotrs", C.c_int, [_ch, _sz, _sz, _vp, _sz, _vp, _sz]),
     ("bola_hip_dpotrs_dev", C.c_int, [_vp, _ch, _sz, _sz, _vp, _sz, _vp, _sz]),
+    ("bola_hip_dgesv_mixed", C.c_int, [C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
+    ("bola_hip_dgesv_mixed_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
+    ("bola_hip_sgemm_lp_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz]),
 ]
 
 
@@ -142,7 +146,7 @@ def _ptr(x, dtype):
         return None
     if _is_dev(x):
         import torch
-        want = torch.float64 if dtype == np.float64 else torch.int32
+        want = {np.float64: torch.float64, np.float32: torch.float32}.get(dtype, torch.int32)
         if x.dtype != want or not x.is_contiguous():
             raise TypeError(f"expected a contiguous {want} CUDA tensor")
         return C.c_void_p(x.data_ptr())
@@ -380,3 +384,41 @@ def potrs(uplo, n, nrhs, f, lda, b, ldb) -> None:
                                          _ptr(b, np.float64), ldb), True)
     else:
         _check(lib().bola_hip_dpotrs(_c(uplo), n, nrhs, _ptr(f, np.float64), lda, _ptr(b, np.float64), ldb), False)
+
+
+# ------------------------------------------------------------------ mixed precision
+# No crate counterpart (the crate solves in FP64 only); see include/bola_hip.h.
+
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+class MixedStats(C.Structure):
+    _fields_ = [("ir_iterations", C.c_int), ("gmres_cycles", C.c_int), ("gmres_iterations", C.c_int),
+                ("fallback", C.c_int), ("converged", C.c_int), ("pad", C.c_int),
+                ("first_berr", C.c_double), ("final_berr", C.c_double), ("tolerance", C.c_double),
+                ("factor_ms", C.c_double), ("refine_ms", C.c_double), ("fallback_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
+def dgesv_mixed(n, nrhs, a, lda, b, ldb, x, ldx, precision="fp32") -> dict:
+    """A X = B to FP64 accuracy from a low-precision LU (fp32 / fp16 / bf16 products),
+    FP64 refinement, GMRES-IR and the FP64 fallback.  Returns the solver statistics."""
+    st = MixedStats()
+    p = PRECISIONS[precision]
+    if _is_dev(a):
+        rc = lib().bola_hip_dgesv_mixed_dev(default_handle().ptr, p, n, nrhs, _ptr(a, np.float64), lda,
+                                            _ptr(b, np.float64), ldb, _ptr(x, np.float64), ldx, C.byref(st))
+        _check(rc, True)
+    else:
+        rc = lib().bola_hip_dgesv_mixed(p, n, nrhs, _ptr(a, np.float64), lda, _ptr(b, np.float64), ldb,
+                                        _ptr(x, np.float64), ldx, C.byref(st))
+        _check(rc, False)
+    return st.as_dict()
+
+
+def sgemm_lp(m, n, k, a, lda, b, ldb, c, ldc, precision="fp32") -> None:
+    """C -= A B on fp32 device tensors, products in `precision` (the mixed LU's update)."""
+    _check(lib().bola_hip_sgemm_lp_dev(default_handle().ptr, PRECISIONS[precision], m, n, k, _ptr(a, np.float32),
+                                       lda, _ptr(b, np.float32), ldb, _ptr(c, np.float32), ldc), True)

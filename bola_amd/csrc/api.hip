@@ -90,12 +90,6 @@ bola_hip_handle_s* default_handle() {
     return t_default;
 }
 
-}  // namespace bola
-
-using namespace bola;
-
-namespace {
-
 int device_failure(bola_hip_handle_s* h, const HipFailure& f) {
     char buf[256];
     snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at line %d", (int)f.err, hipGetErrorString(f.err), f.what,
@@ -103,6 +97,12 @@ int device_failure(bola_hip_handle_s* h, const HipFailure& f) {
     set_error(h, buf);
     return BOLA_HIP_ERR_DEVICE;
 }
+
+}  // namespace bola
+
+using namespace bola;
+
+namespace {
 
 #define API_BEGIN(h) \
     try {
@@ -226,6 +226,8 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     h->ws.release();
     h->ws_aux.release();
     h->stage.release();
+    h->mws.release();
+    h->mws2.release();
     if (h->d_info) (void)hipFree(h->d_info);
     for (auto e : h->sync_events) (void)hipEventDestroy(e);
     if (h->stream2) {
@@ -797,6 +799,31 @@ int bola_hip_dpotrs(char uplo, size_t n, size_t nrhs, const double* f, size_t ld
     d2h_mat(h->stream, b, ldb, s.at<double>(ob), n, n, nrhs);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
+    API_END(h)
+}
+
+int bola_hip_dgesv_mixed(int precision, size_t n, size_t nrhs, const double* a, size_t lda, const double* b,
+                         size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats) {
+    bola_hip_handle_s* h = nullptr;
+    API_BEGIN(h)
+    h = default_handle();
+    if (lda < std::max<size_t>(1, n)) {
+        set_error(h, "Argument 5 to dgesv_mixed had an illegal value");
+        return -5;
+    }
+    if (n == 0 || nrhs == 0) return bola_hip_dgesv_mixed_dev(h, precision, n, nrhs, nullptr, 1, nullptr, 1, nullptr, 1,
+                                                             stats);
+    Stager s{h};
+    size_t oa = s.add(n * n * 8), ob = s.add(n * nrhs * 8), ox = s.add(n * nrhs * 8);
+    s.commit();
+    h2d_mat(h->stream, s.at<double>(oa), n, a, lda, n, n);
+    h2d_mat(h->stream, s.at<double>(ob), n, b, ldb, n, nrhs);
+    int r = bola_hip_dgesv_mixed_dev(h, precision, n, nrhs, s.at<double>(oa), n, s.at<double>(ob), n, s.at<double>(ox),
+                                     n, stats);
+    if (r < 0) return r;
+    d2h_mat(h->stream, x, ldx, s.at<double>(ox), n, n, nrhs);
+    BOLA_CHECK(hipStreamSynchronize(h->stream));
+    return r;
     API_END(h)
 }
 

@@ -88,6 +88,8 @@ struct bola_hip_handle_s {
     bola::Workspace ws;         // factorization scratch (panel flags/slots, row maps)
     bola::Workspace ws_aux;     // LASWP group maps, IDAMAX / DOT partials
     bola::Workspace stage;      // operands of host-pointer calls
+    bola::Workspace mws;        // mixed-precision solver: fp32 factors, vectors, GMRES basis
+    bola::Workspace mws2;       // its FP64 fallback copy of A
     int32_t* d_info = nullptr;  // device status word for factorizations
     std::string last_error;
     // profiling: per-phase GPU time measured with events on the handle's stream
@@ -120,6 +122,20 @@ struct Phase {
 };
 
 void set_error(bola_hip_handle_s* h, const std::string& msg);
+// Records a HIP failure on h (or the thread's host-flavour error) and returns BOLA_HIP_ERR_DEVICE.
+int device_failure(bola_hip_handle_s* h, const HipFailure& f);
+
+// Every C-ABI entry point body sits between these: HIP failures become return codes.
+#define BOLA_API_BEGIN try {
+#define BOLA_API_END(h)                                              \
+    }                                                                \
+    catch (const ::bola::HipFailure& f_) {                           \
+        return ::bola::device_failure(h, f_);                        \
+    }                                                                \
+    catch (...) {                                                    \
+        ::bola::set_error(h, "unexpected host exception");           \
+        return BOLA_HIP_ERR_DEVICE;                                  \
+    }
 bola_hip_handle_s* default_handle();
 void collect_profile(bola_hip_handle_s* h);
 

@@ -1,0 +1,750 @@
+// Mixed-precision LU solve: A X = B to FP64 accuracy from a low-precision factorization.
+// No reference counterpart (the reference solves in FP64 only); the FP64 fallback is the
+// reference-exact GETRF + GETRS of this library.
+//
+//   1. equilibration: row scales s (powers of two bringing every row's max |a| into
+//      [1/2, 1)), then column scales t on S A likewise; A_s = S A C is exact.
+//   2. T = A_s^T in fp32 and the LU of A_s (mixed_lu.hip), trailing products in fp32,
+//      fp16 or bf16 on the MFMA units.
+//   3. refinement in FP64: r = B - A X on the original FP64 A, d = M^-1 r with
+//      M^-1 = C (L U)^-1 P S applied in FP64 arithmetic to the fp32 factors, X += d, until
+//      the backward error |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= n * eps64 in every
+//      column.  Stagnation switches to GMRES-IR: restarted GMRES in FP64 on the
+//      left-preconditioned system M^-1 A d = M^-1 r, one Krylov space per right-hand
+//      side, advanced in lockstep so the matrix is read once per step for all of them.
+//   4. If GMRES-IR stalls too, the FP64 reference-exact GETRF + GETRS solves the system.
+//
+// Triangular solves: one persistent kernel per triangle, blocks of 64 rows, one
+// workgroup per block; a block waits (bounded) for the flags of the blocks it depends on
+// and reads their solved values with device-coherent loads.  Workgroups are dispatched in
+// dependency order, so a waiting workgroup never blocks one it waits for.
+#include "mixed.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace bola {
+
+// lu.hip / getrf.hip
+void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
+               bool recursive_entry);
+
+namespace {
+
+constexpr int TB = 64;        // triangular-solve block
+constexpr int NRMAX = 16;     // right-hand sides per solve call
+constexpr int RCH = 32;       // column chunks of the residual kernel
+constexpr long kTrsvSpin = 1L << 24;
+
+__device__ __forceinline__ void ast(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ald(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pow2_scale(double m) {
+    if (!(m > 0.0) || !(m < INFINITY)) return 1.0;
+    int e;
+    frexp(m, &e);  // m = f 2^e, f in [1/2, 1)
+    return ldexp(1.0, -e);
+}
+__device__ __forceinline__ double rdl(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// ---------------------------------------------------------------- equilibration
+__global__ void eq_row_part(int n, const double* __restrict__ A, size_t lda, double* __restrict__ pmax,
+                            double* __restrict__ psum, int cw) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c0 = blockIdx.y * cw, c1 = min(n, c0 + cw);
+    if (r >= n) return;
+    double mx = 0.0, sm = 0.0;
+    for (int c = c0; c < c1; c++) {
+        const double v = fabs(A[(size_t)r + (size_t)c * lda]);
+        mx = fmax(mx, v);
+        sm += v;
+    }
+    pmax[(size_t)blockIdx.y * n + r] = mx;
+    psum[(size_t)blockIdx.y * n + r] = sm;
+}
+__global__ void eq_row_fin(int n, int nch, const double* __restrict__ pmax, const double* __restrict__ psum,
+                           double* __restrict__ s, double* __restrict__ rsum) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double mx = 0.0, sm = 0.0;
+    for (int ch = 0; ch < nch; ch++) {
+        mx = fmax(mx, pmax[(size_t)ch * n + r]);
+        sm += psum[(size_t)ch * n + r];
+    }
+    s[r] = pow2_scale(mx);
+    rsum[r] = sm;
+}
+__global__ void __launch_bounds__(256) eq_col(int n, const double* __restrict__ A, size_t lda,
+                                              const double* __restrict__ s, double* __restrict__ t) {
+    __shared__ double red[4];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    double mx = 0.0;
+    for (int r = tid; r < n; r += 256) mx = fmax(mx, fabs(s[r] * A[(size_t)r + (size_t)c * lda]));
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) t[c] = pow2_scale(fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+}
+// T[c + r*ldt] = s_r A(r,c) t_c in fp32 (64 x 64 tiles through LDS)
+__global__ void __launch_bounds__(256) to_T(int n, const double* __restrict__ A, size_t lda,
+                                            const double* __restrict__ s, const double* __restrict__ t,
+                                            float* __restrict__ T, size_t ldt) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int cc = ty; cc < 64; cc += 4) {
+        const int r = r0 + tx, c = c0 + cc;
+        tile[cc][tx] = (r < n && c < n) ? (float)(s[r] * A[(size_t)r + (size_t)c * lda] * t[c]) : 0.f;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int r = r0 + rr, c = c0 + tx;
+        if (r < n && c < n) T[(size_t)c + (size_t)r * ldt] = tile[tx][rr];
+    }
+}
+__global__ void k_iota(int n, int* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// ---------------------------------------------------------------- M^-1 pieces
+// Z(i, q) = s[perm[i]] R(perm[i], q)
+__global__ void gather_scale(int n, int nrhs, const double* __restrict__ R, size_t ldr, const int* __restrict__ perm,
+                             const double* __restrict__ s, double* __restrict__ Z, size_t ldz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = perm[i];
+    const double sc = s[p];
+    for (int q = 0; q < nrhs; q++) Z[(size_t)i + (size_t)q * ldz] = sc * R[(size_t)p + (size_t)q * ldr];
+}
+// D(c, q) = t[c] W(c, q)
+__global__ void col_scale(int n, int nrhs, const double* __restrict__ W, size_t ldw, const double* __restrict__ t,
+                          double* __restrict__ D, size_t ldd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double sc = t[i];
+    for (int q = 0; q < nrhs; q++) D[(size_t)i + (size_t)q * ldd] = sc * W[(size_t)i + (size_t)q * ldw];
+}
+
+// Z := L^-1 Z (LOWER, L unit lower) or U^-1 Z (upper), factors in T (fp32), FP64
+// arithmetic; Z is n x nrhs (ld ldz).  Block bi = rows [64 bi, 64 bi + 64).
+template <bool LOWER>
+__global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __restrict__ T, size_t ldt,
+                                                double* __restrict__ Z, size_t ldz, int* __restrict__ flags,
+                                                int epoch, int nb, long spin_limit, int* __restrict__ err) {
+    __shared__ double ys[TB][NRMAX + 1];
+    __shared__ float Ld[TB][TB + 1];
+    __shared__ int ok_sh;
+    const int bi = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
+    const int r0 = bi * TB, rows = min(TB, n - r0);
+    const int tid = threadIdx.x, r = tid & 63, qg = tid >> 6;
+    double acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int q = qg + 4 * u;
+        acc[u] = (r < rows && q < nrhs) ? Z[(size_t)(r0 + r) + (size_t)q * ldz] : 0.0;
+    }
+    // diagonal block, A orientation: Ld[r][c] = factor(r0 + r, r0 + c) = T[(r0+c) + (r0+r) ldt]
+    for (int e = tid; e < TB * TB; e += 256) {
+        const int rr = e >> 6, cc = e & 63;
+        Ld[rr][cc] = (rr < rows && cc < rows) ? T[(size_t)(r0 + cc) + (size_t)(r0 + rr) * ldt] : 0.f;
+    }
+    const int ndep = LOWER ? bi : nb - 1 - bi;
+    for (int jj = 0; jj < ndep; jj++) {
+        const int j = LOWER ? jj : bi + 1 + jj;
+        if (tid == 0) {
+            long sp = 0;
+            int ok = 1;
+            while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (++sp > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+            }
+            ok_sh = ok;
+        }
+        __syncthreads();
+        if (!ok_sh) {
+            if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const int c0 = j * TB, cols = min(TB, n - c0);
+        for (int e = tid; e < TB * NRMAX; e += 256) {
+            const int c = e % TB, q = e / TB;
+            ys[c][q] = (c < cols && q < nrhs) ? ald(Z + (size_t)(c0 + c) + (size_t)q * ldz) : 0.0;
+        }
+        __syncthreads();
+        if (r < rows) {
+            const float* lrow = T + (size_t)c0 + (size_t)(r0 + r) * ldt;  // factor(r0 + r, c0 + c), c contiguous
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int c = 0; c < cols; c++) {
+                const double l = (double)lrow[c];
+                s0 = fma(l, ys[c][qg], s0);
+                s1 = fma(l, ys[c][qg + 4], s1);
+                s2 = fma(l, ys[c][qg + 8], s2);
+                s3 = fma(l, ys[c][qg + 12], s3);
+            }
+            acc[0] -= s0;
+            acc[1] -= s1;
+            acc[2] -= s2;
+            acc[3] -= s3;
+        }
+        __syncthreads();
+    }
+    __syncthreads();  // Ld complete
+    // the diagonal block: each wavefront solves its own right-hand sides, lane = row
+#pragma unroll
+    for (int kk = 0; kk < TB; kk++) {
+        const int k = LOWER ? kk : TB - 1 - kk;
+        if (k < rows) {
+            if (!LOWER && r == k) {
+                const double d = (double)Ld[k][k];
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[u] = acc[u] / d;
+            }
+            double yk[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) yk[u] = rdl(acc[u], k);
+            const bool below = LOWER ? (r > k) : (r < k);
+            if (below && r < rows) {
+                const double l = (double)Ld[r][k];
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[u] = fma(-l, yk[u], acc[u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int q = qg + 4 * u;
+        if (r < rows && q < nrhs) ast(Z + (size_t)(r0 + r) + (size_t)q * ldz, acc[u]);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- residual / products
+// P[(ch * NRMAX + q) * n + r] = sum over the chunk's columns c of A(r, c) X(c, q)
+__global__ void __launch_bounds__(256) prod_part(int n, int nrhs, const double* __restrict__ A, size_t lda,
+                                                 const double* __restrict__ X, size_t ldx, double* __restrict__ P,
+                                                 int cw) {
+    __shared__ double xs[64][NRMAX];
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * cw, c1 = min(n, c0 + cw);
+    double acc[NRMAX];
+#pragma unroll
+    for (int q = 0; q < NRMAX; q++) acc[q] = 0.0;
+    for (int cb = c0; cb < c1; cb += 64) {
+        const int cn = min(64, c1 - cb);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * NRMAX; e += 256) {
+            const int c = e & 63, q = e >> 6;
+            xs[c][q] = (c < cn && q < nrhs) ? X[(size_t)(cb + c) + (size_t)q * ldx] : 0.0;
+        }
+        __syncthreads();
+        if (r < n) {
+            if (nrhs == 1) {
+                for (int c = 0; c < cn; c++) acc[0] = fma(A[(size_t)r + (size_t)(cb + c) * lda], xs[c][0], acc[0]);
+            } else {
+                for (int c = 0; c < cn; c++) {
+                    const double a = A[(size_t)r + (size_t)(cb + c) * lda];
+#pragma unroll
+                    for (int q = 0; q < NRMAX; q++) acc[q] = fma(a, xs[c][q], acc[q]);
+                }
+            }
+        }
+    }
+    if (r < n)
+        for (int q = 0; q < nrhs; q++) P[((size_t)blockIdx.y * NRMAX + q) * n + r] = acc[q];
+}
+// R(r, q) = (B ? B(r, q) : 0) - sum_ch P  (fixed order)
+__global__ void prod_fin(int n, int nrhs, int nch, const double* __restrict__ P, const double* __restrict__ B,
+                         size_t ldb, double* __restrict__ R, size_t ldr) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int q = 0; q < nrhs; q++) {
+        double s = 0.0;
+        for (int ch = 0; ch < nch; ch++) s += P[((size_t)ch * NRMAX + q) * n + r];
+        R[(size_t)r + (size_t)q * ldr] = (B ? B[(size_t)r + (size_t)q * ldb] : 0.0) - s;
+    }
+}
+
+// ---------------------------------------------------------------- vector kernels
+// out[q] = max_i |V(i, q)|
+__global__ void __launch_bounds__(256) col_amax(int n, const double* __restrict__ V, size_t ldv, double* __restrict__ out) {
+    __shared__ double red[4];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    double m = 0.0;
+    for (int i = tid; i < n; i += 256) {
+        const double v = fabs(V[(size_t)i + (size_t)q * ldv]);
+        m = (v > m || v != v) ? v : m;  // NaN propagates
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+        double g = red[0];
+        for (int w = 1; w < 4; w++) g = (red[w] > g || red[w] != red[w]) ? red[w] : g;
+        out[q] = g;
+    }
+}
+// out[j * nrhs + q] = sum_i V_j(i, q) W(i, q), j < nv, V_j at V + j*jstride (column q at + q*qstride)
+__global__ void __launch_bounds__(256) multi_dot(int n, int nrhs, int nv, const double* __restrict__ V, size_t jstride,
+                                                 size_t qstride, const double* __restrict__ W, size_t ldw,
+                                                 double* __restrict__ out) {
+    __shared__ double red[4];
+    const int j = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
+    const double* v = V + (size_t)j * jstride + (size_t)q * qstride;
+    const double* w = W + (size_t)q * ldw;
+    double s = 0.0;
+    for (int i = tid; i < n; i += 256) s = fma(v[i], w[i], s);
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) out[(size_t)j * nrhs + q] = (red[0] + red[1]) + (red[2] + red[3]);
+    (void)nv;
+}
+// W(i, q) -= sum_{j < nv} H[j * nrhs + q] V_j(i, q)
+__global__ void multi_axpy(int n, int nrhs, int nv, const double* __restrict__ V, size_t jstride, size_t qstride,
+                           const double* __restrict__ H, double* __restrict__ W, size_t ldw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    double s = W[(size_t)i + (size_t)q * ldw];
+    for (int j = 0; j < nv; j++) s = fma(-H[(size_t)j * nrhs + q], V[(size_t)j * jstride + (size_t)q * qstride + i], s);
+    W[(size_t)i + (size_t)q * ldw] = s;
+}
+// Y(i, q) = a[q] X(i, q)   (a per column, from device memory)
+__global__ void col_mul(int n, int nrhs, const double* __restrict__ a, const double* __restrict__ X, size_t ldx,
+                        double* __restrict__ Y, size_t ldy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    Y[(size_t)i + (size_t)q * ldy] = a[q] * X[(size_t)i + (size_t)q * ldx];
+}
+// X(i, q) += D(i, q)
+__global__ void mat_add(int n, int nrhs, const double* __restrict__ D, size_t ldd, double* __restrict__ X, size_t ldx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    X[(size_t)i + (size_t)q * ldx] += D[(size_t)i + (size_t)q * ldd];
+}
+__global__ void mat_copy(int n, int nrhs, const double* __restrict__ S, size_t lds, double* __restrict__ D, size_t ldd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    D[(size_t)i + (size_t)q * ldd] = S[(size_t)i + (size_t)q * lds];
+}
+__global__ void mat_zero(int n, int nrhs, double* __restrict__ D, size_t ldd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    D[(size_t)i + (size_t)q * ldd] = 0.0;
+}
+__global__ void k_zero_int(int n, int* p) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---------------------------------------------------------------- host side
+struct Solver {
+    bola_hip_handle_s* h;
+    hipStream_t st;
+    int n, nrhs, lp;
+    const double* A;
+    size_t lda;
+    // device buffers
+    float* T;
+    size_t ldt;
+    int32_t* ipiv;
+    int* perm;
+    double *s, *t, *rsum, *pmax, *psum;
+    double *R, *Z, *D, *W, *P, *nrm, *dots;
+    double* Vb;  // GMRES basis: [q][j][n]
+    int* flags;
+    int* err;
+    void* luws;
+    int epoch = 0;
+    int mgm;  // GMRES restart length
+};
+
+dim3 g1(int n) { return dim3(cdiv((size_t)n, 256)); }
+dim3 g2(int n, int nrhs) { return dim3(cdiv((size_t)n, 256), (unsigned)nrhs); }
+
+// Y = M^-1 Xr (all n x nrhs, ld n); uses Z as scratch
+void apply_minv(Solver& S, const double* Xr, size_t ldx, double* Y) {
+    hipLaunchKernelGGL(gather_scale, g1(S.n), dim3(256), 0, S.st, S.n, S.nrhs, Xr, ldx, S.perm, S.s, S.Z, (size_t)S.n);
+    BOLA_LAUNCH_CHECK();
+    const int nb = (S.n + TB - 1) / TB;
+    S.epoch++;
+    hipLaunchKernelGGL(trsv_blk<true>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
+                       S.flags, S.epoch, nb, kTrsvSpin, S.err);
+    BOLA_LAUNCH_CHECK();
+    S.epoch++;
+    hipLaunchKernelGGL(trsv_blk<false>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
+                       S.flags, S.epoch, nb, kTrsvSpin, S.err);
+    BOLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(col_scale, g1(S.n), dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.t, Y, (size_t)S.n);
+    BOLA_LAUNCH_CHECK();
+}
+
+// R = (B ? B : 0) - A X
+void residual(Solver& S, const double* B, size_t ldb, const double* X, size_t ldx, double* R) {
+    const int cw = (S.n + RCH - 1) / RCH;
+    const int nch = (S.n + cw - 1) / cw;
+    hipLaunchKernelGGL(prod_part, dim3(cdiv((size_t)S.n, 256), (unsigned)nch), dim3(256), 0, S.st, S.n, S.nrhs, S.A,
+                       S.lda, X, ldx, S.P, cw);
+    BOLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(prod_fin, g1(S.n), dim3(256), 0, S.st, S.n, S.nrhs, nch, S.P, B, ldb, R, (size_t)S.n);
+    BOLA_LAUNCH_CHECK();
+}
+
+void amax(Solver& S, const double* V, size_t ldv, double* out_dev) {
+    hipLaunchKernelGGL(col_amax, dim3((unsigned)S.nrhs), dim3(256), 0, S.st, S.n, V, ldv, out_dev);
+    BOLA_LAUNCH_CHECK();
+}
+
+void put(Solver& S, double* dev, const std::vector<double>& v) {
+    BOLA_CHECK(hipMemcpyAsync(dev, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, S.st));
+    BOLA_CHECK(hipStreamSynchronize(S.st));  // v may go out of scope right after
+}
+
+std::vector<double> fetch(Solver& S, const double* dev, int count) {
+    std::vector<double> v(count);
+    BOLA_CHECK(hipMemcpyAsync(v.data(), dev, sizeof(double) * count, hipMemcpyDeviceToHost, S.st));
+    BOLA_CHECK(hipStreamSynchronize(S.st));
+    return v;
+}
+
+// backward error per column of the current X (R must hold B - A X)
+std::vector<double> berr(Solver& S, const double* X, size_t ldx, const std::vector<double>& bn, double anorm) {
+    amax(S, X, ldx, S.nrm);
+    amax(S, S.R, (size_t)S.n, S.nrm + NRMAX);
+    std::vector<double> v = fetch(S, S.nrm, 2 * NRMAX);
+    std::vector<double> e(S.nrhs);
+    for (int q = 0; q < S.nrhs; q++) {
+        const double den = anorm * v[q] + bn[q];
+        e[q] = den > 0 ? v[NRMAX + q] / den : (v[NRMAX + q] == 0 ? 0.0 : INFINITY);
+        if (!(e[q] == e[q])) e[q] = INFINITY;
+    }
+    return e;
+}
+
+double maxv(const std::vector<double>& v) {
+    double m = 0;
+    for (double x : v) m = (x > m || x != x) ? x : m;
+    return m;
+}
+
+// one restarted GMRES cycle on M^-1 A d = M^-1 r for every column; returns steps taken
+int gmres_cycle(Solver& S, double* Dout, double tol_rel) {
+    const int n = S.n, nr = S.nrhs, M = S.mgm;
+    const size_t js = (size_t)n, qs = (size_t)(M + 1) * n;
+    // v0 = M^-1 r / beta
+    apply_minv(S, S.R, (size_t)n, S.W);
+    std::vector<std::vector<double>> H(nr, std::vector<double>((M + 1) * M, 0.0));
+    std::vector<std::vector<double>> cs(nr, std::vector<double>(M)), sn(nr, std::vector<double>(M));
+    std::vector<std::vector<double>> g(nr, std::vector<double>(M + 1, 0.0));
+    std::vector<int> kdone(nr, -1);
+    // beta
+    hipLaunchKernelGGL(multi_dot, dim3(1, (unsigned)nr), dim3(256), 0, S.st, n, nr, 1, S.W, (size_t)0, (size_t)n, S.W,
+                       (size_t)n, S.dots);
+    BOLA_LAUNCH_CHECK();
+    std::vector<double> b2 = fetch(S, S.dots, nr);
+    std::vector<double> beta(nr), inv(nr);
+    for (int q = 0; q < nr; q++) {
+        beta[q] = std::sqrt(std::max(0.0, b2[q]));
+        inv[q] = beta[q] > 0 ? 1.0 / beta[q] : 0.0;
+        g[q][0] = beta[q];
+        if (beta[q] == 0) kdone[q] = 0;
+    }
+    put(S, S.nrm, inv);
+    hipLaunchKernelGGL(col_mul, g2(n, nr), dim3(256), 0, S.st, n, nr, S.nrm, S.W, (size_t)n, S.Vb, qs);
+    BOLA_LAUNCH_CHECK();
+    int k = 0;
+    for (; k < M; k++) {
+        bool all = true;
+        for (int q = 0; q < nr; q++) all = all && kdone[q] >= 0;
+        if (all) break;
+        // W = M^-1 A v_k
+        residual(S, nullptr, 0, S.Vb + (size_t)k * js, qs, S.D);  // D = -A v_k
+        apply_minv(S, S.D, (size_t)n, S.W);                        // W = -M^-1 A v_k
+        // classical Gram-Schmidt, twice; coefficients accumulate in H (sign folded in)
+        std::vector<double> hk(nr * (k + 2), 0.0);
+        for (int pass = 0; pass < 2; pass++) {
+            hipLaunchKernelGGL(multi_dot, dim3((unsigned)(k + 1), (unsigned)nr), dim3(256), 0, S.st, n, nr, k + 1,
+                               S.Vb, js, qs, S.W, (size_t)n, S.dots);
+            BOLA_LAUNCH_CHECK();
+            hipLaunchKernelGGL(multi_axpy, g2(n, nr), dim3(256), 0, S.st, n, nr, k + 1, S.Vb, js, qs, S.dots, S.W,
+                               (size_t)n);
+            BOLA_LAUNCH_CHECK();
+            std::vector<double> d = fetch(S, S.dots, (k + 1) * nr);
+            for (int j = 0; j <= k; j++)
+                for (int q = 0; q < nr; q++) hk[j * nr + q] += d[j * nr + q];
+        }
+        hipLaunchKernelGGL(multi_dot, dim3(1, (unsigned)nr), dim3(256), 0, S.st, n, nr, 1, S.W, (size_t)0, (size_t)n,
+                           S.W, (size_t)n, S.dots);
+        BOLA_LAUNCH_CHECK();
+        std::vector<double> w2 = fetch(S, S.dots, nr);
+        std::vector<double> sc(nr);
+        for (int q = 0; q < nr; q++) {
+            const double hn = std::sqrt(std::max(0.0, w2[q]));
+            // W holds -(M^-1 A v_k - sum h v): the Hessenberg column is the negation
+            double* Hq = H[q].data();
+            for (int j = 0; j <= k; j++) Hq[j * M + k] = -hk[j * nr + q];
+            Hq[(k + 1) * M + k] = hn;
+            sc[q] = hn > 0 ? -1.0 / hn : 0.0;  // v_{k+1} = -W / hn
+            if (kdone[q] >= 0) continue;
+            // previous rotations, then a new one
+            for (int j = 0; j < k; j++) {
+                const double a = Hq[j * M + k], b = Hq[(j + 1) * M + k];
+                Hq[j * M + k] = cs[q][j] * a + sn[q][j] * b;
+                Hq[(j + 1) * M + k] = -sn[q][j] * a + cs[q][j] * b;
+            }
+            const double a = Hq[k * M + k], b = Hq[(k + 1) * M + k];
+            const double rr = std::hypot(a, b);
+            cs[q][k] = rr > 0 ? a / rr : 1.0;
+            sn[q][k] = rr > 0 ? b / rr : 0.0;
+            Hq[k * M + k] = rr;
+            Hq[(k + 1) * M + k] = 0.0;
+            g[q][k + 1] = -sn[q][k] * g[q][k];
+            g[q][k] = cs[q][k] * g[q][k];
+            if (std::fabs(g[q][k + 1]) <= tol_rel * beta[q] || hn == 0.0) kdone[q] = k + 1;
+        }
+        put(S, S.nrm, sc);
+        hipLaunchKernelGGL(col_mul, g2(n, nr), dim3(256), 0, S.st, n, nr, S.nrm, S.W, (size_t)n,
+                           S.Vb + (size_t)(k + 1) * js, qs);
+        BOLA_LAUNCH_CHECK();
+    }
+    // y = R^-1 g per column; d = V y
+    std::vector<double> Y((size_t)M * nr, 0.0);
+    for (int q = 0; q < nr; q++) {
+        const int kq = kdone[q] >= 0 ? kdone[q] : k;
+        const double* Hq = H[q].data();
+        for (int i = kq - 1; i >= 0; i--) {
+            double v = g[q][i];
+            for (int j = i + 1; j < kq; j++) v -= Hq[i * M + j] * Y[(size_t)j * nr + q];
+            Y[(size_t)i * nr + q] = Hq[i * M + i] != 0 ? v / Hq[i * M + i] : 0.0;
+        }
+    }
+    hipLaunchKernelGGL(mat_zero, g2(n, nr), dim3(256), 0, S.st, n, nr, Dout, (size_t)n);
+    BOLA_LAUNCH_CHECK();
+    // Dout -= sum_j (-y_j) v_j
+    std::vector<double> negY(Y.size());
+    for (size_t e = 0; e < Y.size(); e++) negY[e] = -Y[e];
+    put(S, S.dots, negY);
+    hipLaunchKernelGGL(multi_axpy, g2(n, nr), dim3(256), 0, S.st, n, nr, std::max(k, 1), S.Vb, js, qs, S.dots, Dout,
+                       (size_t)n);
+    BOLA_LAUNCH_CHECK();
+    return k;
+}
+
+}  // namespace
+
+}  // namespace bola
+
+using namespace bola;
+
+extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n_, size_t nrhs_, const double* A,
+                                        size_t lda, const double* B, size_t ldb, double* X, size_t ldx,
+                                        bola_hip_mixed_stats* stats) {
+    BOLA_API_BEGIN
+    if (!h) return -1;
+    if (precision < 0 || precision > 2) {
+        set_error(h, "Argument 2 to dgesv_mixed had an illegal value");
+        return -2;
+    }
+    if (lda < std::max<size_t>(1, n_)) {
+        set_error(h, "Argument 6 to dgesv_mixed had an illegal value");
+        return -6;
+    }
+    if (nrhs_ > (size_t)NRMAX) {
+        set_error(h, "Argument 4 to dgesv_mixed had an illegal value (at most 16 right-hand sides)");
+        return -4;
+    }
+    if (n_ > (size_t)mixed_lu_max_n()) {
+        set_error(h, "Argument 3 to dgesv_mixed had an illegal value (n above the supported maximum)");
+        return -3;
+    }
+    bola_hip_mixed_stats loc{};
+    bola_hip_mixed_stats& so = stats ? *stats : loc;
+    so = bola_hip_mixed_stats{};
+    if (n_ == 0 || nrhs_ == 0) return 0;
+    const int n = (int)n_, nrhs = (int)nrhs_;
+    Solver S;
+    S.h = h;
+    S.st = h->stream;
+    S.n = n;
+    S.nrhs = nrhs;
+    S.lp = precision;
+    S.A = A;
+    S.lda = lda;
+    S.mgm = 30;
+    S.ldt = ((size_t)n + 63) & ~(size_t)63;
+    // workspace carve-up
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    const int nch = RCH;
+    const size_t oT = take(sizeof(float) * S.ldt * n);
+    const size_t oipiv = take(sizeof(int32_t) * n), operm = take(sizeof(int) * n);
+    const size_t os = take(8 * (size_t)n), ot = take(8 * (size_t)n), orsum = take(8 * (size_t)n);
+    const size_t opmax = take(8 * (size_t)n * nch), opsum = take(8 * (size_t)n * nch);
+    const size_t nv = (size_t)n * NRMAX;
+    const size_t oR = take(8 * nv), oZ = take(8 * nv), oD = take(8 * nv), oW = take(8 * nv);
+    const size_t oP = take(8 * (size_t)n * NRMAX * nch);
+    const size_t onrm = take(8 * 4 * NRMAX), odots = take(8 * (size_t)(S.mgm + 2) * NRMAX * 2);
+    const size_t oVb = take(8 * (size_t)nrhs * (S.mgm + 1) * n);
+    const size_t oflags = take(sizeof(int) * ((size_t)n / TB + 2)), oerr = take(sizeof(int) * 4);
+    const size_t olu = take(mixed_lu_ws_bytes(n, precision));
+    char* base = (char*)h->mws.get(off);
+    S.T = (float*)(base + oT);
+    S.ipiv = (int32_t*)(base + oipiv);
+    S.perm = (int*)(base + operm);
+    S.s = (double*)(base + os);
+    S.t = (double*)(base + ot);
+    S.rsum = (double*)(base + orsum);
+    S.pmax = (double*)(base + opmax);
+    S.psum = (double*)(base + opsum);
+    S.R = (double*)(base + oR);
+    S.Z = (double*)(base + oZ);
+    S.D = (double*)(base + oD);
+    S.W = (double*)(base + oW);
+    S.P = (double*)(base + oP);
+    S.nrm = (double*)(base + onrm);
+    S.dots = (double*)(base + odots);
+    S.Vb = (double*)(base + oVb);
+    S.flags = (int*)(base + oflags);
+    S.err = (int*)(base + oerr);
+    S.luws = base + olu;
+    hipStream_t st = S.st;
+    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, n / TB + 2, S.flags);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, S.err);
+    BOLA_LAUNCH_CHECK();
+    hipEvent_t ev[4];
+    for (auto& e : ev) BOLA_CHECK(hipEventCreate(&e));
+    BOLA_CHECK(hipEventRecord(ev[0], st));
+    {
+        Phase ph(h, "MIXED_FACTOR", 2.0 * (double)n * n * n / 3.0);
+        // 1. equilibration and the fp32 transpose
+        const int cw = (n + nch - 1) / nch;
+        hipLaunchKernelGGL(eq_row_part, dim3(cdiv((size_t)n, 256), (unsigned)nch), dim3(256), 0, st, n, A, lda, S.pmax,
+                           S.psum, cw);
+        hipLaunchKernelGGL(eq_row_fin, g1(n), dim3(256), 0, st, n, nch, S.pmax, S.psum, S.s, S.rsum);
+        hipLaunchKernelGGL(eq_col, dim3((unsigned)n), dim3(256), 0, st, n, A, lda, S.s, S.t);
+        hipLaunchKernelGGL(to_T, dim3(cdiv((size_t)n, 64), cdiv((size_t)n, 64)), dim3(256), 0, st, n, A, lda, S.s, S.t,
+                           S.T, S.ldt);
+        hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, st, n, S.perm);
+        BOLA_LAUNCH_CHECK();
+        // 2. the low-precision LU
+        mixed_lu_factor(st, precision, n, S.T, S.ldt, S.ipiv, S.perm, S.luws);
+    }
+    BOLA_CHECK(hipEventRecord(ev[1], st));
+    // |A|_inf and |b|_inf
+    hipLaunchKernelGGL(col_amax, dim3(1), dim3(256), 0, st, n, S.rsum, (size_t)n, S.nrm + 2 * NRMAX);
+    BOLA_LAUNCH_CHECK();
+    const double anorm = fetch(S, S.nrm + 2 * NRMAX, 1)[0];
+    hipLaunchKernelGGL(col_amax, dim3((unsigned)nrhs), dim3(256), 0, st, n, B, ldb, S.nrm + 3 * NRMAX);
+    BOLA_LAUNCH_CHECK();
+    const std::vector<double> bn = fetch(S, S.nrm + 3 * NRMAX, nrhs);
+    const double tol = (double)n * 2.220446049250313e-16;
+    so.tolerance = tol;
+    // 3. refinement: X = 0, R = B
+    hipLaunchKernelGGL(mat_zero, g2(n, nrhs), dim3(256), 0, st, n, nrhs, X, ldx);
+    hipLaunchKernelGGL(mat_copy, g2(n, nrhs), dim3(256), 0, st, n, nrhs, B, ldb, S.R, (size_t)n);
+    BOLA_LAUNCH_CHECK();
+    std::vector<double> e(nrhs, INFINITY);
+    double prev = INFINITY;
+    int stall = 0;
+    bool done = false;
+    {
+        Phase ph(h, "MIXED_REFINE");
+        for (int it = 0; it < 30 && !done; it++) {
+            apply_minv(S, S.R, (size_t)n, S.D);
+            hipLaunchKernelGGL(mat_add, g2(n, nrhs), dim3(256), 0, st, n, nrhs, S.D, (size_t)n, X, ldx);
+            BOLA_LAUNCH_CHECK();
+            residual(S, B, ldb, X, ldx, S.R);
+            e = berr(S, X, ldx, bn, anorm);
+            so.ir_iterations = it + 1;
+            const double m = maxv(e);
+            if (it == 0) so.first_berr = m;
+            if (m <= tol) { done = true; break; }
+            if (!(m < 0.5 * prev)) stall++;
+            else stall = 0;
+            prev = std::min(prev, m);
+            if (stall >= 2 || !(m < 1.0)) break;
+        }
+        // GMRES-IR
+        if (!done) {
+            for (int outer = 0; outer < 10 && !done; outer++) {
+                so.gmres_iterations += gmres_cycle(S, S.D, 1e-8);
+                hipLaunchKernelGGL(mat_add, g2(n, nrhs), dim3(256), 0, st, n, nrhs, S.D, (size_t)n, X, ldx);
+                BOLA_LAUNCH_CHECK();
+                residual(S, B, ldb, X, ldx, S.R);
+                e = berr(S, X, ldx, bn, anorm);
+                so.gmres_cycles = outer + 1;
+                const double m = maxv(e);
+                if (m <= tol) { done = true; break; }
+                if (!(m < 0.5 * prev) && outer > 0) break;
+                prev = std::min(prev, m);
+            }
+        }
+    }
+    BOLA_CHECK(hipEventRecord(ev[2], st));
+    {
+        int herr = 0;
+        BOLA_CHECK(hipMemcpyAsync(&herr, S.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        BOLA_CHECK(hipStreamSynchronize(st));
+        if (herr) done = false;  // a triangular solve gave up waiting: trust nothing it produced
+    }
+    // 4. fallback: the FP64 reference-exact path
+    if (!done) {
+        Phase ph(h, "MIXED_FALLBACK");
+        so.fallback = 1;
+        double* Af = (double*)h->mws2.get(sizeof(double) * (size_t)n * n + sizeof(int32_t) * n + 512);
+        int32_t* piv = (int32_t*)(Af + (size_t)n * n);
+        BOLA_CHECK(hipMemcpy2DAsync(Af, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                                    hipMemcpyDeviceToDevice, st));
+        lu_factor(h, n, n, Af, n, piv, h->d_info, false);
+        int32_t info = 0;
+        BOLA_CHECK(hipMemcpyAsync(&info, h->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        BOLA_CHECK(hipStreamSynchronize(st));
+        if (info != 0) {
+            for (auto& x : ev) (void)hipEventDestroy(x);
+            set_error(h, "Matrix is singular to working precision (FP64 fallback factorization failed).");
+            return info > 0 ? info : BOLA_HIP_ERR_DEVICE;
+        }
+        hipLaunchKernelGGL(mat_copy, g2(n, nrhs), dim3(256), 0, st, n, nrhs, B, ldb, X, ldx);
+        BOLA_LAUNCH_CHECK();
+        int r = bola_hip_dgetrs_dev(h, n, nrhs, Af, n, piv, X, ldx);
+        if (r) {
+            for (auto& x : ev) (void)hipEventDestroy(x);
+            return r;
+        }
+        residual(S, B, ldb, X, ldx, S.R);
+        e = berr(S, X, ldx, bn, anorm);
+    }
+    BOLA_CHECK(hipEventRecord(ev[3], st));
+    BOLA_CHECK(hipEventSynchronize(ev[3]));
+    float t01 = 0, t12 = 0, t23 = 0;
+    BOLA_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    BOLA_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    BOLA_CHECK(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    so.factor_ms = t01;
+    so.refine_ms = t12;
+    so.fallback_ms = t23;
+    so.final_berr = maxv(e);
+    so.converged = so.final_berr <= tol ? 1 : 0;
+    return 0;
+    BOLA_API_END(h)
+}

@@ -1,0 +1,311 @@
+// Low-precision trailing-update GEMM for the mixed-precision LU (no reference
+// counterpart: the reference computes in FP64 only).
+//
+//   C (fp32, column-major) -= A * B
+//
+// with the products formed on the MFMA units in one of three precisions:
+//   LP_F32  : A, B fp32, v_mfma_f32_16x16x4_f32 (FP32 matrix peak 157 TF/s)
+//   LP_F16  : A, B rounded to fp16, v_mfma_f32_16x16x32_f16, fp32 accumulation
+//   LP_BF16 : A, B rounded to bf16, v_mfma_f32_16x16x32_bf16, fp32 accumulation
+//
+// Tiles of 128 x 128 per 256-thread workgroup, 2 x 2 wavefronts of 64 x 64 (4 x 4 MFMA
+// blocks each, 16 accumulators); K steps staged through double-buffered LDS.  LDS rows
+// are k-contiguous with a stride of 36 words, which makes every fragment read free of
+// bank conflicts.  The 16-bit variants take A already transposed (k-contiguous rows,
+// produced by the conversion kernel below), so both operands load as 16-byte chunks.
+#include "mixed.hpp"
+
+namespace bola {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+constexpr int GT = 256;
+constexpr int GBM = 128, GBN = 128;
+constexpr int RW = 36;  // LDS row stride in 32-bit words
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    // consecutive tiles to one XCD (8 XCDs, round-robin dispatch)
+    const int per = nblk / 8;
+    if (per == 0 || bid >= per * 8) return bid;
+    return (bid % 8) * per + bid / 8;
+}
+
+// ---- fp32 operands.  A column-major (lda), B column-major (ldb).  BK = 32 floats.
+__global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float* __restrict__ A, size_t lda,
+                                                const float* __restrict__ B, size_t ldb, float* __restrict__ C,
+                                                size_t ldc, int tiles_m, int tiles_n) {
+    constexpr int BK = 32;
+    __shared__ float As[2][GBM * RW];
+    __shared__ float Bs[2][GBN * RW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    const int i0 = tm * GBM, j0 = tn * GBN;
+    const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // loaders: A tile 128 x 32 as 4 rows x 1 k per thread-chunk; B tile 32 x 128 as 4 k x 1 col
+    float ra[4][4], rb[4][4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * GT;       // 0..1023
+            const int ii = (e & 31) * 4;      // row chunk within tile
+            const int kk = e >> 5;            // k within tile
+            const int gi = i0 + ii, gk = k0 + kk;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                ra[q][r] = (gi + r < m && gk < k) ? A[(size_t)(gi + r) + (size_t)gk * lda] : 0.f;
+            const int kb = (e & 7) * 4;       // k chunk
+            const int jj = e >> 3;            // column within tile
+            const int gj = j0 + jj, gkb = k0 + kb;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                rb[q][r] = (gj < n && gkb + r < k) ? B[(size_t)(gkb + r) + (size_t)gj * ldb] : 0.f;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * GT;
+            const int ii = (e & 31) * 4, kk = e >> 5;
+#pragma unroll
+            for (int r = 0; r < 4; r++) As[buf][(ii + r) * RW + kk] = ra[q][r];
+            const int kb = (e & 7) * 4, jj = e >> 3;
+            *(f4*)&Bs[buf][jj * RW + kb] = f4{rb[q][0], rb[q][1], rb[q][2], rb[q][3]};
+        }
+    };
+    const int nk = (k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        const int cur = t & 1;
+        if (t + 1 < nk) gload((t + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            float fa[4], fb[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) fa[a] = As[cur][(wm + 16 * a + (lane & 15)) * RW + kk + (lane >> 4)];
+#pragma unroll
+            for (int b = 0; b < 4; b++) fb[b] = Bs[cur][(wn + 16 * b + (lane & 15)) * RW + kk + (lane >> 4)];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+        if (t + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    // C -= acc; D layout: reg r of lane l -> (row 4*(l/16)+r, col l%16)
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = j0 + wn + 16 * b + (lane & 15);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+            if (j < n) {
+                float* cp = C + (size_t)j * ldc;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (ib + r < m) cp[ib + r] -= acc[a][b][r];
+            }
+        }
+}
+
+// ---- 16-bit operands.  At: A transposed, row i at At + i*lda (k-contiguous); B column
+// j at B + j*ldb (k-contiguous).  BK = 64 elements.
+template <typename T, typename V8>
+__global__ void __launch_bounds__(GT) sgemm_h(int m, int n, int k, const T* __restrict__ At, size_t lda,
+                                              const T* __restrict__ B, size_t ldb, float* __restrict__ C, size_t ldc,
+                                              int tiles_m, int tiles_n) {
+    constexpr int BK = 64;
+    __shared__ uint32_t As[2][GBM * RW];
+    __shared__ uint32_t Bs[2][GBN * RW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    const int i0 = tm * GBM, j0 = tn * GBN;
+    const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    // a tile is 128 rows x 64 elements = 128 x 8 chunks of 16 bytes; 4 chunks per thread
+    uint4 ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * GT;
+            const int row = e >> 3, ch = (e & 7) * 8;
+            const int gk = k0 + ch;
+            const int gi = i0 + row, gj = j0 + row;
+            if (gi < m && gk + 8 <= k) {
+                ra[q] = *(const uint4*)(At + (size_t)gi * lda + gk);
+            } else {
+                uint16_t tmp[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    tmp[r] = (gi < m && gk + r < k) ? ((const uint16_t*)At)[(size_t)gi * lda + gk + r] : (uint16_t)0;
+                ra[q] = uint4{tmp[0] | ((uint32_t)tmp[1] << 16), tmp[2] | ((uint32_t)tmp[3] << 16),
+                              tmp[4] | ((uint32_t)tmp[5] << 16), tmp[6] | ((uint32_t)tmp[7] << 16)};
+            }
+            if (gj < n && gk + 8 <= k) {
+                rb[q] = *(const uint4*)(B + (size_t)gj * ldb + gk);
+            } else {
+                uint16_t tmp[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    tmp[r] = (gj < n && gk + r < k) ? ((const uint16_t*)B)[(size_t)gj * ldb + gk + r] : (uint16_t)0;
+                rb[q] = uint4{tmp[0] | ((uint32_t)tmp[1] << 16), tmp[2] | ((uint32_t)tmp[3] << 16),
+                              tmp[4] | ((uint32_t)tmp[5] << 16), tmp[6] | ((uint32_t)tmp[7] << 16)};
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * GT;
+            const int row = e >> 3, w4 = (e & 7) * 4;
+            *(uint4*)&As[buf][row * RW + w4] = ra[q];
+            *(uint4*)&Bs[buf][row * RW + w4] = rb[q];
+        }
+    };
+    const int nk = (k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        const int cur = t & 1;
+        if (t + 1 < nk) gload((t + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / 2; kk += 16) {  // in words: 32 elements per MFMA
+            V8 fa[4], fb[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const uint4 u = *(const uint4*)&As[cur][(wm + 16 * a + (lane & 15)) * RW + kk + 4 * (lane >> 4)];
+                fa[a] = __builtin_bit_cast(V8, u);
+            }
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint4 u = *(const uint4*)&Bs[cur][(wn + 16 * b + (lane & 15)) * RW + kk + 4 * (lane >> 4)];
+                fb[b] = __builtin_bit_cast(V8, u);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    if constexpr (sizeof(T) == 2 && __is_same(T, _Float16))
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        if (t + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = j0 + wn + 16 * b + (lane & 15);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+            if (j < n) {
+                float* cp = C + (size_t)j * ldc;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (ib + r < m) cp[ib + r] -= acc[a][b][r];
+            }
+        }
+}
+
+// fp32 -> 16-bit, transposed: dst row i (length k, stride ldd) = src row i (src column-major).
+template <typename T>
+__global__ void __launch_bounds__(256) cvt_t16(int m, int k, const float* __restrict__ src, size_t lds,
+                                               T* __restrict__ dst, size_t ldd) {
+    __shared__ float tile[64][65];
+    const int bi = blockIdx.x * 64, bk = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int i = bi + tx, kk = bk + r;
+        tile[r][tx] = (i < m && kk < k) ? src[(size_t)i + (size_t)kk * lds] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int i = bi + r, kk = bk + tx;
+        if (i < m && kk < k) dst[(size_t)i * ldd + kk] = (T)tile[tx][r];
+    }
+}
+
+// fp32 -> 16-bit, same layout (column-major, columns k-contiguous)
+template <typename T>
+__global__ void cvt_n16(int k, int n, const float* __restrict__ src, size_t lds, T* __restrict__ dst, size_t ldd) {
+    const size_t total = (size_t)k * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = e / k, kk = e % k;
+        dst[kk + j * ldd] = (T)src[kk + j * lds];
+    }
+}
+
+}  // namespace
+
+size_t lp_gemm_ws_bytes(int lp, size_t m, size_t n, size_t k) {
+    if (lp == LP_F32) return 0;
+    const size_t kp = (k + 7) & ~(size_t)7;
+    return 2 * (m * kp + n * kp) + 256;
+}
+
+void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A, size_t lda, const float* B,
+                    size_t ldb, float* C, size_t ldc, void* ws) {
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int tiles_m = (m + GBM - 1) / GBM, tiles_n = (n + GBN - 1) / GBN;
+    const unsigned grid = (unsigned)(tiles_m * tiles_n);
+    if (lp == LP_F32) {
+        hipLaunchKernelGGL(sgemm_f32, dim3(grid), dim3(GT), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+        BOLA_LAUNCH_CHECK();
+        return;
+    }
+    // 16-bit copies: At (m rows of kp), Bh (n columns of kp); kp keeps rows 16-byte aligned
+    const size_t kp = ((size_t)k + 7) & ~(size_t)7;
+    char* base = (char*)ws;
+    void* At = base;
+    void* Bh = base + 2 * (size_t)m * kp;
+    const dim3 tg((unsigned)((m + 63) / 64), (unsigned)((k + 63) / 64));
+    const unsigned cg = (unsigned)std::min<size_t>(((size_t)k * n + 255) / 256, 4096);
+    if (lp == LP_F16) {
+        hipLaunchKernelGGL(cvt_t16<_Float16>, tg, dim3(256), 0, st, m, k, A, lda, (_Float16*)At, kp);
+        hipLaunchKernelGGL(cvt_n16<_Float16>, dim3(cg), dim3(256), 0, st, k, n, B, ldb, (_Float16*)Bh, kp);
+        hipLaunchKernelGGL((sgemm_h<_Float16, h8>), dim3(grid), dim3(GT), 0, st, m, n, k, (const _Float16*)At, kp,
+                           (const _Float16*)Bh, kp, C, ldc, tiles_m, tiles_n);
+    } else {
+        hipLaunchKernelGGL(cvt_t16<__bf16>, tg, dim3(256), 0, st, m, k, A, lda, (__bf16*)At, kp);
+        hipLaunchKernelGGL(cvt_n16<__bf16>, dim3(cg), dim3(256), 0, st, k, n, B, ldb, (__bf16*)Bh, kp);
+        hipLaunchKernelGGL((sgemm_h<__bf16, b8>), dim3(grid), dim3(GT), 0, st, m, n, k, (const __bf16*)At, kp,
+                           (const __bf16*)Bh, kp, C, ldc, tiles_m, tiles_n);
+    }
+    BOLA_LAUNCH_CHECK();
+}
+
+}  // namespace bola
+
+using namespace bola;
+
+extern "C" int bola_hip_sgemm_lp_dev(bola_hip_handle_t h, int precision, size_t m, size_t n, size_t k, const float* a,
+                                     size_t lda, const float* b, size_t ldb, float* c, size_t ldc) {
+    BOLA_API_BEGIN
+    if (!h) return -1;
+    if (precision < 0 || precision > 2) return -2;
+    void* ws = lp_gemm_ws_bytes(precision, m, n, k) ? h->mws.get(lp_gemm_ws_bytes(precision, m, n, k)) : nullptr;
+    launch_lp_gemm(h->stream, precision, (int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, ws);
+    return 0;
+    BOLA_API_END(h)
+}

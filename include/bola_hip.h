@@ -158,6 +158,44 @@ int bola_hip_dpotrs(char uplo, size_t n, size_t nrhs, const double* f, size_t ld
 int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, const double* f, size_t lda,
                         double* b, size_t ldb);
 
+/* ------------------------------------------------------------------ mixed precision
+ * No crate counterpart: the crate solves in FP64 only.  A X = B (n x n, nrhs <= 16
+ * right-hand sides, column-major f64) to FP64 accuracy from a low-precision LU:
+ * power-of-two row/column equilibration, LU of the scaled matrix in fp32 storage with
+ * tournament pivoting and trailing products in fp32, fp16 or bf16 on the MFMA units,
+ * iterative refinement in FP64 (r = b - A x on the original A), GMRES-IR when plain
+ * refinement stagnates, and the FP64 GETRF + GETRS of this library as the last resort.
+ * Stops when every column's backward error |r|_inf / (|A|_inf |x|_inf + |b|_inf) is at most
+ * n * 2^-52.  A and B are not modified.  n <= 16384.
+ * Returns 0, -i for an illegal argument i, or +k when even the FP64 fallback meets an
+ * exactly zero pivot in column k. */
+#define BOLA_HIP_PREC_FP32 0
+#define BOLA_HIP_PREC_FP16 1
+#define BOLA_HIP_PREC_BF16 2
+typedef struct bola_hip_mixed_stats {
+    int ir_iterations;    /* refinement steps with the low-precision factors */
+    int gmres_cycles;     /* GMRES-IR restarts */
+    int gmres_iterations; /* GMRES steps in total */
+    int fallback;         /* 1: the FP64 GETRF + GETRS produced the result */
+    int converged;        /* 1: final backward error <= tolerance */
+    int pad;
+    double first_berr;    /* backward error after the first correction */
+    double final_berr;    /* largest over the columns, at return */
+    double tolerance;     /* n * 2^-52 */
+    double factor_ms;     /* equilibration + conversion + factorization (GPU time) */
+    double refine_ms;     /* refinement and GMRES-IR */
+    double fallback_ms;   /* FP64 fallback, 0 when not needed */
+} bola_hip_mixed_stats;
+int bola_hip_dgesv_mixed(int precision, size_t n, size_t nrhs, const double* a, size_t lda, const double* b,
+                         size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats);
+int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n, size_t nrhs, const double* a, size_t lda,
+                             const double* b, size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats);
+/* The trailing-update product of that factorization, exposed for tests and measurement:
+ * C -= A B with A (m x k), B (k x n), C (m x n) column-major fp32 device arrays, the
+ * products in `precision` (BOLA_HIP_PREC_*), accumulated in fp32. */
+int bola_hip_sgemm_lp_dev(bola_hip_handle_t h, int precision, size_t m, size_t n, size_t k, const float* a,
+                          size_t lda, const float* b, size_t ldb, float* c, size_t ldc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,139 @@
+"""Mixed-precision solver (no reference counterpart): the low-precision trailing-update
+GEMM against an FP64 product of the same rounded operands, and the refined solve against
+its own contract (backward error <= n * eps64 in every column) and against the FP64
+oracle's solution (forward error within the conditioning bound).  The FP64 fallback is
+the reference-exact path, so when it is taken the answer must equal the oracle's GETRF +
+GETRS bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+@pytest.fixture(scope="module")
+def B():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import bola_amd
+    bola_amd.lib()
+    return bola_amd
+
+
+@pytest.fixture(scope="module")
+def O():
+    import oracle
+    oracle.lib()
+    return oracle
+
+
+def colmajor(M):
+    return np.asfortranarray(M).ravel(order="F")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 32), (300, 200, 96), (1000, 513, 256), (64, 700, 33), (257, 129, 1)])
+def test_sgemm_lp_matches_fp64_product_of_rounded_operands(B, prec, m, n, k):
+    import torch
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    Bm = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    Cm = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    dA = torch.from_numpy(colmajor(A)).cuda()
+    dB = torch.from_numpy(colmajor(Bm)).cuda()
+    dC = torch.from_numpy(colmajor(Cm)).cuda()
+    B.sgemm_lp(m, n, k, dA, m, dB, k, dC, m, prec)
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy().reshape(n, m).T.astype(np.float64)
+    rnd = {"fp32": torch.float32, "fp16": torch.float16, "bf16": torch.bfloat16}[prec]
+    Ar = torch.from_numpy(A).to(rnd).double().numpy()
+    Br = torch.from_numpy(Bm).to(rnd).double().numpy()
+    want = Cm.astype(np.float64) - Ar @ Br
+    # fp32 accumulation over k products plus the final subtraction
+    bound = (k + 2) * EPS32 * (np.abs(Ar) @ np.abs(Br) + np.abs(Cm))
+    assert np.all(np.abs(got - want) <= bound + 1e-30), float(np.max(np.abs(got - want) - bound))
+
+
+def _solve(B, n, nrhs, kappa, prec, seed=7):
+    import torch
+    from bola_amd import testmat
+    A = testmat.conditioned(n, kappa, seed, device="cuda")
+    rng = np.random.default_rng(seed + 1)
+    b = rng.uniform(-1, 1, n * nrhs)
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros(n * nrhs, dtype=torch.float64, device="cuda")
+    st = B.dgesv_mixed(n, nrhs, A, n, db, n, dx, n, prec)
+    torch.cuda.synchronize()
+    return A.cpu().numpy(), b, dx.cpu().numpy(), st
+
+
+def _berr(Af, b, x, n, nrhs):
+    Am = Af.reshape(n, n).T
+    X = x.reshape(nrhs, n).T
+    Bm = b.reshape(nrhs, n).T
+    R = Bm - Am @ X
+    an = np.max(np.sum(np.abs(Am), axis=1))
+    return np.max(np.max(np.abs(R), axis=0) / (an * np.max(np.abs(X), axis=0) + np.max(np.abs(Bm), axis=0)))
+
+
+@pytest.mark.parametrize("prec,kappa", [("fp32", 1e2), ("fp32", 1e5), ("fp32", 1e8), ("fp16", 1e2),
+                                        ("fp16", 1e5), ("bf16", 1e2), ("bf16", 1e5)])
+@pytest.mark.parametrize("n,nrhs", [(512, 1), (1024, 16), (1000, 3)])
+def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs):
+    Af, b, x, st = _solve(B, n, nrhs, kappa, prec)
+    print(f"n={n} nrhs={nrhs} {prec} kappa={kappa:g}: {st}")
+    assert st["converged"] == 1, st
+    assert st["final_berr"] <= n * EPS64
+    # the same criterion recomputed on the host
+    assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
+    if kappa <= 1e2:
+        assert st["fallback"] == 0, st
+    # forward error against the FP64 oracle's solution, within the conditioning bound
+    a = Af.copy()
+    p = np.zeros(n, np.int32)
+    code, _ = O.getrf(n, n, a, n, p)
+    assert code == 0
+    xo = b.copy()
+    O.getrs(n, nrhs, a, n, p, xo, n)
+    X, XO = x.reshape(nrhs, n), xo.reshape(nrhs, n)
+    ferr = np.max(np.abs(X - XO), axis=1) / np.max(np.abs(XO), axis=1)
+    assert np.all(ferr <= 10 * kappa * n * EPS64), ferr
+
+
+def test_mixed_host_flavour_equals_device(B):
+    import torch
+    n, nrhs = 700, 2
+    Af, b, x_dev, st_dev = _solve(B, n, nrhs, 1e3, "fp32", seed=11)
+    x = np.zeros(n * nrhs)
+    st = B.dgesv_mixed(n, nrhs, Af.copy(), n, b.copy(), n, x, n, "fp32")
+    assert np.array_equal(x, x_dev)
+    assert st["ir_iterations"] == st_dev["ir_iterations"]
+    del torch
+
+
+def test_mixed_fallback_is_the_exact_fp64_path(B, O):
+    # bf16 factors of a kappa = 1e13 matrix cannot refine to n * eps64: the FP64 path answers
+    n, nrhs = 384, 2
+    Af, b, x, st = _solve(B, n, nrhs, 1e13, "bf16", seed=5)
+    assert st["fallback"] == 1, st
+    a = Af.copy()
+    p = np.zeros(n, np.int32)
+    O.getrf(n, n, a, n, p)
+    xo = b.copy()
+    O.getrs(n, nrhs, a, n, p, xo, n)
+    assert np.array_equal(x, xo)
+
+
+def test_mixed_rejects_bad_arguments(B):
+    import torch
+    a = torch.zeros(16, dtype=torch.float64, device="cuda")
+    x = torch.zeros(4, dtype=torch.float64, device="cuda")
+    with pytest.raises(B.BolaError) as e:
+        B.dgesv_mixed(4, 17, a, 4, x, 4, x, 4, "fp32")
+    assert e.value.code == -4
+    with pytest.raises(B.BolaError) as e:
+        B.dgesv_mixed(4, 1, a, 3, x, 4, x, 4, "fp32")
+    assert e.value.code == -6
