@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
     __shared__ double ys[TB][NRMAX + 1];
     __shared__ float Ld[TB][TB + 1];
     __shared__ int ok_sh;
+    for (int e = threadIdx.x; e < TB * (NRMAX + 1); e += 256) (&ys[0][0])[e] = 0.0;
     const int bi = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
     const int r0 = bi * TB, rows = min(TB, n - r0);
     const int tid = threadIdx.x, r = tid & 63, qg = tid >> 6;
@@ -160,7 +161,14 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
     }
     const int ndep = LOWER ? bi : nb - 1 - bi;
     for (int jj = 0; jj < ndep; jj++) {
-        const int j = LOWER ? jj : bi + 1 + jj;
+        // dependencies in the order they are solved: the oldest first
+        const int j = LOWER ? jj : nb - 1 - jj;
+        const int c0 = j * TB, cols = min(TB, n - c0);
+        // this row's factor entries in block j do not depend on the flag: load them first
+        float lr[TB];
+        const float* lrow = T + (size_t)c0 + (size_t)(r0 + min(r, rows - 1)) * ldt;
+#pragma unroll
+        for (int c = 0; c < TB; c++) lr[c] = c < cols ? lrow[c] : 0.f;
         if (tid == 0) {
             long sp = 0;
             int ok = 1;
@@ -177,21 +185,27 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
             if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
-        const int c0 = j * TB, cols = min(TB, n - c0);
-        for (int e = tid; e < TB * NRMAX; e += 256) {
+        for (int e = tid; e < TB * nrhs; e += 256) {
             const int c = e % TB, q = e / TB;
-            ys[c][q] = (c < cols && q < nrhs) ? ald(Z + (size_t)(c0 + c) + (size_t)q * ldz) : 0.0;
+            ys[c][q] = c < cols ? ald(Z + (size_t)(c0 + c) + (size_t)q * ldz) : 0.0;
         }
         __syncthreads();
         if (r < rows) {
-            const float* lrow = T + (size_t)c0 + (size_t)(r0 + r) * ldt;  // factor(r0 + r, c0 + c), c contiguous
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            for (int c = 0; c < cols; c++) {
-                const double l = (double)lrow[c];
-                s0 = fma(l, ys[c][qg], s0);
-                s1 = fma(l, ys[c][qg + 4], s1);
-                s2 = fma(l, ys[c][qg + 8], s2);
-                s3 = fma(l, ys[c][qg + 12], s3);
+            if (nrhs == 1) {
+                if (qg == 0) {
+#pragma unroll
+                    for (int c = 0; c < TB; c++) s0 = fma((double)lr[c], ys[c][0], s0);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < TB; c++) {
+                    const double l = (double)lr[c];
+                    s0 = fma(l, ys[c][qg], s0);
+                    s1 = fma(l, ys[c][qg + 4], s1);
+                    s2 = fma(l, ys[c][qg + 8], s2);
+                    s3 = fma(l, ys[c][qg + 12], s3);
+                }
             }
             acc[0] -= s0;
             acc[1] -= s1;

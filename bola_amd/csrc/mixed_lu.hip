@@ -46,88 +46,133 @@ __device__ __forceinline__ Best wave_best(Best b) {
     return b;
 }
 
-// Partial pivoting on `rows` rows held in LDS (tile[row * CRS + c], w columns); the
-// chosen rows (tile indices) are written to pick[0..min(w, rows)).  NT threads.
-template <int NT>
-__device__ void lds_partial_pivot(float* tile, int rows, int w, int* pick, Best (*red)[2], int* sel) {
-    constexpr int NWV = NT / 64;
-    constexpr int RPT = (CMAX + NT - 1) / NT;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    bool taken[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) taken[q] = false;
-    const int steps = min(w, rows);
-    for (int j = 0; j < steps; j++) {
-        Best b{-1.f, 0x7fffffff};
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int row = tid + q * NT;
-            if (row < rows && !taken[q]) {
-                float v = fabsf(tile[row * CRS + j]);
-                if (!(v == v)) v = -0.5f;  // NaN never wins over a number
-                if (beats(v, row, b.v, b.r)) { b.v = v; b.r = row; }
-            }
-        }
-        b = wave_best(b);
-        if (lane == 0) red[wv][j & 1] = b;
-        __syncthreads();
-        Best g = red[0][j & 1];
-#pragma unroll
-        for (int q = 1; q < NWV; q++)
-            if (beats(red[q][j & 1].v, red[q][j & 1].r, g.v, g.r)) g = red[q][j & 1];
-        const int p = g.r;
-        if (tid == 0) { pick[j] = p; *sel = p; }
-#pragma unroll
-        for (int q = 0; q < RPT; q++)
-            if (tid + q * NT == p) taken[q] = true;
-        const float piv = tile[p * CRS + j];
-        if (piv != 0.f) {
-            const float inv = 1.f / piv;
+// Partial pivoting on the rows held in registers (own[q][c], row tid + q*NT of the set,
+// `rows` rows, w <= 32 columns).  The step loop is unrolled, so column j of a row is a
+// register; the pivot row goes through LDS once per step.  At the end every row holds
+// its L\U row as far as it was factored; pick[s] = set index of the row chosen at step s,
+// and stepof[q] = the step at which this thread's row q was chosen (-1 if never).
+template <int J, int NT, int RPT>
+__device__ __forceinline__ void pp_step(float (&own)[RPT][LPW], bool (&live)[RPT], int steps, int* pick,
+                                        float (*prow)[LPW], Best (*red)[2], int (&stepof)[RPT]) {
+    if constexpr (J < LPW) {
+        constexpr int NWV = NT / 64;
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        if (J < steps) {  // uniform
+            Best b{-1.f, 0x7fffffff};
 #pragma unroll
             for (int q = 0; q < RPT; q++) {
-                const int row = tid + q * NT;
-                if (row < rows && !taken[q]) {
-                    const float l = tile[row * CRS + j] * inv;
-                    for (int c = j + 1; c < w; c++) tile[row * CRS + c] = fmaf(-l, tile[p * CRS + c], tile[row * CRS + c]);
+                if (live[q]) {
+                    float v = fabsf(own[q][J]);
+                    if (!(v == v)) v = -0.5f;  // NaN never wins over a number
+                    if (beats(v, tid + q * NT, b.v, b.r)) { b.v = v; b.r = tid + q * NT; }
+                }
+            }
+            b = wave_best(b);
+            if (lane == 0) red[wv][J & 1] = b;
+            __syncthreads();
+            Best g = red[0][J & 1];
+#pragma unroll
+            for (int q = 1; q < NWV; q++)
+                if (beats(red[q][J & 1].v, red[q][J & 1].r, g.v, g.r)) g = red[q][J & 1];
+            const int p = g.r;
+            if (tid == 0) pick[J] = p;
+            // the owner publishes the pivot row (selected with bit masks, not a branch per row)
+            uint32_t msk[RPT];
+            uint32_t any = 0;
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                msk[q] = (tid + q * NT == p) ? ~0u : 0u;
+                any |= msk[q];
+                live[q] = live[q] && !msk[q];
+                stepof[q] = msk[q] ? J : stepof[q];
+            }
+            if (any) {
+#pragma unroll
+                for (int c = 0; c < LPW; c++) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int q = 0; q < RPT; q++) v |= __builtin_bit_cast(uint32_t, own[q][c]) & msk[q];
+                    prow[J & 1][c] = __builtin_bit_cast(float, v);
+                }
+            }
+            __syncthreads();
+            const float piv = prow[J & 1][J];
+            if (piv != 0.f) {
+                const float inv = 1.f / piv;
+                float pr[LPW];
+#pragma unroll
+                for (int c = J + 1; c < LPW; c++) pr[c] = prow[J & 1][c];
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    const float l = own[q][J] * inv;
+                    own[q][J] = live[q] ? l : own[q][J];
+                    const float lm = live[q] ? -l : 0.f;
+#pragma unroll
+                    for (int c = J + 1; c < LPW; c++) {
+                        const float u = fmaf(lm, pr[c], own[q][c]);
+                        own[q][c] = live[q] ? u : own[q][c];
+                    }
                 }
             }
         }
-        __syncthreads();
+        pp_step<J + 1, NT, RPT>(own, live, steps, pick, prow, red, stepof);
     }
+}
+
+// Partial pivoting on the rows held in registers (own[q][c], row tid + q*NT of the set,
+// `rows` rows, w <= 32 columns).  The steps are expanded at compile time, so column j of a
+// row is a register; the pivot row goes through LDS once per step.  At the end every row
+// holds its L\U row as far as it was factored; pick[s] = set index of the row chosen at
+// step s, stepof[q] = the step at which this thread's row q was chosen (-1 if never).
+template <int NT, int RPT>
+__device__ __forceinline__ void reg_partial_pivot(float (&own)[RPT][LPW], int rows, int w, int* pick,
+                                                  float (*prow)[LPW], Best (*red)[2], int (&stepof)[RPT]) {
+    bool live[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        live[q] = (int)threadIdx.x + q * NT < rows;
+        stepof[q] = -1;
+    }
+    pp_step<0, NT, RPT>(own, live, min(w, rows), pick, prow, red, stepof);
 }
 
 // round 1: every workgroup proposes min(w, its rows) pivot rows (absolute A-row indices)
 __global__ void __launch_bounds__(256) calu_local(int n, int r0, int c0, int w, const float* __restrict__ T,
                                                   size_t ldt, int* __restrict__ prop, int* __restrict__ nprop) {
-    __shared__ float tile[CRB * CRS];
+    constexpr int RPT = CRB / 256;
+    __shared__ float prow[2][LPW];
     __shared__ Best red[4][2];
     __shared__ int pick[LPW];
-    __shared__ int sel;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int rb = r0 + g * CRB;
     const int rows = min(CRB, n - rb);
-    for (int e = tid; e < rows * w; e += 256) {
-        const int row = e / w, c = e % w;
-        tile[row * CRS + c] = T[(size_t)(c0 + c) + (size_t)(rb + row) * ldt];
+    float own[RPT][LPW];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int row = tid + q * 256;
+        const float* src = T + (size_t)c0 + (size_t)(rb + min(row, rows - 1)) * ldt;
+#pragma unroll
+        for (int c = 0; c < LPW; c++) own[q][c] = (row < rows && c < w) ? src[c] : 0.f;
     }
+    int stepof[RPT];
+    reg_partial_pivot<256, RPT>(own, rows, w, pick, prow, red, stepof);
     __syncthreads();
-    lds_partial_pivot<256>(tile, rows, w, pick, red, &sel);
     const int k = min(w, rows);
     if (tid < k) prop[g * LPW + tid] = rb + pick[tid];
     if (tid == 0) nprop[g] = k;
 }
 
-// round 2: one workgroup runs partial pivoting on the proposals' original rows and
-// turns its picks into LAPACK-style interchanges ipiv[r0 .. r0+w) (absolute, 0-based).
+// round 2: one workgroup runs partial pivoting on the proposals' original rows; its picks
+// become LAPACK-style interchanges ipiv[r0 .. r0+w) (absolute, 0-based), and the picked
+// rows' L\U rows, in pick order, are the factored top block (top[s * LPW + c]).
 __global__ void __launch_bounds__(1024) calu_merge(int n, int r0, int c0, int w, const float* __restrict__ T,
                                                    size_t ldt, const int* __restrict__ prop,
                                                    const int* __restrict__ nprop, int nblk,
-                                                   int32_t* __restrict__ ipiv) {
-    __shared__ float tile[CMAX * CRS];
+                                                   int32_t* __restrict__ ipiv, float* __restrict__ top) {
     __shared__ int rowid[CMAX];
+    __shared__ float prow[2][LPW];
     __shared__ Best red[16][2];
     __shared__ int pick[LPW];
-    __shared__ int sel;
     __shared__ int total;
     __shared__ int at_row[2 * LPW], at_src[2 * LPW];  // moved positions -> original rows
     const int tid = threadIdx.x;
@@ -141,12 +186,19 @@ __global__ void __launch_bounds__(1024) calu_merge(int n, int r0, int c0, int w,
     }
     __syncthreads();
     const int rows = total;
-    for (int e = tid; e < rows * w; e += 1024) {
-        const int row = e / w, c = e % w;
-        tile[row * CRS + c] = T[(size_t)(c0 + c) + (size_t)rowid[row] * ldt];
+    float own[1][LPW];
+    {
+        const float* src = T + (size_t)c0 + (size_t)rowid[min(tid, rows - 1)] * ldt;
+#pragma unroll
+        for (int c = 0; c < LPW; c++) own[0][c] = (tid < rows && c < w) ? src[c] : 0.f;
+    }
+    int stepof[1];
+    reg_partial_pivot<1024, 1>(own, rows, w, pick, prow, red, stepof);
+    if (stepof[0] >= 0) {
+#pragma unroll
+        for (int c = 0; c < LPW; c++) top[stepof[0] * LPW + c] = own[0][c];
     }
     __syncthreads();
-    lds_partial_pivot<1024>(tile, rows, w, pick, red, &sel);
     if (tid == 0) {
         // picks -> sequential interchanges: track where each original row currently is
         const int k = min(w, rows);
@@ -200,41 +252,24 @@ __global__ void laswp_t(int n, float* __restrict__ T, size_t ldt, int k1, int k2
     }
 }
 
-// the panel without pivoting: LU of the w x w top block (every workgroup recomputes it
-// in LDS), L21 = A21 U11^-1 for the rows below.  With top_only, one workgroup stores the
-// factored top block instead (a separate launch, after every workgroup has read it).
+// the panel without pivoting, after the interchanges: L21 = A21 U11^-1 for the rows
+// below the top block (one thread per row), with L11\U11 from the merge round; workgroup
+// 0 also stores the factored top block.
 __global__ void __launch_bounds__(256) panel_nopiv(int n, int r0, int c0, int w, float* __restrict__ T, size_t ldt,
-                                                   int top_only) {
+                                                   const float* __restrict__ top) {
     __shared__ float U[LPW][LPW + 1];  // U[row][col] of the top block, A orientation
     const int tid = threadIdx.x;
     for (int e = tid; e < w * w; e += 256) {
         const int r = e / w, c = e % w;
-        U[r][c] = T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt];
+        U[r][c] = top[r * LPW + c];
     }
     __syncthreads();
-    if (tid < 64) {
-        // right-looking LU of the top block by one wavefront: lane = column
-        for (int j = 0; j < w; j++) {
-            const float piv = U[j][j];
-            const float inv = piv != 0.f ? 1.f / piv : 0.f;
-            for (int r = j + 1; r < w; r++) {
-                const float l = U[r][j] * inv;  // every lane reads the same value
-                if (tid > j && tid < w) U[r][tid] = fmaf(-l, U[j][tid], U[r][tid]);
-                __builtin_amdgcn_wave_barrier();
-                if (tid == 0) U[r][j] = l;
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-    }
-    __syncthreads();
-    if (top_only) {
+    if (blockIdx.x == 0) {
         for (int e = tid; e < w * w; e += 256) {
             const int r = e / w, c = e % w;
             T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt] = U[r][c];
         }
-        return;
     }
-    // rows below: l U11 = a (row vector), one thread per row
     const int r = r0 + w + blockIdx.x * 256 + tid;
     if (r >= n) return;
     float x[LPW];
@@ -289,7 +324,7 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
 
 size_t mixed_lu_ws_bytes(int n, int lp) {
     const size_t nblk = (size_t)(n + CRB - 1) / CRB;
-    size_t b = nblk * LPW * sizeof(int) + nblk * sizeof(int) + 1024;
+    size_t b = nblk * LPW * sizeof(int) + nblk * sizeof(int) + LPW * LPW * sizeof(float) + 2048;
     b += lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)n / 2 + 64);
     return b;
 }
@@ -305,6 +340,7 @@ struct LuCtx {
     int* perm;
     int* prop;
     int* nprop;
+    float* top;
     void* gws;
 };
 
@@ -314,19 +350,15 @@ void calu_panel(const LuCtx& c, int r0, int w) {
     hipLaunchKernelGGL(calu_local, dim3(nblk), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.prop, c.nprop);
     BOLA_LAUNCH_CHECK();
     hipLaunchKernelGGL(calu_merge, dim3(1), dim3(1024), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.prop, c.nprop, nblk,
-                       c.ipiv);
+                       c.ipiv, c.top);
     BOLA_LAUNCH_CHECK();
     const int k = std::min(w, rows);
     hipLaunchKernelGGL(laswp_t, dim3(cdiv((size_t)c.n + 1, 256)), dim3(256), 0, c.st, c.n, c.T, c.ldt, r0, r0 + k,
                        c.ipiv, c.perm);
     BOLA_LAUNCH_CHECK();
     const int below = std::max(0, rows - w);
-    if (below > 0) {
-        hipLaunchKernelGGL(panel_nopiv, dim3(cdiv((size_t)below, 256)), dim3(256), 0, c.st, c.n, r0, r0, w, c.T,
-                           c.ldt, 0);
-        BOLA_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(panel_nopiv, dim3(1), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, 1);
+    hipLaunchKernelGGL(panel_nopiv, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n, r0, r0, w,
+                       c.T, c.ldt, c.top);
     BOLA_LAUNCH_CHECK();
 }
 
@@ -381,7 +413,9 @@ void mixed_lu_factor(hipStream_t st, int lp, int n, float* T, size_t ldt, int32_
     char* b = (char*)ws;
     c.prop = (int*)b;
     c.nprop = (int*)(b + nblk * LPW * sizeof(int));
-    c.gws = b + ((nblk * LPW * sizeof(int) + nblk * sizeof(int) + 255) & ~(size_t)255);
+    const size_t o_top = (nblk * LPW * sizeof(int) + nblk * sizeof(int) + 255) & ~(size_t)255;
+    c.top = (float*)(b + o_top);
+    c.gws = b + o_top + ((LPW * LPW * sizeof(float) + 255) & ~(size_t)255);
     lu_rec(c, 0, n);
 }
 
