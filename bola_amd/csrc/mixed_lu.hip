@@ -231,6 +231,264 @@ __global__ void __launch_bounds__(1024) calu_merge(int n, int r0, int c0, int w,
     }
 }
 
+// ---- one-launch tournament: wavefront rounds and a workgroup tree, no waiting
+//
+// Round 1: every wavefront runs partial pivoting on its 128 rows (two per lane) in
+// registers and proposes its pivot rows.  Round 2: wavefront 0 of the workgroup merges
+// the four proposals (original values).  Then a tree over workgroups with fan-in 4: a
+// workgroup posts its proposal and bumps its parent's counter; the last of the siblings
+// to arrive merges them and climbs, the others leave.  The root turns its picks into
+// interchanges and writes the factored top block.  Nobody spins, so the workgroups need
+// not be resident together.
+constexpr int TR = 4;           // tree fan-in
+constexpr int WROWS = 128;      // rows per wavefront in round 1
+
+// 64-bit max over the wavefront: DPP within rows of 16 lanes, lane reads across rows;
+// the result is uniform.
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l), hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint64_t wave_max64(uint64_t k) {
+    k = umax64(k, dpp64<0xB1>(k));   // quad_perm [1,0,3,2]
+    k = umax64(k, dpp64<0x4E>(k));   // quad_perm [2,3,0,1]
+    k = umax64(k, dpp64<0x124>(k));  // row_ror 4
+    k = umax64(k, dpp64<0x128>(k));  // row_ror 8: every lane holds its row's maximum
+    return umax64(umax64(rl64(k, 0), rl64(k, 16)), umax64(rl64(k, 32), rl64(k, 48)));
+}
+// pivot key: larger |x| first, then the smaller set index; NaN above nothing, dead rows 0
+__device__ __forceinline__ uint64_t piv_key(float x, int idx, bool live) {
+    const float v = fabsf(x);
+    const uint32_t kv = !live ? 0u : (v == v ? __builtin_bit_cast(uint32_t, v) + 2u : 1u);
+    return ((uint64_t)kv << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)idx);
+}
+
+template <int J, int RPL>
+__device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RPL], int steps, int (&pick)[LPW],
+                                         int (&stepof)[RPL]) {
+    if constexpr (J < LPW) {
+        const int lane = threadIdx.x & 63;
+        if (J < steps) {
+            uint64_t k = 0;
+#pragma unroll
+            for (int q = 0; q < RPL; q++) k = umax64(k, piv_key(own[q][J], lane + q * 64, live[q]));
+            k = wave_max64(k);
+            const int p = (int)(0xFFFFFFFFu - (uint32_t)k);
+            pick[J] = p;
+            const int qp = p >> 6, lp = p & 63;
+            // the pivot row, uniform: lane lp's registers of row set qp (chosen with masks)
+            float pv[LPW];
+#pragma unroll
+            for (int c = J; c < LPW; c++) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int q = 0; q < RPL; q++) v |= __builtin_bit_cast(uint32_t, own[q][c]) & (qp == q ? ~0u : 0u);
+                pv[c] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(v, lp));
+            }
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                const bool mine = q == qp && lane == lp;
+                stepof[q] = mine ? J : stepof[q];
+                live[q] = live[q] && !mine;
+            }
+            const float piv = pv[J];
+            if (piv != 0.f) {
+                const float inv = 1.f / piv;
+#pragma unroll
+                for (int q = 0; q < RPL; q++) {
+                    const float l = own[q][J] * inv;
+                    own[q][J] = live[q] ? l : own[q][J];
+                    const float lm = live[q] ? -l : 0.f;
+#pragma unroll
+                    for (int c = J + 1; c < LPW; c++) {
+                        const float u = fmaf(lm, pv[c], own[q][c]);
+                        own[q][c] = live[q] ? u : own[q][c];
+                    }
+                }
+            }
+        }
+        wpp_step<J + 1, RPL>(own, live, steps, pick, stepof);
+    }
+}
+
+// partial pivoting of one wavefront on the rows set[0 .. rows) (absolute A-row indices),
+// read from T; picks go to out[0 .. min(w, rows)) as absolute rows.  With top != null the
+// picked rows' L\U rows are stored in pick order (the factored top block).
+template <int RPL>
+__device__ __forceinline__ int wave_tournament(const int* set, int rows, int w, int c0, const float* __restrict__ T,
+                                               size_t ldt, int* out, float* __restrict__ top) {
+    const int lane = threadIdx.x & 63;
+    float own[RPL][LPW];
+    bool live[RPL];
+    int stepof[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const int i = lane + q * 64;
+        live[q] = i < rows;
+        stepof[q] = -1;
+        const int rid = live[q] ? set[i] : c0;  // c0 = r0: a valid row for the idle lanes
+        const float* src = T + (size_t)c0 + (size_t)rid * ldt;
+#pragma unroll
+        for (int c = 0; c < LPW; c++) own[q][c] = (live[q] && c < w) ? src[c] : 0.f;
+    }
+    const int steps = min(w, rows);
+    int pk[LPW];
+    wpp_step<0, RPL>(own, live, steps, pk, stepof);
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
+    const int orow = lane < steps ? set[mine] : 0;  // pick < rows, so set[pick] was written
+    __builtin_amdgcn_wave_barrier();
+    if (lane < steps) out[lane] = orow;
+    if (top) {
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (stepof[q] >= 0)
+#pragma unroll
+                for (int c = 0; c < LPW; c++) top[stepof[q] * LPW + c] = own[q][c];
+    }
+    return steps;
+}
+
+// LAPACK-style interchanges for picks (absolute rows, in order) at rows r0.., one thread
+__device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restrict__ ipiv, int* at_row, int* at_src) {
+    int nm = 0;
+    for (int s = 0; s < k; s++) {
+        const int want = picks[s];
+        int pos = want;
+        for (int q = 0; q < nm; q++)
+            if (at_src[q] == want) { pos = at_row[q]; break; }
+        const int dst = r0 + s;
+        ipiv[dst] = pos;
+        if (pos != dst) {
+            int a = dst;
+            for (int q = 0; q < nm; q++)
+                if (at_row[q] == dst) { a = at_src[q]; break; }
+            bool f = false;
+            for (int q = 0; q < nm; q++)
+                if (at_row[q] == dst) { at_src[q] = want; f = true; break; }
+            if (!f) { at_row[nm] = dst; at_src[nm] = want; nm++; }
+            f = false;
+            for (int q = 0; q < nm; q++)
+                if (at_row[q] == pos) { at_src[q] = a; f = true; break; }
+            if (!f) { at_row[nm] = pos; at_src[nm] = a; nm++; }
+        }
+    }
+}
+
+// lists: [level][node][LPW] absolute rows and counts; cnt: [level][node] arrival counters
+// (zero on entry; the root clears them for the next panel).  G workgroups of 256 threads.
+__global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, const float* __restrict__ T,
+                                                 size_t ldt, int* __restrict__ lists, int* __restrict__ lcount,
+                                                 int* __restrict__ cnt, int lvl_stride, int32_t* __restrict__ ipiv,
+                                                 float* __restrict__ top) {
+    __shared__ int wlist[4][LPW];
+    __shared__ int wcnt[4];
+    __shared__ int merged[TR * LPW];
+    __shared__ int mylist[LPW];
+    __shared__ int at_row[2 * LPW], at_src[2 * LPW];
+    __shared__ int go;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int G = gridDim.x;
+    // round 1: 128 rows per wavefront
+    {
+        const int rb = r0 + (blockIdx.x * 4 + wv) * WROWS;
+        const int rows = max(0, min(WROWS, n - rb));
+        int* set = merged + wv * LPW;  // scratch for the row ids of this wavefront is implicit
+        (void)set;
+        float own[2][LPW];
+        bool live[2];
+        int stepof[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int i = lane + q * 64;
+            live[q] = i < rows;
+            stepof[q] = -1;
+            const float* src = T + (size_t)c0 + (size_t)(live[q] ? rb + i : r0) * ldt;
+#pragma unroll
+            for (int c = 0; c < LPW; c++) own[q][c] = (live[q] && c < w) ? src[c] : 0.f;
+        }
+        const int steps = min(w, rows);
+        int pk[LPW];
+        wpp_step<0, 2>(own, live, steps, pk, stepof);
+        int mine = 0;
+#pragma unroll
+        for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
+        if (lane < steps) wlist[wv][lane] = rb + mine;
+        if (lane == 0) wcnt[wv] = steps;
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    // round 2: the workgroup's four proposals
+    int k;
+    {
+        int tot = 0;
+        for (int q = 0; q < 4; q++) {
+            if (lane < wcnt[q]) merged[tot + lane] = wlist[q][lane];
+            tot += wcnt[q];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const bool root = G == 1;
+        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr);
+        if (root) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) picks_to_ipiv(mylist, k, r0, ipiv, at_row, at_src);
+            return;
+        }
+    }
+    // the tree
+    int node = blockIdx.x, nodes = G;
+    for (int lvl = 0;; lvl++) {
+        int* L = lists + ((size_t)lvl * lvl_stride + node) * LPW;
+        if (lane < k) L[lane] = mylist[lane];
+        if (lane == 0) lcount[lvl * lvl_stride + node] = k;
+        const int parent = node / TR;
+        const int first = parent * TR, nchild = min(TR, nodes - first);
+        __builtin_amdgcn_s_waitcnt(0);
+        if (lane == 0) {
+            const int old = __hip_atomic_fetch_add(cnt + lvl * lvl_stride + parent, 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            go = old == nchild - 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (!go) return;
+        // last to arrive: merge the siblings' proposals
+        int tot = 0;
+        for (int ch = 0; ch < nchild; ch++) {
+            const int cc = __hip_atomic_load(lcount + lvl * lvl_stride + first + ch, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            const int* CL = lists + ((size_t)lvl * lvl_stride + first + ch) * LPW;
+            if (lane < cc) merged[tot + lane] = __hip_atomic_load(CL + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tot += cc;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int pnodes = (nodes + TR - 1) / TR;
+        const bool root = pnodes == 1;
+        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr);
+        node = parent;
+        nodes = pnodes;
+        if (root) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) picks_to_ipiv(mylist, k, r0, ipiv, at_row, at_src);
+            // clear the counters for the next panel
+            for (int l2 = 0; l2 <= lvl; l2++)
+                for (int e = lane; e < lvl_stride; e += 64) cnt[l2 * lvl_stride + e] = 0;
+            return;
+        }
+    }
+}
+
+__global__ void k_zero_cnt(int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // interchanges ipiv[k1 .. k2) applied to whole rows (all columns of A = all rows of T)
 // and to the row-origin vector perm; one thread per column of A.
 __global__ void laswp_t(int n, float* __restrict__ T, size_t ldt, int k1, int k2, const int32_t* __restrict__ ipiv,
@@ -323,8 +581,8 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
 }  // namespace
 
 size_t mixed_lu_ws_bytes(int n, int lp) {
-    const size_t nblk = (size_t)(n + CRB - 1) / CRB;
-    size_t b = nblk * LPW * sizeof(int) + nblk * sizeof(int) + LPW * LPW * sizeof(float) + 2048;
+    const size_t G = (size_t)(n + 4 * WROWS - 1) / (4 * WROWS);
+    size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
     b += lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)n / 2 + 64);
     return b;
 }
@@ -338,19 +596,19 @@ struct LuCtx {
     size_t ldt;
     int32_t* ipiv;
     int* perm;
-    int* prop;
-    int* nprop;
+    int* lists;
+    int* lcount;
+    int* cnt;
+    int lvl_stride;
     float* top;
     void* gws;
 };
 
 void calu_panel(const LuCtx& c, int r0, int w) {
     const int rows = c.n - r0;
-    const int nblk = (rows + CRB - 1) / CRB;
-    hipLaunchKernelGGL(calu_local, dim3(nblk), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.prop, c.nprop);
-    BOLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(calu_merge, dim3(1), dim3(1024), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.prop, c.nprop, nblk,
-                       c.ipiv, c.top);
+    const int G = (rows + 4 * WROWS - 1) / (4 * WROWS);
+    hipLaunchKernelGGL(calu_tree, dim3(G), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.lists, c.lcount, c.cnt,
+                       c.lvl_stride, c.ipiv, c.top);
     BOLA_LAUNCH_CHECK();
     const int k = std::min(w, rows);
     hipLaunchKernelGGL(laswp_t, dim3(cdiv((size_t)c.n + 1, 256)), dim3(256), 0, c.st, c.n, c.T, c.ldt, r0, r0 + k,
@@ -409,16 +667,25 @@ void mixed_lu_factor(hipStream_t st, int lp, int n, float* T, size_t ldt, int32_
     c.ldt = ldt;
     c.ipiv = ipiv;
     c.perm = perm;
-    const size_t nblk = (size_t)(n + CRB - 1) / CRB;
+    const int G = (n + 4 * WROWS - 1) / (4 * WROWS);
+    const int levels = 8;
+    c.lvl_stride = G;
     char* b = (char*)ws;
-    c.prop = (int*)b;
-    c.nprop = (int*)(b + nblk * LPW * sizeof(int));
-    const size_t o_top = (nblk * LPW * sizeof(int) + nblk * sizeof(int) + 255) & ~(size_t)255;
-    c.top = (float*)(b + o_top);
-    c.gws = b + o_top + ((LPW * LPW * sizeof(float) + 255) & ~(size_t)255);
+    size_t o = 0;
+    c.lists = (int*)(b + o);
+    o += ((size_t)levels * G * LPW * sizeof(int) + 255) & ~(size_t)255;
+    c.lcount = (int*)(b + o);
+    o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
+    c.cnt = (int*)(b + o);
+    o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
+    c.top = (float*)(b + o);
+    o += (LPW * LPW * sizeof(float) + 255) & ~(size_t)255;
+    c.gws = b + o;
+    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, st, c.cnt, levels * G);
+    BOLA_LAUNCH_CHECK();
     lu_rec(c, 0, n);
 }
 
-int mixed_lu_max_n() { return CMAX / LPW * CRB; }
+int mixed_lu_max_n() { return 1 << 17; }
 
 }  // namespace bola
