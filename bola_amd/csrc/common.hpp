@@ -141,6 +141,16 @@ void collect_profile(bola_hip_handle_s* h);
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// The i-th ordering event of a handle (created on first use, timing disabled).
+inline hipEvent_t handle_event(bola_hip_handle_s* h, int i) {
+    while ((int)h->sync_events.size() <= i) {
+        hipEvent_t e;
+        BOLA_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->sync_events.push_back(e);
+    }
+    return h->sync_events[i];
+}
+
 // ---------------------------------------------------------------- launchers
 // (defined in the .hip translation units; all enqueue on `st`)
 

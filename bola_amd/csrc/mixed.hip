@@ -660,7 +660,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
         hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, st, n, S.perm);
         BOLA_LAUNCH_CHECK();
         // 2. the low-precision LU
-        mixed_lu_factor(st, precision, n, S.T, S.ldt, S.ipiv, S.perm, S.luws);
+        mixed_lu_factor(h, precision, n, S.T, S.ldt, S.ipiv, S.perm, S.luws);
     }
     BOLA_CHECK(hipEventRecord(ev[1], st));
     // |A|_inf and |b|_inf

@@ -25,6 +25,7 @@ namespace bola {
 namespace {
 
 constexpr int LPW = 32;        // leaf panel width
+constexpr int kMixNB = 256;    // block width of the right-looking factorization
 constexpr int CRB = 512;       // rows per workgroup in the local tournament round
 constexpr int CMAX = 1024;     // proposals one merge workgroup holds
 constexpr int CRS = LPW + 1;   // LDS row stride (floats)
@@ -489,16 +490,16 @@ __global__ void k_zero_cnt(int* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// interchanges ipiv[k1 .. k2) applied to whole rows (all columns of A = all rows of T)
-// and to the row-origin vector perm; one thread per column of A.
-__global__ void laswp_t(int n, float* __restrict__ T, size_t ldt, int k1, int k2, const int32_t* __restrict__ ipiv,
-                        int* __restrict__ perm) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > n) return;
+// interchanges ipiv[k1 .. k2) applied to the columns [cl, ch) of A (rows of T) and, with
+// perm != null, to the row-origin vector; one thread per column of A.
+__global__ void laswp_t(int cl, int ch, float* __restrict__ T, size_t ldt, int k1, int k2,
+                        const int32_t* __restrict__ ipiv, int* __restrict__ perm) {
+    const int c = cl + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ch || (c == ch && !perm)) return;
     for (int i = k1; i < k2; i++) {
         const int p = ipiv[i];
         if (p == i) continue;
-        if (c < n) {
+        if (c < ch) {
             const float t = T[(size_t)c + (size_t)i * ldt];
             T[(size_t)c + (size_t)i * ldt] = T[(size_t)c + (size_t)p * ldt];
             T[(size_t)c + (size_t)p * ldt] = t;
@@ -580,17 +581,20 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
 
 }  // namespace
 
-size_t mixed_lu_ws_bytes(int n, int lp) {
-    const size_t G = (size_t)(n + 4 * WROWS - 1) / (4 * WROWS);
-    size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
-    b += lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)n / 2 + 64);
-    return b;
-}
 
 namespace {
 
+void launch_laswp_t(hipStream_t st, int cl, int ch, float* T, size_t ldt, int k1, int k2, const int32_t* ipiv,
+                    int* perm) {
+    if (ch <= cl && !perm) return;
+    hipLaunchKernelGGL(laswp_t, dim3(cdiv((size_t)(ch - cl) + 1, 256)), dim3(256), 0, st, cl, ch, T, ldt, k1, k2, ipiv,
+                       perm);
+    BOLA_LAUNCH_CHECK();
+}
+
 struct LuCtx {
     hipStream_t st;
+    int swap_lo, swap_hi;  // columns of A that take a leaf's interchanges at once
     int n, lp;
     float* T;
     size_t ldt;
@@ -611,9 +615,7 @@ void calu_panel(const LuCtx& c, int r0, int w) {
                        c.lvl_stride, c.ipiv, c.top);
     BOLA_LAUNCH_CHECK();
     const int k = std::min(w, rows);
-    hipLaunchKernelGGL(laswp_t, dim3(cdiv((size_t)c.n + 1, 256)), dim3(256), 0, c.st, c.n, c.T, c.ldt, r0, r0 + k,
-                       c.ipiv, c.perm);
-    BOLA_LAUNCH_CHECK();
+    launch_laswp_t(c.st, c.swap_lo, c.swap_hi, c.T, c.ldt, r0, r0 + k, c.ipiv, c.perm);
     const int below = std::max(0, rows - w);
     hipLaunchKernelGGL(panel_nopiv, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n, r0, r0, w,
                        c.T, c.ldt, c.top);
@@ -656,11 +658,33 @@ void lu_rec(const LuCtx& c, int r0, int w) {
     lu_rec(c, r0 + w1, nr);
 }
 
+// U12 and the trailing update of panel [k, k+nb) on the columns [c_lo, c_hi) of A
+void update_cols(const LuCtx& c, int k, int nb, int c_lo, int c_hi) {
+    const int ncols = c_hi - c_lo;
+    if (ncols <= 0) return;
+    strsm_rupu(c, ncols, nb, k, k, c_lo);
+    const int below = c.n - k - nb;
+    if (below > 0)
+        launch_lp_gemm(c.st, c.lp, ncols, below, nb, Tp(c, c_lo, k), c.ldt, Tp(c, k, k + nb), c.ldt,
+                       Tp(c, c_lo, k + nb), c.ldt, c.gws);
+}
+
 }  // namespace
 
-void mixed_lu_factor(hipStream_t st, int lp, int n, float* T, size_t ldt, int32_t* ipiv, int* perm, void* ws) {
+size_t mixed_lu_ws_bytes(int n, int lp) {
+    const size_t G = (size_t)(n + 4 * WROWS - 1) / (4 * WROWS);
+    size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
+    b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 256);
+    return b;
+}
+
+// Right-looking blocked LU, panels of kMixNB columns factored recursively (tournament
+// leaves), with look-ahead: stream 1 factors panel k, applies its interchanges and
+// updates the next panel's columns; stream 2 meanwhile applies panel k to all columns
+// further right.  The streams never work on the same columns at the same time.
+void mixed_lu_factor(bola_hip_handle_s* h, int lp, int n, float* T, size_t ldt, int32_t* ipiv, int* perm, void* ws) {
     LuCtx c;
-    c.st = st;
+    c.st = h->stream;
     c.n = n;
     c.lp = lp;
     c.T = T;
@@ -680,10 +704,40 @@ void mixed_lu_factor(hipStream_t st, int lp, int n, float* T, size_t ldt, int32_
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
     c.top = (float*)(b + o);
     o += (LPW * LPW * sizeof(float) + 255) & ~(size_t)255;
+    const size_t gsz = (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 511) & ~(size_t)255;
     c.gws = b + o;
-    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, st, c.cnt, levels * G);
+    LuCtx c2 = c;  // stream 2: its own 16-bit operand buffers
+    c2.st = h->aux_stream();
+    c2.gws = b + o + gsz;
+    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, c.st, c.cnt, levels * G);
     BOLA_LAUNCH_CHECK();
-    lu_rec(c, 0, n);
+    int ev = 0;
+    hipEvent_t bulk_done = nullptr;
+    for (int k = 0; k < n; k += kMixNB) {
+        const int nb = std::min(kMixNB, n - k);
+        // the panel: its own columns take the interchanges at once
+        c.swap_lo = k;
+        c.swap_hi = k + nb;
+        lu_rec(c, k, nb);
+        // the columns right of the panel are stream 2's until its previous bulk update is done
+        if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
+        launch_laswp_t(c.st, 0, k, T, ldt, k, k + nb, ipiv, nullptr);
+        launch_laswp_t(c.st, k + nb, n, T, ldt, k, k + nb, ipiv, nullptr);
+        // look-ahead: the next panel's columns
+        const int c_mid = std::min(n, k + nb + kMixNB);
+        update_cols(c, k, nb, k + nb, c_mid);
+        if (c_mid < n) {
+            hipEvent_t la = handle_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(la, c.st));
+            BOLA_CHECK(hipStreamWaitEvent(c2.st, la, 0));
+            update_cols(c2, k, nb, c_mid, n);
+            bulk_done = handle_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(bulk_done, c2.st));
+        } else {
+            bulk_done = nullptr;
+        }
+    }
+    if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
 }
 
 int mixed_lu_max_n() { return 1 << 17; }
