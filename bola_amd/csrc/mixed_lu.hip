@@ -271,9 +271,13 @@ __device__ __forceinline__ uint64_t piv_key(float x, int idx, bool live) {
     return ((uint64_t)kv << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)idx);
 }
 
-template <int J, int RPL>
+// One step of a wavefront's partial pivoting: DPP argmax, the pivot row through LDS (the
+// owner lane writes it, everybody reads it back), the rank-1 update in registers.  KEEP:
+// rows already chosen keep their values (their L\U rows are needed afterwards); without
+// it they are left to change, since only the chosen indices matter.
+template <int J, int RPL, bool KEEP>
 __device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RPL], int steps, int (&pick)[LPW],
-                                         int (&stepof)[RPL]) {
+                                         int (&stepof)[RPL], float* buf) {
     if constexpr (J < LPW) {
         const int lane = threadIdx.x & 63;
         if (J < steps) {
@@ -284,14 +288,17 @@ __device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RP
             const int p = (int)(0xFFFFFFFFu - (uint32_t)k);
             pick[J] = p;
             const int qp = p >> 6, lp = p & 63;
-            // the pivot row, uniform: lane lp's registers of row set qp (chosen with masks)
-            float pv[LPW];
+            float* row = buf + (J & 1) * LPW;
+            if (lane == lp) {
+                if (qp == 0) {
 #pragma unroll
-            for (int c = J; c < LPW; c++) {
-                uint32_t v = 0;
+                    for (int c = J; c < LPW; c++) row[c] = own[0][c];
+                } else {
+                    if constexpr (RPL > 1) {
 #pragma unroll
-                for (int q = 0; q < RPL; q++) v |= __builtin_bit_cast(uint32_t, own[q][c]) & (qp == q ? ~0u : 0u);
-                pv[c] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(v, lp));
+                        for (int c = J; c < LPW; c++) row[c] = own[RPL - 1][c];
+                    }
+                }
             }
 #pragma unroll
             for (int q = 0; q < RPL; q++) {
@@ -299,23 +306,32 @@ __device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RP
                 stepof[q] = mine ? J : stepof[q];
                 live[q] = live[q] && !mine;
             }
+            __builtin_amdgcn_wave_barrier();
+            float pv[LPW];
+#pragma unroll
+            for (int c = J; c < LPW; c++) pv[c] = row[c];
             const float piv = pv[J];
             if (piv != 0.f) {
                 const float inv = 1.f / piv;
 #pragma unroll
                 for (int q = 0; q < RPL; q++) {
                     const float l = own[q][J] * inv;
-                    own[q][J] = live[q] ? l : own[q][J];
                     const float lm = live[q] ? -l : 0.f;
+                    if (KEEP) {
+                        own[q][J] = live[q] ? l : own[q][J];
 #pragma unroll
-                    for (int c = J + 1; c < LPW; c++) {
-                        const float u = fmaf(lm, pv[c], own[q][c]);
-                        own[q][c] = live[q] ? u : own[q][c];
+                        for (int c = J + 1; c < LPW; c++) {
+                            const float u = fmaf(lm, pv[c], own[q][c]);
+                            own[q][c] = live[q] ? u : own[q][c];
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = J + 1; c < LPW; c++) own[q][c] = fmaf(lm, pv[c], own[q][c]);
                     }
                 }
             }
         }
-        wpp_step<J + 1, RPL>(own, live, steps, pick, stepof);
+        wpp_step<J + 1, RPL, KEEP>(own, live, steps, pick, stepof, buf);
     }
 }
 
@@ -324,7 +340,7 @@ __device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RP
 // picked rows' L\U rows are stored in pick order (the factored top block).
 template <int RPL>
 __device__ __forceinline__ int wave_tournament(const int* set, int rows, int w, int c0, const float* __restrict__ T,
-                                               size_t ldt, int* out, float* __restrict__ top) {
+                                               size_t ldt, int* out, float* __restrict__ top, float* buf) {
     const int lane = threadIdx.x & 63;
     float own[RPL][LPW];
     bool live[RPL];
@@ -341,7 +357,8 @@ __device__ __forceinline__ int wave_tournament(const int* set, int rows, int w, 
     }
     const int steps = min(w, rows);
     int pk[LPW];
-    wpp_step<0, RPL>(own, live, steps, pk, stepof);
+    if (top) wpp_step<0, RPL, true>(own, live, steps, pk, stepof, buf);
+    else wpp_step<0, RPL, false>(own, live, steps, pk, stepof, buf);
     int mine = 0;
 #pragma unroll
     for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
@@ -358,29 +375,16 @@ __device__ __forceinline__ int wave_tournament(const int* set, int rows, int w, 
     return steps;
 }
 
-// LAPACK-style interchanges for picks (absolute rows, in order) at rows r0.., one thread
-__device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restrict__ ipiv, int* at_row, int* at_src) {
-    int nm = 0;
-    for (int s = 0; s < k; s++) {
-        const int want = picks[s];
-        int pos = want;
-        for (int q = 0; q < nm; q++)
-            if (at_src[q] == want) { pos = at_row[q]; break; }
-        const int dst = r0 + s;
-        ipiv[dst] = pos;
-        if (pos != dst) {
-            int a = dst;
-            for (int q = 0; q < nm; q++)
-                if (at_row[q] == dst) { a = at_src[q]; break; }
-            bool f = false;
-            for (int q = 0; q < nm; q++)
-                if (at_row[q] == dst) { at_src[q] = want; f = true; break; }
-            if (!f) { at_row[nm] = dst; at_src[nm] = want; nm++; }
-            f = false;
-            for (int q = 0; q < nm; q++)
-                if (at_row[q] == pos) { at_src[q] = a; f = true; break; }
-            if (!f) { at_row[nm] = pos; at_src[nm] = a; nm++; }
-        }
+// LAPACK-style interchanges for picks (absolute rows, in order) at rows r0..: lane s
+// follows the current position of pick s through the sequence of swaps.
+__device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restrict__ ipiv) {
+    const int lane = threadIdx.x & 63;
+    int cur = lane < k ? picks[lane] : -1;
+    for (int t = 0; t < k; t++) {
+        const int pos = __builtin_amdgcn_readlane(cur, t);
+        const int dst = r0 + t;
+        if (lane == 0) ipiv[dst] = pos;
+        cur = cur == dst ? pos : (cur == pos ? dst : cur);
     }
 }
 
@@ -394,7 +398,7 @@ __global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, c
     __shared__ int wcnt[4];
     __shared__ int merged[TR * LPW];
     __shared__ int mylist[LPW];
-    __shared__ int at_row[2 * LPW], at_src[2 * LPW];
+    __shared__ float rowbuf[4][2 * LPW];
     __shared__ int go;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int G = gridDim.x;
@@ -418,7 +422,7 @@ __global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, c
         }
         const int steps = min(w, rows);
         int pk[LPW];
-        wpp_step<0, 2>(own, live, steps, pk, stepof);
+        wpp_step<0, 2, false>(own, live, steps, pk, stepof, rowbuf[wv]);
         int mine = 0;
 #pragma unroll
         for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
@@ -437,10 +441,10 @@ __global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, c
         }
         __builtin_amdgcn_wave_barrier();
         const bool root = G == 1;
-        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr);
+        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr, rowbuf[0]);
         if (root) {
             __builtin_amdgcn_wave_barrier();
-            if (lane == 0) picks_to_ipiv(mylist, k, r0, ipiv, at_row, at_src);
+            picks_to_ipiv(mylist, k, r0, ipiv);
             return;
         }
     }
@@ -472,12 +476,12 @@ __global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, c
         __builtin_amdgcn_wave_barrier();
         const int pnodes = (nodes + TR - 1) / TR;
         const bool root = pnodes == 1;
-        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr);
+        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr, rowbuf[0]);
         node = parent;
         nodes = pnodes;
         if (root) {
             __builtin_amdgcn_wave_barrier();
-            if (lane == 0) picks_to_ipiv(mylist, k, r0, ipiv, at_row, at_src);
+            picks_to_ipiv(mylist, k, r0, ipiv);
             // clear the counters for the next panel
             for (int l2 = 0; l2 <= lvl; l2++)
                 for (int e = lane; e < lvl_stride; e += 64) cnt[l2 * lvl_stride + e] = 0;
@@ -511,72 +515,104 @@ __global__ void laswp_t(int cl, int ch, float* __restrict__ T, size_t ldt, int k
     }
 }
 
+// Ui = U^-1 for the w x w upper triangle U (unit: ones on the diagonal, not read),
+// both in LDS; one wavefront, lane b computes column b by back substitution.  A zero
+// diagonal entry is taken as one (the factorization is then useless and the refinement
+// will say so).
+__device__ void upper_inverse_lds(const float (*U)[LPW + 1], float (*Ui)[LPW + 1], int w, bool unit) {
+    const int b = threadIdx.x;
+    if (b >= 64) return;
+    float x[LPW];
+#pragma unroll
+    for (int a = LPW - 1; a >= 0; a--) {
+        float v = (a == b) ? 1.f : 0.f;
+#pragma unroll
+        for (int c = a + 1; c < LPW; c++) v = fmaf(-U[a][c], x[c], v);
+        asm volatile("" ::: "memory");
+        const float d = unit ? 1.f : U[a][a];
+        x[a] = (a < w && b < w && a <= b) ? (d != 0.f ? v / d : v) : 0.f;
+    }
+    if (b < LPW)
+#pragma unroll
+        for (int a = 0; a < LPW; a++) Ui[a][b] = x[a];
+}
+
 // the panel without pivoting, after the interchanges: L21 = A21 U11^-1 for the rows
-// below the top block (one thread per row), with L11\U11 from the merge round; workgroup
-// 0 also stores the factored top block.
+// below the top block (one thread per row, a product with the inverse), with L11\U11
+// from the merge round; workgroup 0 also stores the factored top block.
+template <bool FULL>  // FULL: w == 32, loads and stores without tests
 __global__ void __launch_bounds__(256) panel_nopiv(int n, int r0, int c0, int w, float* __restrict__ T, size_t ldt,
                                                    const float* __restrict__ top) {
     __shared__ float U[LPW][LPW + 1];  // U[row][col] of the top block, A orientation
+    __shared__ float Ui[LPW][LPW + 1];
     const int tid = threadIdx.x;
-    for (int e = tid; e < w * w; e += 256) {
-        const int r = e / w, c = e % w;
-        U[r][c] = top[r * LPW + c];
+    for (int e = tid; e < LPW * LPW; e += 256) {
+        const int r = e / LPW, c = e % LPW;
+        U[r][c] = (r < w && c < w) ? top[r * LPW + c] : 0.f;
     }
     __syncthreads();
+    upper_inverse_lds(U, Ui, w, false);
     if (blockIdx.x == 0) {
         for (int e = tid; e < w * w; e += 256) {
             const int r = e / w, c = e % w;
             T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt] = U[r][c];
         }
     }
+    __syncthreads();
     const int r = r0 + w + blockIdx.x * 256 + tid;
     if (r >= n) return;
-    float x[LPW];
+    float x[LPW], l[LPW];
     float* col = T + (size_t)c0 + (size_t)r * ldt;
 #pragma unroll
-    for (int b = 0; b < LPW; b++) x[b] = b < w ? col[b] : 0.f;
-#pragma unroll
     for (int b = 0; b < LPW; b++) {
-        if (b < w) {
-            float v = x[b];
+        x[b] = (FULL || b < w) ? col[b] : 0.f;
+        l[b] = 0.f;
+    }
 #pragma unroll
-            for (int a = 0; a < b; a++) v = fmaf(-x[a], U[a][b], v);
-            x[b] = U[b][b] != 0.f ? v / U[b][b] : v;
-        }
+    for (int a = 0; a < LPW; a++) {
+#pragma unroll
+        for (int b = a; b < LPW; b++) l[b] = fmaf(x[a], Ui[a][b], l[b]);
+        asm volatile("" ::: "memory");  // keeps the Ui reads row by row (registers)
     }
 #pragma unroll
     for (int b = 0; b < LPW; b++)
-        if (b < w) col[b] = x[b];
+        if (FULL || b < w) col[b] = l[b];
 }
 
 // X U = B for unit upper U (w <= 32): X, B the nr x w column-major block at X (ld), U the
-// w x w block at U (ld) (its strictly upper part).  One thread per row.
+// w x w block at U (ld) (its strictly upper part).  X = B U^-1 with the inverse formed in
+// LDS; one thread per row.
+template <bool FULL>  // FULL: w == 32, loads and stores without tests
 __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const float* __restrict__ Ub, float* __restrict__ X,
                                                        size_t ld) {
     __shared__ float U[LPW][LPW + 1];
+    __shared__ float Ui[LPW][LPW + 1];
     const int tid = threadIdx.x;
-    for (int e = tid; e < w * w; e += 256) {
-        const int a = e % w, b = e / w;
-        U[a][b] = Ub[(size_t)a + (size_t)b * ld];
+    for (int e = tid; e < LPW * LPW; e += 256) {
+        const int a = e % LPW, b = e / LPW;
+        U[a][b] = (a < w && b < w) ? Ub[(size_t)a + (size_t)b * ld] : 0.f;
     }
+    __syncthreads();
+    upper_inverse_lds(U, Ui, w, true);
     __syncthreads();
     const int row = blockIdx.x * 256 + tid;
     if (row >= nr) return;
-    float x[LPW];
+    float* xr = X + row;  // element b at xr[b * ld]: a per-lane base and a uniform offset
+    float x[LPW], y[LPW];
 #pragma unroll
-    for (int b = 0; b < LPW; b++) x[b] = b < w ? X[(size_t)row + (size_t)b * ld] : 0.f;
+    for (int b = 0; b < LPW; b++) {
+        x[b] = (FULL || b < w) ? xr[(size_t)b * ld] : 0.f;
+        y[b] = 0.f;
+    }
 #pragma unroll
-    for (int b = 1; b < LPW; b++) {
-        if (b < w) {
-            float v = x[b];
+    for (int a = 0; a < LPW; a++) {
 #pragma unroll
-            for (int a = 0; a < b; a++) v = fmaf(-x[a], U[a][b], v);
-            x[b] = v;
-        }
+        for (int b = a; b < LPW; b++) y[b] = fmaf(x[a], Ui[a][b], y[b]);
+        asm volatile("" ::: "memory");  // keeps the Ui reads row by row (registers)
     }
 #pragma unroll
     for (int b = 0; b < LPW; b++)
-        if (b < w) X[(size_t)row + (size_t)b * ld] = x[b];
+        if (FULL || b < w) xr[(size_t)b * ld] = y[b];
 }
 
 }  // namespace
@@ -617,8 +653,12 @@ void calu_panel(const LuCtx& c, int r0, int w) {
     const int k = std::min(w, rows);
     launch_laswp_t(c.st, c.swap_lo, c.swap_hi, c.T, c.ldt, r0, r0 + k, c.ipiv, c.perm);
     const int below = std::max(0, rows - w);
-    hipLaunchKernelGGL(panel_nopiv, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n, r0, r0, w,
-                       c.T, c.ldt, c.top);
+    if (w == LPW)
+        hipLaunchKernelGGL(panel_nopiv<true>, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n, r0,
+                           r0, w, c.T, c.ldt, c.top);
+    else
+        hipLaunchKernelGGL(panel_nopiv<false>, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n,
+                           r0, r0, w, c.T, c.ldt, c.top);
     BOLA_LAUNCH_CHECK();
 }
 
@@ -628,15 +668,19 @@ float* Tp(const LuCtx& c, int col, int row) { return c.T + (size_t)col + (size_t
 void strsm_rupu(const LuCtx& c, int nr, int w, int uc, int ur, int xc) {
     if (nr <= 0 || w <= 0) return;
     if (w <= LPW) {
-        hipLaunchKernelGGL(strsm_rupu_base, dim3(cdiv((size_t)nr, 256)), dim3(256), 0, c.st, nr, w, Tp(c, uc, ur),
-                           Tp(c, xc, ur), c.ldt);
+        if (w == LPW)
+            hipLaunchKernelGGL(strsm_rupu_base<true>, dim3(cdiv((size_t)nr, 256)), dim3(256), 0, c.st, nr, w,
+                               Tp(c, uc, ur), Tp(c, xc, ur), c.ldt);
+        else
+            hipLaunchKernelGGL(strsm_rupu_base<false>, dim3(cdiv((size_t)nr, 256)), dim3(256), 0, c.st, nr, w,
+                               Tp(c, uc, ur), Tp(c, xc, ur), c.ldt);
         BOLA_LAUNCH_CHECK();
         return;
     }
     const int h1 = ((w / 2) + LPW - 1) / LPW * LPW, h2 = w - h1;
     strsm_rupu(c, nr, h1, uc, ur, xc);
-    // B2 -= X1 * U12 : (nr x h2) -= (nr x h1)(h1 x h2)
-    launch_lp_gemm(c.st, c.lp, nr, h2, h1, Tp(c, xc, ur), c.ldt, Tp(c, uc, ur + h1), c.ldt, Tp(c, xc, ur + h1),
+    // B2 -= X1 * U12 : (nr x h2) -= (nr x h1)(h1 x h2), in fp32 (small)
+    launch_lp_gemm(c.st, LP_F32, nr, h2, h1, Tp(c, xc, ur), c.ldt, Tp(c, uc, ur + h1), c.ldt, Tp(c, xc, ur + h1),
                    c.ldt, c.gws);
     strsm_rupu(c, nr, h2, uc + h1, ur + h1, xc);
 }
@@ -652,8 +696,8 @@ void lu_rec(const LuCtx& c, int r0, int w) {
     strsm_rupu(c, nr, w1, r0, r0, r0 + w1);
     // A22 -= A21 A12
     const int below = c.n - r0 - w1;
-    if (below > 0)
-        launch_lp_gemm(c.st, c.lp, nr, below, w1, Tp(c, r0 + w1, r0), c.ldt, Tp(c, r0, r0 + w1), c.ldt,
+    if (below > 0)  // inside a panel: narrow, kept in fp32
+        launch_lp_gemm(c.st, LP_F32, nr, below, w1, Tp(c, r0 + w1, r0), c.ldt, Tp(c, r0, r0 + w1), c.ldt,
                        Tp(c, r0 + w1, r0 + w1), c.ldt, c.gws);
     lu_rec(c, r0 + w1, nr);
 }
