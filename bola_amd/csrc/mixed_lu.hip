@@ -26,223 +26,6 @@ namespace {
 
 constexpr int LPW = 32;        // leaf panel width
 constexpr int kMixNB = 256;    // block width of the right-looking factorization
-constexpr int CRB = 512;       // rows per workgroup in the local tournament round
-constexpr int CMAX = 1024;     // proposals one merge workgroup holds
-constexpr int CRS = LPW + 1;   // LDS row stride (floats)
-
-struct Best {
-    float v;
-    int r;
-};
-__device__ __forceinline__ bool beats(float v, int r, float bv, int br) {
-    return v > bv || (v == bv && r < br);
-}
-__device__ __forceinline__ Best wave_best(Best b) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const float ov = __shfl_xor(b.v, off, 64);
-        const int orr = __shfl_xor(b.r, off, 64);
-        if (beats(ov, orr, b.v, b.r)) { b.v = ov; b.r = orr; }
-    }
-    return b;
-}
-
-// Partial pivoting on the rows held in registers (own[q][c], row tid + q*NT of the set,
-// `rows` rows, w <= 32 columns).  The step loop is unrolled, so column j of a row is a
-// register; the pivot row goes through LDS once per step.  At the end every row holds
-// its L\U row as far as it was factored; pick[s] = set index of the row chosen at step s,
-// and stepof[q] = the step at which this thread's row q was chosen (-1 if never).
-template <int J, int NT, int RPT>
-__device__ __forceinline__ void pp_step(float (&own)[RPT][LPW], bool (&live)[RPT], int steps, int* pick,
-                                        float (*prow)[LPW], Best (*red)[2], int (&stepof)[RPT]) {
-    if constexpr (J < LPW) {
-        constexpr int NWV = NT / 64;
-        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-        if (J < steps) {  // uniform
-            Best b{-1.f, 0x7fffffff};
-#pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                if (live[q]) {
-                    float v = fabsf(own[q][J]);
-                    if (!(v == v)) v = -0.5f;  // NaN never wins over a number
-                    if (beats(v, tid + q * NT, b.v, b.r)) { b.v = v; b.r = tid + q * NT; }
-                }
-            }
-            b = wave_best(b);
-            if (lane == 0) red[wv][J & 1] = b;
-            __syncthreads();
-            Best g = red[0][J & 1];
-#pragma unroll
-            for (int q = 1; q < NWV; q++)
-                if (beats(red[q][J & 1].v, red[q][J & 1].r, g.v, g.r)) g = red[q][J & 1];
-            const int p = g.r;
-            if (tid == 0) pick[J] = p;
-            // the owner publishes the pivot row (selected with bit masks, not a branch per row)
-            uint32_t msk[RPT];
-            uint32_t any = 0;
-#pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                msk[q] = (tid + q * NT == p) ? ~0u : 0u;
-                any |= msk[q];
-                live[q] = live[q] && !msk[q];
-                stepof[q] = msk[q] ? J : stepof[q];
-            }
-            if (any) {
-#pragma unroll
-                for (int c = 0; c < LPW; c++) {
-                    uint32_t v = 0;
-#pragma unroll
-                    for (int q = 0; q < RPT; q++) v |= __builtin_bit_cast(uint32_t, own[q][c]) & msk[q];
-                    prow[J & 1][c] = __builtin_bit_cast(float, v);
-                }
-            }
-            __syncthreads();
-            const float piv = prow[J & 1][J];
-            if (piv != 0.f) {
-                const float inv = 1.f / piv;
-                float pr[LPW];
-#pragma unroll
-                for (int c = J + 1; c < LPW; c++) pr[c] = prow[J & 1][c];
-#pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    const float l = own[q][J] * inv;
-                    own[q][J] = live[q] ? l : own[q][J];
-                    const float lm = live[q] ? -l : 0.f;
-#pragma unroll
-                    for (int c = J + 1; c < LPW; c++) {
-                        const float u = fmaf(lm, pr[c], own[q][c]);
-                        own[q][c] = live[q] ? u : own[q][c];
-                    }
-                }
-            }
-        }
-        pp_step<J + 1, NT, RPT>(own, live, steps, pick, prow, red, stepof);
-    }
-}
-
-// Partial pivoting on the rows held in registers (own[q][c], row tid + q*NT of the set,
-// `rows` rows, w <= 32 columns).  The steps are expanded at compile time, so column j of a
-// row is a register; the pivot row goes through LDS once per step.  At the end every row
-// holds its L\U row as far as it was factored; pick[s] = set index of the row chosen at
-// step s, stepof[q] = the step at which this thread's row q was chosen (-1 if never).
-template <int NT, int RPT>
-__device__ __forceinline__ void reg_partial_pivot(float (&own)[RPT][LPW], int rows, int w, int* pick,
-                                                  float (*prow)[LPW], Best (*red)[2], int (&stepof)[RPT]) {
-    bool live[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        live[q] = (int)threadIdx.x + q * NT < rows;
-        stepof[q] = -1;
-    }
-    pp_step<0, NT, RPT>(own, live, min(w, rows), pick, prow, red, stepof);
-}
-
-// round 1: every workgroup proposes min(w, its rows) pivot rows (absolute A-row indices)
-__global__ void __launch_bounds__(256) calu_local(int n, int r0, int c0, int w, const float* __restrict__ T,
-                                                  size_t ldt, int* __restrict__ prop, int* __restrict__ nprop) {
-    constexpr int RPT = CRB / 256;
-    __shared__ float prow[2][LPW];
-    __shared__ Best red[4][2];
-    __shared__ int pick[LPW];
-    const int g = blockIdx.x, tid = threadIdx.x;
-    const int rb = r0 + g * CRB;
-    const int rows = min(CRB, n - rb);
-    float own[RPT][LPW];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int row = tid + q * 256;
-        const float* src = T + (size_t)c0 + (size_t)(rb + min(row, rows - 1)) * ldt;
-#pragma unroll
-        for (int c = 0; c < LPW; c++) own[q][c] = (row < rows && c < w) ? src[c] : 0.f;
-    }
-    int stepof[RPT];
-    reg_partial_pivot<256, RPT>(own, rows, w, pick, prow, red, stepof);
-    __syncthreads();
-    const int k = min(w, rows);
-    if (tid < k) prop[g * LPW + tid] = rb + pick[tid];
-    if (tid == 0) nprop[g] = k;
-}
-
-// round 2: one workgroup runs partial pivoting on the proposals' original rows; its picks
-// become LAPACK-style interchanges ipiv[r0 .. r0+w) (absolute, 0-based), and the picked
-// rows' L\U rows, in pick order, are the factored top block (top[s * LPW + c]).
-__global__ void __launch_bounds__(1024) calu_merge(int n, int r0, int c0, int w, const float* __restrict__ T,
-                                                   size_t ldt, const int* __restrict__ prop,
-                                                   const int* __restrict__ nprop, int nblk,
-                                                   int32_t* __restrict__ ipiv, float* __restrict__ top) {
-    __shared__ int rowid[CMAX];
-    __shared__ float prow[2][LPW];
-    __shared__ Best red[16][2];
-    __shared__ int pick[LPW];
-    __shared__ int total;
-    __shared__ int at_row[2 * LPW], at_src[2 * LPW];  // moved positions -> original rows
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        int t = 0;
-        for (int g = 0; g < nblk; g++) {
-            for (int q = 0; q < nprop[g]; q++) rowid[t + q] = prop[g * LPW + q];
-            t += nprop[g];
-        }
-        total = t;
-    }
-    __syncthreads();
-    const int rows = total;
-    float own[1][LPW];
-    {
-        const float* src = T + (size_t)c0 + (size_t)rowid[min(tid, rows - 1)] * ldt;
-#pragma unroll
-        for (int c = 0; c < LPW; c++) own[0][c] = (tid < rows && c < w) ? src[c] : 0.f;
-    }
-    int stepof[1];
-    reg_partial_pivot<1024, 1>(own, rows, w, pick, prow, red, stepof);
-    if (stepof[0] >= 0) {
-#pragma unroll
-        for (int c = 0; c < LPW; c++) top[stepof[0] * LPW + c] = own[0][c];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        // picks -> sequential interchanges: track where each original row currently is
-        const int k = min(w, rows);
-        int nm = 0;
-        auto content = [&](int pos) {
-            for (int q = 0; q < nm; q++)
-                if (at_row[q] == pos) return at_src[q];
-            return pos;
-        };
-        auto set = [&](int pos, int src) {
-            for (int q = 0; q < nm; q++)
-                if (at_row[q] == pos) { at_src[q] = src; return; }
-            at_row[nm] = pos;
-            at_src[nm] = src;
-            nm++;
-        };
-        for (int s = 0; s < k; s++) {
-            const int want = rowid[pick[s]];
-            int pos = want;  // where the original row `want` is now
-            for (int q = 0; q < nm; q++)
-                if (at_src[q] == want) { pos = at_row[q]; break; }
-            const int dst = r0 + s;
-            ipiv[dst] = pos;
-            if (pos != dst) {
-                const int a = content(dst);
-                set(dst, want);
-                set(pos, a);
-            }
-        }
-    }
-}
-
-// ---- one-launch tournament: wavefront rounds and a workgroup tree, no waiting
-//
-// Round 1: every wavefront runs partial pivoting on its 128 rows (two per lane) in
-// registers and proposes its pivot rows.  Round 2: wavefront 0 of the workgroup merges
-// the four proposals (original values).  Then a tree over workgroups with fan-in 4: a
-// workgroup posts its proposal and bumps its parent's counter; the last of the siblings
-// to arrive merges them and climbs, the others leave.  The root turns its picks into
-// interchanges and writes the factored top block.  Nobody spins, so the workgroups need
-// not be resident together.
-constexpr int TR = 4;           // tree fan-in
-constexpr int WROWS = 128;      // rows per wavefront in round 1
 
 // 64-bit max over the wavefront: DPP within rows of 16 lanes, lane reads across rows;
 // the result is uniform.
@@ -271,50 +54,63 @@ __device__ __forceinline__ uint64_t piv_key(float x, int idx, bool live) {
     return ((uint64_t)kv << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)idx);
 }
 
-// One step of a wavefront's partial pivoting: DPP argmax, the pivot row through LDS (the
-// owner lane writes it, everybody reads it back), the rank-1 update in registers.  KEEP:
-// rows already chosen keep their values (their L\U rows are needed afterwards); without
-// it they are left to change, since only the chosen indices matter.
-template <int J, int RPL, bool KEEP>
-__device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RPL], int steps, int (&pick)[LPW],
-                                         int (&stepof)[RPL], float* buf) {
+// LAPACK-style interchanges for picks (absolute rows, in order) at rows r0..: lane s
+// follows the current position of pick s through the sequence of swaps.
+__device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restrict__ ipiv) {
+    const int lane = threadIdx.x & 63;
+    int cur = lane < k ? picks[lane] : -1;
+    for (int t = 0; t < k; t++) {
+        const int pos = __builtin_amdgcn_readlane(cur, t);
+        const int dst = r0 + t;
+        if (lane == 0) ipiv[dst] = pos;
+        cur = cur == dst ? pos : (cur == pos ? dst : cur);
+    }
+}
+
+// ---- the workgroup tournament: 4 wavefronts, RPT rows per thread in registers, two
+// barriers per step (argmax across wavefronts, then the pivot row).  KEEP as above.
+template <int J, int RPT, bool KEEP>
+__device__ __forceinline__ void gstep(float (&own)[RPT][LPW], bool (&live)[RPT], int steps, int (&pick)[LPW],
+                                      int (&stepof)[RPT], uint64_t (*red)[4], float (*prow)[LPW]) {
     if constexpr (J < LPW) {
-        const int lane = threadIdx.x & 63;
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
         if (J < steps) {
             uint64_t k = 0;
 #pragma unroll
-            for (int q = 0; q < RPL; q++) k = umax64(k, piv_key(own[q][J], lane + q * 64, live[q]));
+            for (int q = 0; q < RPT; q++) k = umax64(k, piv_key(own[q][J], tid + q * 256, live[q]));
             k = wave_max64(k);
+            if (lane == 0) red[J & 1][wv] = k;
+            __syncthreads();
+            k = umax64(umax64(red[J & 1][0], red[J & 1][1]), umax64(red[J & 1][2], red[J & 1][3]));
             const int p = (int)(0xFFFFFFFFu - (uint32_t)k);
             pick[J] = p;
-            const int qp = p >> 6, lp = p & 63;
-            float* row = buf + (J & 1) * LPW;
-            if (lane == lp) {
-                if (qp == 0) {
+            const int qp = p >> 8, tp = p & 255;
+            if (tid == tp) {
+                // the owner publishes the pivot row, its slot chosen with bit masks (a branch
+                // per slot would put the rows in memory)
 #pragma unroll
-                    for (int c = J; c < LPW; c++) row[c] = own[0][c];
-                } else {
-                    if constexpr (RPL > 1) {
+                for (int c = J; c < LPW; c++) {
+                    uint32_t v = 0;
 #pragma unroll
-                        for (int c = J; c < LPW; c++) row[c] = own[RPL - 1][c];
-                    }
+                    for (int q = 0; q < RPT; q++) v |= __builtin_bit_cast(uint32_t, own[q][c]) & (q == qp ? ~0u : 0u);
+                    prow[J & 1][c] = __builtin_bit_cast(float, v);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < RPL; q++) {
-                const bool mine = q == qp && lane == lp;
+            for (int q = 0; q < RPT; q++) {
+                const bool mine = q == qp && tid == tp;
                 stepof[q] = mine ? J : stepof[q];
                 live[q] = live[q] && !mine;
             }
-            __builtin_amdgcn_wave_barrier();
+            __syncthreads();
             float pv[LPW];
 #pragma unroll
-            for (int c = J; c < LPW; c++) pv[c] = row[c];
+            for (int c = J; c < LPW; c++) pv[c] = prow[J & 1][c];
             const float piv = pv[J];
             if (piv != 0.f) {
                 const float inv = 1.f / piv;
 #pragma unroll
-                for (int q = 0; q < RPL; q++) {
+                for (int q = 0; q < RPT; q++) {
                     const float l = own[q][J] * inv;
                     const float lm = live[q] ? -l : 0.f;
                     if (KEEP) {
@@ -331,160 +127,114 @@ __device__ __forceinline__ void wpp_step(float (&own)[RPL][LPW], bool (&live)[RP
                 }
             }
         }
-        wpp_step<J + 1, RPL, KEEP>(own, live, steps, pick, stepof, buf);
+        gstep<J + 1, RPT, KEEP>(own, live, steps, pick, stepof, red, prow);
     }
 }
 
-// partial pivoting of one wavefront on the rows set[0 .. rows) (absolute A-row indices),
-// read from T; picks go to out[0 .. min(w, rows)) as absolute rows.  With top != null the
-// picked rows' L\U rows are stored in pick order (the factored top block).
-template <int RPL>
-__device__ __forceinline__ int wave_tournament(const int* set, int rows, int w, int c0, const float* __restrict__ T,
-                                               size_t ldt, int* out, float* __restrict__ top, float* buf) {
-    const int lane = threadIdx.x & 63;
-    float own[RPL][LPW];
-    bool live[RPL];
-    int stepof[RPL];
+// Partial pivoting of the workgroup on rows set(i), i < rows (absolute A-rows; set == null:
+// rows base + i); the picks (absolute rows) go to out[], and with top != null the picked
+// rows' L\U rows to top[] in pick order.  Returns the number of picks.
+template <int RPT>
+__device__ __forceinline__ int wg_tournament(const int* set, int base, int rows, int w, int c0,
+                                             const float* __restrict__ T, size_t ldt, int* out, float* __restrict__ top,
+                                             uint64_t (*red)[4], float (*prow)[LPW], int* pk_sh) {
+    const int tid = threadIdx.x;
+    float own[RPT][LPW];
+    bool live[RPT];
+    int stepof[RPT];
 #pragma unroll
-    for (int q = 0; q < RPL; q++) {
-        const int i = lane + q * 64;
+    for (int q = 0; q < RPT; q++) {
+        const int i = tid + q * 256;
         live[q] = i < rows;
         stepof[q] = -1;
-        const int rid = live[q] ? set[i] : c0;  // c0 = r0: a valid row for the idle lanes
+        const int rid = live[q] ? (set ? set[i] : base + i) : c0;  // c0 = r0: a valid row for idle threads
         const float* src = T + (size_t)c0 + (size_t)rid * ldt;
 #pragma unroll
         for (int c = 0; c < LPW; c++) own[q][c] = (live[q] && c < w) ? src[c] : 0.f;
     }
     const int steps = min(w, rows);
     int pk[LPW];
-    if (top) wpp_step<0, RPL, true>(own, live, steps, pk, stepof, buf);
-    else wpp_step<0, RPL, false>(own, live, steps, pk, stepof, buf);
-    int mine = 0;
+    if (top) gstep<0, RPT, true>(own, live, steps, pk, stepof, red, prow);
+    else gstep<0, RPT, false>(own, live, steps, pk, stepof, red, prow);
+    if (tid == 0)
 #pragma unroll
-    for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
-    const int orow = lane < steps ? set[mine] : 0;  // pick < rows, so set[pick] was written
-    __builtin_amdgcn_wave_barrier();
-    if (lane < steps) out[lane] = orow;
+        for (int j = 0; j < LPW; j++) pk_sh[j] = pk[j];
     if (top) {
 #pragma unroll
-        for (int q = 0; q < RPL; q++)
+        for (int q = 0; q < RPT; q++)
             if (stepof[q] >= 0)
 #pragma unroll
                 for (int c = 0; c < LPW; c++) top[stepof[q] * LPW + c] = own[q][c];
     }
+    __syncthreads();
+    if (tid < steps) out[tid] = set ? set[pk_sh[tid]] : base + pk_sh[tid];
+    __syncthreads();
     return steps;
 }
 
-// LAPACK-style interchanges for picks (absolute rows, in order) at rows r0..: lane s
-// follows the current position of pick s through the sequence of swaps.
-__device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restrict__ ipiv) {
-    const int lane = threadIdx.x & 63;
-    int cur = lane < k ? picks[lane] : -1;
-    for (int t = 0; t < k; t++) {
-        const int pos = __builtin_amdgcn_readlane(cur, t);
-        const int dst = r0 + t;
-        if (lane == 0) ipiv[dst] = pos;
-        cur = cur == dst ? pos : (cur == pos ? dst : cur);
-    }
-}
+constexpr int GROWS = 1024;  // rows per workgroup in the first round (4 per thread)
+constexpr int GFAN = 16;     // tree fan-in: 16 proposals of 32 rows, 2 per thread
 
 // lists: [level][node][LPW] absolute rows and counts; cnt: [level][node] arrival counters
-// (zero on entry; the root clears them for the next panel).  G workgroups of 256 threads.
-__global__ void __launch_bounds__(256) calu_tree(int n, int r0, int c0, int w, const float* __restrict__ T,
-                                                 size_t ldt, int* __restrict__ lists, int* __restrict__ lcount,
-                                                 int* __restrict__ cnt, int lvl_stride, int32_t* __restrict__ ipiv,
-                                                 float* __restrict__ top) {
-    __shared__ int wlist[4][LPW];
-    __shared__ int wcnt[4];
-    __shared__ int merged[TR * LPW];
+// (zero on entry; the root clears them for the next panel).  G workgroups of 256 threads:
+// round 1 on GROWS rows each, then a tree with fan-in GFAN where the last workgroup of a
+// group to arrive merges the group's proposals.  Nobody waits for anybody.
+__global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, const float* __restrict__ T, size_t ldt,
+                                               int* __restrict__ lists, int* __restrict__ lcount, int* __restrict__ cnt,
+                                               int lvl_stride, int32_t* __restrict__ ipiv, float* __restrict__ top) {
+    __shared__ uint64_t red[2][4];
+    __shared__ float prow[2][LPW];
+    __shared__ int pk_sh[LPW];
     __shared__ int mylist[LPW];
-    __shared__ float rowbuf[4][2 * LPW];
+    __shared__ int merged[GFAN * LPW];
     __shared__ int go;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x;
     const int G = gridDim.x;
-    // round 1: 128 rows per wavefront
-    {
-        const int rb = r0 + (blockIdx.x * 4 + wv) * WROWS;
-        const int rows = max(0, min(WROWS, n - rb));
-        int* set = merged + wv * LPW;  // scratch for the row ids of this wavefront is implicit
-        (void)set;
-        float own[2][LPW];
-        bool live[2];
-        int stepof[2];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int i = lane + q * 64;
-            live[q] = i < rows;
-            stepof[q] = -1;
-            const float* src = T + (size_t)c0 + (size_t)(live[q] ? rb + i : r0) * ldt;
-#pragma unroll
-            for (int c = 0; c < LPW; c++) own[q][c] = (live[q] && c < w) ? src[c] : 0.f;
-        }
-        const int steps = min(w, rows);
-        int pk[LPW];
-        wpp_step<0, 2, false>(own, live, steps, pk, stepof, rowbuf[wv]);
-        int mine = 0;
-#pragma unroll
-        for (int j = 0; j < LPW; j++) mine = lane == j ? pk[j] : mine;
-        if (lane < steps) wlist[wv][lane] = rb + mine;
-        if (lane == 0) wcnt[wv] = steps;
+    const bool root1 = G == 1;
+    const int rb = r0 + blockIdx.x * GROWS;
+    const int rows = max(0, min(GROWS, n - rb));
+    int k = wg_tournament<GROWS / 256>(nullptr, rb, rows, w, c0, T, ldt, mylist, root1 ? top : nullptr, red, prow,
+                                      pk_sh);
+    if (root1) {
+        if (tid < 64) picks_to_ipiv(mylist, k, r0, ipiv);
+        return;
     }
-    __syncthreads();
-    if (wv != 0) return;
-    // round 2: the workgroup's four proposals
-    int k;
-    {
-        int tot = 0;
-        for (int q = 0; q < 4; q++) {
-            if (lane < wcnt[q]) merged[tot + lane] = wlist[q][lane];
-            tot += wcnt[q];
-        }
-        __builtin_amdgcn_wave_barrier();
-        const bool root = G == 1;
-        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr, rowbuf[0]);
-        if (root) {
-            __builtin_amdgcn_wave_barrier();
-            picks_to_ipiv(mylist, k, r0, ipiv);
-            return;
-        }
-    }
-    // the tree
     int node = blockIdx.x, nodes = G;
     for (int lvl = 0;; lvl++) {
         int* L = lists + ((size_t)lvl * lvl_stride + node) * LPW;
-        if (lane < k) L[lane] = mylist[lane];
-        if (lane == 0) lcount[lvl * lvl_stride + node] = k;
-        const int parent = node / TR;
-        const int first = parent * TR, nchild = min(TR, nodes - first);
+        if (tid < k) L[tid] = mylist[tid];
+        if (tid == 0) lcount[lvl * lvl_stride + node] = k;
+        const int parent = node / GFAN;
+        const int first = parent * GFAN, nchild = min(GFAN, nodes - first);
         __builtin_amdgcn_s_waitcnt(0);
-        if (lane == 0) {
+        __syncthreads();
+        if (tid == 0) {
             const int old = __hip_atomic_fetch_add(cnt + lvl * lvl_stride + parent, 1, __ATOMIC_ACQ_REL,
                                                    __HIP_MEMORY_SCOPE_AGENT);
             go = old == nchild - 1;
         }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
         if (!go) return;
-        // last to arrive: merge the siblings' proposals
+        // the last to arrive merges the group's proposals
         int tot = 0;
         for (int ch = 0; ch < nchild; ch++) {
             const int cc = __hip_atomic_load(lcount + lvl * lvl_stride + first + ch, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
             const int* CL = lists + ((size_t)lvl * lvl_stride + first + ch) * LPW;
-            if (lane < cc) merged[tot + lane] = __hip_atomic_load(CL + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid < cc) merged[tot + tid] = __hip_atomic_load(CL + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             tot += cc;
         }
-        __builtin_amdgcn_wave_barrier();
-        const int pnodes = (nodes + TR - 1) / TR;
+        __syncthreads();
+        const int pnodes = (nodes + GFAN - 1) / GFAN;
         const bool root = pnodes == 1;
-        k = wave_tournament<2>(merged, tot, w, c0, T, ldt, mylist, root ? top : nullptr, rowbuf[0]);
+        k = wg_tournament<GFAN * LPW / 256>(merged, 0, tot, w, c0, T, ldt, mylist, root ? top : nullptr, red, prow,
+                                           pk_sh);
         node = parent;
         nodes = pnodes;
         if (root) {
-            __builtin_amdgcn_wave_barrier();
-            picks_to_ipiv(mylist, k, r0, ipiv);
-            // clear the counters for the next panel
+            if (tid < 64) picks_to_ipiv(mylist, k, r0, ipiv);
             for (int l2 = 0; l2 <= lvl; l2++)
-                for (int e = lane; e < lvl_stride; e += 64) cnt[l2 * lvl_stride + e] = 0;
+                for (int e = tid; e < lvl_stride; e += 256) cnt[l2 * lvl_stride + e] = 0;
             return;
         }
     }
@@ -646,8 +396,8 @@ struct LuCtx {
 
 void calu_panel(const LuCtx& c, int r0, int w) {
     const int rows = c.n - r0;
-    const int G = (rows + 4 * WROWS - 1) / (4 * WROWS);
-    hipLaunchKernelGGL(calu_tree, dim3(G), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.lists, c.lcount, c.cnt,
+    const int G = (rows + GROWS - 1) / GROWS;
+    hipLaunchKernelGGL(calu_wg, dim3(G), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.lists, c.lcount, c.cnt,
                        c.lvl_stride, c.ipiv, c.top);
     BOLA_LAUNCH_CHECK();
     const int k = std::min(w, rows);
@@ -716,7 +466,7 @@ void update_cols(const LuCtx& c, int k, int nb, int c_lo, int c_hi) {
 }  // namespace
 
 size_t mixed_lu_ws_bytes(int n, int lp) {
-    const size_t G = (size_t)(n + 4 * WROWS - 1) / (4 * WROWS);
+    const size_t G = (size_t)(n + GROWS - 1) / GROWS;
     size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
     b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 256);
     return b;
@@ -735,7 +485,7 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, int n, float* T, size_t ldt, 
     c.ldt = ldt;
     c.ipiv = ipiv;
     c.perm = perm;
-    const int G = (n + 4 * WROWS - 1) / (4 * WROWS);
+    const int G = (n + GROWS - 1) / GROWS;
     const int levels = 8;
     c.lvl_stride = G;
     char* b = (char*)ws;
