@@ -402,11 +402,16 @@ class MixedStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
 
 
-def dgesv_mixed(n, nrhs, a, lda, b, ldb, x, ldx, precision="fp32") -> dict:
+MIX_RBT = 0x100
+
+
+def dgesv_mixed(n, nrhs, a, lda, b, ldb, x, ldx, precision="fp32", rbt=False) -> dict:
     """A X = B to FP64 accuracy from a low-precision LU (fp32 / fp16 / bf16 products),
-    FP64 refinement, GMRES-IR and the FP64 fallback.  Returns the solver statistics."""
+    FP64 refinement, GMRES-IR and the FP64 fallback.  ``rbt``: randomise with random
+    butterflies and factor without interchanges instead of tournament pivoting.  Returns
+    the solver statistics."""
     st = MixedStats()
-    p = PRECISIONS[precision]
+    p = PRECISIONS[precision] | (MIX_RBT if rbt else 0)
     if _is_dev(a):
         rc = lib().bola_hip_dgesv_mixed_dev(default_handle().ptr, p, n, nrhs, _ptr(a, np.float64), lda,
                                             _ptr(b, np.float64), ldb, _ptr(x, np.float64), ldx, C.byref(st))

@@ -116,6 +116,59 @@ __global__ void k_iota(int n, int* p) {
     if (i < n) p[i] = i;
 }
 
+// ---------------------------------------------------------------- butterflies
+// One level of a recursive butterfly transformation (Parker): blocks of 2h rows (or
+// columns); inside a block the pair (a, b) = (x_i, x_{i+h}) becomes
+//   transposed:  (R_i (a + b), R_{i+h} (a - b)) / sqrt 2
+//   plain:       (R_i a + R_{i+h} b, R_i a - R_{i+h} b) / sqrt 2
+// with R an n-vector of random factors near one.  Randomising A this way makes LU
+// without interchanges safe with high probability.
+constexpr double kInvSqrt2 = 0.70710678118654752440;
+
+// On T (fp32).  LEFT: pairs of rows of A (columns of T, contiguous); RIGHT: pairs of
+// columns of A (rows of T).  Both use the transposed pair rule: the left factor is W_L^T,
+// and the right factor's plain form on the columns of A is the transposed rule on the
+// rows of T.  Threads run along T's contiguous index in both cases.
+template <bool LEFT>
+__global__ void rbt_T(int n, float* __restrict__ T, size_t ldt, int h, const double* __restrict__ R) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int o = blockIdx.y; o < (LEFT ? n / 2 : n); o += gridDim.y) {
+        float *pa, *pb;
+        int i;
+        if (LEFT) {  // o: pair, t: element along the column of T
+            if (t >= n) return;
+            i = (o / h) * 2 * h + o % h;
+            pa = T + (size_t)t + (size_t)i * ldt;
+            pb = pa + (size_t)h * ldt;
+        } else {  // o: column of T, t: pair
+            if (t >= n / 2) return;
+            i = (t / h) * 2 * h + t % h;
+            pa = T + (size_t)i + (size_t)o * ldt;
+            pb = pa + h;
+        }
+        const float r0 = (float)(R[i] * kInvSqrt2), r1 = (float)(R[i + h] * kInvSqrt2);
+        const float a = *pa, b = *pb;
+        *pa = r0 * (a + b);
+        *pb = r1 * (a - b);
+    }
+}
+// on n x nrhs FP64 columns: TRANS: the transposed form (W^T z); else the plain form (W y)
+template <bool TRANS>
+__global__ void rbt_vec(int n, int nrhs, double* __restrict__ Z, size_t ldz, int h, const double* __restrict__ R) {
+    const int pr = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (pr >= n / 2) return;
+    const int blk = pr / h, i = blk * 2 * h + pr % h;
+    double* z = Z + (size_t)q * ldz;
+    const double a = z[i], b = z[i + h], r0 = R[i] * kInvSqrt2, r1 = R[i + h] * kInvSqrt2;
+    if (TRANS) {
+        z[i] = r0 * (a + b);
+        z[i + h] = r1 * (a - b);
+    } else {
+        z[i] = r0 * a + r1 * b;
+        z[i + h] = r0 * a - r1 * b;
+    }
+}
+
 // ---------------------------------------------------------------- M^-1 pieces
 // Z(i, q) = s[perm[i]] R(perm[i], q)
 __global__ void gather_scale(int n, int nrhs, const double* __restrict__ R, size_t ldr, const int* __restrict__ perm,
@@ -386,6 +439,9 @@ struct Solver {
     void* luws;
     int epoch = 0;
     int mgm;  // GMRES restart length
+    bool rbt = false;       // butterfly-randomised, factored without interchanges
+    double* rl[2] = {};     // left butterflies, levels 1 (h = n/2) and 2 (h = n/4)
+    double* rr[2] = {};     // right butterflies
 };
 
 dim3 g1(int n) { return dim3(cdiv((size_t)n, 256)); }
@@ -395,6 +451,12 @@ dim3 g2(int n, int nrhs) { return dim3(cdiv((size_t)n, 256), (unsigned)nrhs); }
 void apply_minv(Solver& S, const double* Xr, size_t ldx, double* Y) {
     hipLaunchKernelGGL(gather_scale, g1(S.n), dim3(256), 0, S.st, S.n, S.nrhs, Xr, ldx, S.perm, S.s, S.Z, (size_t)S.n);
     BOLA_LAUNCH_CHECK();
+    const dim3 gp(cdiv((size_t)S.n / 2, 256), (unsigned)S.nrhs);
+    if (S.rbt) {  // W_L^T z = B1^T (B2^T z)
+        hipLaunchKernelGGL(rbt_vec<true>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 4, S.rl[1]);
+        hipLaunchKernelGGL(rbt_vec<true>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 2, S.rl[0]);
+        BOLA_LAUNCH_CHECK();
+    }
     const int nb = (S.n + TB - 1) / TB;
     S.epoch++;
     hipLaunchKernelGGL(trsv_blk<true>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
@@ -404,6 +466,11 @@ void apply_minv(Solver& S, const double* Xr, size_t ldx, double* Y) {
     hipLaunchKernelGGL(trsv_blk<false>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
                        S.flags, S.epoch, nb, kTrsvSpin, S.err);
     BOLA_LAUNCH_CHECK();
+    if (S.rbt) {  // W_R y = B2 (B1 y)
+        hipLaunchKernelGGL(rbt_vec<false>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 2, S.rr[0]);
+        hipLaunchKernelGGL(rbt_vec<false>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 4, S.rr[1]);
+        BOLA_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(col_scale, g1(S.n), dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.t, Y, (size_t)S.n);
     BOLA_LAUNCH_CHECK();
 }
@@ -570,7 +637,9 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
                                         bola_hip_mixed_stats* stats) {
     BOLA_API_BEGIN
     if (!h) return -1;
-    if (precision < 0 || precision > 2) {
+    const bool want_rbt = (precision & BOLA_HIP_MIX_RBT) != 0;
+    precision &= 0xFF;
+    if (precision > 2) {
         set_error(h, "Argument 2 to dgesv_mixed had an illegal value");
         return -2;
     }
@@ -601,6 +670,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     S.lda = lda;
     S.mgm = 30;
     S.ldt = ((size_t)n + 63) & ~(size_t)63;
+    S.rbt = want_rbt && n % 4 == 0 && n >= 8;  // two butterfly levels need n divisible by 4
     // workspace carve-up
     size_t off = 0;
     auto take = [&](size_t b) {
@@ -620,6 +690,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     const size_t oVb = take(8 * (size_t)nrhs * (S.mgm + 1) * n);
     const size_t oflags = take(sizeof(int) * ((size_t)n / TB + 2)), oerr = take(sizeof(int) * 4);
     const size_t olu = take(mixed_lu_ws_bytes(n, precision));
+    const size_t orb = take(8 * 4 * (size_t)n);
     char* base = (char*)h->mws.get(off);
     S.T = (float*)(base + oT);
     S.ipiv = (int32_t*)(base + oipiv);
@@ -640,6 +711,24 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     S.flags = (int*)(base + oflags);
     S.err = (int*)(base + oerr);
     S.luws = base + olu;
+    for (int l = 0; l < 2; l++) {
+        S.rl[l] = (double*)(base + orb) + (size_t)l * n;
+        S.rr[l] = (double*)(base + orb) + (size_t)(2 + l) * n;
+    }
+    if (S.rbt) {
+        // random factors exp(u / 10), u uniform in [-1/2, 1/2), from a fixed-seed splitmix64
+        std::vector<double> rv(4 * (size_t)n);
+        uint64_t x = 0x5EEDB077E8F1ull;
+        for (auto& v : rv) {
+            uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            v = std::exp(((double)(z >> 11) * 0x1.0p-53 - 0.5) / 10.0);
+        }
+        BOLA_CHECK(hipMemcpyAsync(S.rl[0], rv.data(), sizeof(double) * rv.size(), hipMemcpyHostToDevice, S.st));
+        BOLA_CHECK(hipStreamSynchronize(S.st));
+    }
     hipStream_t st = S.st;
     hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, n / TB + 2, S.flags);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, S.err);
@@ -659,8 +748,17 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
                            S.T, S.ldt);
         hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, st, n, S.perm);
         BOLA_LAUNCH_CHECK();
+        if (S.rbt) {  // T <- (W_L^T A_s W_R)^T: left B2^T then B1^T, right B2 then B1
+            const dim3 gl(cdiv((size_t)n, 256), (unsigned)std::min(n / 2, 16384));
+            const dim3 gr(cdiv((size_t)n / 2, 256), (unsigned)std::min(n, 16384));
+            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, S.T, S.ldt, n / 4, S.rl[1]);
+            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, S.T, S.ldt, n / 2, S.rl[0]);
+            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, S.T, S.ldt, n / 4, S.rr[1]);
+            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, S.T, S.ldt, n / 2, S.rr[0]);
+            BOLA_LAUNCH_CHECK();
+        }
         // 2. the low-precision LU
-        mixed_lu_factor(h, precision, n, S.T, S.ldt, S.ipiv, S.perm, S.luws);
+        mixed_lu_factor(h, precision, !S.rbt, n, S.T, S.ldt, S.ipiv, S.perm, S.luws, S.err);
     }
     BOLA_CHECK(hipEventRecord(ev[1], st));
     // |A|_inf and |b|_inf

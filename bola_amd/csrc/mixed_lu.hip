@@ -68,10 +68,11 @@ __device__ void picks_to_ipiv(const int* picks, int k, int r0, int32_t* __restri
 }
 
 // ---- the workgroup tournament: 4 wavefronts, RPT rows per thread in registers, two
-// barriers per step (argmax across wavefronts, then the pivot row).  KEEP as above.
-template <int J, int RPT, bool KEEP>
+// barriers per step (argmax across wavefronts, then the pivot row).  Fully unrolled over
+// the 32 steps, so columns already eliminated leave the registers.
+template <int J, int RPT>
 __device__ __forceinline__ void gstep(float (&own)[RPT][LPW], bool (&live)[RPT], int steps, int (&pick)[LPW],
-                                      int (&stepof)[RPT], uint64_t (*red)[4], float (*prow)[LPW]) {
+                                      uint64_t (*red)[4], float (*prow)[LPW]) {
     if constexpr (J < LPW) {
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
         if (J < steps) {
@@ -97,11 +98,7 @@ __device__ __forceinline__ void gstep(float (&own)[RPT][LPW], bool (&live)[RPT],
                 }
             }
 #pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                const bool mine = q == qp && tid == tp;
-                stepof[q] = mine ? J : stepof[q];
-                live[q] = live[q] && !mine;
-            }
+            for (int q = 0; q < RPT; q++) live[q] = live[q] && !(q == qp && tid == tp);
             __syncthreads();
             float pv[LPW];
 #pragma unroll
@@ -111,42 +108,28 @@ __device__ __forceinline__ void gstep(float (&own)[RPT][LPW], bool (&live)[RPT],
                 const float inv = 1.f / piv;
 #pragma unroll
                 for (int q = 0; q < RPT; q++) {
-                    const float l = own[q][J] * inv;
-                    const float lm = live[q] ? -l : 0.f;
-                    if (KEEP) {
-                        own[q][J] = live[q] ? l : own[q][J];
+                    const float lm = live[q] ? -(own[q][J] * inv) : 0.f;
 #pragma unroll
-                        for (int c = J + 1; c < LPW; c++) {
-                            const float u = fmaf(lm, pv[c], own[q][c]);
-                            own[q][c] = live[q] ? u : own[q][c];
-                        }
-                    } else {
-#pragma unroll
-                        for (int c = J + 1; c < LPW; c++) own[q][c] = fmaf(lm, pv[c], own[q][c]);
-                    }
+                    for (int c = J + 1; c < LPW; c++) own[q][c] = fmaf(lm, pv[c], own[q][c]);
                 }
             }
         }
-        gstep<J + 1, RPT, KEEP>(own, live, steps, pick, stepof, red, prow);
+        gstep<J + 1, RPT>(own, live, steps, pick, red, prow);
     }
 }
 
 // Partial pivoting of the workgroup on rows set(i), i < rows (absolute A-rows; set == null:
-// rows base + i); the picks (absolute rows) go to out[], and with top != null the picked
-// rows' L\U rows to top[] in pick order.  Returns the number of picks.
+// rows base + i); the picks (absolute rows) go to out[].  Returns the number of picks.
 template <int RPT>
 __device__ __forceinline__ int wg_tournament(const int* set, int base, int rows, int w, int c0,
-                                             const float* __restrict__ T, size_t ldt, int* out, float* __restrict__ top,
-                                             uint64_t (*red)[4], float (*prow)[LPW], int* pk_sh) {
+                                             const float* __restrict__ T, size_t ldt, int* out, uint64_t (*red)[4], float (*prow)[LPW], int* pk_sh) {
     const int tid = threadIdx.x;
     float own[RPT][LPW];
     bool live[RPT];
-    int stepof[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int i = tid + q * 256;
         live[q] = i < rows;
-        stepof[q] = -1;
         const int rid = live[q] ? (set ? set[i] : base + i) : c0;  // c0 = r0: a valid row for idle threads
         const float* src = T + (size_t)c0 + (size_t)rid * ldt;
 #pragma unroll
@@ -154,18 +137,10 @@ __device__ __forceinline__ int wg_tournament(const int* set, int base, int rows,
     }
     const int steps = min(w, rows);
     int pk[LPW];
-    if (top) gstep<0, RPT, true>(own, live, steps, pk, stepof, red, prow);
-    else gstep<0, RPT, false>(own, live, steps, pk, stepof, red, prow);
+    gstep<0, RPT>(own, live, steps, pk, red, prow);
     if (tid == 0)
 #pragma unroll
         for (int j = 0; j < LPW; j++) pk_sh[j] = pk[j];
-    if (top) {
-#pragma unroll
-        for (int q = 0; q < RPT; q++)
-            if (stepof[q] >= 0)
-#pragma unroll
-                for (int c = 0; c < LPW; c++) top[stepof[q] * LPW + c] = own[q][c];
-    }
     __syncthreads();
     if (tid < steps) out[tid] = set ? set[pk_sh[tid]] : base + pk_sh[tid];
     __syncthreads();
@@ -178,10 +153,11 @@ constexpr int GFAN = 16;     // tree fan-in: 16 proposals of 32 rows, 2 per thre
 // lists: [level][node][LPW] absolute rows and counts; cnt: [level][node] arrival counters
 // (zero on entry; the root clears them for the next panel).  G workgroups of 256 threads:
 // round 1 on GROWS rows each, then a tree with fan-in GFAN where the last workgroup of a
-// group to arrive merges the group's proposals.  Nobody waits for anybody.
+// group to arrive merges the group's proposals.  Nobody waits for anybody.  The root
+// writes the panel's interchanges ipiv[r0 .. r0+k).
 __global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, const float* __restrict__ T, size_t ldt,
                                                int* __restrict__ lists, int* __restrict__ lcount, int* __restrict__ cnt,
-                                               int lvl_stride, int32_t* __restrict__ ipiv, float* __restrict__ top) {
+                                               int lvl_stride, int32_t* __restrict__ ipiv) {
     __shared__ uint64_t red[2][4];
     __shared__ float prow[2][LPW];
     __shared__ int pk_sh[LPW];
@@ -190,12 +166,10 @@ __global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, con
     __shared__ int go;
     const int tid = threadIdx.x;
     const int G = gridDim.x;
-    const bool root1 = G == 1;
     const int rb = r0 + blockIdx.x * GROWS;
     const int rows = max(0, min(GROWS, n - rb));
-    int k = wg_tournament<GROWS / 256>(nullptr, rb, rows, w, c0, T, ldt, mylist, root1 ? top : nullptr, red, prow,
-                                      pk_sh);
-    if (root1) {
+    int k = wg_tournament<GROWS / 256>(nullptr, rb, rows, w, c0, T, ldt, mylist, red, prow, pk_sh);
+    if (G == 1) {
         if (tid < 64) picks_to_ipiv(mylist, k, r0, ipiv);
         return;
     }
@@ -227,8 +201,7 @@ __global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, con
         __syncthreads();
         const int pnodes = (nodes + GFAN - 1) / GFAN;
         const bool root = pnodes == 1;
-        k = wg_tournament<GFAN * LPW / 256>(merged, 0, tot, w, c0, T, ldt, mylist, root ? top : nullptr, red, prow,
-                                           pk_sh);
+        k = wg_tournament<GFAN * LPW / 256>(merged, 0, tot, w, c0, T, ldt, mylist, red, prow, pk_sh);
         node = parent;
         nodes = pnodes;
         if (root) {
@@ -238,6 +211,12 @@ __global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, con
             return;
         }
     }
+}
+
+
+__global__ void k_iota_piv(int n, int32_t* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 
 __global__ void k_zero_cnt(int* p, int n) {
@@ -288,26 +267,49 @@ __device__ void upper_inverse_lds(const float (*U)[LPW + 1], float (*Ui)[LPW + 1
 }
 
 // the panel without pivoting, after the interchanges: L21 = A21 U11^-1 for the rows
-// below the top block (one thread per row, a product with the inverse), with L11\U11
-// from the merge round; workgroup 0 also stores the factored top block.
+// below the top block (one thread per row, a product with the inverse).  Every workgroup
+// factors the top block itself (LU without pivoting: its rows are the chosen pivots, in
+// order); with top_only, one workgroup stores it instead, in a launch after the others
+// have read the unfactored block.
 template <bool FULL>  // FULL: w == 32, loads and stores without tests
 __global__ void __launch_bounds__(256) panel_nopiv(int n, int r0, int c0, int w, float* __restrict__ T, size_t ldt,
-                                                   const float* __restrict__ top) {
+                                                   int top_only) {
     __shared__ float U[LPW][LPW + 1];  // U[row][col] of the top block, A orientation
     __shared__ float Ui[LPW][LPW + 1];
     const int tid = threadIdx.x;
-    for (int e = tid; e < LPW * LPW; e += 256) {
-        const int r = e / LPW, c = e % LPW;
-        U[r][c] = (r < w && c < w) ? top[r * LPW + c] : 0.f;
+    if (tid < 64) {
+        // LU without pivoting of the top block (its rows are the chosen pivots, in order):
+        // lane c holds column c; multipliers come from lane j by lane reads
+        const int c = tid;
+        float col[LPW];
+#pragma unroll
+        for (int r = 0; r < LPW; r++)
+            col[r] = (c < w && r < w) ? T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt] : (r == c ? 1.f : 0.f);
+#pragma unroll
+        for (int j = 0; j < LPW; j++) {
+            const float piv = __builtin_amdgcn_readlane(__builtin_bit_cast(int, col[j]), j) == 0
+                                  ? 0.f
+                                  : __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, col[j]), j));
+            const float inv = piv != 0.f ? 1.f / piv : 0.f;
+#pragma unroll
+            for (int r = j + 1; r < LPW; r++) {
+                const float l = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, col[r]), j)) * inv;
+                col[r] = c > j ? fmaf(-l, col[j], col[r]) : (c == j ? l : col[r]);
+            }
+        }
+        if (c < LPW)
+#pragma unroll
+            for (int r = 0; r < LPW; r++) U[r][c] = col[r];
     }
     __syncthreads();
-    upper_inverse_lds(U, Ui, w, false);
-    if (blockIdx.x == 0) {
+    if (top_only) {
         for (int e = tid; e < w * w; e += 256) {
             const int r = e / w, c = e % w;
             T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt] = U[r][c];
         }
+        return;
     }
+    upper_inverse_lds(U, Ui, w, false);
     __syncthreads();
     const int r = r0 + w + blockIdx.x * 256 + tid;
     if (r >= n) return;
@@ -365,6 +367,256 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
         if (FULL || b < w) xr[(size_t)b * ld] = y[b];
 }
 
+// ---- the butterfly mode: LU without interchanges, one launch per panel ----------------
+
+constexpr int NPLD = kMixNB + 4;      // LDS row stride (floats) of panel rows
+constexpr long kPanelSpin = 1L << 24;  // bound on the polls of one flag
+constexpr int TRC = 64;               // columns per workgroup of trsm_rbt
+constexpr size_t kInvPanel = 2 * (size_t)(kMixNB / LPW) * LPW * LPW;  // floats of one panel's inverses
+
+// column b (a lane) of the inverse of d's unit lower triangle (LOWER) or of its upper
+// triangle, substituted in o's column b.  d is the identity outside its leading block, so
+// the inverse is too.  Rolled loops over LDS: unrolled, the compiler hoists the d reads
+// and runs out of registers.
+template <bool LOWER>
+__device__ void tri_inv_col(const float (*d)[LPW + 1], float (*o)[LPW + 1], int b) {
+    if (b >= LPW) return;
+#pragma unroll 1
+    for (int i = 0; i < LPW; i++) {
+        const int a = LOWER ? i : LPW - 1 - i;
+        float v = a == b ? 1.f : 0.f;
+        if (LOWER) {
+            for (int c = 0; c < a; c++) v = fmaf(-d[a][c], o[c][b], v);
+            o[a][b] = v;
+        } else {
+            for (int c = a + 1; c < LPW; c++) v = fmaf(-d[a][c], o[c][b], v);
+            const float dd = d[a][a];
+            o[a][b] = dd != 0.f ? v / dd : v;
+        }
+    }
+}
+
+// one leaf (columns cj .. cj+32, zero-padded past the panel) applied to the 32 rows in a:
+// L = A[:, leaf] U11^-1, then A[:, right of the leaf] -= L U12 with U12 in ub
+__device__ __forceinline__ void rbt_apply_leaf(float (*a)[NPLD], const float (*ub)[NPLD], const float (*ui)[LPW + 1],
+                                               int cj, int lw, int w) {
+    const int tid = threadIdx.x;
+    {
+        const int r = tid >> 3, q = (tid & 7) * 4;
+        float l[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+        for (int s = 0; s < LPW; s++) {
+            const float x = a[r][cj + s];
+#pragma unroll
+            for (int i = 0; i < 4; i++) l[i] = fmaf(x, ui[s][q + i], l[i]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (q + i < lw) a[r][cj + q + i] = l[i];
+    }
+    __syncthreads();
+    const int c0 = cj + lw;
+    if (c0 < w) {
+        const int rq = (tid >> 6) * 8, cl = tid & 63;
+        float acc[8][4];
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[rr][i] = 0.f;
+#pragma unroll 4
+        for (int s = 0; s < LPW; s++) {
+            float u[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int c = c0 + cl + 64 * i;
+                u[i] = c < w ? ub[s][c] : 0.f;
+            }
+#pragma unroll
+            for (int rr = 0; rr < 8; rr++) {
+                const float x = a[rq + rr][cj + s];
+#pragma unroll
+                for (int i = 0; i < 4; i++) acc[rr][i] = fmaf(x, u[i], acc[rr][i]);
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int c = c0 + cl + 64 * i;
+                if (c < w) a[rq + rr][c] -= acc[rr][i];
+            }
+    }
+    __syncthreads();
+}
+
+// LU without interchanges of the 32 x 32 block a[0..32)[cj..cj+32) (identity past lw) into
+// d (L\U), by lanes 0..31 of one wavefront, lane r holding row r; the pivot row goes
+// through LDS (one wavefront: its LDS accesses stay in program order).  Kept out of line:
+// inlined beside the inverses it costs the kernel hundreds of registers.
+__device__ __forceinline__ void diag_lu_rows(const float (*a)[NPLD], float (*d)[LPW + 1], int cj, int lw) {
+    const int r = threadIdx.x;
+    float rv[LPW];
+#pragma unroll
+    for (int c = 0; c < LPW; c++) rv[c] = (r < lw && c < lw) ? a[r][cj + c] : (r == c ? 1.f : 0.f);
+#pragma unroll
+    for (int j = 0; j < LPW; j++) {
+        if (r == j)
+#pragma unroll
+            for (int c = j; c < LPW; c++) d[j][c] = rv[c];
+        asm volatile("" ::: "memory");
+        const float piv = d[j][j];
+        const float iv = piv != 0.f ? 1.f / piv : 0.f;
+        const float l = rv[j] * iv;
+        if (r > j) {
+            rv[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < LPW; c++) rv[c] = fmaf(-l, d[j][c], rv[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < LPW; c++)
+        if (c < r) d[r][c] = rv[c];
+}
+
+// One launch factors the panel A[r0:n, r0:r0+w) (w <= kMixNB) without interchanges.  Each
+// workgroup holds 32 rows in LDS.  Workgroup j < leaves owns leaf j (panel columns
+// 32j ..): it applies leaves 0..j-1 to its rows, factors its diagonal block, forms
+// U12 = L11^-1 A12 for the panel columns right of the leaf, stores its rows and the block
+// inverses (inv[j] = U11^-1 then L11^-1) and raises flags[j] to epoch.  Every other
+// workgroup applies the leaves as their flags rise.  Workgroups wait only for
+// lower-numbered ones, which are dispatched first; every wait is bounded and a timeout
+// raises err (the factors are then not to be trusted).
+__global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __restrict__ T, size_t ldt,
+                                                 float* __restrict__ inv, int* __restrict__ flags, int epoch,
+                                                 int* __restrict__ err) {
+    __shared__ float a[LPW][NPLD];   // this workgroup's rows, panel columns 0 .. w
+    __shared__ float ub[LPW][NPLD];  // the current leaf's rows
+    __shared__ float ui[LPW][LPW + 1], li[LPW][LPW + 1], d[LPW][LPW + 1];
+    __shared__ int bail;
+    const int tid = threadIdx.x, wb = blockIdx.x;
+    const int row0 = r0 + wb * LPW;
+    const int nr = min(LPW, n - row0);
+    const int leaves = (w + LPW - 1) / LPW;
+    const int col = tid;  // one panel column per thread in loads and stores
+    float* const P = T + (size_t)r0;  // column r0 of A
+    for (int r = 0; r < LPW; r++) a[r][col] = (col < w && r < nr) ? P[col + (size_t)(row0 + r) * ldt] : 0.f;
+    __syncthreads();
+    const int napply = min(wb, leaves);
+    for (int j = 0; j < napply; j++) {
+        const int cj = j * LPW, lw = min(LPW, w - cj);
+        if (tid == 0) {
+            long sp = 0;
+            int ok = 1;
+            while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                if (++sp > kPanelSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bail = !ok;
+        }
+        __syncthreads();
+        if (bail) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        for (int r = 0; r < LPW; r++)
+            ub[r][col] = (r < lw && col >= cj && col < w) ? P[col + (size_t)(r0 + cj + r) * ldt] : 0.f;
+        for (int e = tid; e < LPW * LPW; e += 256) ui[e / LPW][e % LPW] = inv[(size_t)j * 2 * LPW * LPW + e];
+        __syncthreads();
+        rbt_apply_leaf(a, ub, ui, cj, lw, w);
+    }
+    if (wb < leaves) {
+        // the owner of leaf wb: its rows are the leaf's diagonal block (a partial leaf only
+        // occurs in a square last panel, where it is also the last rows)
+        const int cj = wb * LPW, lw = min(LPW, w - cj);
+        if (tid < LPW) diag_lu_rows(a, d, cj, lw);
+        __syncthreads();
+        if (tid < 64) tri_inv_col<false>(d, ui, tid);
+        else if (tid < 128) tri_inv_col<true>(d, li, tid - 64);
+        __syncthreads();
+        const int c = cj + lw + tid;
+        if (c < w) {  // U12 = L11^-1 A12 by forward substitution, one column per thread
+#pragma unroll 1
+            for (int r = 1; r < LPW; r++) {
+                float v = a[r][c];
+                for (int s2 = 0; s2 < r; s2++) v = fmaf(-d[r][s2], a[s2][c], v);
+                a[r][c] = v;
+            }
+        }
+        if (tid < lw)
+            for (int r = 0; r < lw; r++) a[r][cj + tid] = d[r][tid];
+        __syncthreads();
+        for (int r = 0; r < nr; r++)
+            if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
+        float* io = inv + (size_t)wb * 2 * LPW * LPW;
+        for (int e = tid; e < LPW * LPW; e += 256) {
+            io[e] = ui[e / LPW][e % LPW];
+            io[LPW * LPW + e] = li[e / LPW][e % LPW];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flags + wb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int r = 0; r < nr; r++)
+        if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
+}
+
+// U12 = L11^-1 A12 in place on the A-columns [c_lo, c_hi) of block rows [k, k+nb), L11 the
+// unit lower factor of the panel at A(k.., k..) with its diagonal-block inverses in inv.
+// One workgroup per 64 columns holds its nb x 64 block in LDS: blocked forward
+// substitution, right-looking over the 32-row blocks; L is read with wavefront-uniform
+// (scalar) loads.
+__global__ void __launch_bounds__(256) trsm_rbt(int k, int nb, int c_lo, int c_hi, float* __restrict__ T, size_t ldt,
+                                                const float* __restrict__ inv) {
+    __shared__ float x[kMixNB][TRC + 1];
+    const int tid = threadIdx.x, cl = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = c_lo + blockIdx.x * TRC + cl;
+    const bool cv = c < c_hi;
+    for (int r = wv; r < nb; r += 4) x[r][cl] = cv ? T[(size_t)c + (size_t)(k + r) * ldt] : 0.f;
+    __syncthreads();
+    const int nl = (nb + LPW - 1) / LPW;
+    for (int j = 0; j < nl; j++) {
+        const int rj = j * LPW;
+        const float* li = inv + (size_t)j * 2 * LPW * LPW + LPW * LPW;
+        float xs[LPW];
+#pragma unroll
+        for (int s = 0; s < LPW; s++) xs[s] = rj + s < nb ? x[rj + s][cl] : 0.f;
+        float y[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int r = wv * 8 + i;
+            float v = 0.f;
+#pragma unroll
+            for (int s = 0; s < LPW; s++) v = fmaf(li[r * LPW + s], xs[s], v);
+            y[i] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (rj + wv * 8 + i < nb) x[rj + wv * 8 + i][cl] = y[i];
+        __syncthreads();
+        if (rj + LPW < nb) {
+            float xj[LPW];
+#pragma unroll
+            for (int s = 0; s < LPW; s++) xj[s] = x[rj + s][cl];
+            for (int r = rj + LPW + wv; r < nb; r += 4) {
+                const float* lr = T + (size_t)(k + rj) + (size_t)(k + r) * ldt;  // L(k+r, k+rj ..)
+                float v = 0.f;
+#pragma unroll
+                for (int s = 0; s < LPW; s++) v = fmaf(lr[s], xj[s], v);
+                x[r][cl] -= v;
+            }
+            __syncthreads();
+        }
+    }
+    for (int r = wv; r < nb; r += 4)
+        if (cv) T[(size_t)c + (size_t)(k + r) * ldt] = x[r][cl];
+}
+
 }  // namespace
 
 
@@ -380,6 +632,7 @@ void launch_laswp_t(hipStream_t st, int cl, int ch, float* T, size_t ldt, int k1
 
 struct LuCtx {
     hipStream_t st;
+    bool pivot;            // tournament pivoting; false: no interchanges (butterfly-randomised input)
     int swap_lo, swap_hi;  // columns of A that take a leaf's interchanges at once
     int n, lp;
     float* T;
@@ -390,25 +643,33 @@ struct LuCtx {
     int* lcount;
     int* cnt;
     int lvl_stride;
-    float* top;
     void* gws;
+    float* inv;  // the butterfly mode: the current panel's diagonal-block inverses
+    int* flags;  // the butterfly mode: per-leaf flags of panel_rbt
+    int* err;
 };
 
 void calu_panel(const LuCtx& c, int r0, int w) {
     const int rows = c.n - r0;
-    const int G = (rows + GROWS - 1) / GROWS;
-    hipLaunchKernelGGL(calu_wg, dim3(G), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.lists, c.lcount, c.cnt,
-                       c.lvl_stride, c.ipiv, c.top);
-    BOLA_LAUNCH_CHECK();
-    const int k = std::min(w, rows);
-    launch_laswp_t(c.st, c.swap_lo, c.swap_hi, c.T, c.ldt, r0, r0 + k, c.ipiv, c.perm);
+    if (c.pivot) {
+        const int G = (rows + GROWS - 1) / GROWS;
+        hipLaunchKernelGGL(calu_wg, dim3(G), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, c.lists, c.lcount, c.cnt,
+                           c.lvl_stride, c.ipiv);
+        BOLA_LAUNCH_CHECK();
+        const int k = std::min(w, rows);
+        launch_laswp_t(c.st, c.swap_lo, c.swap_hi, c.T, c.ldt, r0, r0 + k, c.ipiv, c.perm);
+    }
     const int below = std::max(0, rows - w);
-    if (w == LPW)
-        hipLaunchKernelGGL(panel_nopiv<true>, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n, r0,
-                           r0, w, c.T, c.ldt, c.top);
-    else
-        hipLaunchKernelGGL(panel_nopiv<false>, dim3(std::max(1u, cdiv((size_t)below, 256))), dim3(256), 0, c.st, c.n,
-                           r0, r0, w, c.T, c.ldt, c.top);
+    if (below > 0) {
+        if (w == LPW)
+            hipLaunchKernelGGL(panel_nopiv<true>, dim3(cdiv((size_t)below, 256)), dim3(256), 0, c.st, c.n, r0, r0, w,
+                               c.T, c.ldt, 0);
+        else
+            hipLaunchKernelGGL(panel_nopiv<false>, dim3(cdiv((size_t)below, 256)), dim3(256), 0, c.st, c.n, r0, r0, w,
+                               c.T, c.ldt, 0);
+        BOLA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(panel_nopiv<false>, dim3(1), dim3(256), 0, c.st, c.n, r0, r0, w, c.T, c.ldt, 1);
     BOLA_LAUNCH_CHECK();
 }
 
@@ -456,7 +717,13 @@ void lu_rec(const LuCtx& c, int r0, int w) {
 void update_cols(const LuCtx& c, int k, int nb, int c_lo, int c_hi) {
     const int ncols = c_hi - c_lo;
     if (ncols <= 0) return;
-    strsm_rupu(c, ncols, nb, k, k, c_lo);
+    if (c.pivot) {
+        strsm_rupu(c, ncols, nb, k, k, c_lo);
+    } else {
+        hipLaunchKernelGGL(trsm_rbt, dim3(cdiv((size_t)ncols, TRC)), dim3(256), 0, c.st, k, nb, c_lo, c_hi, c.T,
+                           c.ldt, c.inv);
+        BOLA_LAUNCH_CHECK();
+    }
     const int below = c.n - k - nb;
     if (below > 0)
         launch_lp_gemm(c.st, c.lp, ncols, below, nb, Tp(c, c_lo, k), c.ldt, Tp(c, k, k + nb), c.ldt,
@@ -469,6 +736,7 @@ size_t mixed_lu_ws_bytes(int n, int lp) {
     const size_t G = (size_t)(n + GROWS - 1) / GROWS;
     size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
     b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 256);
+    b += 2 * kInvPanel * sizeof(float) + 1024;
     return b;
 }
 
@@ -476,9 +744,11 @@ size_t mixed_lu_ws_bytes(int n, int lp) {
 // leaves), with look-ahead: stream 1 factors panel k, applies its interchanges and
 // updates the next panel's columns; stream 2 meanwhile applies panel k to all columns
 // further right.  The streams never work on the same columns at the same time.
-void mixed_lu_factor(bola_hip_handle_s* h, int lp, int n, float* T, size_t ldt, int32_t* ipiv, int* perm, void* ws) {
+void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, size_t ldt, int32_t* ipiv, int* perm,
+                     void* ws, int* err) {
     LuCtx c;
     c.st = h->stream;
+    c.pivot = pivot;
     c.n = n;
     c.lp = lp;
     c.T = T;
@@ -496,15 +766,21 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, int n, float* T, size_t ldt, 
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
     c.cnt = (int*)(b + o);
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
-    c.top = (float*)(b + o);
-    o += (LPW * LPW * sizeof(float) + 255) & ~(size_t)255;
     const size_t gsz = (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 511) & ~(size_t)255;
     c.gws = b + o;
+    float* const invb = (float*)(b + o + 2 * gsz);  // two panels' inverses, by panel parity
+    c.flags = (int*)(invb + 2 * kInvPanel);
+    c.err = err;
     LuCtx c2 = c;  // stream 2: its own 16-bit operand buffers
     c2.st = h->aux_stream();
     c2.gws = b + o + gsz;
     hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, c.st, c.cnt, levels * G);
+    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(64), 0, c.st, c.flags, kMixNB / LPW);
     BOLA_LAUNCH_CHECK();
+    if (!pivot) {  // identity interchanges: the solve path stays the same
+        hipLaunchKernelGGL(k_iota_piv, dim3(cdiv((size_t)n, 256)), dim3(256), 0, c.st, n, ipiv);
+        BOLA_LAUNCH_CHECK();
+    }
     int ev = 0;
     hipEvent_t bulk_done = nullptr;
     for (int k = 0; k < n; k += kMixNB) {
@@ -512,11 +788,21 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, int n, float* T, size_t ldt, 
         // the panel: its own columns take the interchanges at once
         c.swap_lo = k;
         c.swap_hi = k + nb;
-        lu_rec(c, k, nb);
+        if (pivot) {
+            lu_rec(c, k, nb);
+        } else {
+            // stream 2 may still read the previous panel's inverses: alternate two buffers
+            c.inv = c2.inv = invb + (size_t)((k / kMixNB) & 1) * kInvPanel;
+            hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(n - k), LPW)), dim3(256), 0, c.st, n, k, nb, T, ldt, c.inv,
+                               c.flags, k / kMixNB + 1, err);
+            BOLA_LAUNCH_CHECK();
+        }
         // the columns right of the panel are stream 2's until its previous bulk update is done
         if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
-        launch_laswp_t(c.st, 0, k, T, ldt, k, k + nb, ipiv, nullptr);
-        launch_laswp_t(c.st, k + nb, n, T, ldt, k, k + nb, ipiv, nullptr);
+        if (c.pivot) {
+            launch_laswp_t(c.st, 0, k, T, ldt, k, k + nb, ipiv, nullptr);
+            launch_laswp_t(c.st, k + nb, n, T, ldt, k, k + nb, ipiv, nullptr);
+        }
         // look-ahead: the next panel's columns
         const int c_mid = std::min(n, k + nb + kMixNB);
         update_cols(c, k, nb, k + nb, c_mid);

@@ -162,7 +162,8 @@ int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, c
  * No crate counterpart: the crate solves in FP64 only.  A X = B (n x n, nrhs <= 16
  * right-hand sides, column-major f64) to FP64 accuracy from a low-precision LU:
  * power-of-two row/column equilibration, LU of the scaled matrix in fp32 storage with
- * tournament pivoting and trailing products in fp32, fp16 or bf16 on the MFMA units,
+ * tournament pivoting (or, with BOLA_HIP_MIX_RBT, butterfly randomisation and no
+ * interchanges) and trailing products in fp32, fp16 or bf16 on the MFMA units,
  * iterative refinement in FP64 (r = b - A x on the original A), GMRES-IR when plain
  * refinement stagnates, and the FP64 GETRF + GETRS of this library as the last resort.
  * Stops when every column's backward error |r|_inf / (|A|_inf |x|_inf + |b|_inf) is at most
@@ -172,6 +173,10 @@ int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, c
 #define BOLA_HIP_PREC_FP32 0
 #define BOLA_HIP_PREC_FP16 1
 #define BOLA_HIP_PREC_BF16 2
+/* Or-ed into `precision`: randomise A with two-level random butterflies on both sides
+ * and factor without interchanges (instead of tournament pivoting); for n divisible by 4,
+ * otherwise ignored.  The refinement and the fallback are unchanged. */
+#define BOLA_HIP_MIX_RBT 0x100
 typedef struct bola_hip_mixed_stats {
     int ir_iterations;    /* refinement steps with the low-precision factors */
     int gmres_cycles;     /* GMRES-IR restarts */

@@ -57,7 +57,7 @@ def test_sgemm_lp_matches_fp64_product_of_rounded_operands(B, prec, m, n, k):
     assert np.all(np.abs(got - want) <= bound + 1e-30), float(np.max(np.abs(got - want) - bound))
 
 
-def _solve(B, n, nrhs, kappa, prec, seed=7):
+def _solve(B, n, nrhs, kappa, prec, seed=7, rbt=False):
     import torch
     from bola_amd import testmat
     A = testmat.conditioned(n, kappa, seed, device="cuda")
@@ -65,7 +65,7 @@ def _solve(B, n, nrhs, kappa, prec, seed=7):
     b = rng.uniform(-1, 1, n * nrhs)
     db = torch.from_numpy(b).cuda()
     dx = torch.zeros(n * nrhs, dtype=torch.float64, device="cuda")
-    st = B.dgesv_mixed(n, nrhs, A, n, db, n, dx, n, prec)
+    st = B.dgesv_mixed(n, nrhs, A, n, db, n, dx, n, prec, rbt=rbt)
     torch.cuda.synchronize()
     return A.cpu().numpy(), b, dx.cpu().numpy(), st
 
@@ -79,12 +79,13 @@ def _berr(Af, b, x, n, nrhs):
     return np.max(np.max(np.abs(R), axis=0) / (an * np.max(np.abs(X), axis=0) + np.max(np.abs(Bm), axis=0)))
 
 
+@pytest.mark.parametrize("rbt", [False, True], ids=["tournament", "butterfly"])
 @pytest.mark.parametrize("prec,kappa", [("fp32", 1e2), ("fp32", 1e5), ("fp32", 1e8), ("fp16", 1e2),
                                         ("fp16", 1e5), ("bf16", 1e2), ("bf16", 1e5)])
 @pytest.mark.parametrize("n,nrhs", [(512, 1), (1024, 16), (1000, 3)])
-def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs):
-    Af, b, x, st = _solve(B, n, nrhs, kappa, prec)
-    print(f"n={n} nrhs={nrhs} {prec} kappa={kappa:g}: {st}")
+def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs, rbt):
+    Af, b, x, st = _solve(B, n, nrhs, kappa, prec, rbt=rbt)
+    print(f"n={n} nrhs={nrhs} {prec} kappa={kappa:g} rbt={rbt}: {st}")
     assert st["converged"] == 1, st
     assert st["final_berr"] <= n * EPS64
     # the same criterion recomputed on the host
@@ -137,3 +138,21 @@ def test_mixed_rejects_bad_arguments(B):
     with pytest.raises(B.BolaError) as e:
         B.dgesv_mixed(4, 1, a, 3, x, 4, x, 4, "fp32")
     assert e.value.code == -6
+
+
+def test_mixed_butterfly_needs_no_pivoting(B, O):
+    # a matrix whose leading entry is zero cannot be factored without interchanges, but
+    # its butterfly-randomised image can
+    import torch
+    n, nrhs = 256, 1
+    rng = np.random.default_rng(3)
+    M = rng.uniform(-1, 1, (n, n))
+    M[0, 0] = 0.0
+    M[:, 0] = 0.0
+    M[1, 0] = 1.0
+    a = torch.from_numpy(colmajor(M)).cuda()
+    b = rng.uniform(-1, 1, n)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = B.dgesv_mixed(n, nrhs, a, n, torch.from_numpy(b).cuda(), n, x, n, "fp32", rbt=True)
+    assert st["converged"] == 1 and st["fallback"] == 0, st
+    assert _berr(colmajor(M), b, x.cpu().numpy(), n, nrhs) <= 2 * n * EPS64
