@@ -141,6 +141,22 @@ void collect_profile(bola_hip_handle_s* h);
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// v when keep, else +0.  Written as a bitwise AND with a mask the compiler cannot see
+// through, so that the load feeding v is not moved into a branch on keep (where it would
+// be waited for on its own): loads from clamped addresses then all stay in flight at once.
+__device__ __forceinline__ float keep_or_zero(float v, bool keep) {
+    uint32_t m = keep ? 0xFFFFFFFFu : 0u;
+    asm volatile("" : "+v"(m));
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
+}
+__device__ __forceinline__ double keep_or_zero(double v, bool keep) {
+    uint32_t m = keep ? 0xFFFFFFFFu : 0u;
+    asm volatile("" : "+v"(m));
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t mm = ((uint64_t)m << 32) | m;
+    return __builtin_bit_cast(double, b & mm);
+}
+
 // The i-th ordering event of a handle (created on first use, timing disabled).
 inline hipEvent_t handle_event(bola_hip_handle_s* h, int i) {
     while ((int)h->sync_events.size() <= i) {

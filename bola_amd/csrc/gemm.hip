@@ -93,23 +93,28 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
     // running C (reference: C after beta, updated once per 64-slab) and slab accumulator
     d4 crun[Cfg::TM][Cfg::TN];
     d4 acc[Cfg::TM][Cfg::TN];
+    // C from clamped addresses, every load in flight at once (C is not read when beta == 0)
 #pragma unroll
     for (int a = 0; a < Cfg::TM; a++)
 #pragma unroll
         for (int b = 0; b < Cfg::TN; b++) {
-            int gc = col0 + wn + 16 * b + lr;
+            const int gc = col0 + wn + 16 * b + lr;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int gr = row0 + wm + 16 * a + lk + 4 * r;
+                const int gr = row0 + wm + 16 * a + lk + 4 * r;
                 double v = 0.0;
-                if (gr < m && gc < n && beta != 0.0) {
-                    v = C[gr + (size_t)gc * ldc];
-                    if (beta != 1.0) v = v * beta;
-                }
+                if (beta != 0.0) v = keep_or_zero(C[min(gr, m - 1) + (size_t)min(gc, n - 1) * ldc], gr < m && gc < n);
                 crun[a][b][r] = v;
                 acc[a][b][r] = 0.0;
             }
         }
+    if (beta != 0.0 && beta != 1.0)
+#pragma unroll
+        for (int a = 0; a < Cfg::TM; a++)
+#pragma unroll
+            for (int b = 0; b < Cfg::TN; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) crun[a][b][r] = crun[a][b][r] * beta;
 
     double ra[Cfg::A_LOADS], rb[Cfg::B_LOADS];
     auto gload = [&](int k0) {
@@ -118,18 +123,18 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
             int i, kk;
             a_coord<BM, TA>(t, s, i, kk);
             int gi = row0 + i, gk = k0 + kk;
-            double v = 0.0;
-            if (gi < m && gk < k) v = TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda];
-            ra[s] = v;
+            const int ci = min(gi, m - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
+            const double v = TA ? A[ck + (size_t)ci * lda] : A[ci + (size_t)ck * lda];
+            ra[s] = keep_or_zero(v, gi < m && gk < k);
         }
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
             b_coord<BN, TB>(t, s, kk, j);
             int gj = col0 + j, gk = k0 + kk;
-            double v = 0.0;
-            if (gj < n && gk < k) v = TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb];
-            rb[s] = v;
+            const int cj = min(gj, n - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
+            const double v = TB ? B[cj + (size_t)ck * ldb] : B[ck + (size_t)cj * ldb];
+            rb[s] = keep_or_zero(v, gj < n && gk < k);
         }
     };
     auto lstore = [&](int buf) {

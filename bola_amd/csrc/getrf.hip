@@ -251,7 +251,10 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
         valid[s] = r < m;
         pos[s] = (int)r;
 #pragma unroll
-        for (int c = 0; c < PW; c++) a[s][c] = (valid[s] && c < w) ? A[r + (size_t)c * lda] : 0.0;
+        for (int c = 0; c < PW; c++) {  // clamped addresses: every load in flight at once
+            const double v = A[min(r, (long)m - 1) + (size_t)min(c, w - 1) * lda];
+            a[s][c] = keep_or_zero(v, valid[s] && c < w);
+        }
     }
     for (int e = tid; e < 2 * (2 * PW + 2); e += PT) (&prow[0][0])[e] = 0.0;
     if (tid < 2) pzero[tid] = ~0ull;
@@ -659,9 +662,14 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
         rowid[s] = r;
         valid[s] = r < m;
         pos[s] = (int)r;
+        const long rc = min(r, (long)m - 1);  // clamped addresses: every load in flight at once
 #pragma unroll
-        for (int c = 0; c < WW; c++) a[s][c] = (valid[s] && c < wl) ? A[r + (size_t)c * lda] : 0.0;
-        al[s] = valid[s] ? A[r + (size_t)wl * lda] : 0.0;
+        for (int c = 0; c < WW; c++) {
+            const double v = A[rc + (size_t)min(c, max(wl - 1, 0)) * lda];
+            a[s][c] = keep_or_zero(v, valid[s] && c < wl);
+        }
+        const double vl = A[rc + (size_t)wl * lda];
+        al[s] = keep_or_zero(vl, valid[s]);
     }
     for (int e = lane; e < 2 * (2 * PW + 2); e += WV) (&prow[0][0])[e] = 0.0;
     uint64_t pz0 = ~0ull, pz1 = ~0ull;  // zero masks of prow[0], prow[1]

@@ -32,10 +32,10 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
 
 namespace {
 
-constexpr int TB = 64;        // triangular-solve block
+constexpr int TB = 128;       // triangular-solve block (rows per workgroup of trsv_blk)
 constexpr int NRMAX = 16;     // right-hand sides per solve call
 constexpr int RCH = 32;       // column chunks of the residual kernel
-constexpr long kTrsvSpin = 1L << 24;
+constexpr long kTrsvSpin = 1L << 22;
 
 __device__ __forceinline__ void ast(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -188,29 +188,68 @@ __global__ void col_scale(int n, int nrhs, const double* __restrict__ W, size_t 
     for (int q = 0; q < nrhs; q++) D[(size_t)i + (size_t)q * ldd] = sc * W[(size_t)i + (size_t)q * ldw];
 }
 
+// Inverses of the TB x TB diagonal blocks of L (unit lower, blockIdx.y == 0) and U
+// (blockIdx.y == 1) in T, in fp32, row-major per block: D[((y nb + bi) TB + r) TB + c].
+// Blocks past n are completed with the identity.  Lane b substitutes column b of the
+// inverse, two accumulators per row.
+__global__ void __launch_bounds__(TB) diag_inv(int n, const float* __restrict__ T, size_t ldt, float* __restrict__ D,
+                                               int nb) {
+    __shared__ float f[TB][TB + 1];
+    __shared__ float x[TB][TB + 1];
+    const int bi = blockIdx.x, up = blockIdx.y, b = threadIdx.x;
+    const int r0 = bi * TB, rows = min(TB, n - r0);
+    for (int r = 0; r < TB; r++)  // f[r][c] = factor(r0 + r, r0 + c)
+        f[r][b] = (r < rows && b < rows) ? T[(size_t)(r0 + b) + (size_t)(r0 + r) * ldt] : (r == b ? 1.f : 0.f);
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < TB; i++) {
+        const int a = up ? TB - 1 - i : i;
+        const int lo = up ? a + 1 : 0, hi = up ? TB : a;
+        float v0 = a == b ? 1.f : 0.f, v1 = 0.f;
+        int c = lo;
+#pragma unroll 4
+        for (; c + 1 < hi; c += 2) {
+            v0 = fmaf(-f[a][c], x[c][b], v0);
+            v1 = fmaf(-f[a][c + 1], x[c + 1][b], v1);
+        }
+        if (c < hi) v0 = fmaf(-f[a][c], x[c][b], v0);
+        v0 += v1;
+        x[a][b] = up ? v0 / f[a][a] : v0;
+    }
+    __syncthreads();
+    float* o = D + ((size_t)up * nb + bi) * TB * TB;
+    for (int r = 0; r < TB; r++) o[(size_t)r * TB + b] = x[r][b];
+}
+
 // Z := L^-1 Z (LOWER, L unit lower) or U^-1 Z (upper), factors in T (fp32), FP64
-// arithmetic; Z is n x nrhs (ld ldz).  Block bi = rows [64 bi, 64 bi + 64).
+// arithmetic; Z is n x nrhs (ld ldz).  Block bi = rows [TB bi, TB bi + TB), one workgroup
+// each, dispatched in the order the blocks are solved.  A block subtracts the products
+// with the blocks it depends on as their flags rise (the oldest first), then applies the
+// inverse of its diagonal block (diag_inv) and raises its own flag.  Thread (r, h) holds
+// row r's entries in columns [64 h, 64 h + 64) of a block; the two halves meet in LDS.
 template <bool LOWER>
 __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __restrict__ T, size_t ldt,
-                                                double* __restrict__ Z, size_t ldz, int* __restrict__ flags,
-                                                int epoch, int nb, long spin_limit, int* __restrict__ err) {
-    __shared__ double ys[TB][NRMAX + 1];
-    __shared__ float Ld[TB][TB + 1];
+                                                const float* __restrict__ Dinv, double* __restrict__ Z, size_t ldz,
+                                                int* __restrict__ flags, int epoch, int nb, long spin_limit,
+                                                int* __restrict__ err) {
+    constexpr int HW = TB / 2;
+    __shared__ double ys[TB][NRMAX + 1];       // the incoming block of the solution
+    __shared__ double acc[TB][NRMAX + 1];      // this block's right-hand side, updated
+    __shared__ double part[2][TB][NRMAX + 1];  // the halves' partial products
     __shared__ int ok_sh;
-    for (int e = threadIdx.x; e < TB * (NRMAX + 1); e += 256) (&ys[0][0])[e] = 0.0;
     const int bi = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
     const int r0 = bi * TB, rows = min(TB, n - r0);
-    const int tid = threadIdx.x, r = tid & 63, qg = tid >> 6;
-    double acc[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int q = qg + 4 * u;
-        acc[u] = (r < rows && q < nrhs) ? Z[(size_t)(r0 + r) + (size_t)q * ldz] : 0.0;
+    const int tid = threadIdx.x, r = tid % TB, h = tid / TB;
+    for (int e = tid; e < TB * nrhs; e += 256) {
+        const int rr = e % TB, q = e / TB;
+        acc[rr][q] = rr < rows ? Z[(size_t)(r0 + rr) + (size_t)q * ldz] : 0.0;
     }
-    // diagonal block, A orientation: Ld[r][c] = factor(r0 + r, r0 + c) = T[(r0+c) + (r0+r) ldt]
-    for (int e = tid; e < TB * TB; e += 256) {
-        const int rr = e >> 6, cc = e & 63;
-        Ld[rr][cc] = (rr < rows && cc < rows) ? T[(size_t)(r0 + cc) + (size_t)(r0 + rr) * ldt] : 0.f;
+    // this row's half of the diagonal block's inverse, wanted last: loaded first
+    float dr[HW];
+    {
+        const float* drow = Dinv + (((size_t)(LOWER ? 0 : 1) * nb + bi) * TB + r) * TB + h * HW;
+#pragma unroll
+        for (int c = 0; c < HW; c++) dr[c] = drow[c];
     }
     const int ndep = LOWER ? bi : nb - 1 - bi;
     for (int jj = 0; jj < ndep; jj++) {
@@ -218,18 +257,25 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
         const int j = LOWER ? jj : nb - 1 - jj;
         const int c0 = j * TB, cols = min(TB, n - c0);
         // this row's factor entries in block j do not depend on the flag: load them first
-        float lr[TB];
+        // (clamped addresses, so that all the loads are in flight at once; masked after)
+        float lr[HW];
         const float* lrow = T + (size_t)c0 + (size_t)(r0 + min(r, rows - 1)) * ldt;
 #pragma unroll
-        for (int c = 0; c < TB; c++) lr[c] = c < cols ? lrow[c] : 0.f;
+        for (int c = 0; c < HW; c++) lr[c] = lrow[min(h * HW + c, cols - 1)];
+#pragma unroll
+        for (int c = 0; c < HW; c++) lr[c] = keep_or_zero(lr[c], h * HW + c < cols);
         if (tid == 0) {
             long sp = 0;
             int ok = 1;
+            // the block solved next polls eagerly; the others sparingly, so that the
+            // waiting blocks do not crowd the flag's cache line
+            const bool next = j == (LOWER ? bi - 1 : bi + 1);
             while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
                 if (++sp > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                     ok = 0;
                     break;
                 }
+                if (!next) __builtin_amdgcn_s_sleep(32);
             }
             ok_sh = ok;
         }
@@ -243,56 +289,36 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
             ys[c][q] = c < cols ? ald(Z + (size_t)(c0 + c) + (size_t)q * ldz) : 0.0;
         }
         __syncthreads();
-        if (r < rows) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            if (nrhs == 1) {
-                if (qg == 0) {
+        for (int q = 0; q < nrhs; q++) {
+            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-                    for (int c = 0; c < TB; c++) s0 = fma((double)lr[c], ys[c][0], s0);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < TB; c++) {
-                    const double l = (double)lr[c];
-                    s0 = fma(l, ys[c][qg], s0);
-                    s1 = fma(l, ys[c][qg + 4], s1);
-                    s2 = fma(l, ys[c][qg + 8], s2);
-                    s3 = fma(l, ys[c][qg + 12], s3);
-                }
+            for (int c = 0; c < HW; c += 2) {
+                s0 = fma((double)lr[c], ys[h * HW + c][q], s0);
+                s1 = fma((double)lr[c + 1], ys[h * HW + c + 1][q], s1);
             }
-            acc[0] -= s0;
-            acc[1] -= s1;
-            acc[2] -= s2;
-            acc[3] -= s3;
+            part[h][r][q] = s0 + s1;
         }
         __syncthreads();
-    }
-    __syncthreads();  // Ld complete
-    // the diagonal block: each wavefront solves its own right-hand sides, lane = row
-#pragma unroll
-    for (int kk = 0; kk < TB; kk++) {
-        const int k = LOWER ? kk : TB - 1 - kk;
-        if (k < rows) {
-            if (!LOWER && r == k) {
-                const double d = (double)Ld[k][k];
-#pragma unroll
-                for (int u = 0; u < 4; u++) acc[u] = acc[u] / d;
-            }
-            double yk[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) yk[u] = rdl(acc[u], k);
-            const bool below = LOWER ? (r > k) : (r < k);
-            if (below && r < rows) {
-                const double l = (double)Ld[r][k];
-#pragma unroll
-                for (int u = 0; u < 4; u++) acc[u] = fma(-l, yk[u], acc[u]);
-            }
+        for (int e = tid; e < TB * nrhs; e += 256) {
+            const int rr = e % TB, q = e / TB;
+            acc[rr][q] -= part[0][rr][q] + part[1][rr][q];
         }
     }
+    __syncthreads();
+    // the diagonal block: y = D^-1 acc
+    for (int q = 0; q < nrhs; q++) {
+        double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int q = qg + 4 * u;
-        if (r < rows && q < nrhs) ast(Z + (size_t)(r0 + r) + (size_t)q * ldz, acc[u]);
+        for (int c = 0; c < HW; c += 2) {
+            s0 = fma((double)dr[c], acc[h * HW + c][q], s0);
+            s1 = fma((double)dr[c + 1], acc[h * HW + c + 1][q], s1);
+        }
+        part[h][r][q] = s0 + s1;
+    }
+    __syncthreads();
+    for (int e = tid; e < TB * nrhs; e += 256) {
+        const int rr = e % TB, q = e / TB;
+        if (rr < rows) ast(Z + (size_t)(r0 + rr) + (size_t)q * ldz, part[0][rr][q] + part[1][rr][q]);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -428,6 +454,7 @@ struct Solver {
     size_t lda;
     // device buffers
     float* T;
+    float* Dinv;  // inverses of the diagonal blocks of L and U (diag_inv)
     size_t ldt;
     int32_t* ipiv;
     int* perm;
@@ -459,12 +486,12 @@ void apply_minv(Solver& S, const double* Xr, size_t ldx, double* Y) {
     }
     const int nb = (S.n + TB - 1) / TB;
     S.epoch++;
-    hipLaunchKernelGGL(trsv_blk<true>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
-                       S.flags, S.epoch, nb, kTrsvSpin, S.err);
+    hipLaunchKernelGGL(trsv_blk<true>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Dinv, S.Z,
+                       (size_t)S.n, S.flags, S.epoch, nb, kTrsvSpin, S.err);
     BOLA_LAUNCH_CHECK();
     S.epoch++;
-    hipLaunchKernelGGL(trsv_blk<false>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Z, (size_t)S.n,
-                       S.flags, S.epoch, nb, kTrsvSpin, S.err);
+    hipLaunchKernelGGL(trsv_blk<false>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Dinv, S.Z,
+                       (size_t)S.n, S.flags, S.epoch, nb, kTrsvSpin, S.err);
     BOLA_LAUNCH_CHECK();
     if (S.rbt) {  // W_R y = B2 (B1 y)
         hipLaunchKernelGGL(rbt_vec<false>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 2, S.rr[0]);
@@ -691,6 +718,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     const size_t oflags = take(sizeof(int) * ((size_t)n / TB + 2)), oerr = take(sizeof(int) * 4);
     const size_t olu = take(mixed_lu_ws_bytes(n, precision));
     const size_t orb = take(8 * 4 * (size_t)n);
+    const size_t odi = take(sizeof(float) * 2 * (size_t)((n + TB - 1) / TB) * TB * TB);
     char* base = (char*)h->mws.get(off);
     S.T = (float*)(base + oT);
     S.ipiv = (int32_t*)(base + oipiv);
@@ -711,6 +739,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     S.flags = (int*)(base + oflags);
     S.err = (int*)(base + oerr);
     S.luws = base + olu;
+    S.Dinv = (float*)(base + odi);
     for (int l = 0; l < 2; l++) {
         S.rl[l] = (double*)(base + orb) + (size_t)l * n;
         S.rr[l] = (double*)(base + orb) + (size_t)(2 + l) * n;
@@ -759,6 +788,9 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
         }
         // 2. the low-precision LU
         mixed_lu_factor(h, precision, !S.rbt, n, S.T, S.ldt, S.ipiv, S.perm, S.luws, S.err);
+        const int nbk = (n + TB - 1) / TB;
+        hipLaunchKernelGGL(diag_inv, dim3((unsigned)nbk, 2), dim3(TB), 0, st, n, S.T, S.ldt, S.Dinv, nbk);
+        BOLA_LAUNCH_CHECK();
     }
     BOLA_CHECK(hipEventRecord(ev[1], st));
     // |A|_inf and |b|_inf

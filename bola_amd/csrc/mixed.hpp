@@ -12,6 +12,9 @@ enum { LP_F32 = 0, LP_F16 = 1, LP_BF16 = 2 };
 // C (fp32, column-major) -= A * B with the products in precision lp; ws holds the 16-bit
 // operand copies (lp_gemm_ws_bytes).
 size_t lp_gemm_ws_bytes(int lp, size_t m, size_t n, size_t k);
+// C = -(A * B), fp32 throughout (C's old contents are not read)
+void launch_sgemm_set(hipStream_t st, int m, int n, int k, const float* A, size_t lda, const float* B, size_t ldb,
+                      float* C, size_t ldc);
 void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A, size_t lda, const float* B,
                     size_t ldb, float* C, size_t ldc, void* ws);
 

@@ -37,7 +37,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // ---- fp32 operands.  A column-major (lda), B column-major (ldb).  BK = 32 floats.
 __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float* __restrict__ A, size_t lda,
                                                 const float* __restrict__ B, size_t ldb, float* __restrict__ C,
-                                                size_t ldc, int tiles_m, int tiles_n) {
+                                                size_t ldc, int tiles_m, int tiles_n, int overwrite) {
     constexpr int BK = 32;
     __shared__ float As[2][GBM * RW];
     __shared__ float Bs[2][GBN * RW];
@@ -63,13 +63,17 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
             const int gi = i0 + ii, gk = k0 + kk;
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                ra[q][r] = (gi + r < m && gk < k) ? A[(size_t)(gi + r) + (size_t)gk * lda] : 0.f;
+                ra[q][r] = A[(size_t)min(gi + r, m - 1) + (size_t)min(gk, k - 1) * lda];  // clamped
+#pragma unroll
+            for (int r = 0; r < 4; r++) ra[q][r] = keep_or_zero(ra[q][r], gi + r < m && gk < k);
             const int kb = (e & 7) * 4;       // k chunk
             const int jj = e >> 3;            // column within tile
             const int gj = j0 + jj, gkb = k0 + kb;
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                rb[q][r] = (gj < n && gkb + r < k) ? B[(size_t)(gkb + r) + (size_t)gj * ldb] : 0.f;
+                rb[q][r] = B[(size_t)min(gkb + r, k - 1) + (size_t)min(gj, n - 1) * ldb];  // clamped
+#pragma unroll
+            for (int r = 0; r < 4; r++) rb[q][r] = keep_or_zero(rb[q][r], gj < n && gkb + r < k);
         }
     };
     auto sstore = [&](int buf) {
@@ -105,7 +109,19 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
         if (t + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
-    // C -= acc; D layout: reg r of lane l -> (row 4*(l/16)+r, col l%16)
+    // C -= acc (C = -acc with overwrite); D layout: reg r of lane l -> (row 4*(l/16)+r,
+    // col l%16).  C is read from clamped addresses first, every load in flight at once.
+    float cv[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                cv[a][b][r] = overwrite ? 0.f : C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+        }
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -116,7 +132,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
                 float* cp = C + (size_t)j * ldc;
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] -= acc[a][b][r];
+                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - acc[a][b][r];
             }
         }
 }
@@ -213,6 +229,17 @@ __global__ void __launch_bounds__(GT) sgemm_h(int m, int n, int k, const T* __re
         if (t + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
+    // C -= acc, C read from clamped addresses first (every load in flight at once)
+    float cv[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; r++) cv[a][b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+        }
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -223,7 +250,7 @@ __global__ void __launch_bounds__(GT) sgemm_h(int m, int n, int k, const T* __re
                 float* cp = C + (size_t)j * ldc;
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] -= acc[a][b][r];
+                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - acc[a][b][r];
             }
         }
 }
@@ -258,6 +285,15 @@ __global__ void cvt_n16(int k, int n, const float* __restrict__ src, size_t lds,
 
 }  // namespace
 
+void launch_sgemm_set(hipStream_t st, int m, int n, int k, const float* A, size_t lda, const float* B, size_t ldb,
+                      float* C, size_t ldc) {
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int tiles_m = (m + GBM - 1) / GBM, tiles_n = (n + GBN - 1) / GBN;
+    hipLaunchKernelGGL(sgemm_f32, dim3((unsigned)(tiles_m * tiles_n)), dim3(GT), 0, st, m, n, k, A, lda, B, ldb, C, ldc,
+                       tiles_m, tiles_n, 1);
+    BOLA_LAUNCH_CHECK();
+}
+
 size_t lp_gemm_ws_bytes(int lp, size_t m, size_t n, size_t k) {
     if (lp == LP_F32) return 0;
     const size_t kp = (k + 7) & ~(size_t)7;
@@ -270,7 +306,8 @@ void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A,
     const int tiles_m = (m + GBM - 1) / GBM, tiles_n = (n + GBN - 1) / GBN;
     const unsigned grid = (unsigned)(tiles_m * tiles_n);
     if (lp == LP_F32) {
-        hipLaunchKernelGGL(sgemm_f32, dim3(grid), dim3(GT), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n);
+        hipLaunchKernelGGL(sgemm_f32, dim3(grid), dim3(GT), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tiles_m, tiles_n,
+                           0);
         BOLA_LAUNCH_CHECK();
         return;
     }

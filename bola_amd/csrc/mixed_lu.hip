@@ -133,7 +133,9 @@ __device__ __forceinline__ int wg_tournament(const int* set, int base, int rows,
         const int rid = live[q] ? (set ? set[i] : base + i) : c0;  // c0 = r0: a valid row for idle threads
         const float* src = T + (size_t)c0 + (size_t)rid * ldt;
 #pragma unroll
-        for (int c = 0; c < LPW; c++) own[q][c] = (live[q] && c < w) ? src[c] : 0.f;
+        for (int c = 0; c < LPW; c++) own[q][c] = src[min(c, w - 1)];  // clamped: loads in flight together
+#pragma unroll
+        for (int c = 0; c < LPW; c++) own[q][c] = keep_or_zero(own[q][c], live[q] && c < w);
     }
     const int steps = min(w, rows);
     int pk[LPW];
@@ -284,7 +286,9 @@ __global__ void __launch_bounds__(256) panel_nopiv(int n, int r0, int c0, int w,
         float col[LPW];
 #pragma unroll
         for (int r = 0; r < LPW; r++)
-            col[r] = (c < w && r < w) ? T[(size_t)(c0 + c) + (size_t)(r0 + r) * ldt] : (r == c ? 1.f : 0.f);
+            col[r] = T[(size_t)(c0 + min(c, w - 1)) + (size_t)(r0 + min(r, w - 1)) * ldt];  // clamped
+#pragma unroll
+        for (int r = 0; r < LPW; r++) col[r] = (c < w && r < w) ? col[r] : (r == c ? 1.f : 0.f);
 #pragma unroll
         for (int j = 0; j < LPW; j++) {
             const float piv = __builtin_amdgcn_readlane(__builtin_bit_cast(int, col[j]), j) == 0
@@ -317,7 +321,8 @@ __global__ void __launch_bounds__(256) panel_nopiv(int n, int r0, int c0, int w,
     float* col = T + (size_t)c0 + (size_t)r * ldt;
 #pragma unroll
     for (int b = 0; b < LPW; b++) {
-        x[b] = (FULL || b < w) ? col[b] : 0.f;
+        x[b] = col[FULL ? b : min(b, w - 1)];  // clamped: loads in flight together
+        x[b] = FULL ? x[b] : keep_or_zero(x[b], b < w);
         l[b] = 0.f;
     }
 #pragma unroll
@@ -370,7 +375,7 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
 // ---- the butterfly mode: LU without interchanges, one launch per panel ----------------
 
 constexpr int NPLD = kMixNB + 4;      // LDS row stride (floats) of panel rows
-constexpr long kPanelSpin = 1L << 24;  // bound on the polls of one flag
+constexpr long kPanelSpin = 1L << 22;  // bound on the polls of one flag
 constexpr int TRC = 64;               // columns per workgroup of trsm_rbt
 constexpr size_t kInvPanel = 2 * (size_t)(kMixNB / LPW) * LPW * LPW;  // floats of one panel's inverses
 
@@ -384,14 +389,21 @@ __device__ void tri_inv_col(const float (*d)[LPW + 1], float (*o)[LPW + 1], int 
 #pragma unroll 1
     for (int i = 0; i < LPW; i++) {
         const int a = LOWER ? i : LPW - 1 - i;
-        float v = a == b ? 1.f : 0.f;
+        const int lo = LOWER ? 0 : a + 1, hi = LOWER ? a : LPW;
+        float v0 = a == b ? 1.f : 0.f, v1 = 0.f;
+        int c = lo;
+#pragma unroll 4
+        for (; c + 1 < hi; c += 2) {
+            v0 = fmaf(-d[a][c], o[c][b], v0);
+            v1 = fmaf(-d[a][c + 1], o[c + 1][b], v1);
+        }
+        if (c < hi) v0 = fmaf(-d[a][c], o[c][b], v0);
+        v0 += v1;
         if (LOWER) {
-            for (int c = 0; c < a; c++) v = fmaf(-d[a][c], o[c][b], v);
-            o[a][b] = v;
+            o[a][b] = v0;
         } else {
-            for (int c = a + 1; c < LPW; c++) v = fmaf(-d[a][c], o[c][b], v);
             const float dd = d[a][a];
-            o[a][b] = dd != 0.f ? v / dd : v;
+            o[a][b] = dd != 0.f ? v0 / dd : v0;
         }
     }
 }
@@ -500,7 +512,15 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
     const int leaves = (w + LPW - 1) / LPW;
     const int col = tid;  // one panel column per thread in loads and stores
     float* const P = T + (size_t)r0;  // column r0 of A
-    for (int r = 0; r < LPW; r++) a[r][col] = (col < w && r < nr) ? P[col + (size_t)(row0 + r) * ldt] : 0.f;
+    {
+        // loads from clamped (valid) addresses, all in flight at once, then the selects
+        const int cc = min(col, w - 1);
+        float v[LPW];
+#pragma unroll
+        for (int r = 0; r < LPW; r++) v[r] = P[cc + (size_t)(row0 + min(r, nr - 1)) * ldt];
+#pragma unroll
+        for (int r = 0; r < LPW; r++) a[r][col] = (col < w && r < nr) ? v[r] : 0.f;
+    }
     __syncthreads();
     const int napply = min(wb, leaves);
     for (int j = 0; j < napply; j++) {
@@ -508,12 +528,16 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         if (tid == 0) {
             long sp = 0;
             int ok = 1;
+            // the next owner polls eagerly; the others sparingly, so that hundreds of
+            // pollers do not crowd the flag's cache line
+            const bool next = wb == j + 1;
             while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
                 if (++sp > kPanelSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                     ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                if (next) __builtin_amdgcn_s_sleep(1);
+                else __builtin_amdgcn_s_sleep(32);
             }
             if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bail = !ok;
@@ -521,8 +545,14 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         __syncthreads();
         if (bail) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        for (int r = 0; r < LPW; r++)
-            ub[r][col] = (r < lw && col >= cj && col < w) ? P[col + (size_t)(r0 + cj + r) * ldt] : 0.f;
+        {
+            const int cc = min(max(col, cj), w - 1);
+            float v[LPW];
+#pragma unroll
+            for (int r = 0; r < LPW; r++) v[r] = P[cc + (size_t)(r0 + cj + min(r, lw - 1)) * ldt];
+#pragma unroll
+            for (int r = 0; r < LPW; r++) ub[r][col] = (r < lw && col >= cj && col < w) ? v[r] : 0.f;
+        }
         for (int e = tid; e < LPW * LPW; e += 256) ui[e / LPW][e % LPW] = inv[(size_t)j * 2 * LPW * LPW + e];
         __syncthreads();
         rbt_apply_leaf(a, ub, ui, cj, lw, w);
@@ -533,18 +563,29 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         const int cj = wb * LPW, lw = min(LPW, w - cj);
         if (tid < LPW) diag_lu_rows(a, d, cj, lw);
         __syncthreads();
-        if (tid < 64) tri_inv_col<false>(d, ui, tid);
-        else if (tid < 128) tri_inv_col<true>(d, li, tid - 64);
-        __syncthreads();
-        const int c = cj + lw + tid;
-        if (c < w) {  // U12 = L11^-1 A12 by forward substitution, one column per thread
+        // side by side: U11^-1 (wavefront 0), L11^-1 (wavefront 1) and U12 = L11^-1 A12 by
+        // forward substitution, a column per thread (wavefronts 2 and 3)
+        if (tid < 64) {
+            tri_inv_col<false>(d, ui, tid);
+        } else if (tid < 128) {
+            tri_inv_col<true>(d, li, tid - 64);
+        } else {
+            for (int c = cj + lw + tid - 128; c < w; c += 128) {
 #pragma unroll 1
-            for (int r = 1; r < LPW; r++) {
-                float v = a[r][c];
-                for (int s2 = 0; s2 < r; s2++) v = fmaf(-d[r][s2], a[s2][c], v);
-                a[r][c] = v;
+                for (int r = 1; r < LPW; r++) {
+                    float v0 = a[r][c], v1 = 0.f;
+                    int s2 = 0;
+#pragma unroll 4
+                    for (; s2 + 1 < r; s2 += 2) {
+                        v0 = fmaf(-d[r][s2], a[s2][c], v0);
+                        v1 = fmaf(-d[r][s2 + 1], a[s2 + 1][c], v1);
+                    }
+                    if (s2 < r) v0 = fmaf(-d[r][s2], a[s2][c], v0);
+                    a[r][c] = v0 + v1;
+                }
             }
         }
+        __syncthreads();
         if (tid < lw)
             for (int r = 0; r < lw; r++) a[r][cj + tid] = d[r][tid];
         __syncthreads();
@@ -564,57 +605,95 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
 }
 
-// U12 = L11^-1 A12 in place on the A-columns [c_lo, c_hi) of block rows [k, k+nb), L11 the
-// unit lower factor of the panel at A(k.., k..) with its diagonal-block inverses in inv.
-// One workgroup per 64 columns holds its nb x 64 block in LDS: blocked forward
-// substitution, right-looking over the 32-row blocks; L is read with wavefront-uniform
-// (scalar) loads.
-__global__ void __launch_bounds__(256) trsm_rbt(int k, int nb, int c_lo, int c_hi, float* __restrict__ T, size_t ldt,
-                                                const float* __restrict__ inv) {
+// X := L11^-1 X for the columns [c_lo, c_hi) of the nb-row block X (X(r, c) at
+// X[c + r ldx]), L11 the unit lower factor of the panel at A(k.., k..) in T with its
+// diagonal-block inverses in inv; with eye, X starts as -I (and is not read).  One
+// workgroup per 64 columns holds its nb x 64 block in LDS: blocked forward substitution,
+// right-looking over the 32-row blocks, each block's inverse and the L entries below it
+// staged in LDS (read as wavefront broadcasts).
+__global__ void __launch_bounds__(256) trsm_rbt(int k, int nb, int c_lo, int c_hi, const float* __restrict__ T,
+                                                size_t ldt, float* __restrict__ X, size_t ldx,
+                                                const float* __restrict__ inv, int eye) {
     __shared__ float x[kMixNB][TRC + 1];
+    __shared__ float lb[kMixNB - LPW][LPW + 1];  // L rows below block j, its 32 columns
+    __shared__ float ls[LPW][LPW + 1];           // L_jj^-1
     const int tid = threadIdx.x, cl = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = c_lo + blockIdx.x * TRC + cl;
     const bool cv = c < c_hi;
-    for (int r = wv; r < nb; r += 4) x[r][cl] = cv ? T[(size_t)c + (size_t)(k + r) * ldt] : 0.f;
-    __syncthreads();
+    if (eye) {
+        for (int r = wv; r < nb; r += 4) x[r][cl] = r == c ? -1.f : 0.f;
+    } else {
+        const size_t cc = (size_t)min(c, c_hi - 1);
+        for (int r0 = wv; r0 < nb; r0 += 32) {  // eight loads in flight per thread
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = X[cc + (size_t)min(r0 + 4 * i, nb - 1) * ldx];
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (r0 + 4 * i < nb) x[r0 + 4 * i][cl] = cv ? v[i] : 0.f;
+        }
+    }
     const int nl = (nb + LPW - 1) / LPW;
     for (int j = 0; j < nl; j++) {
-        const int rj = j * LPW;
+        const int rj = j * LPW, nbelow = max(0, nb - rj - LPW);
         const float* li = inv + (size_t)j * 2 * LPW * LPW + LPW * LPW;
+        for (int e = tid; e < LPW * LPW; e += 256) ls[e / LPW][e % LPW] = li[e];
+        if (nbelow > 0) {
+            const int s = tid & 31;
+            for (int r0 = tid >> 5; r0 < nbelow; r0 += 64) {  // eight loads in flight per thread
+                float v[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    v[i] = T[(size_t)(k + rj + s) + (size_t)(k + rj + LPW + min(r0 + 8 * i, nbelow - 1)) * ldt];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (r0 + 8 * i < nbelow) lb[r0 + 8 * i][s] = v[i];
+            }
+        }
+        __syncthreads();
+        // x_j = L_jj^-1 x_j: rows wv*8 .. +8 of the block, column cl
         float xs[LPW];
 #pragma unroll
         for (int s = 0; s < LPW; s++) xs[s] = rj + s < nb ? x[rj + s][cl] : 0.f;
         float y[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int r = wv * 8 + i;
             float v = 0.f;
 #pragma unroll
-            for (int s = 0; s < LPW; s++) v = fmaf(li[r * LPW + s], xs[s], v);
+            for (int s = 0; s < LPW; s++) v = fmaf(ls[wv * 8 + i][s], xs[s], v);
             y[i] = v;
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 8; i++)
             if (rj + wv * 8 + i < nb) x[rj + wv * 8 + i][cl] = y[i];
+        // the rows below: x_i -= L_ij x_j, with x_j = y (this thread's column) exchanged
+        // through LDS
         __syncthreads();
-        if (rj + LPW < nb) {
-            float xj[LPW];
 #pragma unroll
-            for (int s = 0; s < LPW; s++) xj[s] = x[rj + s][cl];
-            for (int r = rj + LPW + wv; r < nb; r += 4) {
-                const float* lr = T + (size_t)(k + rj) + (size_t)(k + r) * ldt;  // L(k+r, k+rj ..)
-                float v = 0.f;
+        for (int s = 0; s < LPW; s++) xs[s] = x[rj + s][cl];
+        for (int r = wv; r < nbelow; r += 4) {
+            float v0 = 0.f, v1 = 0.f;
 #pragma unroll
-                for (int s = 0; s < LPW; s++) v = fmaf(lr[s], xj[s], v);
-                x[r][cl] -= v;
+            for (int s = 0; s < LPW; s += 2) {
+                v0 = fmaf(lb[r][s], xs[s], v0);
+                v1 = fmaf(lb[r][s + 1], xs[s + 1], v1);
             }
-            __syncthreads();
+            x[rj + LPW + r][cl] -= v0 + v1;
         }
+        __syncthreads();
     }
     for (int r = wv; r < nb; r += 4)
-        if (cv) T[(size_t)c + (size_t)(k + r) * ldt] = x[r][cl];
+        if (cv) X[(size_t)c + (size_t)r * ldx] = x[r][cl];
+}
+
+// dst (rows x cols, column-major, ld ldd) = src (ld lds)
+__global__ void copy2d(int rows, int cols, const float* __restrict__ src, size_t lds, float* __restrict__ dst,
+                       size_t ldd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    for (int j = blockIdx.y; j < cols; j += gridDim.y) dst[(size_t)i + (size_t)j * ldd] = src[(size_t)i + (size_t)j * lds];
 }
 
 }  // namespace
@@ -645,6 +724,8 @@ struct LuCtx {
     int lvl_stride;
     void* gws;
     float* inv;  // the butterfly mode: the current panel's diagonal-block inverses
+    float* nlt;  // the butterfly mode: -(L11^-1)^T of the current panel (kMixNB x kMixNB)
+    float* ubuf; // the butterfly mode: U12^T blocks, row c for A-column c (ld n)
     int* flags;  // the butterfly mode: per-leaf flags of panel_rbt
     int* err;
 };
@@ -717,17 +798,24 @@ void lu_rec(const LuCtx& c, int r0, int w) {
 void update_cols(const LuCtx& c, int k, int nb, int c_lo, int c_hi) {
     const int ncols = c_hi - c_lo;
     if (ncols <= 0) return;
+    const int below = c.n - k - nb;
     if (c.pivot) {
         strsm_rupu(c, ncols, nb, k, k, c_lo);
-    } else {
-        hipLaunchKernelGGL(trsm_rbt, dim3(cdiv((size_t)ncols, TRC)), dim3(256), 0, c.st, k, nb, c_lo, c_hi, c.T,
-                           c.ldt, c.inv);
-        BOLA_LAUNCH_CHECK();
+        if (below > 0)
+            launch_lp_gemm(c.st, c.lp, ncols, below, nb, Tp(c, c_lo, k), c.ldt, Tp(c, k, k + nb), c.ldt,
+                           Tp(c, c_lo, k + nb), c.ldt, c.gws);
+        return;
     }
-    const int below = c.n - k - nb;
+    // U12^T = A12^T L11^-T as one product into ubuf (copied into place afterwards), and
+    // the trailing update reads it from there
+    float* ub = c.ubuf + c_lo;
+    launch_sgemm_set(c.st, ncols, nb, nb, Tp(c, c_lo, k), c.ldt, c.nlt, (size_t)nb, ub, (size_t)c.n);
     if (below > 0)
-        launch_lp_gemm(c.st, c.lp, ncols, below, nb, Tp(c, c_lo, k), c.ldt, Tp(c, k, k + nb), c.ldt,
-                       Tp(c, c_lo, k + nb), c.ldt, c.gws);
+        launch_lp_gemm(c.st, c.lp, ncols, below, nb, ub, (size_t)c.n, Tp(c, k, k + nb), c.ldt, Tp(c, c_lo, k + nb),
+                       c.ldt, c.gws);
+    hipLaunchKernelGGL(copy2d, dim3(cdiv((size_t)ncols, 256), (unsigned)std::min(nb, 256)), dim3(256), 0, c.st, ncols,
+                       nb, ub, (size_t)c.n, Tp(c, c_lo, k), c.ldt);
+    BOLA_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -737,6 +825,7 @@ size_t mixed_lu_ws_bytes(int n, int lp) {
     size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
     b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 256);
     b += 2 * kInvPanel * sizeof(float) + 1024;
+    b += (2 * (size_t)kMixNB * kMixNB + (size_t)n * kMixNB) * sizeof(float) + 1024;
     return b;
 }
 
@@ -770,6 +859,8 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
     c.gws = b + o;
     float* const invb = (float*)(b + o + 2 * gsz);  // two panels' inverses, by panel parity
     c.flags = (int*)(invb + 2 * kInvPanel);
+    float* const nltb = (float*)((char*)c.flags + 1024);  // two panels' -(L11^-1)^T, by parity
+    c.ubuf = nltb + 2 * (size_t)kMixNB * kMixNB;
     c.err = err;
     LuCtx c2 = c;  // stream 2: its own 16-bit operand buffers
     c2.st = h->aux_stream();
@@ -792,9 +883,15 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
             lu_rec(c, k, nb);
         } else {
             // stream 2 may still read the previous panel's inverses: alternate two buffers
-            c.inv = c2.inv = invb + (size_t)((k / kMixNB) & 1) * kInvPanel;
+            const int par = (k / kMixNB) & 1;
+            c.inv = c2.inv = invb + (size_t)par * kInvPanel;
+            c.nlt = c2.nlt = nltb + (size_t)par * kMixNB * kMixNB;
+            c2.ubuf = c.ubuf;
             hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(n - k), LPW)), dim3(256), 0, c.st, n, k, nb, T, ldt, c.inv,
                                c.flags, k / kMixNB + 1, err);
+            // -(L11^-1)^T, column-major nb x nb: L11^-1 applied to -I, stored row by row
+            hipLaunchKernelGGL(trsm_rbt, dim3(cdiv((size_t)nb, TRC)), dim3(256), 0, c.st, k, nb, 0, nb, T, ldt, c.nlt,
+                               (size_t)nb, c.inv, 1);
             BOLA_LAUNCH_CHECK();
         }
         // the columns right of the panel are stream 2's until its previous bulk update is done

@@ -49,9 +49,11 @@ __global__ void __launch_bounds__(256) trsm_ln_lanes(int unit, int t, int nvec, 
 #pragma unroll
     for (int k = 0; k < TW; k++) {
         const bool need = LOWER ? k <= i : k >= i;
-        arow[k] = (i < t && k < t && need) ? A[(size_t)i + (size_t)k * lda] : 0.0;
+        const double a = A[(size_t)min(i, t - 1) + (size_t)min(k, t - 1) * lda];  // clamped: loads in flight together
+        arow[k] = keep_or_zero(a, i < t && k < t && need);
     }
-    double x = act ? B[i + (size_t)v * ldb] : 0.0;
+    const double xb = B[min(i, t - 1) + (size_t)min(v, nvec - 1) * ldb];
+    double x = keep_or_zero(xb, act);
 #pragma unroll
     for (int kk = 0; kk < TW; kk++) {
         const int k = LOWER ? kk : (TW - 1 - kk);
@@ -102,7 +104,9 @@ __global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, c
     double x[MAXT];
 #pragma unroll
     for (int q = 0; q < MAXT; q++)
-        x[q] = (q < t) ? (LEFT ? B[q + (size_t)v * ldb] : B[v + (size_t)q * ldb]) : 0.0;
+        x[q] = LEFT ? B[min(q, t - 1) + (size_t)v * ldb] : B[v + (size_t)min(q, t - 1) * ldb];  // clamped
+#pragma unroll
+    for (int q = 0; q < MAXT; q++) x[q] = keep_or_zero(x[q], q < t);
     if (V == LTL) {
 #pragma unroll
         for (int ii = 0; ii < MAXT; ii++) {
