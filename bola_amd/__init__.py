@@ -29,6 +29,8 @@ __all__ = [
     "gemm", "getrf", "getrf2", "potrf", "potrf2", "trsm", "syrk", "laswp", "idamax",
     "ger", "scal", "daxpy", "dot_product", "getrs", "potrs",
     "MixedStats", "PRECISIONS", "dgesv_mixed", "sgemm_lp",
+    "BATCH_LU", "BATCH_CHOL", "BATCH_FP64", "BATCH_FP32_IR", "BATCH_DESC_DTYPE", "BatchConfig",
+    "batch_solve", "batch_tune", "BatchGraph",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -112,6 +114,12 @@ _SIGS = [
     ("bola_hip_dgesv_mixed", C.c_int, [C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
     ("bola_hip_dgesv_mixed_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
     ("bola_hip_sgemm_lp_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz]),
+    ("bola_hip_batch_solve_dev", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("bola_hip_batch_tune_dev", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp]),
+    ("bola_hip_batch_workspace_bytes", _sz, [_vp, _vp]),
+    ("bola_hip_batch_graph_create", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    ("bola_hip_batch_graph_launch", C.c_int, [_vp]),
+    ("bola_hip_batch_graph_destroy", C.c_int, [_vp]),
 ]
 
 
@@ -427,3 +435,79 @@ def sgemm_lp(m, n, k, a, lda, b, ldb, c, ldc, precision="fp32") -> None:
     """C -= A B on fp32 device tensors, products in `precision` (the mixed LU's update)."""
     _check(lib().bola_hip_sgemm_lp_dev(default_handle().ptr, PRECISIONS[precision], m, n, k, _ptr(a, np.float32),
                                        lda, _ptr(b, np.float32), ldb, _ptr(c, np.float32), ldc), True)
+
+
+# ------------------------------------------------------------------ batched solves
+
+BATCH_LU, BATCH_CHOL = 0, 1
+BATCH_FP64, BATCH_FP32_IR = 0, 1
+# the C struct bola_hip_batch_desc, field for field
+BATCH_DESC_DTYPE = np.dtype([("n", "<i4"), ("nrhs", "<i4"), ("kind", "<i4"), ("pad", "<i4"),
+                             ("a", "<i8"), ("b", "<i8"), ("piv", "<i8")])
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("nb", C.c_int32), ("wg_per_cu", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.nb, self.wg_per_cu)
+
+
+def _descs(descs):
+    d = np.ascontiguousarray(descs, dtype=BATCH_DESC_DTYPE)
+    return d, C.c_void_p(d.ctypes.data)
+
+
+def batch_solve(variant, descs, arena, piv, info, iters=None, config=None) -> None:
+    """Solve every system of the batch in one persistent launch (see include/bola_hip.h).
+    descs: numpy array of BATCH_DESC_DTYPE (host); arena (float64), piv and info (int32)
+    and the optional iters (int32): torch CUDA tensors.  config: (nb, wg_per_cu) or None
+    for the tuned / default choice."""
+    d, dp = _descs(descs)
+    cfg = None
+    if config is not None:
+        cfg = C.byref(BatchConfig(*config))
+    rc = lib().bola_hip_batch_solve_dev(default_handle().ptr, variant, len(d), dp, _ptr(arena, np.float64),
+                                        _ptr(piv, np.int32), _ptr(info, np.int32), _ptr(iters, np.int32), cfg)
+    _check(rc, True)
+
+
+def batch_tune(variant, descs, arena):
+    """Time every launch configuration on a sample of the batch; the fastest is kept for
+    batches like this one and returned as (nb, wg_per_cu).  The arena is not modified."""
+    d, dp = _descs(descs)
+    out = BatchConfig()
+    rc = lib().bola_hip_batch_tune_dev(default_handle().ptr, variant, len(d), dp, _ptr(arena, np.float64),
+                                       C.byref(out))
+    _check(rc, True)
+    return out.as_tuple()
+
+
+class BatchGraph:
+    """A captured batched solve (hipGraph) of fixed descriptors and buffers; launch() replays
+    it on the handle's stream."""
+
+    def __init__(self, variant, descs, arena, piv, info, iters=None, config=None):
+        d, dp = _descs(descs)
+        cfg = C.byref(BatchConfig(*config)) if config is not None else None
+        g = C.c_void_p()
+        self._keep = (arena, piv, info, iters)
+        rc = lib().bola_hip_batch_graph_create(default_handle().ptr, variant, len(d), dp, _ptr(arena, np.float64),
+                                               _ptr(piv, np.int32), _ptr(info, np.int32), _ptr(iters, np.int32),
+                                               cfg, C.byref(g))
+        _check(rc, True)
+        self._g = g
+
+    def launch(self) -> None:
+        _check(lib().bola_hip_batch_graph_launch(self._g), True)
+
+    def close(self) -> None:
+        if getattr(self, "_g", None):
+            lib().bola_hip_batch_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

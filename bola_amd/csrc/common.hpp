@@ -90,6 +90,7 @@ struct bola_hip_handle_s {
     bola::Workspace stage;      // operands of host-pointer calls
     bola::Workspace mws;        // mixed-precision solver: fp32 factors, vectors, GMRES basis
     bola::Workspace mws2;       // its FP64 fallback copy of A
+    bola::Workspace bws;        // batched solves: staged descriptors, per-workgroup scratch
     int32_t* d_info = nullptr;  // device status word for factorizations
     std::string last_error;
     // profiling: per-phase GPU time measured with events on the handle's stream

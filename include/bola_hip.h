@@ -201,6 +201,52 @@ int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n, size_
 int bola_hip_sgemm_lp_dev(bola_hip_handle_t h, int precision, size_t m, size_t n, size_t k, const float* a,
                           size_t lda, const float* b, size_t ldb, float* c, size_t ldc);
 
+/* ---- Batched solves (no reference counterpart: BOLA solves one system per call) ------
+ * Many independent small systems in one persistent launch.  Every system lives in one
+ * device arena of doubles: A (n x n, column-major, lda = n) at element offset `a`, B
+ * (n x nrhs, ldb = n) at offset `b`; LU systems also own ipiv[n] (1-based, LAPACK
+ * meaning) at offset `piv` of an int32 arena.  Descriptors are host memory. */
+#define BOLA_HIP_BATCH_LU 0   /* general: LU with partial pivoting (dgesv) */
+#define BOLA_HIP_BATCH_CHOL 1 /* symmetric positive definite, lower triangle (dposv) */
+#define BOLA_HIP_BATCH_FP64 0     /* FP64 factors: A := its factors, B := X */
+#define BOLA_HIP_BATCH_FP32_IR 1  /* FP32 factors, FP64 refinement to n * 2^-52 (FP64 refactor if
+                                     it stalls): A unchanged, B := X */
+typedef struct bola_hip_batch_desc {
+    int32_t n;    /* 1 .. 512 */
+    int32_t nrhs; /* 1 .. 16 */
+    int32_t kind; /* BOLA_HIP_BATCH_LU or BOLA_HIP_BATCH_CHOL */
+    int32_t pad;
+    int64_t a, b, piv;
+} bola_hip_batch_desc;
+/* Launch shape: panel width nb (8 or 16) and workgroups per CU (1 .. 4). */
+typedef struct bola_hip_batch_config {
+    int32_t nb;
+    int32_t wg_per_cu;
+} bola_hip_batch_config;
+/* info[i] (device, count entries): 0, or the LAPACK info of system i (first zero pivot /
+ * first non-positive leading minor, 1-based).  iters (device, optional): refinement steps
+ * of system i, -1 when it was refactored in FP64, 0 for the FP64 variant.  cfg NULL: the
+ * configuration bola_hip_batch_tune_dev chose for a batch like this one, else a default. */
+int bola_hip_batch_solve_dev(bola_hip_handle_t h, int variant, size_t count, const bola_hip_batch_desc* desc,
+                             double* arena, int32_t* piv_arena, int32_t* info, int32_t* iters,
+                             const bola_hip_batch_config* cfg);
+/* Autotuner: times every configuration on a sample of the batch (copied, so the arena is
+ * not modified), keeps the fastest for batches of this variant, largest order and count,
+ * and returns it in *out. */
+int bola_hip_batch_tune_dev(bola_hip_handle_t h, int variant, size_t count, const bola_hip_batch_desc* desc,
+                            const double* arena, bola_hip_batch_config* out);
+/* Device scratch a batched solve takes from the handle for a configuration (NULL: the
+ * largest). */
+size_t bola_hip_batch_workspace_bytes(bola_hip_handle_t h, const bola_hip_batch_config* cfg);
+/* hipGraph capture of one batched solve with fixed descriptors and buffers: replay it with
+ * new contents in the same arena.  Launches go to the handle's stream. */
+typedef struct bola_hip_batch_graph_s* bola_hip_batch_graph_t;
+int bola_hip_batch_graph_create(bola_hip_handle_t h, int variant, size_t count, const bola_hip_batch_desc* desc,
+                                double* arena, int32_t* piv_arena, int32_t* info, int32_t* iters,
+                                const bola_hip_batch_config* cfg, bola_hip_batch_graph_t* out);
+int bola_hip_batch_graph_launch(bola_hip_batch_graph_t g);
+int bola_hip_batch_graph_destroy(bola_hip_batch_graph_t g);
+
 #ifdef __cplusplus
 }
 #endif
