@@ -31,6 +31,7 @@ __all__ = [
     "MixedStats", "PRECISIONS", "dgesv_mixed", "sgemm_lp",
     "BATCH_LU", "BATCH_CHOL", "BATCH_FP64", "BATCH_FP32_IR", "BATCH_DESC_DTYPE", "BatchConfig",
     "batch_solve", "batch_tune", "BatchGraph",
+    "SERVE_LU", "SERVE_CHOL", "Server",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -120,6 +121,14 @@ _SIGS = [
     ("bola_hip_batch_graph_create", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     ("bola_hip_batch_graph_launch", C.c_int, [_vp]),
     ("bola_hip_batch_graph_destroy", C.c_int, [_vp]),
+    ("bola_hip_server_create", C.c_int, [_sz, C.c_int, C.POINTER(_vp)]),
+    ("bola_hip_server_submit", C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
+    ("bola_hip_server_wait", C.c_int, [_vp, C.c_uint64, _vp]),
+    ("bola_hip_server_poll", C.c_int, [_vp, C.c_uint64, _vp]),
+    ("bola_hip_server_drain", C.c_int, [_vp]),
+    ("bola_hip_server_get_stats", C.c_int, [_vp, _vp]),
+    ("bola_hip_server_reset_stats", C.c_int, [_vp]),
+    ("bola_hip_server_destroy", C.c_int, [_vp]),
 ]
 
 
@@ -505,6 +514,110 @@ class BatchGraph:
         if getattr(self, "_g", None):
             lib().bola_hip_batch_graph_destroy(self._g)
             self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ serving
+
+SERVE_LU, SERVE_CHOL = 0, 1
+ERR_CANCELLED = -1001
+
+
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("n", C.c_size_t), ("nrhs", C.c_size_t),
+                ("a", C.c_void_p), ("b", C.c_void_p), ("x", C.c_void_p), ("factor_key", C.c_uint64)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("info", C.c_int32), ("cache_hit", C.c_int32), ("slot", C.c_int32),
+                ("queue_us", C.c_double), ("latency_us", C.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("cache_hits", C.c_uint64), ("evictions", C.c_uint64), ("p50_us", C.c_double),
+                ("p99_us", C.c_double), ("mean_us", C.c_double), ("solves_per_s", C.c_double),
+                ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t), ("hbm_budget", C.c_size_t),
+                ("hbm_cached", C.c_size_t)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class Server:
+    """The C ABI's solve server: `slots` workers with their own streams admit requests as
+    slots and the HBM budget allow, overlap them on the device, and keep factorizations
+    with a nonzero key resident for later right-hand sides.  Operands are numpy arrays
+    or torch CUDA tensors (float64, column-major flat); they must stay alive until the
+    ticket is waited for."""
+
+    def __init__(self, hbm_budget: int, slots: int = 4):
+        lib()
+        s = C.c_void_p()
+        rc = lib().bola_hip_server_create(hbm_budget, slots, C.byref(s))
+        if rc != 0:
+            raise BolaError(lib().bola_hip_last_error().decode(), rc)
+        self._s = s
+        self._live = {}
+
+    def submit(self, kind, n, nrhs, a, b, x, key=0) -> int:
+        r = _Request(kind, 0, n, nrhs, _ptr(a, np.float64) if a is not None else None, _ptr(b, np.float64),
+                     _ptr(x, np.float64), key)
+        t = C.c_uint64()
+        rc = lib().bola_hip_server_submit(self._s, C.byref(r), C.byref(t))
+        if rc != 0:
+            raise BolaError(lib().bola_hip_last_error().decode(), rc)
+        self._live[t.value] = (a, b, x)
+        return t.value
+
+    def wait(self, ticket) -> dict:
+        res = _Result()
+        rc = lib().bola_hip_server_wait(self._s, ticket, C.byref(res))
+        self._live.pop(ticket, None)
+        if rc != 0:
+            raise BolaError("unknown ticket", rc)
+        return res.as_dict()
+
+    def poll(self, ticket):
+        res = _Result()
+        rc = lib().bola_hip_server_poll(self._s, ticket, C.byref(res))
+        if rc == 1:
+            return None
+        self._live.pop(ticket, None)
+        if rc != 0:
+            raise BolaError("unknown ticket", rc)
+        return res.as_dict()
+
+    def drain(self) -> None:
+        lib().bola_hip_server_drain(self._s)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        lib().bola_hip_server_get_stats(self._s, C.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        lib().bola_hip_server_reset_stats(self._s)
+
+    def close(self) -> None:
+        if getattr(self, "_s", None):
+            lib().bola_hip_server_destroy(self._s)
+            self._s = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -155,8 +155,10 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
     };
 
     const int nsteps = (k + BK - 1) / BK;
-    gload(0);
-    lstore(0);
+    if (nsteps > 0) {  // k == 0: C = beta C, and A, B are not read
+        gload(0);
+        lstore(0);
+    }
     __syncthreads();
     for (int step = 0; step < nsteps; step++) {
         const int buf = step & 1;

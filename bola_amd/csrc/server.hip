@@ -1,0 +1,399 @@
+// A solve server (no reference counterpart: BOLA is a library of one-shot calls).
+//
+// Requests (LU or Cholesky solves of mixed size, FP64, bit-identical to the one-shot
+// entry points) are submitted from any host thread and return a ticket.  A fixed number
+// of slots, each a worker thread with its own handle and HIP stream, take requests as
+// soon as they are free: a request is admitted when a slot is idle and its device
+// buffers fit in the HBM budget (factorizations kept for later right-hand sides count
+// against the budget too, and the least recently used idle ones are evicted to make
+// room).  Admission scans the queue in arrival order and may pass over a request that
+// does not fit yet, but only a bounded number of times, so large requests are not
+// starved.  The slots' work overlaps on the device; each request's result is the same
+// as when it runs alone, since every slot runs the same deterministic kernels.
+#include "common.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace bola {
+namespace {
+
+double now_us() {
+    using namespace std::chrono;
+    return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() * 1e-3;
+}
+
+struct Ticket {
+    uint64_t id = 0;
+    bola_hip_request req{};
+    bola_hip_result res{};
+    double t_submit = 0.0;
+    bool done = false;
+    int passed = 0;  // times admission passed over it while it was first in line
+};
+
+struct Factor {
+    int kind = 0;
+    size_t n = 0;
+    double* f = nullptr;
+    int32_t* ipiv = nullptr;
+    size_t bytes = 0;
+    int users = 0;
+    uint64_t last_use = 0;
+};
+
+size_t factor_bytes(size_t n) { return n * n * 8 + ((n * 4 + 255) & ~(size_t)255); }
+
+}  // namespace
+}  // namespace bola
+
+using namespace bola;
+
+struct bola_hip_server_s {
+    int device = 0;
+    size_t budget = 0, in_use = 0, peak = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Ticket*> queue;
+    std::unordered_map<uint64_t, Ticket*> tickets;
+    std::unordered_map<uint64_t, Factor> cache;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    uint64_t next_id = 1, clock = 0;
+    int running = 0;
+    // statistics
+    uint64_t submitted = 0, completed = 0, rejected = 0, hits = 0, evictions = 0;
+    std::vector<double> lat;
+    double t_first = -1.0, t_last = 0.0;
+
+    // bytes a request needs beyond what the cache already holds for it
+    size_t need(const bola_hip_request& r, bool& hit) {
+        hit = false;
+        if (r.factor_key) {
+            auto it = cache.find(r.factor_key);
+            if (it != cache.end() && it->second.kind == r.kind && it->second.n == r.n) hit = true;
+        }
+        const size_t rhs = r.n * r.nrhs * 8;
+        return hit ? rhs : factor_bytes(r.n) + rhs;
+    }
+    // evicts idle cached factors, least recently used first, until `bytes` fit
+    bool make_room(size_t bytes, hipStream_t st) {
+        while (in_use + bytes > budget) {
+            auto victim = cache.end();
+            for (auto it = cache.begin(); it != cache.end(); ++it)
+                if (it->second.users == 0 && (victim == cache.end() || it->second.last_use < victim->second.last_use))
+                    victim = it;
+            if (victim == cache.end()) return false;
+            hipFreeAsync(victim->second.f, st);
+            hipFreeAsync(victim->second.ipiv, st);
+            in_use -= victim->second.bytes;
+            evictions++;
+            cache.erase(victim);
+        }
+        return true;
+    }
+    // the next admissible request (the lock is held)
+    Ticket* admit(hipStream_t st, size_t& reserved, bool& hit) {
+        for (size_t qi = 0; qi < queue.size(); qi++) {
+            Ticket* t = queue[qi];
+            const size_t nb = need(t->req, hit);
+            if (in_use + nb <= budget || make_room(nb, st)) {
+                in_use += nb;
+                peak = std::max(peak, in_use);
+                reserved = nb;
+                queue.erase(queue.begin() + (long)qi);
+                if (hit) cache[t->req.factor_key].users++;
+                return t;
+            }
+            // the first in line does not fit yet: others may pass it, a bounded number of times
+            if (qi == 0 && ++t->passed > 4 * (int)workers.size()) return nullptr;
+        }
+        return nullptr;
+    }
+    void work(int slot);
+};
+
+void bola_hip_server_s::work(int slot) {
+    hipSetDevice(device);
+    bola_hip_handle_t h = nullptr;
+    if (bola_hip_create(&h) != 0) return;
+    hipStream_t st = h->stream;
+    int32_t* dinfo = nullptr;
+    if (hipMalloc(&dinfo, 4 * sizeof(int32_t)) != hipSuccess) {
+        bola_hip_destroy(h);
+        return;
+    }
+    for (;;) {
+        Ticket* t = nullptr;
+        size_t reserved = 0;
+        bool hit = false;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] {
+                if (stop) return true;
+                t = admit(st, reserved, hit);
+                return t != nullptr;
+            });
+            if (!t) break;  // stopping
+            running++;
+        }
+        bola_hip_request& r = t->req;
+        bola_hip_result& res = t->res;
+        res.slot = slot;
+        res.cache_hit = hit ? 1 : 0;
+        res.queue_us = now_us() - t->t_submit;
+        double* fa = nullptr;
+        int32_t* ipiv = nullptr;
+        double* xb = nullptr;
+        bool keep = false;
+        int status = 0, info = 0;
+        const size_t n = r.n;
+        try {
+            if (hit) {
+                std::lock_guard<std::mutex> lk(mu);
+                Factor& f = cache[r.factor_key];
+                fa = f.f;
+                ipiv = f.ipiv;
+                f.last_use = ++clock;
+            } else {
+                BOLA_CHECK(hipMallocAsync((void**)&fa, n * n * 8, st));
+                BOLA_CHECK(hipMallocAsync((void**)&ipiv, std::max<size_t>(n, 1) * 4, st));
+                BOLA_CHECK(hipMemcpyAsync(fa, r.a, n * n * 8, hipMemcpyDefault, st));
+                int rc = r.kind == BOLA_HIP_SERVE_LU ? bola_hip_dgetrf_dev(h, n, n, fa, n, ipiv, dinfo)
+                                                     : bola_hip_dpotrf_dev(h, 'L', n, fa, n, dinfo);
+                if (rc < 0) throw HipFailure{hipErrorUnknown, "factorization", __LINE__};
+                BOLA_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                BOLA_CHECK(hipStreamSynchronize(st));
+            }
+            if (info == 0) {
+                BOLA_CHECK(hipMallocAsync((void**)&xb, std::max<size_t>(n * r.nrhs, 1) * 8, st));
+                BOLA_CHECK(hipMemcpyAsync(xb, r.b, n * r.nrhs * 8, hipMemcpyDefault, st));
+                int rc = r.kind == BOLA_HIP_SERVE_LU ? bola_hip_dgetrs_dev(h, n, r.nrhs, fa, n, ipiv, xb, n)
+                                                     : bola_hip_dpotrs_dev(h, 'L', n, r.nrhs, fa, n, xb, n);
+                if (rc < 0) throw HipFailure{hipErrorUnknown, "solve", __LINE__};
+                BOLA_CHECK(hipMemcpyAsync(r.x, xb, n * r.nrhs * 8, hipMemcpyDefault, st));
+                BOLA_CHECK(hipFreeAsync(xb, st));
+                xb = nullptr;
+                BOLA_CHECK(hipStreamSynchronize(st));
+                keep = !hit && r.factor_key != 0;
+            }
+        } catch (const HipFailure& f) {
+            status = BOLA_HIP_ERR_DEVICE;
+            hipStreamSynchronize(st);
+            if (xb) hipFreeAsync(xb, st);
+            keep = false;
+            (void)f;
+        }
+        if (!hit && !keep) {
+            if (fa) hipFreeAsync(fa, st);
+            if (ipiv) hipFreeAsync(ipiv, st);
+        }
+        hipStreamSynchronize(st);
+        const double t_done = now_us();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            size_t kept = 0;
+            if (keep) {
+                auto it = cache.find(r.factor_key);
+                if (it != cache.end() && it->second.users == 0) {  // a stale entry under this key
+                    hipFreeAsync(it->second.f, st);
+                    hipFreeAsync(it->second.ipiv, st);
+                    in_use -= it->second.bytes;
+                    cache.erase(it);
+                }
+                Factor f;
+                f.kind = r.kind;
+                f.n = n;
+                f.f = fa;
+                f.ipiv = ipiv;
+                f.bytes = factor_bytes(n);
+                f.last_use = ++clock;
+                if (cache.find(r.factor_key) == cache.end()) {
+                    cache[r.factor_key] = f;
+                    kept = f.bytes;
+                } else {  // the key is busy with another factorization: drop this one
+                    hipFreeAsync(fa, st);
+                    hipFreeAsync(ipiv, st);
+                }
+            }
+            if (hit) {
+                cache[r.factor_key].users--;
+                hits++;
+            }
+            in_use -= reserved - kept;
+            res.status = status;
+            res.info = info;
+            res.latency_us = t_done - t->t_submit;
+            t->done = true;
+            completed++;
+            lat.push_back(res.latency_us);
+            t_last = std::max(t_last, t_done);
+            running--;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+    hipFree(dinfo);
+    bola_hip_destroy(h);
+}
+
+extern "C" {
+
+int bola_hip_server_create(size_t hbm_budget, int slots, bola_hip_server_t* out) {
+    if (!out) return -3;
+    *out = nullptr;
+    if (slots < 1 || slots > 16) {
+        set_error(nullptr, "Argument 2 to server_create had an illegal value (1 <= slots <= 16)");
+        return -2;
+    }
+    if (hbm_budget < ((size_t)1 << 20)) {
+        set_error(nullptr, "Argument 1 to server_create had an illegal value (at least 1 MiB)");
+        return -1;
+    }
+    auto* s = new bola_hip_server_s;
+    hipGetDevice(&s->device);
+    s->budget = hbm_budget;
+    for (int i = 0; i < slots; i++) s->workers.emplace_back([s, i] { s->work(i); });
+    *out = s;
+    return 0;
+}
+
+int bola_hip_server_submit(bola_hip_server_t s, const bola_hip_request* r, uint64_t* ticket) {
+    if (!s || !r || !ticket) return -1;
+    if ((r->kind != BOLA_HIP_SERVE_LU && r->kind != BOLA_HIP_SERVE_CHOL) || r->n == 0 || r->nrhs == 0 || !r->b ||
+        !r->x || (!r->a && !r->factor_key)) {
+        set_error(nullptr, "server_submit: invalid request (kind, n > 0, nrhs > 0, a, b and x)");
+        return -2;
+    }
+    auto* t = new Ticket;
+    t->req = *r;
+    t->t_submit = now_us();
+    std::lock_guard<std::mutex> lk(s->mu);
+    bool hit = false;
+    const size_t nb = s->need(*r, hit);
+    if (nb > s->budget || (!hit && !r->a)) {
+        delete t;
+        s->rejected++;
+        set_error(nullptr, !r->a ? "server_submit: no matrix and no cached factorization under this key"
+                                 : "server_submit: the request alone exceeds the HBM budget");
+        return -3;
+    }
+    t->id = s->next_id++;
+    *ticket = t->id;
+    s->tickets[t->id] = t;
+    s->queue.push_back(t);
+    s->submitted++;
+    if (s->t_first < 0.0) s->t_first = t->t_submit;
+    s->cv_work.notify_all();
+    return 0;
+}
+
+static int finish(bola_hip_server_t s, Ticket* t, bola_hip_result* res) {
+    if (res) *res = t->res;
+    s->tickets.erase(t->id);
+    delete t;
+    return 0;
+}
+
+int bola_hip_server_wait(bola_hip_server_t s, uint64_t ticket, bola_hip_result* res) {
+    if (!s) return -1;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return -2;
+    Ticket* t = it->second;
+    s->cv_done.wait(lk, [&] { return t->done; });
+    return finish(s, t, res);
+}
+
+int bola_hip_server_poll(bola_hip_server_t s, uint64_t ticket, bola_hip_result* res) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return -2;
+    if (!it->second->done) return 1;
+    return finish(s, it->second, res);
+}
+
+int bola_hip_server_drain(bola_hip_server_t s) {
+    if (!s) return -1;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->queue.empty() && s->running == 0; });
+    return 0;
+}
+
+int bola_hip_server_get_stats(bola_hip_server_t s, bola_hip_server_stats* st) {
+    if (!s || !st) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = bola_hip_server_stats{};
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->rejected = s->rejected;
+    st->cache_hits = s->hits;
+    st->evictions = s->evictions;
+    st->hbm_in_use = s->in_use;
+    st->hbm_peak = s->peak;
+    st->hbm_budget = s->budget;
+    size_t cached = 0;
+    for (auto& kv : s->cache) cached += kv.second.bytes;
+    st->hbm_cached = cached;
+    if (!s->lat.empty()) {
+        std::vector<double> v = s->lat;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            const size_t i = (size_t)std::min<double>((double)v.size() - 1, std::ceil(p * (double)v.size()) - 1);
+            return v[i];
+        };
+        st->p50_us = pct(0.50);
+        st->p99_us = pct(0.99);
+        double sum = 0.0;
+        for (double x : v) sum += x;
+        st->mean_us = sum / (double)v.size();
+        const double span = s->t_last - s->t_first;
+        st->solves_per_s = span > 0.0 ? (double)s->completed / (span * 1e-6) : 0.0;
+    }
+    return 0;
+}
+
+int bola_hip_server_reset_stats(bola_hip_server_t s) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->submitted = s->completed = s->rejected = s->hits = s->evictions = 0;
+    s->lat.clear();
+    s->t_first = -1.0;
+    s->t_last = 0.0;
+    s->peak = s->in_use;
+    return 0;
+}
+
+int bola_hip_server_destroy(bola_hip_server_t s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        // requests not yet admitted are cancelled
+        for (Ticket* t : s->queue) {
+            t->res.status = BOLA_HIP_ERR_CANCELLED;
+            t->done = true;
+        }
+        s->queue.clear();
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    for (auto& w : s->workers) w.join();
+    hipSetDevice(s->device);
+    for (auto& kv : s->cache) {
+        hipFree(kv.second.f);
+        hipFree(kv.second.ipiv);
+    }
+    for (auto& kv : s->tickets) delete kv.second;
+    delete s;
+    return 0;
+}
+
+}  // extern "C"

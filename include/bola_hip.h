@@ -38,6 +38,7 @@ extern "C" {
 #endif
 
 #define BOLA_HIP_ERR_DEVICE (-1000)
+#define BOLA_HIP_ERR_CANCELLED (-1001) /* a served request dropped when its server closed */
 
 typedef struct bola_hip_handle_s* bola_hip_handle_t;
 
@@ -246,6 +247,61 @@ int bola_hip_batch_graph_create(bola_hip_handle_t h, int variant, size_t count, 
                                 const bola_hip_batch_config* cfg, bola_hip_batch_graph_t* out);
 int bola_hip_batch_graph_launch(bola_hip_batch_graph_t g);
 int bola_hip_batch_graph_destroy(bola_hip_batch_graph_t g);
+
+/* ---- Serving (no reference counterpart: the crate is a library of one-shot calls) ----
+ * A server owns `slots` worker threads, each with its own handle and HIP stream.  Solves
+ * submitted from any thread are admitted in arrival order as soon as a slot is free and
+ * their device buffers fit in the HBM budget; a request that does not fit yet may be
+ * passed by later ones a bounded number of times.  The slots' work overlaps on the
+ * device.  Factorizations submitted with a nonzero factor_key stay resident (counted
+ * against the budget, least recently used idle ones evicted first) and serve later
+ * requests with the same key, kind and n, which then only upload B and solve.  Results
+ * are those of bola_hip_dgetrf_dev + dgetrs_dev (LU) or dpotrf_dev + dpotrs_dev ('L'),
+ * bit for bit, whatever else is running. */
+#define BOLA_HIP_SERVE_LU 0
+#define BOLA_HIP_SERVE_CHOL 1
+typedef struct bola_hip_request {
+    int32_t kind; /* BOLA_HIP_SERVE_LU or BOLA_HIP_SERVE_CHOL (lower triangle used) */
+    int32_t pad;
+    size_t n, nrhs;
+    const double* a;     /* n x n column-major, lda = n, host or device memory; may be NULL on a
+                            cache hit.  A request with a key must present the same matrix. */
+    const double* b;     /* n x nrhs, ldb = n */
+    double* x;           /* n x nrhs, ldx = n: receives the solution */
+    uint64_t factor_key; /* 0: not cached */
+} bola_hip_request;
+typedef struct bola_hip_result {
+    int32_t status;    /* 0, or BOLA_HIP_ERR_DEVICE / BOLA_HIP_ERR_CANCELLED */
+    int32_t info;      /* the factorization's info (x is not written when nonzero) */
+    int32_t cache_hit; /* 1: served from a resident factorization */
+    int32_t slot;
+    double queue_us;   /* submission -> admission */
+    double latency_us; /* submission -> solution in x */
+} bola_hip_result;
+typedef struct bola_hip_server_stats {
+    uint64_t submitted, completed, rejected, cache_hits, evictions;
+    double p50_us, p99_us, mean_us; /* submission -> result */
+    double solves_per_s;            /* completed / (last completion - first submission) */
+    size_t hbm_in_use, hbm_peak, hbm_budget, hbm_cached;
+} bola_hip_server_stats;
+typedef struct bola_hip_server_s* bola_hip_server_t;
+/* The server runs on the current device.  hbm_budget: bytes of request buffers and
+ * resident factorizations (each slot's handle keeps its own small workspace besides);
+ * 1 <= slots <= 16. */
+int bola_hip_server_create(size_t hbm_budget, int slots, bola_hip_server_t* out);
+/* Queues a request; its buffers must stay valid until the ticket completes.  -3: the
+ * request alone exceeds the budget (or names no matrix and no resident factorization). */
+int bola_hip_server_submit(bola_hip_server_t s, const bola_hip_request* r, uint64_t* ticket);
+/* Blocks until the ticket completes and releases it. */
+int bola_hip_server_wait(bola_hip_server_t s, uint64_t ticket, bola_hip_result* res);
+/* 1: still pending; 0: complete (*res filled, the ticket released). */
+int bola_hip_server_poll(bola_hip_server_t s, uint64_t ticket, bola_hip_result* res);
+/* Blocks until nothing is queued or running. */
+int bola_hip_server_drain(bola_hip_server_t s);
+int bola_hip_server_get_stats(bola_hip_server_t s, bola_hip_server_stats* st);
+int bola_hip_server_reset_stats(bola_hip_server_t s);
+/* Cancels what is still queued, finishes what is running, frees everything. */
+int bola_hip_server_destroy(bola_hip_server_t s);
 
 #ifdef __cplusplus
 }
