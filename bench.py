@@ -1,14 +1,29 @@
-"""Benchmark: LU solve at FP64 accuracy on one MI355X.
+"""Benchmark: LU solve at FP64 accuracy on one MI355X, plain FP64 and mixed-precision IR.
 
-Workload (one step): DGETRF + DGETRS (laswp, trsm(L,L,N,U), trsm(L,U,N,N)) of a
-column-major n x n matrix with entries uniform in [-1, 1) and nrhs = 64 right-hand
-sides, n = 8192, inputs resident in HBM before the timed region (each step restores A
-and B from HBM copies first; the copy is outside the timed interval of that step).
-Throughput counts 2n^3/3 + 2n^2*nrhs flops per solve.  Per-step latencies give p50/p99.
+Headline workload (one step, the `value`): DGETRF + DGETRS (laswp, trsm(L,L,N,U),
+trsm(L,U,N,N)) of a column-major n x n matrix with entries uniform in [-1, 1) and nrhs = 64
+right-hand sides, n = 8192, inputs resident in HBM before the timed region (each step
+restores A and B from HBM copies first; the copy is outside the timed interval of that
+step).  Throughput counts 2n^3/3 + 2n^2*nrhs flops per solve.  Per-step latencies give
+p50/p99.
 
-The dominant kernel is the rank-32 trailing-update DGEMM; its achieved rate is measured
-with HIP events on the library's stream over the timed steps (phase "GEMM"), against
-the MI355X FP64 matrix peak.
+Beside it, in the same JSON line:
+  mixed_ir  -- A X = b at n = 16384, nrhs = 1, A = U diag(sigma) V^T with kappa = 1e2
+               (seeded Householder products), solved to backward error <= n * 2^-52 by the
+               mixed-precision solver (FP32 or FP16 factors, FP64 refinement / GMRES-IR,
+               FP64 fallback), each mode timed once after a warm-up, next to the plain FP64
+               GETRF + GETRS of the same system; the forward error is stated against the
+               plain FP64 solution (which is bit-identical to the oracle's).
+  batched   -- 20,000 independent systems, orders uniform in [8, 512], half LU and half
+               SPD (Cholesky), nrhs uniform in [1, 16], one persistent launch each for the
+               FP64 and the FP32 + FP64-refinement variants, with the autotuned launch
+               configuration.
+
+The dominant kernel of the headline step is the LU panel (phase GETRF2_IT, the persistent
+wavefront panel kernel); its algorithmic traffic is the panel read and written once per
+launch (2 * rows * 32 * 8 bytes), measured with HIP events on the library's stream over the
+timed steps, against the HBM peak.  The trailing-update DGEMM is reported beside it
+against the FP64 matrix peak.
 
 CPU baseline: the oracle (a C restatement of the reference's own algorithm, OpenMP
 standing in for rayon), timed on this host on one factorization + solve of the same
@@ -42,7 +57,19 @@ def main():
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=0, help="size of the CPU baseline sample (default: n)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
                                    "this build (the factorization shards by column blocks, see DESIGN.md)"}))
@@ -90,6 +117,11 @@ def main():
         print(f"# step {len(times)}: {dt * 1e3:.2f} ms", file=sys.stderr, flush=True)
     gemm_ms, gemm_calls, gemm_flops = h.profile_get("GEMM")
     panel_ms, panel_calls, _ = h.profile_get("GETRF2_IT")
+    # algorithmic bytes of the panel launches: each panel of w <= 32 columns and the rows at
+    # and below its diagonal, read once and written once
+    panel_bytes = 0.0
+    for j in range(0, n, 32):
+        panel_bytes += 2.0 * (n - j) * min(32, n - j) * 8.0
     report = h.profile_report(args.steps)
     h.profile(False)
 
@@ -125,6 +157,12 @@ def main():
                "kind": "port", "sample": f"one DGETRF+DGETRS n={cn} nrhs={nrhs} of the same seeded inputs, "
                                          f"{ct:.2f} s; bit-identical to the GPU result: {same}"}
 
+    extra_mixed = extra_batched = None
+    if not args.no_extra:
+        extra_mixed = mixed_section(B, torch)
+        extra_batched = batched_section(B, torch)
+
+    panel_gbs = panel_bytes * args.steps / (panel_ms * 1e-3) / 1e9 if panel_ms > 0 else 0.0
     out = {
         "metric": "LU solve GFLOP/s at FP64 accuracy (plain FP64 DGETRF+DGETRS)",
         "value": round(value, 2),
@@ -141,16 +179,104 @@ def main():
         "config": {"workload": f"DGETRF+DGETRS n={n} nrhs={nrhs} column-major f64", "n": n, "nrhs": nrhs},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "backward_error": berr,
-        "roofline": {"bound": "mfma", "kernel": "dgemm_mfma (rank-32 trailing update)",
-                     "achieved": round(gemm_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(gemm_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "gemm_ms_per_step": round(gemm_ms / args.steps, 3), "gemm_launches_per_step":
-                         gemm_calls // max(args.steps, 1),
-                     "panel_ms_per_step": round(panel_ms / args.steps, 3)},
+        "roofline": {"bound": "hbm", "kernel": "getf2_wave (LU panel, 32 columns, persistent)",
+                     "achieved": round(panel_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(panel_gbs / HBM_PEAK_GBS, 6), "traffic": None,
+                     "panel_ms_per_step": round(panel_ms / args.steps, 3),
+                     "panel_launches_per_step": panel_calls // max(args.steps, 1),
+                     "panel_avg_launch_us": round(1e3 * panel_ms / max(panel_calls, 1), 2),
+                     "algorithmic_bytes_per_step": panel_bytes},
+        "gemm": {"bound": "mfma", "kernel": "dgemm_mfma (trailing update)", "achieved": round(gemm_tflops, 3),
+                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(gemm_tflops / FP64_PEAK_TFLOPS, 4),
+                 "gemm_ms_per_step": round(gemm_ms / args.steps, 3),
+                 "gemm_launches_per_step": gemm_calls // max(args.steps, 1)},
+        "mixed_ir": extra_mixed,
+        "batched": extra_batched,
+        "serving": None,
         "cpu_baseline": cpu,
     }
     print(report, file=sys.stderr)
     print(json.dumps(out))
+
+
+def mixed_section(B, torch, n=16384, kappa=1e2):
+    """Config: mixed-precision IR at n = 16384, nrhs = 1, kappa = 1e2."""
+    from bola_amd import testmat
+    A = testmat.conditioned(n, kappa, 7, device="cuda")
+    b = torch.from_numpy(np.random.default_rng(8).uniform(-1, 1, n)).cuda()
+    flops = lu_flops(n, 1)
+    # plain FP64 on the same system
+    a = A.clone()
+    p = torch.zeros(n, dtype=torch.int32, device="cuda")
+    xp = b.clone()
+    B.getrf(n, n, a, n, p)
+    B.getrs(n, 1, a, n, p, xp, n)
+    torch.cuda.synchronize()
+    a.copy_(A)
+    xp.copy_(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.getrf(n, n, a, n, p)
+    B.getrs(n, 1, a, n, p, xp, n)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    del a
+    out = {"workload": f"A X = b, n={n} nrhs=1, A = U diag(sigma) V^T, kappa={kappa:g}, b uniform [-1,1)",
+           "plain_fp64": {"ms": round(plain_s * 1e3, 2), "gflops": round(flops / plain_s / 1e9, 1)},
+           "modes": {}}
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    xpn = float(xp.abs().max())
+    for name, prec, rbt in [("fp32_butterfly", "fp32", True), ("fp32_tournament", "fp32", False),
+                            ("fp16_tournament", "fp16", False)]:
+        B.dgesv_mixed(n, 1, A, n, b, n, x, n, prec, rbt=rbt)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = B.dgesv_mixed(n, 1, A, n, b, n, x, n, prec, rbt=rbt)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ferr = float((x - xp).abs().max()) / xpn
+        out["modes"][name] = {"ms": round(dt * 1e3, 2), "gflops": round(flops / dt / 1e9, 1),
+                              "speedup_vs_plain_fp64": round(plain_s / dt, 3),
+                              "final_berr": st["final_berr"], "tolerance": st["tolerance"],
+                              "converged": st["converged"], "ir_iterations": st["ir_iterations"],
+                              "gmres_iterations": st["gmres_iterations"], "fallback": st["fallback"],
+                              "factor_ms": round(st["factor_ms"], 2), "refine_ms": round(st["refine_ms"], 2),
+                              "forward_error_vs_fp64": ferr}
+        print(f"# mixed {name}: {dt * 1e3:.1f} ms, berr {st['final_berr']:.2e}", file=sys.stderr, flush=True)
+    best = max(out["modes"].values(), key=lambda m: m["gflops"] if m["converged"] else 0.0)
+    out["best_gflops"] = best["gflops"]
+    out["best_speedup_vs_plain_fp64"] = best["speedup_vs_plain_fp64"]
+    del A
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_section(B, torch, count=20000):
+    """Config: 20,000 uneven systems (orders 8..512, half LU / half SPD, nrhs 1..16)."""
+    from bola_amd import batchgen
+    d, arena, ptot = batchgen.make_batch(count, 8, 512, 16, seed=1)
+    flops = batchgen.cost_flops(d)
+    piv = torch.zeros(ptot, dtype=torch.int32, device="cuda")
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    iters = torch.zeros(count, dtype=torch.int32, device="cuda")
+    out = {"workload": f"{count} systems, n uniform [8,512], half LU / half SPD, nrhs uniform [1,16], "
+                       f"{arena.numel() * 8 / 2**30:.2f} GiB in HBM", "fp64_equiv_gflop": round(flops / 1e9, 1)}
+    for variant, name in [(B.BATCH_FP64, "fp64"), (B.BATCH_FP32_IR, "fp32_ir")]:
+        cfg = B.batch_tune(variant, d, arena)
+        work = arena.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.batch_solve(variant, d, work, piv, info, iters, cfg)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        it = iters.cpu().numpy()
+        out[name] = {"ms": round(dt * 1e3, 2), "gflops": round(flops / dt / 1e9, 1), "config": list(cfg),
+                     "failed": int((info != 0).sum()), "refined_fallbacks": int((it == -1).sum())}
+        print(f"# batched {name}: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        del work
+    del arena
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":
