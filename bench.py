@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=0, help="size of the CPU baseline sample (default: n)")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
+    ap.add_argument("--only", choices=["mixed", "batched"], default=None,
+                    help="run one section alone (for profiling) and print it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -77,6 +79,12 @@ def main():
 
     import torch
     import bola_amd as B
+
+    if args.only:
+        torch.cuda.set_device(0)
+        sec = mixed_section(B, torch) if args.only == "mixed" else batched_section(B, torch)
+        print(json.dumps({args.only: sec}))
+        return
 
     n, nrhs = args.n, args.nrhs
     torch.cuda.set_device(0)
@@ -181,7 +189,8 @@ def main():
         "backward_error": berr,
         "roofline": {"bound": "hbm", "kernel": "getf2_wave (LU panel, 32 columns, persistent)",
                      "achieved": round(panel_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(panel_gbs / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(panel_gbs / HBM_PEAK_GBS, 6),
+                     "traffic": panel_traffic() if n == 8192 else None,
                      "panel_ms_per_step": round(panel_ms / args.steps, 3),
                      "panel_launches_per_step": panel_calls // max(args.steps, 1),
                      "panel_avg_launch_us": round(1e3 * panel_ms / max(panel_calls, 1), 2),
@@ -197,6 +206,19 @@ def main():
     }
     print(report, file=sys.stderr)
     print(json.dumps(out))
+
+
+def panel_traffic():
+    """HBM bytes per panel launch at n = 8192, from the FETCH_SIZE / WRITE_SIZE counter runs
+    (scaled by a plain copy's counts) recorded in profiles/getf2_wave_lu_n8192_hbm_traffic.md."""
+    path = os.path.join(ROOT, "profiles", "getf2_wave_lu_n8192_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def mixed_section(B, torch, n=16384, kappa=1e2):
