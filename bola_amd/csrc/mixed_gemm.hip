@@ -46,15 +46,42 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
     const int tm = bid % tiles_m, tn = bid / tiles_m;
     const int i0 = tm * GBM, j0 = tn * GBN;
     const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
+    // The accumulators start from C (clamped addresses; zero with overwrite) and A enters
+    // negated, so they end as C - A B.  D layout: reg r of lane l -> (row 4*(l/16)+r, col l%16).
     f4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < 4; b++) {
+            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                acc[a][b][r] = overwrite ? 0.f : C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+        }
 
-    // loaders: A tile 128 x 32 as 4 rows x 1 k per thread-chunk; B tile 32 x 128 as 4 k x 1 col
+    // loaders: A tile 128 x 32 as 4 rows x 1 k per thread-chunk; B tile 32 x 128 as 4 k x 1 col.
+    // Tiles wholly inside the operands with 16-byte aligned columns load 4 floats at once.
+    const bool vec = i0 + GBM <= m && j0 + GBN <= n && (k % 32) == 0 && (lda % 4) == 0 && (ldb % 4) == 0 &&
+                     (((uintptr_t)A | (uintptr_t)B) & 15) == 0;
     float ra[4][4], rb[4][4];
     auto gload = [&](int k0) {
+        if (vec) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int e = tid + q * GT;
+                const int ii = (e & 31) * 4, kk = e >> 5;
+                const f4 va = *(const f4*)(A + (size_t)(i0 + ii) + (size_t)(k0 + kk) * lda);
+                const int kb = (e & 7) * 4, jj = e >> 3;
+                const f4 vb = *(const f4*)(B + (size_t)(k0 + kb) + (size_t)(j0 + jj) * ldb);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    ra[q][r] = va[r];
+                    rb[q][r] = vb[r];
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int e = tid + q * GT;       // 0..1023
@@ -82,7 +109,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
             const int e = tid + q * GT;
             const int ii = (e & 31) * 4, kk = e >> 5;
 #pragma unroll
-            for (int r = 0; r < 4; r++) As[buf][(ii + r) * RW + kk] = ra[q][r];
+            for (int r = 0; r < 4; r++) As[buf][(ii + r) * RW + kk] = -ra[q][r];
             const int kb = (e & 7) * 4, jj = e >> 3;
             *(f4*)&Bs[buf][jj * RW + kb] = f4{rb[q][0], rb[q][1], rb[q][2], rb[q][3]};
         }
@@ -109,19 +136,6 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
         if (t + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
-    // C -= acc (C = -acc with overwrite); D layout: reg r of lane l -> (row 4*(l/16)+r,
-    // col l%16).  C is read from clamped addresses first, every load in flight at once.
-    float cv[4][4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
-            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                cv[a][b][r] = overwrite ? 0.f : C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
-        }
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -132,7 +146,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
                 float* cp = C + (size_t)j * ldc;
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - acc[a][b][r];
+                    if (ib + r < m) cp[ib + r] = acc[a][b][r];
             }
         }
 }
