@@ -376,7 +376,6 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
 
 constexpr int NPLD = kMixNB + 4;      // LDS row stride (floats) of panel rows
 constexpr long kPanelSpin = 1L << 22;  // bound on the polls of one flag
-constexpr int TRC = 64;               // columns per workgroup of trsm_rbt
 constexpr size_t kInvPanel = 2 * (size_t)(kMixNB / LPW) * LPW * LPW;  // floats of one panel's inverses
 
 // column b (a lane) of the inverse of d's unit lower triangle (LOWER) or of its upper
@@ -501,7 +500,7 @@ __device__ __forceinline__ void diag_lu_rows(const float (*a)[NPLD], float (*d)[
 // raises err (the factors are then not to be trusted).
 __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __restrict__ T, size_t ldt,
                                                  float* __restrict__ inv, int* __restrict__ flags, int epoch,
-                                                 int* __restrict__ err) {
+                                                 int* __restrict__ err, float* __restrict__ nlt) {
     __shared__ float a[LPW][NPLD];   // this workgroup's rows, panel columns 0 .. w
     __shared__ float ub[LPW][NPLD];  // the current leaf's rows
     __shared__ float ui[LPW][LPW + 1], li[LPW][LPW + 1], d[LPW][LPW + 1];
@@ -599,93 +598,77 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (tid == 0) __hip_atomic_store(flags + wb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // Then, off the leaves' chain: row block wb of L11^-1 (the panel's unit lower factor
+        // inverted), for the U12 products.  Block (wb, j) = -Li_wb,wb sum_{m=j}^{wb-1}
+        // L_wb,m Li_m,j, with the rows of earlier owners read back from nlt once their flags
+        // (flags[8 + m]) rise.  nlt holds -(L11^-1)^T column-major, i.e. nlt[c + r w] =
+        // -L11^-1(r, c).
+        float* const nrow = nlt + (size_t)cj * w;  // rows cj .. cj + lw of -L11^-1
+        float tacc[kMixNB / LPW][4];
+#pragma unroll
+        for (int j = 0; j < kMixNB / LPW; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) tacc[j][q] = 0.f;
+        const int er = tid >> 3, ec = (tid & 7) * 4;  // this thread's 4 elements of a 32 x 32 block
+        for (int m = 0; m < wb; m++) {
+            if (tid == 0) {
+                long sp = 0;
+                int ok = 1;
+                while (__hip_atomic_load(flags + kMixNB / LPW + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       epoch) {
+                    if (++sp > kPanelSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bail = !ok;
+            }
+            __syncthreads();
+            if (bail) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // ub = L11^-1 rows of block m, columns 0 .. 32(m+1) (nlt holds the negatives)
+            for (int r = 0; r < LPW; r++)
+                ub[r][col] = col < LPW * (m + 1) ? -nlt[(size_t)col + (size_t)(LPW * m + r) * w] : 0.f;
+            __syncthreads();
+            // tacc[j] += L_wb,m Li_m,j for j <= m
+#pragma unroll
+            for (int j = 0; j < kMixNB / LPW; j++) {
+                if (j > m) break;
+#pragma unroll 4
+                for (int s2 = 0; s2 < LPW; s2++) {
+                    const float l = a[er][LPW * m + s2];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) tacc[j][q] = fmaf(l, ub[s2][LPW * j + ec + q], tacc[j][q]);
+                }
+            }
+            __syncthreads();
+        }
+        // block j < wb: -Li_wb,wb T_j (through LDS d), then the diagonal block and zeros
+        for (int j = 0; j < wb; j++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) d[er][ec + q] = tacc[j][q];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float v = 0.f;
+#pragma unroll 8
+                for (int t = 0; t < LPW; t++) v = fmaf(li[er][t], d[t][ec + q], v);
+                if (er < lw) nrow[(size_t)(LPW * j + ec + q) + (size_t)er * w] = v;  // -(-Li T) stored negated
+            }
+            __syncthreads();
+        }
+        for (int r = 0; r < lw; r++)
+            if (col >= cj && col < w)
+                nrow[(size_t)col + (size_t)r * w] = col < cj + lw ? -li[r][col - cj] : 0.f;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flags + kMixNB / LPW + wb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     for (int r = 0; r < nr; r++)
         if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
-}
-
-// X := L11^-1 X for the columns [c_lo, c_hi) of the nb-row block X (X(r, c) at
-// X[c + r ldx]), L11 the unit lower factor of the panel at A(k.., k..) in T with its
-// diagonal-block inverses in inv; with eye, X starts as -I (and is not read).  One
-// workgroup per 64 columns holds its nb x 64 block in LDS: blocked forward substitution,
-// right-looking over the 32-row blocks, each block's inverse and the L entries below it
-// staged in LDS (read as wavefront broadcasts).
-__global__ void __launch_bounds__(256) trsm_rbt(int k, int nb, int c_lo, int c_hi, const float* __restrict__ T,
-                                                size_t ldt, float* __restrict__ X, size_t ldx,
-                                                const float* __restrict__ inv, int eye) {
-    __shared__ float x[kMixNB][TRC + 1];
-    __shared__ float lb[kMixNB - LPW][LPW + 1];  // L rows below block j, its 32 columns
-    __shared__ float ls[LPW][LPW + 1];           // L_jj^-1
-    const int tid = threadIdx.x, cl = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int c = c_lo + blockIdx.x * TRC + cl;
-    const bool cv = c < c_hi;
-    if (eye) {
-        for (int r = wv; r < nb; r += 4) x[r][cl] = r == c ? -1.f : 0.f;
-    } else {
-        const size_t cc = (size_t)min(c, c_hi - 1);
-        for (int r0 = wv; r0 < nb; r0 += 32) {  // eight loads in flight per thread
-            float v[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) v[i] = X[cc + (size_t)min(r0 + 4 * i, nb - 1) * ldx];
-#pragma unroll
-            for (int i = 0; i < 8; i++)
-                if (r0 + 4 * i < nb) x[r0 + 4 * i][cl] = cv ? v[i] : 0.f;
-        }
-    }
-    const int nl = (nb + LPW - 1) / LPW;
-    for (int j = 0; j < nl; j++) {
-        const int rj = j * LPW, nbelow = max(0, nb - rj - LPW);
-        const float* li = inv + (size_t)j * 2 * LPW * LPW + LPW * LPW;
-        for (int e = tid; e < LPW * LPW; e += 256) ls[e / LPW][e % LPW] = li[e];
-        if (nbelow > 0) {
-            const int s = tid & 31;
-            for (int r0 = tid >> 5; r0 < nbelow; r0 += 64) {  // eight loads in flight per thread
-                float v[8];
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-                    v[i] = T[(size_t)(k + rj + s) + (size_t)(k + rj + LPW + min(r0 + 8 * i, nbelow - 1)) * ldt];
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-                    if (r0 + 8 * i < nbelow) lb[r0 + 8 * i][s] = v[i];
-            }
-        }
-        __syncthreads();
-        // x_j = L_jj^-1 x_j: rows wv*8 .. +8 of the block, column cl
-        float xs[LPW];
-#pragma unroll
-        for (int s = 0; s < LPW; s++) xs[s] = rj + s < nb ? x[rj + s][cl] : 0.f;
-        float y[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            float v = 0.f;
-#pragma unroll
-            for (int s = 0; s < LPW; s++) v = fmaf(ls[wv * 8 + i][s], xs[s], v);
-            y[i] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-            if (rj + wv * 8 + i < nb) x[rj + wv * 8 + i][cl] = y[i];
-        // the rows below: x_i -= L_ij x_j, with x_j = y (this thread's column) exchanged
-        // through LDS
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < LPW; s++) xs[s] = x[rj + s][cl];
-        for (int r = wv; r < nbelow; r += 4) {
-            float v0 = 0.f, v1 = 0.f;
-#pragma unroll
-            for (int s = 0; s < LPW; s += 2) {
-                v0 = fmaf(lb[r][s], xs[s], v0);
-                v1 = fmaf(lb[r][s + 1], xs[s + 1], v1);
-            }
-            x[rj + LPW + r][cl] -= v0 + v1;
-        }
-        __syncthreads();
-    }
-    for (int r = wv; r < nb; r += 4)
-        if (cv) X[(size_t)c + (size_t)r * ldx] = x[r][cl];
 }
 
 // dst (rows x cols, column-major, ld ldd) = src (ld lds)
@@ -866,7 +849,7 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
     c2.st = h->aux_stream();
     c2.gws = b + o + gsz;
     hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, c.st, c.cnt, levels * G);
-    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(64), 0, c.st, c.flags, kMixNB / LPW);
+    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(64), 0, c.st, c.flags, 2 * kMixNB / LPW);
     BOLA_LAUNCH_CHECK();
     if (!pivot) {  // identity interchanges: the solve path stays the same
         hipLaunchKernelGGL(k_iota_piv, dim3(cdiv((size_t)n, 256)), dim3(256), 0, c.st, n, ipiv);
@@ -888,10 +871,7 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
             c.nlt = c2.nlt = nltb + (size_t)par * kMixNB * kMixNB;
             c2.ubuf = c.ubuf;
             hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(n - k), LPW)), dim3(256), 0, c.st, n, k, nb, T, ldt, c.inv,
-                               c.flags, k / kMixNB + 1, err);
-            // -(L11^-1)^T, column-major nb x nb: L11^-1 applied to -I, stored row by row
-            hipLaunchKernelGGL(trsm_rbt, dim3(cdiv((size_t)nb, TRC)), dim3(256), 0, c.st, k, nb, 0, nb, T, ldt, c.nlt,
-                               (size_t)nb, c.inv, 1);
+                               c.flags, k / kMixNB + 1, err, c.nlt);
             BOLA_LAUNCH_CHECK();
         }
         // the columns right of the panel are stream 2's until its previous bulk update is done
