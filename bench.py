@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=0, help="size of the CPU baseline sample (default: n)")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
-    ap.add_argument("--only", choices=["mixed", "batched"], default=None,
+    ap.add_argument("--only", choices=["mixed", "batched", "dgemm"], default=None,
                     help="run one section alone (for profiling) and print it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
@@ -82,7 +82,7 @@ def main():
 
     if args.only:
         torch.cuda.set_device(0)
-        sec = mixed_section(B, torch) if args.only == "mixed" else batched_section(B, torch)
+        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section}[args.only](B, torch)
         print(json.dumps({args.only: sec}))
         return
 
@@ -165,8 +165,9 @@ def main():
                "kind": "port", "sample": f"one DGETRF+DGETRS n={cn} nrhs={nrhs} of the same seeded inputs, "
                                          f"{ct:.2f} s; bit-identical to the GPU result: {same}"}
 
-    extra_mixed = extra_batched = None
+    extra_mixed = extra_batched = extra_dgemm = None
     if not args.no_extra:
+        extra_dgemm = dgemm_section(B, torch)
         extra_mixed = mixed_section(B, torch)
         extra_batched = batched_section(B, torch)
 
@@ -199,6 +200,7 @@ def main():
                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(gemm_tflops / FP64_PEAK_TFLOPS, 4),
                  "gemm_ms_per_step": round(gemm_ms / args.steps, 3),
                  "gemm_launches_per_step": gemm_calls // max(args.steps, 1)},
+        "dgemm": extra_dgemm,
         "mixed_ir": extra_mixed,
         "batched": extra_batched,
         "serving": None,
@@ -219,6 +221,35 @@ def panel_traffic():
     except OSError:
         pass
     return None
+
+
+def dgemm_section(B, torch):
+    """Plain DGEMM 'N','N', alpha = beta = 1, A, B, C uniform [0, 1) (the crate's gemm
+    bench inputs, drawn here with numpy from seed 1): m = n = k = 1024 (the crate's bench
+    size) and 8192, each the mean of repeated calls between events."""
+    out = {}
+    rng = np.random.default_rng(1)
+    for n, reps in [(1024, 50), (8192, 5)]:
+        a = torch.from_numpy(rng.random(n * n)).cuda()
+        b = torch.from_numpy(rng.random(n * n)).cuda()
+        c0 = torch.from_numpy(rng.random(n * n)).cuda()
+        c = c0.clone()
+        B.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 1.0, c, n)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            B.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 1.0, c, n)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+        out[f"n{n}"] = {"ms": round(ms, 4), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)}
+        print(f"# dgemm {n}: {ms:.3f} ms, {tf:.2f} TF/s", file=sys.stderr, flush=True)
+        del a, b, c, c0
+    torch.cuda.empty_cache()
+    return out
 
 
 def mixed_section(B, torch, n=16384, kappa=1e2):

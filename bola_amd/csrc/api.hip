@@ -185,7 +185,7 @@ void gemm_dev(bola_hip_handle_s* h, char transa, char transb, size_t m, size_t n
         else if (beta != 1.0) launch_scale(h->stream, c, ldc, m, n, beta);
         return;
     }
-    launch_dgemm(h->stream, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nullptr);
+    launch_dgemm(h->stream, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nullptr, true);
 }
 
 }  // namespace

@@ -203,14 +203,16 @@ void launch_rowmap_apply(hipStream_t st, double* a, size_t lda, size_t left, siz
 // (used inside factorizations so that work queued behind a failed pivot is skipped).
 void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                  const int32_t* abort = nullptr);
+                  const int32_t* abort = nullptr, bool fast = false);
 // slab: depth of one accumulate-then-fold step (64 = gemm.rs KC; 32 = the syrk.rs block
 // loop, or one LU panel per step); mask: 0 all of C, 1 only rows in a later 32-block than
 // the column, 2 earlier; edge_m / edge_n: first row / column (relative to C) that takes
-// the reference's unfused edge path, -1 for this call's own m - m%8 / n - n%4.
+// the reference's unfused edge path, -1 for this call's own m - m%8 / n - n%4.  fast: the
+// register budget of two workgroups per CU (stand-alone products); otherwise one, which
+// leaves more of the machine to a factorization's panel running beside it.
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask, long edge_m = -1, long edge_n = -1);
+                     const int32_t* abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
 void set_num_cus(int n);
 
 // trsm.hip

@@ -57,8 +57,11 @@ __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
     else { j = e % BN; kk = e / BN; }           // B^T: n fastest
 }
 
-template <int BM, int BN, bool TA, bool TB>
-__global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
+// OCC: workgroups per CU the register budget is fitted to.  2 for stand-alone products;
+// 1 (more registers, lighter on the machine) for the updates that run beside a
+// factorization's latency-bound panel kernel.
+template <int BM, int BN, bool TA, bool TB, int OCC>
+__global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
                                                   int edge_m, int edge_n, const int32_t* __restrict__ abort,
@@ -116,8 +119,10 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
 #pragma unroll
                 for (int r = 0; r < 4; r++) crun[a][b][r] = crun[a][b][r] * beta;
 
-    double ra[Cfg::A_LOADS], rb[Cfg::B_LOADS];
-    auto gload = [&](int k0) {
+    // Two register stages: the global loads for step t+2 are issued while step t computes,
+    // so each load has two steps of MFMA work to arrive in.
+    double ra0[Cfg::A_LOADS], rb0[Cfg::B_LOADS], ra1[Cfg::A_LOADS], rb1[Cfg::B_LOADS];
+    auto gload = [&](int k0, double (&ra)[Cfg::A_LOADS], double (&rb)[Cfg::B_LOADS]) {
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
             rb[s] = keep_or_zero(v, gj < n && gk < k);
         }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, const double (&ra)[Cfg::A_LOADS], const double (&rb)[Cfg::B_LOADS]) {
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
@@ -153,16 +158,9 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
             else sB[buf][kk * BN + j] = rb[s];
         }
     };
-
     const int nsteps = (k + BK - 1) / BK;
-    if (nsteps > 0) {  // k == 0: C = beta C, and A, B are not read
-        gload(0);
-        lstore(0);
-    }
-    __syncthreads();
-    for (int step = 0; step < nsteps; step++) {
-        const int buf = step & 1;
-        if (step + 1 < nsteps) gload((step + 1) * BK);
+    // one BK-deep step from LDS buffer buf, then the slab fold when a 64-deep slab (or k) ends
+    auto compute = [&](int buf, int step) {
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             double fa[Cfg::TM], fb[Cfg::TN];
@@ -182,7 +180,6 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
                 for (int b = 0; b < Cfg::TN; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
-        // end of a 64-deep slab (or of k): fold the slab into C, restart the accumulator
         const int kdone = (step + 1) * BK;
         if ((kdone % slab) == 0 || step + 1 == nsteps) {
 #pragma unroll
@@ -195,8 +192,26 @@ __global__ void __launch_bounds__(256) dgemm_mfma(int m, int n, int k, double al
                         acc[a][b][r] = 0.0;
                     }
         }
-        if (step + 1 < nsteps) {
-            lstore(buf ^ 1);
+    };
+
+    if (nsteps > 0) {  // k == 0: C = beta C, and A, B are not read
+        gload(0, ra0, rb0);
+        lstore(0, ra0, rb0);
+        if (nsteps > 1) gload(BK, ra1, rb1);
+    }
+    __syncthreads();
+    for (int step = 0; step < nsteps; step += 2) {
+        // even step: LDS 0 holds step, registers 1 hold step + 1
+        if (step + 2 < nsteps) gload((step + 2) * BK, ra0, rb0);
+        compute(0, step);
+        if (step + 1 >= nsteps) break;
+        lstore(1, ra1, rb1);
+        __syncthreads();
+        // odd step: LDS 1 holds step + 1, registers 0 hold step + 2
+        if (step + 3 < nsteps) gload((step + 3) * BK, ra1, rb1);
+        compute(1, step + 1);
+        if (step + 2 < nsteps) {
+            lstore(0, ra0, rb0);
             __syncthreads();
         }
     }
@@ -256,29 +271,33 @@ __global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double al
 template <int BM, int BN, bool TA, bool TB>
 void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
                  size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, const int32_t* abort,
-                 int slab, int mask) {
+                 int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
-    hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+    if (fast)
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 2>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+    else
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B,
+                           ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
     BOLA_LAUNCH_CHECK();
 }
 
 template <bool TA, bool TB>
 void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
               size_t ldb, double beta, double* C, size_t ldc, int num_cus, const int32_t* abort, int slab,
-              int mask, int edge_m, int edge_n) {
+              int mask, int edge_m, int edge_n, bool fast) {
     if (edge_m > 0 && edge_n > 0) {
         const long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
         const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
         if (t128 >= 2L * num_cus)
             launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
-                                         slab, mask);
+                                         slab, mask, fast);
         else if (t64 >= num_cus || m * (long)n <= 64L * 64)
             launch_tile<64, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
-                                        mask);
+                                        mask, fast);
         else
             launch_tile<32, 32, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
-                                        mask);
+                                        mask, fast);
     }
     long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
     if (nedge > 0) {
@@ -298,13 +317,13 @@ void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
 // Requires m, n, k < 2^31.  alpha == 0 must be handled by the caller (beta-only pass).
 void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                  const int32_t* abort) {
-    launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0, -1, -1);
+                  const int32_t* abort, bool fast) {
+    launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0, -1, -1, fast);
 }
 
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask, long edge_m, long edge_n) {
+                     const int32_t* abort, int slab, int mask, long edge_m, long edge_n, bool fast) {
     bool ta = !(transa == 'N' || transa == 'n');
     bool tb = !(transb == 'N' || transb == 'n');
     int M = (int)m, N = (int)n, K = (int)k;
@@ -312,10 +331,10 @@ void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t 
     // by default the last m % 8 rows and n % 4 columns of this call (gemm.rs:177-190)
     const int em = edge_m < 0 ? M - (M % 8) : (int)std::min<long>(std::max<long>(edge_m, 0), M);
     const int en = edge_n < 0 ? N - (N % 4) : (int)std::min<long>(std::max<long>(edge_n, 0), N);
-    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
-    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
-    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
-    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en);
+    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
+    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
+    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
+    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
 }
 
 }  // namespace bola
