@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=0, help="size of the CPU baseline sample (default: n)")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
-    ap.add_argument("--only", choices=["mixed", "batched", "dgemm"], default=None,
+    ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "serving"], default=None,
                     help="run one section alone (for profiling) and print it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
@@ -82,7 +82,8 @@ def main():
 
     if args.only:
         torch.cuda.set_device(0)
-        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section}[args.only](B, torch)
+        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section,
+               "serving": serving_section}[args.only](B, torch)
         print(json.dumps({args.only: sec}))
         return
 
@@ -165,11 +166,12 @@ def main():
                "kind": "port", "sample": f"one DGETRF+DGETRS n={cn} nrhs={nrhs} of the same seeded inputs, "
                                          f"{ct:.2f} s; bit-identical to the GPU result: {same}"}
 
-    extra_mixed = extra_batched = extra_dgemm = None
+    extra_mixed = extra_batched = extra_dgemm = extra_serving = None
     if not args.no_extra:
         extra_dgemm = dgemm_section(B, torch)
         extra_mixed = mixed_section(B, torch)
         extra_batched = batched_section(B, torch)
+        extra_serving = serving_section(B, torch)
 
     panel_gbs = panel_bytes * args.steps / (panel_ms * 1e-3) / 1e9 if panel_ms > 0 else 0.0
     out = {
@@ -203,7 +205,7 @@ def main():
         "dgemm": extra_dgemm,
         "mixed_ir": extra_mixed,
         "batched": extra_batched,
-        "serving": None,
+        "serving": extra_serving,
         "cpu_baseline": cpu,
     }
     print(report, file=sys.stderr)
@@ -301,6 +303,60 @@ def mixed_section(B, torch, n=16384, kappa=1e2):
     out["best_speedup_vs_plain_fp64"] = best["speedup_vs_plain_fp64"]
     del A
     torch.cuda.empty_cache()
+    return out
+
+
+def serving_section(B, torch, nmax=600):
+    """Solve server under Poisson arrivals: orders log-uniform in [32, nmax] (quantised to 16
+    sizes), 70 % LU / 30 % SPD, nrhs uniform in [1, 16], 30 % of requests reuse a resident
+    factorization with a new right-hand side; host matrices (uploaded by the server), 4 slots,
+    16 GiB budget.  Two loads: 200 requests/s for 5 s (latency) and 2000/s for 2 s
+    (throughput).  The full [32, 8192] range of the serving configuration is not run."""
+    import math
+    rng = np.random.default_rng(1)
+    sizes = np.unique(np.round(np.exp(np.linspace(math.log(32), math.log(nmax), 16))).astype(int))
+    pool = []
+    for n in sizes:
+        for kind in (B.SERVE_LU, B.SERVE_CHOL):
+            G = rng.uniform(-1, 1, (n, n))
+            A = G @ G.T + n * np.eye(n) if kind == B.SERVE_CHOL else G
+            pool.append((int(n), kind, np.asfortranarray(A).ravel(order="F").copy()))
+    out = {"workload": f"Poisson arrivals, n log-uniform [32,{nmax}], 70% LU / 30% SPD, nrhs uniform [1,16], "
+                       f"30% reuse a resident factorization, FP64, 4 slots, host operands"}
+    with B.Server(16 << 30, 4) as s:
+        for (n, kind, A) in pool:  # warm-up, one at a time
+            s.wait(s.submit(kind, n, 1, A, np.ones(n), np.zeros(n), 0))
+        for name, qps, dur in [("latency", 200.0, 5.0), ("throughput", 2000.0, 2.0)]:
+            nreq = int(qps * dur)
+            arr = np.cumsum(rng.exponential(1.0 / qps, nreq))
+            ln = np.exp(rng.uniform(math.log(32), math.log(nmax), nreq))
+            reqs = []
+            for k in range(nreq):
+                n = sizes[np.argmin(np.abs(np.log(sizes) - math.log(ln[k])))]
+                kind = B.SERVE_LU if rng.random() < 0.7 else B.SERVE_CHOL
+                idx = [j for j, p in enumerate(pool) if p[0] == n and p[1] == kind][0]
+                nrhs = int(rng.integers(1, 17))
+                reqs.append((arr[k], idx, nrhs, rng.random() < 0.3, rng.uniform(-1, 1, n * nrhs)))
+            s.reset_stats()
+            live = []
+            t0 = time.perf_counter()
+            for (ta, idx, nrhs, reuse, b) in reqs:
+                n, kind, A = pool[idx]
+                while time.perf_counter() - t0 < ta:
+                    time.sleep(0.0002)
+                x = np.zeros(n * nrhs)
+                live.append((s.submit(kind, n, nrhs, A, b, x, (idx + 1) if reuse else 0), n, kind, nrhs))
+            res = [s.wait(t[0]) for t in live]
+            span = time.perf_counter() - t0
+            st = s.stats()
+            flops = sum(((2 * n ** 3 / 3 if kind == B.SERVE_LU else n ** 3 / 3) if not r["cache_hit"] else 0.0)
+                        + 2.0 * n * n * nrhs for (t, n, kind, nrhs), r in zip(live, res))
+            out[name] = {"offered_per_s": qps, "requests": nreq, "p50_ms": round(st["p50_us"] / 1e3, 3),
+                         "p99_ms": round(st["p99_us"] / 1e3, 3), "solves_per_s": round(st["solves_per_s"], 1),
+                         "gflops": round(flops / span / 1e9, 2), "cache_hits": st["cache_hits"],
+                         "failed": sum(1 for r in res if r["status"] != 0 or r["info"] != 0)}
+            print(f"# serving {name}: p50 {out[name]['p50_ms']} ms p99 {out[name]['p99_ms']} ms, "
+                  f"{out[name]['solves_per_s']} solves/s", file=sys.stderr, flush=True)
     return out
 
 
