@@ -274,8 +274,8 @@ void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     if (fast)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 2>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B,
-                           ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : 3)>), dim3(tm * tn), dim3(256), 0, st, m, n, k,
+                           alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
     else
         hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B,
                            ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
