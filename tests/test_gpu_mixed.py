@@ -156,3 +156,12 @@ def test_mixed_butterfly_needs_no_pivoting(B, O):
     st = B.dgesv_mixed(n, nrhs, a, n, torch.from_numpy(b).cuda(), n, x, n, "fp32", rbt=True)
     assert st["converged"] == 1 and st["fallback"] == 0, st
     assert _berr(colmajor(M), b, x.cpu().numpy(), n, nrhs) <= 2 * n * EPS64
+
+
+@pytest.mark.parametrize("n,nrhs", [(514, 16), (1001, 2)])
+def test_mixed_butterfly_request_falls_back_to_tournament_when_n_is_not_a_multiple_of_4(B, O, n, nrhs):
+    # two butterfly levels need n divisible by 4: other orders are factored with
+    # tournament pivoting and must meet the same contract
+    Af, b, x, st = _solve(B, n, nrhs, 1e3, "fp32", seed=21, rbt=True)
+    assert st["converged"] == 1 and st["fallback"] == 0, st
+    assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
