@@ -76,6 +76,76 @@ __device__ __forceinline__ void block_max_abs(double v, int i, Lds& s, int& out)
     out = s.piv_sh;
 }
 
+// One 64 x 64 block of a trailing update, F_blk -= L(64 x nb) U(nb x 64), on the matrix
+// cores: wavefront w owns the 32 x 32 quadrant (wm, wn), 2 x 2 MFMA tiles of 16 x 16.
+// L(r, j) = Lp[j BMAX + lr0 + r] (clamped to the panel), U(j, c) = Up[j ldu + c] (uc
+// columns valid).  The block is rows [r0, r0 + 64) and columns [q0, q0 + 64) of the
+// trailing matrix at F (ld n), m2 x m2; LOWER: only its lower triangle.  F is read from
+// clamped addresses first, so the loads are in flight together.
+typedef double bd4 __attribute__((ext_vector_type(4)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int NB, bool LOWER>
+__device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, int ldu, int uc, int nb, T* F,
+                                             size_t n, int m2, int r0, int q0) {
+    using V = typename std::conditional<sizeof(T) == 8, bd4, bf4>::type;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    V acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
+#pragma unroll
+    for (int k4 = 0; k4 < NB; k4 += 4) {
+        const int k = k4 + lk;
+        T fa[2], fb[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            const T v = Lp[k * BMAX + min(lr0 + wm + 16 * a + lr, BMAX - 1)];
+            fa[a] = k < nb ? v : (T)0;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const T v = Up[k * ldu + min(wn + 16 * b + lr, uc - 1)];
+            fb[b] = k < nb ? v : (T)0;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                if constexpr (sizeof(T) == 8)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                else
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a], fb[b], acc[a][b], 0, 0, 0);
+            }
+    }
+    // result element i of tile (a, b): row lk + 4 i (FP64) or 4 lk + i (FP32), column lr
+    T v[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? lk + 4 * i : 4 * lk + i);
+                const int cc = q0 + wn + 16 * b + lr;
+                v[a][b][i] = F[(size_t)min(rr, m2 - 1) + (size_t)min(cc, m2 - 1) * n];
+            }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? lk + 4 * i : 4 * lk + i);
+                const int cc = q0 + wn + 16 * b + lr;
+                if (rr < m2 && cc < m2 && (!LOWER || cc <= rr))
+                    F[(size_t)rr + (size_t)cc * n] = v[a][b][i] - acc[a][b][i];
+            }
+}
+
 // LU with partial pivoting of the n x n matrix F (ld n) in place; ipiv 1-based.  Returns
 // the LAPACK info (first zero pivot, 1-based, or 0).
 template <typename T, int NB>
@@ -156,39 +226,16 @@ __device__ __forceinline__ int lu_factor(int n, T* __restrict__ F, int32_t* __re
                 if (j < nb) col[j] = x[j];
         }
         __syncthreads();
-        // A22 -= L21 U12
-        const int tr = tid & 15, tc = tid >> 4;
+        // A22 -= L21 U12, by 64 x 64 blocks on the matrix cores
+        T* F22 = F + c2 + (size_t)c2 * n;
         for (int cb = 0; cb < m2; cb += 64) {
             for (int e = tid; e < NB * 64; e += BT) {
                 const int j = e >> 6, cc = e & 63;
                 Us[j * 65 + cc] = (j < nb && cb + cc < m2) ? F[(size_t)(k0 + j) + (size_t)(c2 + cb + cc) * n] : (T)0;
             }
             __syncthreads();
-            for (int rb = 0; rb < m2; rb += 64) {
-                T acc[4][4] = {};
-#pragma unroll 2
-                for (int j = 0; j < nb; j++) {
-                    T l[4], u[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) l[i] = P[j * BMAX + min(nb + rb + tr + 16 * i, BMAX - 1)];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) u[q] = Us[j * 65 + tc + 16 * q];
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int q = 0; q < 4; q++) acc[i][q] += l[i] * u[q];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int r = rb + tr + 16 * i;
-                    if (r >= m2) continue;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int c = cb + tc + 16 * q;
-                        if (c < m2) F[(size_t)(c2 + r) + (size_t)(c2 + c) * n] -= acc[i][q];
-                    }
-                }
-            }
+            for (int rb = 0; rb < m2; rb += 64)
+                block_update<T, NB, false>(P, nb + rb, Us, 65, 64, nb, F22, n, m2, rb, cb);
             __syncthreads();
         }
     }
@@ -223,34 +270,12 @@ __device__ __forceinline__ int chol_factor(int n, T* __restrict__ F, T* P, Lds& 
         for (int c = 0; c < nb; c++)
             for (int r = c + tid; r < mp; r += BT) F[(size_t)(k0 + r) + (size_t)(k0 + c) * n] = P[c * BMAX + r];
         const int c2 = k0 + nb, m2 = n - c2;
-        // A22 -= L21 L21^T, lower triangle only
-        const int tr = tid & 15, tc = tid >> 4;
+        // A22 -= L21 L21^T, lower triangle only, by 64 x 64 blocks on the matrix cores
+        T* F22 = F + c2 + (size_t)c2 * n;
         for (int cb = 0; cb < m2; cb += 64)
-            for (int rb = cb; rb < m2; rb += 64) {
-                T acc[4][4] = {};
-#pragma unroll 2
-                for (int j = 0; j < nb; j++) {
-                    T l[4], u[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) l[i] = P[j * BMAX + min(nb + rb + tr + 16 * i, BMAX - 1)];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) u[q] = P[j * BMAX + min(nb + cb + tc + 16 * q, BMAX - 1)];
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int q = 0; q < 4; q++) acc[i][q] += l[i] * u[q];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int r = rb + tr + 16 * i;
-                    if (r >= m2) continue;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int c = cb + tc + 16 * q;
-                        if (c < m2 && c <= r) F[(size_t)(c2 + r) + (size_t)(c2 + c) * n] -= acc[i][q];
-                    }
-                }
-            }
+            for (int rb = cb; rb < m2; rb += 64)
+                block_update<T, NB, true>(P, nb + rb, P + min(nb + cb, BMAX - 1), BMAX, BMAX - min(nb + cb, BMAX - 1),
+                                          nb, F22, n, m2, rb, cb);
         __syncthreads();
     }
     return 0;
@@ -411,7 +436,8 @@ __device__ __forceinline__ double residual_berr(int n, int nrhs, bool spd, const
         double asum = 0.0;
         for (int c = 0; c < n; c++) {
             // the SPD systems keep only their lower triangle meaningful: mirror it
-            const double a = (spd && c > r) ? A[(size_t)c + (size_t)r * n] : A[(size_t)r + (size_t)c * n];
+            const size_t ia = (spd && c > r) ? (size_t)c + (size_t)r * n : (size_t)r + (size_t)c * n;
+            const double a = A[ia];
             asum += fabs(a);
 #pragma unroll
             for (int q = 0; q < BNR; q++)
