@@ -571,16 +571,66 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
 constexpr int WV = 64;
 constexpr int WW = PW - 1;  // window slots
 
-__device__ __forceinline__ void wave_reduce2(Cand& a, Cand& b) {
+// Best (v, k) under `better` over the wavefront, every lane receiving it, with the lane
+// that held it (idx: that lane's value of idx).  The largest v comes from shuffles; when
+// one lane holds it (the usual case) its key is read from that lane, when several do
+// (equal magnitudes) the smallest of their keys decides.  Candidates are never NaN and
+// an empty lane holds (-1, NOKEY), so an empty wavefront yields (-1, NOKEY) again.
+__device__ __forceinline__ uint64_t lane_u64(uint64_t x, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+// x from another lane by a DPP row operation (control C, rows in RM; lanes outside RM
+// keep their own x)
+template <int C, int RM = 0xF>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), C, RM, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), C, RM, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// the maxima of a and b over the wavefront, in every lane: quad swaps and row rotations
+// leave each row of 16 with its maximum, two row broadcasts carry it to lane 63
+__device__ __forceinline__ void wave_max2(double& a, double& b) {
+    a = fmax(a, dpp_f64<0xB1>(a)); b = fmax(b, dpp_f64<0xB1>(b));                // quad_perm 1,0,3,2
+    a = fmax(a, dpp_f64<0x4E>(a)); b = fmax(b, dpp_f64<0x4E>(b));                // quad_perm 2,3,0,1
+    a = fmax(a, dpp_f64<0x124>(a)); b = fmax(b, dpp_f64<0x124>(b));              // row_ror 4
+    a = fmax(a, dpp_f64<0x128>(a)); b = fmax(b, dpp_f64<0x128>(b));              // row_ror 8
+    a = fmax(a, dpp_f64<0x142, 0xA>(a)); b = fmax(b, dpp_f64<0x142, 0xA>(b));    // row_bcast 15
+    a = fmax(a, dpp_f64<0x143, 0xC>(a)); b = fmax(b, dpp_f64<0x143, 0xC>(b));    // row_bcast 31
+    a = __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(a), 63));
+    b = __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(b), 63));
+}
+// the lane holding the best candidate once the maximum m is known
+__device__ __forceinline__ int best_lane(const Cand& a, double m) {
+    uint64_t h = __ballot(a.v == m);
+    if (__popcll(h) != 1) {  // equal magnitudes in several lanes: the smallest key
+        uint64_t k = a.v == m ? a.k : NOKEY;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        double ov = __shfl_xor(a.v, off, 64);
-        uint64_t ok = (uint64_t)__shfl_xor((long long)a.k, off, 64);
-        if (better(ov, ok, a.v, a.k)) { a.v = ov; a.k = ok; }
-        ov = __shfl_xor(b.v, off, 64);
-        ok = (uint64_t)__shfl_xor((long long)b.k, off, 64);
-        if (better(ov, ok, b.v, b.k)) { b.v = ov; b.k = ok; }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const uint64_t o = (uint64_t)__shfl_xor((long long)k, off, 64);
+            k = o < k ? o : k;
+        }
+        h = __ballot(a.v == m && a.k == k);
     }
+    return __ffsll((unsigned long long)h) - 1;
+}
+// best (v, k) of two candidate streams over the wavefront, every lane receiving both;
+// la, lb: the lanes that held them
+__device__ __forceinline__ void wave_best2(Cand& a, Cand& b, int& la, int& lb) {
+    double ma = a.v, mb = b.v;
+    wave_max2(ma, mb);
+    la = best_lane(a, ma);
+    lb = best_lane(b, mb);
+    a.k = lane_u64(a.k, la);
+    b.k = lane_u64(b.k, lb);
+    a.v = ma;
+    b.v = mb;
+}
+
+__device__ __forceinline__ void wave_reduce2(Cand& a, Cand& b) {
+    int la, lb;
+    wave_best2(a, b, la, lb);
 }
 
 // Deferred part of step dj's GER (ger.rs:28-90): window slots 1.. (column dj+1+c, rc = c)
@@ -776,23 +826,32 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
             const WaveSlot* base = slots + (size_t)par * nwv * 2;
             Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
             int gvs = -1, gts = -1;
-            for (int q = lane; q < nwv; q += WV) {
-                const double v0 = xld(&base[2 * q].v), v1 = xld(&base[2 * q + 1].v);
-                const uint64_t k0 = xldu(&base[2 * q].key), k1 = xldu(&base[2 * q + 1].key);
-                if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
-                if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
-            }
+            for (int q0 = 0; q0 < nwv; q0 += 2 * WV) {  // two wavefronts' keys per lane in flight
+                double v[4];
+                uint64_t k[4];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                double ov = __shfl_xor(gv.v, off, 64);
-                uint64_t okk = (uint64_t)__shfl_xor((long long)gv.k, off, 64);
-                int os = __shfl_xor(gvs, off, 64);
-                if (better(ov, okk, gv.v, gv.k)) { gv.v = ov; gv.k = okk; gvs = os; }
-                ov = __shfl_xor(gt.v, off, 64);
-                okk = (uint64_t)__shfl_xor((long long)gt.k, off, 64);
-                os = __shfl_xor(gts, off, 64);
-                if (better(ov, okk, gt.v, gt.k)) { gt.v = ov; gt.k = okk; gts = os; }
+                for (int h = 0; h < 2; h++) {
+                    const int q = min(q0 + h * WV + lane, nwv - 1);
+                    v[2 * h] = xld(&base[2 * q].v);
+                    v[2 * h + 1] = xld(&base[2 * q + 1].v);
+                    k[2 * h] = xldu(&base[2 * q].key);
+                    k[2 * h + 1] = xldu(&base[2 * q + 1].key);
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int q = q0 + h * WV + lane;
+                    if (q < nwv) {
+                        if (better(v[2 * h], k[2 * h], gv.v, gv.k)) { gv.v = v[2 * h]; gv.k = k[2 * h]; gvs = 2 * q; }
+                        if (better(v[2 * h + 1], k[2 * h + 1], gt.v, gt.k)) {
+                            gt.v = v[2 * h + 1]; gt.k = k[2 * h + 1]; gts = 2 * q + 1;
+                        }
+                    }
+                }
             }
+            int lv, lt;
+            wave_best2(gv, gt, lv, lt);
+            gvs = __builtin_amdgcn_readlane(gvs, lv);
+            gts = __builtin_amdgcn_readlane(gts, lt);
             bool ft;
             rel = idamax_rule(len, gv, gt, ft);
             const WaveSlot* P = rel == 0 ? cur + par : base + (ft ? gts : gvs);
@@ -1005,7 +1064,7 @@ int panel_nwg(int m) {
 }
 // the wavefront panel: rows per lane and wavefronts; taller panels use the workgroup kernel
 constexpr int kWaveMax = 512;
-int wave_rows_per_lane(int m) { return m <= 4096 ? 1 : 2; }
+int wave_rows_per_lane(int m) { return m <= 8192 ? 1 : 2; }
 int wave_count(int m) {
     int r = wave_rows_per_lane(m);
     return (m + WV * r - 1) / (WV * r);

@@ -350,6 +350,26 @@ def test_getrf_ties_and_zero_rows(B, O):
         assert e.value.message == msg
 
 
+@pytest.mark.parametrize("m,n,ints", [(4500, 64, False), (8192, 40, False), (8192, 36, True), (9000, 40, False),
+                                      (9000, 33, True)])
+def test_getrf_tall_panels_bitexact(B, O, m, n, ints):
+    # tall panels spread over 71..141 wavefronts (one row per lane up to 8192 rows, two
+    # above), with and without equal-magnitude candidates in many wavefronts at once
+    rng = np.random.default_rng(m + n)
+    a = rng.integers(-3, 4, m * n).astype(np.float64) if ints else rng.uniform(-1, 1, m * n)
+    ipiv = np.zeros(n, np.int32)
+    ref, rp = a.copy(), ipiv.copy()
+    code, msg = O.getrf(m, n, ref, m, rp)
+    if code != 0:
+        with pytest.raises(B.BolaError) as e:
+            B.getrf(m, n, a, m, ipiv)
+        assert e.value.message == msg
+        return
+    B.getrf(m, n, a, m, ipiv)
+    same(ipiv, rp, "ipiv")
+    same(a, ref, f"getrf {m}x{n}")
+
+
 def test_getrf_errors(B, O):
     with pytest.raises(B.BolaError) as e:
         B.getrf(4, 4, np.zeros(16), 3, np.zeros(4, np.int32))
