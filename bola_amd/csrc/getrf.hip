@@ -60,6 +60,9 @@ struct WaveSlot {
     double win[PW];
 };
 static_assert(sizeof(WaveSlot) == sizeof(PanelSlot), "slot layouts share the workspace");
+// A wavefront slot in global memory: each 32-bit half of a WaveSlot word travels in a
+// 64-bit word whose upper half is the step's epoch, so readers poll the data itself.
+constexpr int TW = 2 * (int)(sizeof(WaveSlot) / 8);
 
 __device__ __forceinline__ bool better(double v, uint64_t k, double cv, uint64_t ck) {
     return v > cv || (v == cv && k < ck);
@@ -558,10 +561,13 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
 // ---- the panel with one wavefront per workgroup and no workgroup barriers
 //
 // Every wavefront holds 64*R rows and takes part in the pivot search on its own: it
-// reduces its candidates with shuffles, stages the candidate rows in its LDS, copies them
-// to its global slots and raises its flag; after the deferred columns of the previous
-// step it waits for every flag, reads every wavefront's keys, picks the same pivot as
-// all the others and reads the winning row.  With one wavefront there is no exchange.
+// reduces its candidates with DPP row operations, stages the candidate rows in its LDS
+// and copies them to its global slots, every word tagged with the step's epoch; after
+// the deferred columns of the previous step it polls every wavefront's keys until they
+// carry the epoch, picks the same pivot as all the others and reads the winning row
+// (again until its words carry the epoch).  No store waits for another: a word is
+// valid when its tag is, so the exchange costs the stores' visibility, not their
+// acknowledgement.  With one wavefront there is no exchange.
 //
 // Register layout of a row: a window of columns j .. w-2 that shifts left by one column
 // per step (slot 0 is the current column, every index is a compile-time constant) and
@@ -689,11 +695,15 @@ __device__ __forceinline__ void wave_ger_rest(double (&a)[R][WW], double (&al)[R
     }
 }
 
+__device__ __forceinline__ uint64_t join32(uint64_t lo, uint64_t hi) { return (lo & 0xffffffffull) | (hi << 32); }
+
+// slots: [2 parities][nwv][2 candidates][TW] tagged words, cur: [2 parities][TW]; a step's
+// epoch is col_off + j + 1, unique within a factorization whose slots start zeroed.
 template <int R, bool MULTI>
 __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restrict__ A, size_t lda,
                                                  int32_t* __restrict__ ipiv, int32_t* __restrict__ info, int col_off,
-                                                 long row_off, int* __restrict__ flags, WaveSlot* __restrict__ slots,
-                                                 WaveSlot* __restrict__ cur, RowMap* __restrict__ map, int nwv,
+                                                 long row_off, uint64_t* __restrict__ slots,
+                                                 uint64_t* __restrict__ cur, RowMap* __restrict__ map, int nwv,
                                                  long spin_limit) {
     __shared__ double sret[PW * WV * R];  // retired columns: sret[c * WV*R + s*WV + lane]
     __shared__ WaveSlot stg[3];           // candidates of this wavefront, then the row at position j
@@ -726,7 +736,6 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
     __syncthreads();
 
     const int min_mn = min(m, w);
-    constexpr int NW = sizeof(WaveSlot) / 8;
     double xprev[R];
     bool fprev[R], uprev[R];
 #pragma unroll
@@ -788,17 +797,20 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
             stg[1].v = tb.v; stg[1].key = tb.k;
         }
         __syncthreads();  // one wavefront: orders the staging writes before the reads below
+        const uint32_t ep = (uint32_t)(col_off + j + 1);
         if (MULTI) {
-            WaveSlot* mine = slots + ((size_t)par * nwv + g) * 2;
-            for (int e = lane; e < 3 * NW; e += WV) {
-                const int q = e / NW, f = e % NW;
-                if (q < 2 || hascur) {
-                    uint64_t* dst = (uint64_t*)(q < 2 ? mine + q : cur + par);
-                    xstu(dst + f, ((const uint64_t*)(stg + q))[f]);
+            // publish without waiting for the stores: the keys always, a row only for a
+            // candidate that exists (an empty one cannot win), the row at j when held here
+            const uint64_t tag = (uint64_t)ep << 32;
+            uint64_t* mine = slots + ((size_t)par * nwv + g) * 2 * TW;
+            for (int e = lane; e < 3 * TW; e += WV) {
+                const int q = e / TW, f2 = e % TW;
+                const bool need = q < 2 ? (f2 < 4 || (q == 0 ? vb.v : tb.v) >= 0.0) : hascur;
+                if (need) {
+                    uint64_t* dst = q < 2 ? mine + q * TW : cur + (size_t)par * TW;
+                    xstu(dst + f2, tag | ((const uint32_t*)(stg + q))[f2]);
                 }
             }
-            __builtin_amdgcn_s_waitcnt(0);  // the slot stores have completed
-            if (lane == 0) __hip_atomic_store(flags + g, j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         // the deferred columns of the previous step, while the other wavefronts catch up
         if (dj >= 0) wave_ger_rest<R>(a, al, xprev, fprev, uprev, prow[dj & 1], (dj & 1) ? pz1 : pz0, dj, dnp, dnu);
@@ -809,13 +821,43 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
         long rel;
         if (MULTI) {
             int ok = 1;
-            for (int q = lane; q < nwv; q += WV) {
+            const uint64_t* base = slots + (size_t)par * nwv * 2 * TW;
+            Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
+            int gvs = -1, gts = -1;
+            for (int q0 = 0; q0 < nwv && ok; q0 += 2 * WV) {  // two wavefronts' keys per lane
+                uint64_t t[2][8];
                 long sp = 0;
-                while (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
+                for (;;) {  // until every word carries this step's epoch
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const uint64_t* s0 = base + (size_t)2 * min(q0 + h * WV + lane, nwv - 1) * TW;
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            t[h][i] = xldu(s0 + i);
+                            t[h][4 + i] = xldu(s0 + TW + i);
+                        }
+                    }
+                    bool good = true;
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int i = 0; i < 8; i++) good = good && (uint32_t)(t[h][i] >> 32) == ep;
+                    if (good) break;
                     if (++sp > spin_limit ||
                         __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                         ok = 0;
                         break;
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int q = q0 + h * WV + lane;
+                    if (q < nwv) {
+                        const double v0 = __longlong_as_double((long long)join32(t[h][0], t[h][1]));
+                        const double v1 = __longlong_as_double((long long)join32(t[h][4], t[h][5]));
+                        const uint64_t k0 = join32(t[h][2], t[h][3]), k1 = join32(t[h][6], t[h][7]);
+                        if (better(v0, k0, gv.v, gv.k)) { gv.v = v0; gv.k = k0; gvs = 2 * q; }
+                        if (better(v1, k1, gt.v, gt.k)) { gt.v = v1; gt.k = k1; gts = 2 * q + 1; }
                     }
                 }
             }
@@ -823,41 +865,40 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                 if (lane == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
-            const WaveSlot* base = slots + (size_t)par * nwv * 2;
-            Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
-            int gvs = -1, gts = -1;
-            for (int q0 = 0; q0 < nwv; q0 += 2 * WV) {  // two wavefronts' keys per lane in flight
-                double v[4];
-                uint64_t k[4];
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int q = min(q0 + h * WV + lane, nwv - 1);
-                    v[2 * h] = xld(&base[2 * q].v);
-                    v[2 * h + 1] = xld(&base[2 * q + 1].v);
-                    k[2 * h] = xldu(&base[2 * q].key);
-                    k[2 * h + 1] = xldu(&base[2 * q + 1].key);
-                }
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int q = q0 + h * WV + lane;
-                    if (q < nwv) {
-                        if (better(v[2 * h], k[2 * h], gv.v, gv.k)) { gv.v = v[2 * h]; gv.k = k[2 * h]; gvs = 2 * q; }
-                        if (better(v[2 * h + 1], k[2 * h + 1], gt.v, gt.k)) {
-                            gt.v = v[2 * h + 1]; gt.k = k[2 * h + 1]; gts = 2 * q + 1;
-                        }
-                    }
-                }
-            }
             int lv, lt;
             wave_best2(gv, gt, lv, lt);
             gvs = __builtin_amdgcn_readlane(gvs, lv);
             gts = __builtin_amdgcn_readlane(gts, lt);
             bool ft;
             rel = idamax_rule(len, gv, gt, ft);
-            const WaveSlot* P = rel == 0 ? cur + par : base + (ft ? gts : gvs);
-            if (c >= j && c < w) v = xld(&P->win[c - j]);
-            px = xld(&P->x);
-            pf = xldu(&P->flags);
+            const uint64_t* P = rel == 0 ? cur + (size_t)par * TW : base + (size_t)(ft ? gts : gvs) * TW;
+            const bool inw = c >= j && c < w;
+            const int wi = 2 * (4 + (inw ? c - j : 0));  // win[c - j]; x and flags are words 2 and 3
+            uint64_t r0, r1, x0, x1, f0, f1;
+            long sp = 0;
+            for (;;) {
+                r0 = xldu(P + wi);
+                r1 = xldu(P + wi + 1);
+                x0 = xldu(P + 4);
+                x1 = xldu(P + 5);
+                f0 = xldu(P + 6);
+                f1 = xldu(P + 7);
+                const bool good = (!inw || ((uint32_t)(r0 >> 32) == ep && (uint32_t)(r1 >> 32) == ep)) &&
+                                  (uint32_t)(x0 >> 32) == ep && (uint32_t)(x1 >> 32) == ep &&
+                                  (uint32_t)(f0 >> 32) == ep && (uint32_t)(f1 >> 32) == ep;
+                if (good) break;
+                if (++sp > spin_limit || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    break;
+                }
+            }
+            if (!__all(ok)) {
+                if (lane == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            if (inw) v = __longlong_as_double((long long)join32(r0, r1));
+            px = __longlong_as_double((long long)join32(x0, x1));
+            pf = join32(f0, f1);
         } else {
             bool ft;
             rel = idamax_rule(len, vb, tb, ft);
@@ -1081,12 +1122,15 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
+    // the slots start zeroed: the wavefront panel's epochs are unique only within one
+    // factorization
+    const size_t slot_bytes = std::max(sizeof(PanelSlot), sizeof(uint64_t) * TW);
     size_t o_flags = take(sizeof(int) * (size_t)w.npanels * w.nwg_max);
     size_t o_maps = take(sizeof(RowMap) * (size_t)w.npanels);
     size_t o_jp = take(sizeof(int32_t) * 4);
+    size_t o_slots = take(slot_bytes * 4 * (size_t)w.nwg_max);
+    size_t o_cur = take(slot_bytes * 2);
     size_t zero_bytes = off;
-    size_t o_slots = take(sizeof(PanelSlot) * 4 * (size_t)w.nwg_max);
-    size_t o_cur = take(sizeof(PanelSlot) * 2);
     char* base = (char*)h->ws.get(off);
     w.flags = (int*)(base + o_flags);
     w.maps = (RowMap*)(base + o_maps);
@@ -1108,24 +1152,23 @@ void getf2(hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv, i
     if (n <= PW && wave_count(m) <= kWaveMax) {
         const int r = wave_rows_per_lane(m);
         const int nwv = wave_count(m);
-        WaveSlot* wslots = reinterpret_cast<WaveSlot*>(w.slots);
-        WaveSlot* wcur = reinterpret_cast<WaveSlot*>(w.cur);
-        int* flags = w.flags + (size_t)panel_idx * w.nwg_max;
+        uint64_t* wslots = reinterpret_cast<uint64_t*>(w.slots);
+        uint64_t* wcur = reinterpret_cast<uint64_t*>(w.cur);
         RowMap* map = w.maps + panel_idx;
         if (nwv == 1) {
             if (r == 1)
                 hipLaunchKernelGGL((getf2_wave<1, false>), dim3(1), dim3(WV), 0, st, m, n, a, lda, ipiv, info, col_off,
-                                   row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+                                   row_off, wslots, wcur, map, nwv, kSpinLimit);
             else
                 hipLaunchKernelGGL((getf2_wave<2, false>), dim3(1), dim3(WV), 0, st, m, n, a, lda, ipiv, info, col_off,
-                                   row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+                                   row_off, wslots, wcur, map, nwv, kSpinLimit);
         } else {
             if (r == 1)
                 hipLaunchKernelGGL((getf2_wave<1, true>), dim3(nwv), dim3(WV), 0, st, m, n, a, lda, ipiv, info,
-                                   col_off, row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+                                   col_off, row_off, wslots, wcur, map, nwv, kSpinLimit);
             else
                 hipLaunchKernelGGL((getf2_wave<2, true>), dim3(nwv), dim3(WV), 0, st, m, n, a, lda, ipiv, info,
-                                   col_off, row_off, flags, wslots, wcur, map, nwv, kSpinLimit);
+                                   col_off, row_off, wslots, wcur, map, nwv, kSpinLimit);
         }
         BOLA_LAUNCH_CHECK();
         return;
