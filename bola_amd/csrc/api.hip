@@ -63,6 +63,7 @@ void collect_profile(bola_hip_handle_s* h) {
     if (h->pending.empty()) return;
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     if (h->stream2) BOLA_CHECK(hipStreamSynchronize(h->stream2));
+    if (h->stream3) BOLA_CHECK(hipStreamSynchronize(h->stream3));
     for (auto& p : h->pending) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, p.start, p.stop);
@@ -234,6 +235,10 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     if (h->stream2) {
         (void)hipStreamSynchronize(h->stream2);
         (void)hipStreamDestroy(h->stream2);
+    }
+    if (h->stream3) {
+        (void)hipStreamSynchronize(h->stream3);
+        (void)hipStreamDestroy(h->stream3);
     }
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     if (t_default == h) t_default = nullptr;

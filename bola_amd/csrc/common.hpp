@@ -100,12 +100,19 @@ struct bola_hip_handle_s {
     std::vector<hipEvent_t> event_pool;
     // second stream for work that overlaps the critical path, and ordering events
     hipStream_t stream2 = nullptr;
+    hipStream_t stream3 = nullptr;  // a third one: side work that must not wait for the second
     std::vector<hipEvent_t> sync_events;
     hipStream_t aux_stream() {
         if (!stream2) {
             if (hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking) != hipSuccess) stream2 = nullptr;
         }
         return stream2 ? stream2 : stream;
+    }
+    hipStream_t aux_stream3() {
+        if (!stream3) {
+            if (hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking) != hipSuccess) stream3 = nullptr;
+        }
+        return stream3 ? stream3 : stream;
     }
 };
 

@@ -803,12 +803,21 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
             // candidate that exists (an empty one cannot win), the row at j when held here
             const uint64_t tag = (uint64_t)ep << 32;
             uint64_t* mine = slots + ((size_t)par * nwv + g) * 2 * TW;
-            for (int e = lane; e < 3 * TW; e += WV) {
+            constexpr int NE = (3 * TW + WV - 1) / WV;
+            uint32_t hv[NE];
+#pragma unroll
+            for (int i = 0; i < NE; i++) {  // every LDS read in flight before the first store
+                const int e = min(lane + i * WV, 3 * TW - 1);
+                hv[i] = ((const uint32_t*)stg)[e];  // half-word f2 of staged slot q at q * TW + f2
+            }
+#pragma unroll
+            for (int i = 0; i < NE; i++) {
+                const int e = lane + i * WV;
                 const int q = e / TW, f2 = e % TW;
-                const bool need = q < 2 ? (f2 < 4 || (q == 0 ? vb.v : tb.v) >= 0.0) : hascur;
+                const bool need = e < 3 * TW && (q < 2 ? (f2 < 4 || (q == 0 ? vb.v : tb.v) >= 0.0) : hascur);
                 if (need) {
                     uint64_t* dst = q < 2 ? mine + q * TW : cur + (size_t)par * TW;
-                    xstu(dst + f2, tag | ((const uint32_t*)(stg + q))[f2]);
+                    xstu(dst + f2, tag | hv[i]);
                 }
             }
         }
@@ -827,14 +836,20 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
             for (int q0 = 0; q0 < nwv && ok; q0 += 2 * WV) {  // two wavefronts' keys per lane
                 uint64_t t[2][8];
                 long sp = 0;
+                const bool two = q0 + WV < nwv;  // a second wavefront for some lane
                 for (;;) {  // until every word carries this step's epoch
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
                         const uint64_t* s0 = base + (size_t)2 * min(q0 + h * WV + lane, nwv - 1) * TW;
+                        if (h == 0 || two) {
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            t[h][i] = xldu(s0 + i);
-                            t[h][4 + i] = xldu(s0 + TW + i);
+                            for (int i = 0; i < 4; i++) {
+                                t[h][i] = xldu(s0 + i);
+                                t[h][4 + i] = xldu(s0 + TW + i);
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 8; i++) t[h][i] = t[0][i];
                         }
                     }
                     bool good = true;
@@ -1311,8 +1326,10 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
 
 // The columns [s0, s0+sw) themselves, panel by panel: the reference's loop restricted
 // to these columns (their left/right interchanges, TRSM and GEMM stay inside them).
+// after_first (may be null): an event the columns right of the first panel wait for; the
+// first panel itself runs before it.
 void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv,
-                  int32_t* info, const LuWs& w, int s0, int sw) {
+                  int32_t* info, const LuWs& w, int s0, int sw, hipEvent_t after_first) {
     const int G = (sw + PW - 1) / PW;
     for (int q = 0; q < G; q++) {
         const int j = s0 + q * PW;
@@ -1322,6 +1339,7 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
             Phase pp(h, "GETRF2_IT", 0.0, st);
             getf2(st, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
         }
+        if (q == 0 && after_first) BOLA_CHECK(hipStreamWaitEvent(st, after_first, 0));
         {
             Phase pl(h, "LASWP", 0.0, st);
             launch_rowmap_apply(st, a + (size_t)s0 * lda, lda, (size_t)(j - s0), (size_t)(j + jb - s0),
@@ -1420,21 +1438,34 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
         }
         return;
     }
+    // Three streams.  Stream 1 (critical): the super-panel's panels, then the next
+    // super-panel's columns.  Stream 2 (bulk): every column after the next super-panel.
+    // Stream 3: the super-panel's interchanges on the columns to its left, which nothing
+    // after it reads until the end.  Each column still receives the same updates in the
+    // same order; the events only order the streams that touch the same columns.
     const int W = min_mn >= 2048 ? 256 : 128;
     hipStream_t s2 = h->aux_stream();
+    hipStream_t s3 = h->aux_stream3();
     int ev = 0;
-    hipEvent_t bulk_done = nullptr;  // the previous super-panel's bulk update
+    hipEvent_t bulk_done = nullptr;  // the previous super-panel's bulk update (stream 2)
+    hipEvent_t left_done = nullptr;  // the last left interchanges (stream 3)
     for (int s0 = 0; s0 < min_mn; s0 += W) {
         const int sw = std::min(W, min_mn - s0);
-        factor_super(h, st, m, n, a, lda, ipiv, info, w, s0, sw);
+        factor_super(h, st, m, n, a, lda, ipiv, info, w, s0, sw, nullptr);
         hipEvent_t fact_done = lu_event(h, ev++);
         BOLA_CHECK(hipEventRecord(fact_done, st));
+        if (s0 > 0) {
+            // the rows it permutes include the previous super-panel, which the previous
+            // bulk update reads
+            BOLA_CHECK(hipStreamWaitEvent(s3, fact_done, 0));
+            if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(s3, bulk_done, 0));
+            Phase pl(h, "LASWP", 0.0, s3);
+            launch_rowmap_apply(s3, a, lda, (size_t)s0, 0, 0, w.maps + s0 / PW, (sw + PW - 1) / PW, info, nullptr, 0, 0);
+            left_done = lu_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(left_done, s3));
+        }
         // from here on stream 1 touches columns the previous bulk update wrote
         if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
-        if (s0 > 0) {
-            Phase pl(h, "LASWP", 0.0, st);
-            launch_rowmap_apply(st, a, lda, (size_t)s0, 0, 0, w.maps + s0 / PW, (sw + PW - 1) / PW, info, nullptr, 0, 0);
-        }
         const int c_lo = s0 + sw;
         const int c_mid = std::min(n, c_lo + W);
         update_cols(h, st, m, n, a, lda, info, w, s0, sw, c_lo, c_mid);
@@ -1448,6 +1479,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
         }
     }
     if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
+    if (left_done) BOLA_CHECK(hipStreamWaitEvent(st, left_done, 0));
 }
 
 }  // namespace bola
