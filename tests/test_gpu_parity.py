@@ -208,6 +208,23 @@ def test_gemm_bitexact(B, O, ta, tb, m, n, k, alpha, beta):
     same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (2.5, -0.3), (0.7, 0.0)])
+def test_gemm_large_tiles_bitexact(B, O, ta, tb, alpha, beta):
+    # products large enough for the 128 x 64 tiles at two workgroups per CU; ragged edges
+    # and k = 150 (two full 64-deep slabs and a partial one)
+    m, n, k = 2048 + 37, 2048 + 5, 150
+    rng = np.random.default_rng(7 + (ta == "T") * 2 + (tb == "T"))
+    a, lda = _rand_op(rng, ta, m, k, 3)
+    b, ldb = _rand_op(rng, tb, k, n, 1)
+    ldc = m + 2
+    c = rng.uniform(-1, 1, ldc * n)
+    ref = c.copy()
+    O.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, ref, ldc)
+    B.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+    same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
+
+
 def test_gemm_alpha_zero_beta_rules(B, O):
     c = np.array([np.nan, 1.0, 2.0, 3.0])
     B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
