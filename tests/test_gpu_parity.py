@@ -418,6 +418,20 @@ def test_getrs_bitexact(B, O):
     same(b, ref, "getrs")
 
 
+@pytest.mark.parametrize("n", [2600, 2601])
+def test_getrf_large_square_bitexact(B, O, n):
+    # 256-column super-panels with a partial last one (2600), and the reference's own
+    # panel-by-panel order for an order that is not a multiple of 8 (2601)
+    rng = np.random.default_rng(n)
+    a = rng.uniform(-1, 1, n * n)
+    ref, rp = a.copy(), np.zeros(n, np.int32)
+    assert O.getrf(n, n, ref, n, rp)[0] == 0
+    ipiv = np.zeros(n, np.int32)
+    B.getrf(n, n, a, n, ipiv)
+    same(ipiv, rp, f"ipiv {n}")
+    same(a, ref, f"lu {n}")
+
+
 def test_getrf_getrs_2048_bitexact(B, O):
     n, nrhs = 2048, 16
     rng = np.random.default_rng(15)
@@ -445,6 +459,19 @@ def _spd(rng, n):
 @pytest.mark.parametrize("n", [1, 5, 32, 33, 100, 300])
 def test_potrf_bitexact(B, O, uplo, n):
     rng = np.random.default_rng(n * 3 + (uplo == "U"))
+    a = _spd(rng, n)
+    ref = a.copy()
+    assert O.potrf(uplo, n, ref, n)[0] == 0
+    B.potrf(uplo, n, a, n)
+    same(a, ref, f"potrf {uplo} {n}")
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n", [1000, 2085])
+def test_potrf_large_bitexact(B, O, uplo, n):
+    # the blocked loop over many 32-column blocks: SYRK against thousands of previous
+    # columns, GEMMs with k up to n - 32, a ragged last block
+    rng = np.random.default_rng(n + (uplo == "U"))
     a = _spd(rng, n)
     ref = a.copy()
     assert O.potrf(uplo, n, ref, n)[0] == 0
