@@ -14,9 +14,9 @@
 //      side, advanced in lockstep so the matrix is read once per step for all of them.
 //   4. If GMRES-IR stalls too, the FP64 reference-exact GETRF + GETRS solves the system.
 //
-// Triangular solves: one persistent kernel per triangle, blocks of 64 rows, one
-// workgroup per block; a block waits (bounded) for the flags of the blocks it depends on
-// and reads their solved values with device-coherent loads.  Workgroups are dispatched in
+// Triangular solves: one persistent kernel per triangle, blocks of 128 rows, one
+// workgroup per block; a block polls (bounded) the epoch-tagged solved values of the
+// blocks it depends on with device-coherent loads.  Workgroups are dispatched in
 // dependency order, so a waiting workgroup never blocks one it waits for.
 #include "mixed.hpp"
 
@@ -42,6 +42,12 @@ __device__ __forceinline__ void ast(double* p, double v) {
 }
 __device__ __forceinline__ double ald(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t tld(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void tst(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double pow2_scale(double m) {
     if (!(m > 0.0) || !(m < INFINITY)) return 1.0;
@@ -224,13 +230,18 @@ __global__ void __launch_bounds__(TB) diag_inv(int n, const float* __restrict__ 
 // Z := L^-1 Z (LOWER, L unit lower) or U^-1 Z (upper), factors in T (fp32), FP64
 // arithmetic; Z is n x nrhs (ld ldz).  Block bi = rows [TB bi, TB bi + TB), one workgroup
 // each, dispatched in the order the blocks are solved.  A block subtracts the products
-// with the blocks it depends on as their flags rise (the oldest first), then applies the
-// inverse of its diagonal block (diag_inv) and raises its own flag.  Thread (r, h) holds
-// row r's entries in columns [64 h, 64 h + 64) of a block; the two halves meet in LDS.
+// with the blocks it depends on (the oldest first), then applies the inverse of its
+// diagonal block (diag_inv).  A solved block publishes its rows twice: plainly in Z, for
+// the kernels that follow, and in zt (same indexing, two 64-bit words per double), each
+// word carrying the call's epoch in its upper half beside 32 bits of the value.  A block
+// that depends on it polls those words until they carry the epoch: no store waits for
+// another and there is no flag.  zt starts zeroed and epochs are unique per solver.
+// Thread (r, h) holds row r's entries in columns [64 h, 64 h + 64) of a block; the two
+// halves meet in LDS.
 template <bool LOWER>
 __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __restrict__ T, size_t ldt,
                                                 const float* __restrict__ Dinv, double* __restrict__ Z, size_t ldz,
-                                                int* __restrict__ flags, int epoch, int nb, long spin_limit,
+                                                uint64_t* __restrict__ zt, int epoch, int nb, long spin_limit,
                                                 int* __restrict__ err) {
     constexpr int HW = TB / 2;
     __shared__ double ys[TB][NRMAX + 1];       // the incoming block of the solution
@@ -247,9 +258,16 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
     // this row's half of the diagonal block's inverse, wanted last: loaded first
     float dr[HW];
     {
-        const float* drow = Dinv + (((size_t)(LOWER ? 0 : 1) * nb + bi) * TB + r) * TB + h * HW;
+        const float4* drow = reinterpret_cast<const float4*>(Dinv + (((size_t)(LOWER ? 0 : 1) * nb + bi) * TB + r) * TB +
+                                                             h * HW);
 #pragma unroll
-        for (int c = 0; c < HW; c++) dr[c] = drow[c];
+        for (int c4 = 0; c4 < HW / 4; c4++) {
+            const float4 v = drow[c4];
+            dr[4 * c4] = v.x;
+            dr[4 * c4 + 1] = v.y;
+            dr[4 * c4 + 2] = v.z;
+            dr[4 * c4 + 3] = v.w;
+        }
     }
     const int ndep = LOWER ? bi : nb - 1 - bi;
     for (int jj = 0; jj < ndep; jj++) {
@@ -260,35 +278,52 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
         // (clamped addresses, so that all the loads are in flight at once; masked after)
         float lr[HW];
         const float* lrow = T + (size_t)c0 + (size_t)(r0 + min(r, rows - 1)) * ldt;
+        if (cols == TB) {  // a full block: the row's 64 entries as 16-byte loads (rows of T are 256-byte aligned)
+            const float4* p4 = reinterpret_cast<const float4*>(lrow + h * HW);
 #pragma unroll
-        for (int c = 0; c < HW; c++) lr[c] = lrow[min(h * HW + c, cols - 1)];
-#pragma unroll
-        for (int c = 0; c < HW; c++) lr[c] = keep_or_zero(lr[c], h * HW + c < cols);
-        if (tid == 0) {
-            long sp = 0;
-            int ok = 1;
-            // the block solved next polls eagerly; the others sparingly, so that the
-            // waiting blocks do not crowd the flag's cache line
-            const bool next = j == (LOWER ? bi - 1 : bi + 1);
-            while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                if (++sp > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = 0;
-                    break;
-                }
-                if (!next) __builtin_amdgcn_s_sleep(32);
+            for (int c4 = 0; c4 < HW / 4; c4++) {
+                const float4 v = p4[c4];
+                lr[4 * c4] = v.x;
+                lr[4 * c4 + 1] = v.y;
+                lr[4 * c4 + 2] = v.z;
+                lr[4 * c4 + 3] = v.w;
             }
-            ok_sh = ok;
+        } else {
+#pragma unroll
+            for (int c = 0; c < HW; c++) lr[c] = lrow[min(h * HW + c, cols - 1)];
+#pragma unroll
+            for (int c = 0; c < HW; c++) lr[c] = keep_or_zero(lr[c], h * HW + c < cols);
         }
-        __syncthreads();
-        if (!ok_sh) {
+        // block j's solution, word by word until every word carries the epoch; the block
+        // solved next polls eagerly, the others (usually long solved) sparingly
+        const bool next = j == (LOWER ? bi - 1 : bi + 1);
+        int ok = 1;
+        for (int e = tid; e < TB * nrhs; e += 256) {
+            const int c = e % TB, q = e / TB;
+            double v = 0.0;
+            if (c < cols) {
+                const uint64_t* w = zt + 2 * ((size_t)(c0 + c) + (size_t)q * ldz);
+                uint64_t lo, hi;
+                long sp = 0;
+                for (;;) {
+                    lo = tld(w);
+                    hi = tld(w + 1);
+                    if ((uint32_t)(lo >> 32) == (uint32_t)epoch && (uint32_t)(hi >> 32) == (uint32_t)epoch) break;
+                    if (++sp > spin_limit ||
+                        __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        ok = 0;
+                        break;
+                    }
+                    if (!next) __builtin_amdgcn_s_sleep(32);
+                }
+                v = __longlong_as_double((long long)((lo & 0xffffffffull) | (hi << 32)));
+            }
+            ys[c][q] = v;
+        }
+        if (!__syncthreads_and(ok)) {
             if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
-        for (int e = tid; e < TB * nrhs; e += 256) {
-            const int c = e % TB, q = e / TB;
-            ys[c][q] = c < cols ? ald(Z + (size_t)(c0 + c) + (size_t)q * ldz) : 0.0;
-        }
-        __syncthreads();
         for (int q = 0; q < nrhs; q++) {
             double s0 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -316,13 +351,18 @@ __global__ void __launch_bounds__(256) trsv_blk(int n, int nrhs, const float* __
         part[h][r][q] = s0 + s1;
     }
     __syncthreads();
+    const uint64_t tag = (uint64_t)(uint32_t)epoch << 32;
     for (int e = tid; e < TB * nrhs; e += 256) {
         const int rr = e % TB, q = e / TB;
-        if (rr < rows) ast(Z + (size_t)(r0 + rr) + (size_t)q * ldz, part[0][rr][q] + part[1][rr][q]);
+        if (rr < rows) {
+            const double y = part[0][rr][q] + part[1][rr][q];
+            const size_t at = (size_t)(r0 + rr) + (size_t)q * ldz;
+            const uint64_t b = (uint64_t)__double_as_longlong(y);
+            Z[at] = y;
+            tst(zt + 2 * at, tag | (b & 0xffffffffull));
+            tst(zt + 2 * at + 1, tag | (b >> 32));
+        }
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- residual / products
@@ -461,7 +501,7 @@ struct Solver {
     double *s, *t, *rsum, *pmax, *psum;
     double *R, *Z, *D, *W, *P, *nrm, *dots;
     double* Vb;  // GMRES basis: [q][j][n]
-    int* flags;
+    uint64_t* zt;  // tagged copy of Z for the blocked triangular solves
     int* err;
     void* luws;
     int epoch = 0;
@@ -487,11 +527,11 @@ void apply_minv(Solver& S, const double* Xr, size_t ldx, double* Y) {
     const int nb = (S.n + TB - 1) / TB;
     S.epoch++;
     hipLaunchKernelGGL(trsv_blk<true>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Dinv, S.Z,
-                       (size_t)S.n, S.flags, S.epoch, nb, kTrsvSpin, S.err);
+                       (size_t)S.n, S.zt, S.epoch, nb, kTrsvSpin, S.err);
     BOLA_LAUNCH_CHECK();
     S.epoch++;
     hipLaunchKernelGGL(trsv_blk<false>, dim3(nb), dim3(256), 0, S.st, S.n, S.nrhs, S.T, S.ldt, S.Dinv, S.Z,
-                       (size_t)S.n, S.flags, S.epoch, nb, kTrsvSpin, S.err);
+                       (size_t)S.n, S.zt, S.epoch, nb, kTrsvSpin, S.err);
     BOLA_LAUNCH_CHECK();
     if (S.rbt) {  // W_R y = B2 (B1 y)
         hipLaunchKernelGGL(rbt_vec<false>, gp, dim3(256), 0, S.st, S.n, S.nrhs, S.Z, (size_t)S.n, S.n / 2, S.rr[0]);
@@ -715,7 +755,8 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     const size_t oP = take(8 * (size_t)n * NRMAX * nch);
     const size_t onrm = take(8 * 4 * NRMAX), odots = take(8 * (size_t)(S.mgm + 2) * NRMAX * 2);
     const size_t oVb = take(8 * (size_t)nrhs * (S.mgm + 1) * n);
-    const size_t oflags = take(sizeof(int) * ((size_t)n / TB + 2)), oerr = take(sizeof(int) * 4);
+    const size_t oerr = take(sizeof(int) * 4);
+    const size_t ozt = take(2 * 8 * nv);
     const size_t olu = take(mixed_lu_ws_bytes(n, precision));
     const size_t orb = take(8 * 4 * (size_t)n);
     const size_t odi = take(sizeof(float) * 2 * (size_t)((n + TB - 1) / TB) * TB * TB);
@@ -736,7 +777,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
     S.nrm = (double*)(base + onrm);
     S.dots = (double*)(base + odots);
     S.Vb = (double*)(base + oVb);
-    S.flags = (int*)(base + oflags);
+    S.zt = (uint64_t*)(base + ozt);
     S.err = (int*)(base + oerr);
     S.luws = base + olu;
     S.Dinv = (float*)(base + odi);
@@ -759,7 +800,7 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
         BOLA_CHECK(hipStreamSynchronize(S.st));
     }
     hipStream_t st = S.st;
-    hipLaunchKernelGGL(k_zero_int, dim3(64), dim3(256), 0, st, n / TB + 2, S.flags);
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (int)(4 * nv), (int*)S.zt);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, S.err);
     BOLA_LAUNCH_CHECK();
     hipEvent_t ev[4];
