@@ -85,7 +85,7 @@ __device__ __forceinline__ void block_max_abs(double v, int i, Lds& s, int& out)
 typedef double bd4 __attribute__((ext_vector_type(4)));
 typedef float bf4 __attribute__((ext_vector_type(4)));
 
-template <typename T, int NB, bool LOWER>
+template <typename T, int NB, bool LOWER, bool COLS>
 __device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, int ldu, int uc, int nb, T* F,
                                              size_t n, int m2, int r0, int q0) {
     using V = typename std::conditional<sizeof(T) == 8, bd4, bf4>::type;
@@ -115,13 +115,20 @@ __device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, 
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                if constexpr (sizeof(T) == 8)
+                // COLS (FP64): U first, so the tile comes out transposed and a lane's
+                // results run down a column of F (whole 128-byte lines) rather than across
+                // 16 columns; it costs registers, so the kernel holding the FP64 fallback
+                // of the refined variant keeps the plain form
+                if constexpr (sizeof(T) == 8 && COLS)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[b], fa[a], acc[a][b], 0, 0, 0);
+                else if constexpr (sizeof(T) == 8)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
                 else
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a], fb[b], acc[a][b], 0, 0, 0);
             }
     }
-    // result element i of tile (a, b): row lk + 4 i (FP64) or 4 lk + i (FP32), column lr
+    // result element i of tile (a, b): row lr, column lk + 4 i (FP64, COLS); row lk + 4 i,
+    // column lr (FP64); row 4 lk + i, column lr (FP32)
     T v[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -129,8 +136,8 @@ __device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, 
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? lk + 4 * i : 4 * lk + i);
-                const int cc = q0 + wn + 16 * b + lr;
+                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? (COLS ? lr : lk + 4 * i) : 4 * lk + i);
+                const int cc = q0 + wn + 16 * b + (sizeof(T) == 8 && COLS ? lk + 4 * i : lr);
                 v[a][b][i] = F[(size_t)min(rr, m2 - 1) + (size_t)min(cc, m2 - 1) * n];
             }
 #pragma unroll
@@ -139,8 +146,8 @@ __device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, 
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? lk + 4 * i : 4 * lk + i);
-                const int cc = q0 + wn + 16 * b + lr;
+                const int rr = r0 + wm + 16 * a + (sizeof(T) == 8 ? (COLS ? lr : lk + 4 * i) : 4 * lk + i);
+                const int cc = q0 + wn + 16 * b + (sizeof(T) == 8 && COLS ? lk + 4 * i : lr);
                 if (rr < m2 && cc < m2 && (!LOWER || cc <= rr))
                     F[(size_t)rr + (size_t)cc * n] = v[a][b][i] - acc[a][b][i];
             }
@@ -148,7 +155,7 @@ __device__ __forceinline__ void block_update(const T* Lp, int lr0, const T* Up, 
 
 // LU with partial pivoting of the n x n matrix F (ld n) in place; ipiv 1-based.  Returns
 // the LAPACK info (first zero pivot, 1-based, or 0).
-template <typename T, int NB>
+template <typename T, int NB, bool COLS>
 __device__ __forceinline__ int lu_factor(int n, T* __restrict__ F, int32_t* __restrict__ ipiv, T* P, T* Us, int* spiv, Lds& s) {
     const int tid = threadIdx.x;
     int info = 0;
@@ -235,7 +242,7 @@ __device__ __forceinline__ int lu_factor(int n, T* __restrict__ F, int32_t* __re
             }
             __syncthreads();
             for (int rb = 0; rb < m2; rb += 64)
-                block_update<T, NB, false>(P, nb + rb, Us, 65, 64, nb, F22, n, m2, rb, cb);
+                block_update<T, NB, false, COLS>(P, nb + rb, Us, 65, 64, nb, F22, n, m2, rb, cb);
             __syncthreads();
         }
     }
@@ -245,7 +252,7 @@ __device__ __forceinline__ int lu_factor(int n, T* __restrict__ F, int32_t* __re
 // Cholesky A = L L^T of the lower triangle of F (ld n) in place; the strict upper triangle
 // is not referenced.  Returns the LAPACK info (order of the first non-positive leading
 // minor, or 0); the factorization stops there.
-template <typename T, int NB>
+template <typename T, int NB, bool COLS>
 __device__ __forceinline__ int chol_factor(int n, T* __restrict__ F, T* P, Lds& s) {
     const int tid = threadIdx.x;
     for (int k0 = 0; k0 < n; k0 += NB) {
@@ -274,7 +281,7 @@ __device__ __forceinline__ int chol_factor(int n, T* __restrict__ F, T* P, Lds& 
         T* F22 = F + c2 + (size_t)c2 * n;
         for (int cb = 0; cb < m2; cb += 64)
             for (int rb = cb; rb < m2; rb += 64)
-                block_update<T, NB, true>(P, nb + rb, P + min(nb + cb, BMAX - 1), BMAX, BMAX - min(nb + cb, BMAX - 1),
+                block_update<T, NB, true, COLS>(P, nb + rb, P + min(nb + cb, BMAX - 1), BMAX, BMAX - min(nb + cb, BMAX - 1),
                                           nb, F22, n, m2, rb, cb);
         __syncthreads();
     }
@@ -506,7 +513,8 @@ __global__ void __launch_bounds__(BT) batch_solve(int count, const Desc* __restr
         int32_t* ipiv = lu ? piva + d.piv : pw;
         int inf = 0, its = 0;
         if (VAR == 0) {
-            inf = lu ? lu_factor<double, NB>(n, A, ipiv, P, Us, spiv, s) : chol_factor<double, NB>(n, A, P, s);
+            inf = lu ? lu_factor<double, NB, true>(n, A, ipiv, P, Us, spiv, s)
+                     : chol_factor<double, NB, true>(n, A, P, s);
             if (inf == 0) {
                 if (lu) piv_to_perm(n, ipiv, perm);
                 solve_with<double, NB>(n, nrhs, lu, A, perm, B, Xw, Dd, xs);
@@ -516,8 +524,8 @@ __global__ void __launch_bounds__(BT) batch_solve(int count, const Desc* __restr
             // FP32 factors of A
             for (int e = tid; e < n * n; e += BT) Fw[e] = (float)A[e];
             __syncthreads();
-            inf = lu ? lu_factor<float, NB>(n, Fw, pw, (float*)P, (float*)Us, spiv, s)
-                     : chol_factor<float, NB>(n, Fw, (float*)P, s);
+            inf = lu ? lu_factor<float, NB, false>(n, Fw, pw, (float*)P, (float*)Us, spiv, s)
+                     : chol_factor<float, NB, false>(n, Fw, (float*)P, s);
             bool done = false;
             const double tol = (double)n * 0x1.0p-52;
             if (inf == 0) {
@@ -541,7 +549,8 @@ __global__ void __launch_bounds__(BT) batch_solve(int count, const Desc* __restr
                 // the FP64 path on a copy of A
                 for (int e = tid; e < n * n; e += BT) Fd[e] = A[e];
                 __syncthreads();
-                inf = lu ? lu_factor<double, NB>(n, Fd, pw, P, Us, spiv, s) : chol_factor<double, NB>(n, Fd, P, s);
+                inf = lu ? lu_factor<double, NB, false>(n, Fd, pw, P, Us, spiv, s)
+                         : chol_factor<double, NB, false>(n, Fd, P, s);
                 its = -1;
                 if (inf == 0) {
                     if (lu) piv_to_perm(n, pw, perm);
