@@ -582,6 +582,12 @@ constexpr int WW = PW - 1;  // window slots
 // one lane holds it (the usual case) its key is read from that lane, when several do
 // (equal magnitudes) the smallest of their keys decides.  Candidates are never NaN and
 // an empty lane holds (-1, NOKEY), so an empty wavefront yields (-1, NOKEY) again.
+__device__ __forceinline__ double rdlane_d(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ uint64_t lane_u64(uint64_t x, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
@@ -1003,6 +1009,100 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
     }
 }
 
+// ---- the U rows of a factored super-panel for one block of 32 columns, in one launch
+//
+// For panels q = 0 .. G-1 of the super-panel (rows j0 = 32 q): first the GEMM of those
+// rows against the U rows of the earlier panels, then the unit-lower TRSM with the
+// panel's L11 -- the operations the separate launches perform, in their order and with
+// their arithmetic:
+//   * the GEMM as dgemm_mfma<32, 32> computes it with a 32-deep slab (getrf.rs:75-86):
+//     each wavefront owns a 16 x 16 tile, every 32-deep slab is an 8-step
+//     mfma_f64_16x16x4f64 chain over k = 0, 4, 8, .. from a zero accumulator, folded as
+//     C = fma(-1, acc, C);
+//   * the TRSM as trsm_ln_lanes<true, 32> computes it (trsm.rs:95-171, unit diagonal):
+//     a lane per row, columns eliminated in order, a zero multiplier skipped, the first
+//     (31 - k) & ~3 rows below the pivot fused, the rest unfused.
+// The 32 x 32G block lives in LDS between the steps.  L: the super-panel's first row and
+// column (a + s0 + s0 lda); Bm: its first row in the first of the columns.
+constexpr int UR_CB = 32;  // columns per workgroup
+typedef double ur_d4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) urows_fused(int G, const double* __restrict__ L, size_t lda,
+                                                   double* __restrict__ Bm, int ncols,
+                                                   const int32_t* __restrict__ abort) {
+    if (abort && *abort != 0) return;
+    __shared__ double sb[8 * PW][UR_CB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int cb0 = blockIdx.x * UR_CB;
+    const int rows = G * PW;
+    for (int e = tid; e < rows * UR_CB; e += 256) {
+        const int r = e % rows, c = e / rows;
+        sb[r][c] = cb0 + c < ncols ? Bm[(size_t)r + (size_t)(cb0 + c) * lda] : 0.0;
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = lane >> 4;
+    const int wm = (w & 1) * 16, wn = (w >> 1) * 16;
+    const int i = lane & 31, sub = lane >> 5;
+    for (int q = 0; q < G; q++) {
+        const int j0 = q * PW;
+        if (q > 0) {
+            ur_d4 crun, acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < 4; r++) crun[r] = sb[j0 + wm + lk + 4 * r][wn + lr];
+            const double* Lr = L + (size_t)(j0 + wm + lr);
+            for (int s = 0; s < j0; s += PW) {  // one 32-deep slab per earlier panel
+                double fa[PW / 4], fb[PW / 4];
+#pragma unroll
+                for (int t = 0; t < PW / 4; t++) {
+                    fa[t] = Lr[(size_t)(s + 4 * t + lk) * lda];
+                    fb[t] = sb[s + 4 * t + lk][wn + lr];
+                }
+#pragma unroll
+                for (int t = 0; t < PW / 4; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], fb[t], acc, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    crun[r] = fma(-1.0, acc[r], crun[r]);
+                    acc[r] = 0.0;
+                }
+            }
+            __syncthreads();  // every wavefront has read the rows it multiplies
+#pragma unroll
+            for (int r = 0; r < 4; r++) sb[j0 + wm + lk + 4 * r][wn + lr] = crun[r];
+            __syncthreads();
+        }
+        // TRSM of rows j0 .. j0+31 with the panel's unit-lower L11: lane i holds row i of
+        // L11; each wavefront solves two columns at a time (one per half)
+        double arow[PW];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const double a = L[(size_t)(j0 + i) + (size_t)(j0 + min(k, i)) * lda];  // clamped: loads in flight together
+            arow[k] = keep_or_zero(a, k <= i);
+        }
+        for (int pass = 0; pass < UR_CB / 8; pass++) {
+            const int c = w * (UR_CB / 4) + 2 * pass + sub;
+            double x = sb[j0 + i][c];
+#pragma unroll
+            for (int k = 0; k < PW; k++) {
+                double bk = rdlane_d(x, k);
+                const double b1 = rdlane_d(x, 32 + k);
+                bk = sub ? b1 : bk;
+                if (bk != 0.0) {
+                    const double al = -bk;
+                    if (al != 0.0) {
+                        const int nf = (PW - k - 1) & ~3;
+                        if (i > k) x = (i - k - 1 < nf) ? fma(al, arow[k], x) : x + al * arow[k];
+                    }
+                }
+            }
+            sb[j0 + i][c] = x;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < rows * UR_CB; e += 256) {
+        const int r = e % rows, c = e / rows;
+        if (cb0 + c < ncols) Bm[(size_t)r + (size_t)(cb0 + c) * lda] = sb[r][c];
+    }
+}
+
 __global__ void k_zero_i32(int32_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -1286,6 +1386,14 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
     const bool uniform = (sw % PW) == 0;
     if (uniform) {
         const Edge e = panel_edge(m, n, s0, PW);  // the same for every full-width panel
+        if (e.row >= s0 + sw && e.col >= c1 && G <= 8) {
+            // no edge elements among these rows and columns: every panel's GEMM and TRSM in
+            // one launch per 32 columns
+            Phase pt(h, "TRSM", (double)sw * sw * ncols, st);
+            hipLaunchKernelGGL(urows_fused, dim3((unsigned)((ncols + UR_CB - 1) / UR_CB)), dim3(256), 0, st, G,
+                               a + s0 + (size_t)s0 * lda, lda, a + s0 + (size_t)c0 * lda, ncols, info);
+            BOLA_LAUNCH_CHECK();
+        } else
         for (int q = 0; q < G; q++) {
             const int j = s0 + q * PW;
             if (q > 0) {
