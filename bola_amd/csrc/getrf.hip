@@ -1049,19 +1049,25 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
 #pragma unroll
             for (int r = 0; r < 4; r++) crun[r] = sb[j0 + wm + lk + 4 * r][wn + lr];
             const double* Lr = L + (size_t)(j0 + wm + lr);
-            for (int s = 0; s < j0; s += PW) {  // one 32-deep slab per earlier panel
-                double fa[PW / 4], fb[PW / 4];
+            // every slab's L entries in flight at once (at most 7 earlier panels)
+            double fa[7][PW / 4];
 #pragma unroll
-                for (int t = 0; t < PW / 4; t++) {
-                    fa[t] = Lr[(size_t)(s + 4 * t + lk) * lda];
-                    fb[t] = sb[s + 4 * t + lk][wn + lr];
-                }
+            for (int sl = 0; sl < 7; sl++)
 #pragma unroll
-                for (int t = 0; t < PW / 4; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], fb[t], acc, 0, 0, 0);
+                for (int t = 0; t < PW / 4; t++)
+                    fa[sl][t] = Lr[(size_t)(min(sl, q - 1) * PW + 4 * t + lk) * lda];  // clamped; slabs >= q unused
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    crun[r] = fma(-1.0, acc[r], crun[r]);
-                    acc[r] = 0.0;
+            for (int sl = 0; sl < 7; sl++) {  // one 32-deep slab per earlier panel
+                if (sl < q) {
+#pragma unroll
+                    for (int t = 0; t < PW / 4; t++)
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[sl][t], sb[sl * PW + 4 * t + lk][wn + lr], acc, 0,
+                                                                   0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        crun[r] = fma(-1.0, acc[r], crun[r]);
+                        acc[r] = 0.0;
+                    }
                 }
             }
             __syncthreads();  // every wavefront has read the rows it multiplies
@@ -1070,31 +1076,34 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
             __syncthreads();
         }
         // TRSM of rows j0 .. j0+31 with the panel's unit-lower L11: lane i holds row i of
-        // L11; each wavefront solves two columns at a time (one per half)
+        // L11; each wavefront solves its 8 columns together, 4 per half, whose
+        // eliminations are independent and interleave
         double arow[PW];
 #pragma unroll
         for (int k = 0; k < PW; k++) {
             const double a = L[(size_t)(j0 + i) + (size_t)(j0 + min(k, i)) * lda];  // clamped: loads in flight together
             arow[k] = keep_or_zero(a, k <= i);
         }
-        for (int pass = 0; pass < UR_CB / 8; pass++) {
-            const int c = w * (UR_CB / 4) + 2 * pass + sub;
-            double x = sb[j0 + i][c];
+        constexpr int NC = UR_CB / 8;  // columns per lane
+        const int cbase = w * (UR_CB / 4) + sub * NC;
+        double x[NC];
 #pragma unroll
-            for (int k = 0; k < PW; k++) {
-                double bk = rdlane_d(x, k);
-                const double b1 = rdlane_d(x, 32 + k);
+        for (int m2 = 0; m2 < NC; m2++) x[m2] = sb[j0 + i][cbase + m2];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const int nf = (PW - k - 1) & ~3;
+#pragma unroll
+            for (int m2 = 0; m2 < NC; m2++) {
+                double bk = rdlane_d(x[m2], k);
+                const double b1 = rdlane_d(x[m2], 32 + k);
                 bk = sub ? b1 : bk;
-                if (bk != 0.0) {
-                    const double al = -bk;
-                    if (al != 0.0) {
-                        const int nf = (PW - k - 1) & ~3;
-                        if (i > k) x = (i - k - 1 < nf) ? fma(al, arow[k], x) : x + al * arow[k];
-                    }
-                }
+                const double al = -bk;
+                // a zero multiplier (bk == 0, so al == 0) leaves the column as it is
+                if (bk != 0.0 && al != 0.0 && i > k) x[m2] = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
             }
-            sb[j0 + i][c] = x;
         }
+#pragma unroll
+        for (int m2 = 0; m2 < NC; m2++) sb[j0 + i][cbase + m2] = x[m2];
         __syncthreads();
     }
     for (int e = tid; e < rows * UR_CB; e += 256) {
