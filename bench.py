@@ -23,7 +23,7 @@ The dominant kernel of the headline step is the LU panel (phase GETRF2_IT, the p
 wavefront panel kernel); its algorithmic traffic is the panel read and written once per
 launch (2 * rows * 32 * 8 bytes), measured with HIP events on the library's stream, against
 the HBM peak.  The events run in a second pass of as many steps as the timed ones: inside
-the timed steps they would add about 12 ms per step.  The trailing-update DGEMM is
+the timed steps they would add 10 ms or more per step.  The trailing-update DGEMM is
 reported beside it against the FP64 matrix peak.
 
 CPU baseline: the oracle (a C restatement of the reference's own algorithm, OpenMP
@@ -111,7 +111,7 @@ def main():
     torch.cuda.synchronize()
 
     # the timed steps run without per-phase events: recording them (two per kernel call,
-    # about 4000 per factorization at n = 8192) adds about 12 ms to a step
+    # thousands per factorization at n = 8192) adds 10 ms or more to a step
     times = []
     t_all = 0.0
     for _ in range(args.steps):
