@@ -712,7 +712,11 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                                                  uint64_t* __restrict__ cur, RowMap* __restrict__ map, int nwv,
                                                  long spin_limit) {
     __shared__ double sret[PW * WV * R];  // retired columns: sret[c * WV*R + s*WV + lane]
-    __shared__ WaveSlot stg[3];           // candidates of this wavefront, then the row at position j
+    // every lane's rows as they stand before the deferred update: [0] multiplier x, [1] form
+    // flags, [2 + c] window slot c (column j + c); 35 doubles a row, so that the lanes' stores
+    // of one slot meet at most two to a bank and a row reads without conflicts
+    constexpr int RS = 35;
+    __shared__ double rst[R][WV][RS];
     __shared__ double prow[2][2 * PW + 2];
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
@@ -770,8 +774,10 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                 }
             }
         }
-        wave_reduce2(vb, tb);
-        // stage the candidate rows (pre-update form) and the row at position j
+        int lv, lt;  // the lanes holding the wavefront's best candidates
+        wave_best2(vb, tb, lv, lt);
+        // every lane stages its rows (pre-update form); the candidates' and the row at
+        // position j are read from there by lane
         int sv = -1, st = -1, sc = -1;
 #pragma unroll
         for (int s = 0; s < R; s++) {
@@ -782,25 +788,25 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                 if (rel == 0) sc = s;
             }
         }
-        const bool hascur = __ballot(sc >= 0) != 0;
+        const uint64_t hc = __ballot(sc >= 0);
+        const bool hascur = hc != 0;
+        const int lc = hascur ? __ffsll((unsigned long long)hc) - 1 : 0;
+        // (lane, row slot) of each staged row: the vector-phase candidate, the tail one, the row at j
+        int ql[3], qs[3];
+        ql[0] = lv;
+        ql[1] = lt;
+        ql[2] = lc;
+        qs[0] = R == 1 ? 0 : max(__builtin_amdgcn_readlane(sv, lv), 0);
+        qs[1] = R == 1 ? 0 : max(__builtin_amdgcn_readlane(st, lt), 0);
+        qs[2] = R == 1 ? 0 : max(__builtin_amdgcn_readlane(sc, lc), 0);
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int sx = q == 0 ? sv : (q == 1 ? st : sc);
-            WaveSlot* d = stg + q;
+        for (int s = 0; s < R; s++) {
+            double* d = rst[s][lane];
+            d[0] = xprev[s];
+            d[1] = __longlong_as_double((long long)((fprev[s] ? 1ull : 0ull) | (uprev[s] ? 2ull : 0ull)));
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                if (sx == s) {  // a branch per row, not a select, keeps the rows in registers
-                    d->x = xprev[s];
-                    d->flags = (fprev[s] ? 1ull : 0ull) | (uprev[s] ? 2ull : 0ull);
-#pragma unroll
-                    for (int c = 0; c < WW; c++) d->win[c] = a[s][c];  // slots past the window are never read
-                    d->win[wl - j] = al[s];
-                }
-            }
-        }
-        if (lane == 0) {
-            stg[0].v = vb.v; stg[0].key = vb.k;
-            stg[1].v = tb.v; stg[1].key = tb.k;
+            for (int c = 0; c < WW; c++) d[2 + c] = a[s][c];  // slots past the window are never read
+            d[2 + wl - j] = al[s];
         }
         __syncthreads();  // one wavefront: orders the staging writes before the reads below
         const uint32_t ep = (uint32_t)(col_off + j + 1);
@@ -814,7 +820,15 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
 #pragma unroll
             for (int i = 0; i < NE; i++) {  // every LDS read in flight before the first store
                 const int e = min(lane + i * WV, 3 * TW - 1);
-                hv[i] = ((const uint32_t*)stg)[e];  // half-word f2 of staged slot q at q * TW + f2
+                const int q = e / TW, f2 = e % TW, f = f2 >> 1;
+                // words 0, 1: the candidate's magnitude and key; from 2 on: the staged row
+                const int qi = q == 0 ? 0 : (q == 1 ? 1 : 2);
+                const uint64_t head = f == 0 ? (uint64_t)__double_as_longlong(qi == 0 ? vb.v : tb.v)
+                                             : (qi == 0 ? vb.k : tb.k);
+                const double* rrow = rst[qi == 0 ? qs[0] : (qi == 1 ? qs[1] : qs[2])]
+                                        [qi == 0 ? ql[0] : (qi == 1 ? ql[1] : ql[2])];
+                const uint32_t body = ((const uint32_t*)rrow)[max(f2 - 4, 0)];
+                hv[i] = f >= 2 ? body : (uint32_t)(head >> (32 * (f2 & 1)));
             }
 #pragma unroll
             for (int i = 0; i < NE; i++) {
@@ -923,10 +937,11 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
         } else {
             bool ft;
             rel = idamax_rule(len, vb, tb, ft);
-            const WaveSlot* P = stg + (rel == 0 ? 2 : (ft ? 1 : 0));
-            if (c >= j && c < w) v = P->win[c - j];
-            px = P->x;
-            pf = P->flags;
+            const int qi = rel == 0 ? 2 : (ft ? 1 : 0);
+            const double* P = rst[qs[qi]][ql[qi]];
+            if (c >= j && c < w) v = P[2 + c - j];
+            px = P[0];
+            pf = (uint64_t)__double_as_longlong(P[1]);
         }
         // the winning row receives the previous step's deferred update and becomes the pivot row
         if (dj >= 0 && c > j && c < w) {
