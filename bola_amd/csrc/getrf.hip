@@ -1127,6 +1127,112 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
     }
 }
 
+// ---- after a panel inside its super-panel: the TRSM of its U rows and the GEMM of the
+// rows below, in one launch
+//
+// U := L11^-1 U (the panel's 32 rows in the super-panel's nr columns right of it) as
+// trsm_ln_lanes<true, 32> computes it, then C -= L21 U for the mb rows below as
+// dgemm_mfma computes a 32-deep single-slab product (a zero accumulator, the 8-step
+// mfma_f64_16x16x4f64 chain over k = 0, 4, .., then C = fma(-1, acc, C)).  Workgroup
+// (ct, 0) solves the 32 columns [32 ct, 32 ct + 32), stores them with device-coherent
+// stores and raises flags[ct]; workgroups (ct, 1 ..) each update 64 rows of those columns
+// once the flag is up.  Each column block's solver precedes its updaters in dispatch
+// order, so a waiting workgroup never holds back the one it waits for.
+__global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nrt, const double* __restrict__ L11,
+                                                          double* __restrict__ U, const double* __restrict__ L21,
+                                                          double* __restrict__ C, size_t lda,
+                                                          int* __restrict__ flags, long spin_limit,
+                                                          int32_t* __restrict__ info) {
+    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int ct = blockIdx.x / (nrt + 1), rt = blockIdx.x % (nrt + 1);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c0 = ct * UR_CB;
+    if (rt == 0) {
+        __shared__ double sb[PW][UR_CB + 1];
+        for (int e = tid; e < PW * UR_CB; e += 256) {
+            const int r = e % PW, c = e / PW;
+            sb[r][c] = c0 + c < nr ? U[(size_t)r + (size_t)(c0 + c) * lda] : 0.0;
+        }
+        const int i = lane & 31, sub = lane >> 5;
+        double arow[PW];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const double a = L11[(size_t)i + (size_t)min(k, i) * lda];  // clamped: loads in flight together
+            arow[k] = keep_or_zero(a, k <= i);
+        }
+        __syncthreads();
+        constexpr int NC = UR_CB / 8;
+        const int cbase = w * (UR_CB / 4) + sub * NC;
+        double x[NC];
+#pragma unroll
+        for (int m2 = 0; m2 < NC; m2++) x[m2] = sb[i][cbase + m2];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const int nf = (PW - k - 1) & ~3;
+#pragma unroll
+            for (int m2 = 0; m2 < NC; m2++) {
+                double bk = rdlane_d(x[m2], k);
+                const double b1 = rdlane_d(x[m2], 32 + k);
+                bk = sub ? b1 : bk;
+                const double al = -bk;
+                if (bk != 0.0 && al != 0.0 && i > k) x[m2] = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
+            }
+        }
+#pragma unroll
+        for (int m2 = 0; m2 < NC; m2++)
+            if (c0 + cbase + m2 < nr) xst(U + (size_t)i + (size_t)(c0 + cbase + m2) * lda, x[m2]);
+        __builtin_amdgcn_s_waitcnt(0);  // the solved rows are stored before the flag rises
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flags + ct, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    // 64 rows of the update: wavefront w owns rows [16 w, 16 w + 16) as two 16 x 16 tiles
+    const int r0 = (rt - 1) * 64 + 16 * w;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int row = min(r0 + lr, mb - 1);
+    double fa[PW / 4];
+#pragma unroll
+    for (int t = 0; t < PW / 4; t++) fa[t] = L21[(size_t)row + (size_t)(4 * t + lk) * lda];
+    ur_d4 crun[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            crun[b][r] = C[(size_t)min(r0 + lk + 4 * r, mb - 1) + (size_t)min(c0 + 16 * b + lr, nr - 1) * lda];
+    __shared__ int ok_sh;
+    if (tid == 0) {
+        long sp = 0;
+        int ok = 1;
+        while (__hip_atomic_load(flags + ct, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            if (++sp > spin_limit || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        ok_sh = ok;
+    }
+    __syncthreads();
+    if (!ok_sh) {
+        if (tid == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        const int col = min(c0 + 16 * b + lr, nr - 1);
+        ur_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < PW / 4; t++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], xld(U + (size_t)(4 * t + lk) + (size_t)col * lda), acc,
+                                                       0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int rr = r0 + lk + 4 * r, cc = c0 + 16 * b + lr;
+            if (rr < mb && cc < nr) C[(size_t)rr + (size_t)cc * lda] = fma(-1.0, acc[r], crun[b][r]);
+        }
+    }
+}
+
 __global__ void k_zero_i32(int32_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -1229,6 +1335,7 @@ namespace {
 
 struct LuWs {
     int* flags;         // [npanels][nwg_max]
+    int* uflags;        // [npanels][8]: the fused panel update's column-block flags
     PanelSlot* slots;   // [2][nwg_max][2]
     PanelSlot* cur;     // [2]
     RowMap* maps;       // [npanels]
@@ -1265,6 +1372,7 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
     // factorization
     const size_t slot_bytes = std::max(sizeof(PanelSlot), sizeof(uint64_t) * TW);
     size_t o_flags = take(sizeof(int) * (size_t)w.npanels * w.nwg_max);
+    size_t o_uflags = take(sizeof(int) * (size_t)w.npanels * 8);
     size_t o_maps = take(sizeof(RowMap) * (size_t)w.npanels);
     size_t o_jp = take(sizeof(int32_t) * 4);
     size_t o_slots = take(slot_bytes * 4 * (size_t)w.nwg_max);
@@ -1272,6 +1380,7 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
     size_t zero_bytes = off;
     char* base = (char*)h->ws.get(off);
     w.flags = (int*)(base + o_flags);
+    w.uflags = (int*)(base + o_uflags);
     w.maps = (RowMap*)(base + o_maps);
     w.jp = (int32_t*)(base + o_jp);
     w.slots = (PanelSlot*)(base + o_slots);
@@ -1478,7 +1587,16 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
                                 (size_t)(s0 + sw - j - jb), w.maps + p, 1, info, ipiv + j, jb, j);
         }
         const int nr = s0 + sw - j - jb;
-        if (nr > 0) {
+        const Edge ef = panel_edge(m, n, j, jb);
+        if (nr > 0 && jb == PW && j + jb < m && nr <= 8 * UR_CB && ef.row >= m && ef.col >= j + jb + nr) {
+            // no edge element among the rows below and these columns: both in one launch
+            const int mb = m - j - jb, nrt = (mb + 63) / 64, ncb = (nr + UR_CB - 1) / UR_CB;
+            Phase pt(h, "TRSM", (double)jb * jb * nr + 2.0 * (double)mb * nr * jb, st);
+            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + 1))), dim3(256), 0, st, mb, nr, nrt,
+                               a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda, a + (j + jb) + (size_t)j * lda,
+                               a + (j + jb) + (size_t)(j + jb) * lda, lda, w.uflags + (size_t)p * 8, kSpinLimit, info);
+            BOLA_LAUNCH_CHECK();
+        } else if (nr > 0) {
             {
                 Phase pt(h, "TRSM", (double)jb * jb * nr, st);
                 launch_dtrsm(st, 'L', 'L', 'N', 'U', jb, nr, 1.0, a + j + (size_t)j * lda, lda,
