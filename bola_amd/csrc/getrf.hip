@@ -1133,26 +1133,23 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
 // U := L11^-1 U (the panel's 32 rows in the super-panel's nr columns right of it) as
 // trsm_ln_lanes<true, 32> computes it, then C -= L21 U for the mb rows below as
 // dgemm_mfma computes a 32-deep single-slab product (a zero accumulator, the 8-step
-// mfma_f64_16x16x4f64 chain over k = 0, 4, .., then C = fma(-1, acc, C)).  Workgroup
-// (ct, 0) solves the 32 columns [32 ct, 32 ct + 32), stores them with device-coherent
-// stores and raises flags[ct]; workgroups (ct, 1 ..) each update 64 rows of those columns
-// once the flag is up.  Each column block's solver precedes its updaters in dispatch
-// order, so a waiting workgroup never holds back the one it waits for.
+// mfma_f64_16x16x4f64 chain over k = 0, 4, .., then C = fma(-1, acc, C)).  The first
+// PSU_S workgroups of column block ct each solve 8 of its 32 columns (a wavefront per two,
+// one per half, as the TRSM launch does), store them with device-coherent stores and raise
+// their flag; the block's other workgroups each update 64 rows of its columns once all
+// PSU_S flags are up.  A block's solvers precede its updaters in dispatch order, so a
+// waiting workgroup never holds back the ones it waits for.
+constexpr int PSU_S = UR_CB / 8;  // solver workgroups per column block
 __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nrt, const double* __restrict__ L11,
                                                           double* __restrict__ U, const double* __restrict__ L21,
                                                           double* __restrict__ C, size_t lda,
                                                           int* __restrict__ flags, long spin_limit,
                                                           int32_t* __restrict__ info) {
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    const int ct = blockIdx.x / (nrt + 1), rt = blockIdx.x % (nrt + 1);
+    const int ct = blockIdx.x / (nrt + PSU_S), rt = blockIdx.x % (nrt + PSU_S);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int c0 = ct * UR_CB;
-    if (rt == 0) {
-        __shared__ double sb[PW][UR_CB + 1];
-        for (int e = tid; e < PW * UR_CB; e += 256) {
-            const int r = e % PW, c = e / PW;
-            sb[r][c] = c0 + c < nr ? U[(size_t)r + (size_t)(c0 + c) * lda] : 0.0;
-        }
+    if (rt < PSU_S) {
         const int i = lane & 31, sub = lane >> 5;
         double arow[PW];
 #pragma unroll
@@ -1160,12 +1157,10 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
             const double a = L11[(size_t)i + (size_t)min(k, i) * lda];  // clamped: loads in flight together
             arow[k] = keep_or_zero(a, k <= i);
         }
-        __syncthreads();
-        constexpr int NC = UR_CB / 8;
-        const int cbase = w * (UR_CB / 4) + sub * NC;
+        constexpr int NC = 1;
+        const int cbase = rt * 8 + w * 2 + sub;  // this lane's column within the block
         double x[NC];
-#pragma unroll
-        for (int m2 = 0; m2 < NC; m2++) x[m2] = sb[i][cbase + m2];
+        x[0] = keep_or_zero(U[(size_t)i + (size_t)min(c0 + cbase, nr - 1) * lda], c0 + cbase < nr);
 #pragma unroll
         for (int k = 0; k < PW; k++) {
             const int nf = (PW - k - 1) & ~3;
@@ -1183,11 +1178,11 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
             if (c0 + cbase + m2 < nr) xst(U + (size_t)i + (size_t)(c0 + cbase + m2) * lda, x[m2]);
         __builtin_amdgcn_s_waitcnt(0);  // the solved rows are stored before the flag rises
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(flags + ct, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_store(flags + ct * PSU_S + rt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     // 64 rows of the update: wavefront w owns rows [16 w, 16 w + 16) as two 16 x 16 tiles
-    const int r0 = (rt - 1) * 64 + 16 * w;
+    const int r0 = (rt - PSU_S) * 64 + 16 * w;
     const int lr = lane & 15, lk = lane >> 4;
     const int row = min(r0 + lr, mb - 1);
     double fa[PW / 4];
@@ -1200,17 +1195,19 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
         for (int r = 0; r < 4; r++)
             crun[b][r] = C[(size_t)min(r0 + lk + 4 * r, mb - 1) + (size_t)min(c0 + 16 * b + lr, nr - 1) * lda];
     __shared__ int ok_sh;
-    if (tid == 0) {
+    if (tid == 0) ok_sh = 1;
+    __syncthreads();
+    if (tid < PSU_S) {  // one thread per solver flag
         long sp = 0;
         int ok = 1;
-        while (__hip_atomic_load(flags + ct, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        while (__hip_atomic_load(flags + ct * PSU_S + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
             if (++sp > spin_limit || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                 ok = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
         }
-        ok_sh = ok;
+        if (!ok) ok_sh = 0;
     }
     __syncthreads();
     if (!ok_sh) {
@@ -1335,7 +1332,7 @@ namespace {
 
 struct LuWs {
     int* flags;         // [npanels][nwg_max]
-    int* uflags;        // [npanels][8]: the fused panel update's column-block flags
+    int* uflags;        // [npanels][8 * 4]: the fused panel update's solver flags
     PanelSlot* slots;   // [2][nwg_max][2]
     PanelSlot* cur;     // [2]
     RowMap* maps;       // [npanels]
@@ -1372,7 +1369,7 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
     // factorization
     const size_t slot_bytes = std::max(sizeof(PanelSlot), sizeof(uint64_t) * TW);
     size_t o_flags = take(sizeof(int) * (size_t)w.npanels * w.nwg_max);
-    size_t o_uflags = take(sizeof(int) * (size_t)w.npanels * 8);
+    size_t o_uflags = take(sizeof(int) * (size_t)w.npanels * 32);
     size_t o_maps = take(sizeof(RowMap) * (size_t)w.npanels);
     size_t o_jp = take(sizeof(int32_t) * 4);
     size_t o_slots = take(slot_bytes * 4 * (size_t)w.nwg_max);
@@ -1592,9 +1589,10 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
             // no edge element among the rows below and these columns: both in one launch
             const int mb = m - j - jb, nrt = (mb + 63) / 64, ncb = (nr + UR_CB - 1) / UR_CB;
             Phase pt(h, "TRSM", (double)jb * jb * nr + 2.0 * (double)mb * nr * jb, st);
-            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + 1))), dim3(256), 0, st, mb, nr, nrt,
-                               a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda, a + (j + jb) + (size_t)j * lda,
-                               a + (j + jb) + (size_t)(j + jb) * lda, lda, w.uflags + (size_t)p * 8, kSpinLimit, info);
+            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + PSU_S))), dim3(256), 0, st, mb, nr,
+                               nrt, a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda,
+                               a + (j + jb) + (size_t)j * lda, a + (j + jb) + (size_t)(j + jb) * lda, lda,
+                               w.uflags + (size_t)p * 32, kSpinLimit, info);
             BOLA_LAUNCH_CHECK();
         } else if (nr > 0) {
             {
