@@ -1037,32 +1037,40 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
 //   * the TRSM as trsm_ln_lanes<true, 32> computes it (trsm.rs:95-171, unit diagonal):
 //     a lane per row, columns eliminated in order, a zero multiplier skipped, the first
 //     (31 - k) & ~3 rows below the pivot fused, the rest unfused.
-// The 32 x 32G block lives in LDS between the steps.  L: the super-panel's first row and
+// A workgroup takes CB columns; their 32G x CB block lives in LDS between the steps.  L: the super-panel's first row and
 // column (a + s0 + s0 lda); Bm: its first row in the first of the columns.
-constexpr int UR_CB = 32;  // columns per workgroup
+constexpr int UR_CB = 32;  // columns per column block of the panel update below
+constexpr int UR_W = 8;    // columns per workgroup of urows_fused: one per half-wavefront in the TRSM
 typedef double ur_d4 __attribute__((ext_vector_type(4)));
+template <int CB>
 __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restrict__ L, size_t lda,
                                                    double* __restrict__ Bm, int ncols,
                                                    const int32_t* __restrict__ abort) {
     if (abort && *abort != 0) return;
-    __shared__ double sb[8 * PW][UR_CB + 1];
+    __shared__ double sb[8 * PW][CB + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int cb0 = blockIdx.x * UR_CB;
+    const int cb0 = blockIdx.x * CB;
     const int rows = G * PW;
-    for (int e = tid; e < rows * UR_CB; e += 256) {
+    for (int e = tid; e < rows * CB; e += 256) {
         const int r = e % rows, c = e / rows;
         sb[r][c] = cb0 + c < ncols ? Bm[(size_t)r + (size_t)(cb0 + c) * lda] : 0.0;
     }
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
     const int wm = (w & 1) * 16, wn = (w >> 1) * 16;
+    // a 16-column MFMA tile: with CB < 32 the wavefronts past the block sit the GEMM out and
+    // the lanes past it read a clamped column and store nothing
+    const bool gemm_wave = wn < CB;
+    const int gc = min(wn + lr, CB - 1);
+    const bool gcol = wn + lr < CB;
     const int i = lane & 31, sub = lane >> 5;
     for (int q = 0; q < G; q++) {
         const int j0 = q * PW;
-        if (q > 0) {
-            ur_d4 crun, acc = {0.0, 0.0, 0.0, 0.0};
+        ur_d4 crun = {0.0, 0.0, 0.0, 0.0};
+        if (q > 0 && gemm_wave) {
+            ur_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int r = 0; r < 4; r++) crun[r] = sb[j0 + wm + lk + 4 * r][wn + lr];
+            for (int r = 0; r < 4; r++) crun[r] = sb[j0 + wm + lk + 4 * r][gc];
             const double* Lr = L + (size_t)(j0 + wm + lr);
             // every slab's L entries in flight at once (at most 7 earlier panels)
             double fa[7][PW / 4];
@@ -1076,7 +1084,7 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
                 if (sl < q) {
 #pragma unroll
                     for (int t = 0; t < PW / 4; t++)
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[sl][t], sb[sl * PW + 4 * t + lk][wn + lr], acc, 0,
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[sl][t], sb[sl * PW + 4 * t + lk][gc], acc, 0,
                                                                    0, 0);
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
@@ -1085,13 +1093,16 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
                     }
                 }
             }
+        }
+        if (q > 0) {
             __syncthreads();  // every wavefront has read the rows it multiplies
+            if (gemm_wave && gcol)
 #pragma unroll
-            for (int r = 0; r < 4; r++) sb[j0 + wm + lk + 4 * r][wn + lr] = crun[r];
+                for (int r = 0; r < 4; r++) sb[j0 + wm + lk + 4 * r][gc] = crun[r];
             __syncthreads();
         }
         // TRSM of rows j0 .. j0+31 with the panel's unit-lower L11: lane i holds row i of
-        // L11; each wavefront solves its 8 columns together, 4 per half, whose
+        // L11; each wavefront solves its CB / 4 columns together, CB / 8 per half, whose
         // eliminations are independent and interleave
         double arow[PW];
 #pragma unroll
@@ -1099,8 +1110,8 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
             const double a = L[(size_t)(j0 + i) + (size_t)(j0 + min(k, i)) * lda];  // clamped: loads in flight together
             arow[k] = keep_or_zero(a, k <= i);
         }
-        constexpr int NC = UR_CB / 8;  // columns per lane
-        const int cbase = w * (UR_CB / 4) + sub * NC;
+        constexpr int NC = CB / 8;  // columns per lane
+        const int cbase = w * (CB / 4) + sub * NC;
         double x[NC];
 #pragma unroll
         for (int m2 = 0; m2 < NC; m2++) x[m2] = sb[j0 + i][cbase + m2];
@@ -1121,7 +1132,7 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
         for (int m2 = 0; m2 < NC; m2++) sb[j0 + i][cbase + m2] = x[m2];
         __syncthreads();
     }
-    for (int e = tid; e < rows * UR_CB; e += 256) {
+    for (int e = tid; e < rows * CB; e += 256) {
         const int r = e % rows, c = e / rows;
         if (cb0 + c < ncols) Bm[(size_t)r + (size_t)(cb0 + c) * lda] = sb[r][c];
     }
@@ -1518,9 +1529,9 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
         const Edge e = panel_edge(m, n, s0, PW);  // the same for every full-width panel
         if (e.row >= s0 + sw && e.col >= c1 && G <= 8) {
             // no edge elements among these rows and columns: every panel's GEMM and TRSM in
-            // one launch per 32 columns
+            // one launch, a workgroup per 8 columns
             Phase pt(h, "TRSM", (double)sw * sw * ncols, st);
-            hipLaunchKernelGGL(urows_fused, dim3((unsigned)((ncols + UR_CB - 1) / UR_CB)), dim3(256), 0, st, G,
+            hipLaunchKernelGGL(urows_fused<UR_W>, dim3((unsigned)((ncols + UR_W - 1) / UR_W)), dim3(256), 0, st, G,
                                a + s0 + (size_t)s0 * lda, lda, a + s0 + (size_t)c0 * lda, ncols, info);
             BOLA_LAUNCH_CHECK();
         } else
