@@ -59,6 +59,12 @@ struct bola_hip_server_s {
     int device = 0;
     size_t budget = 0, in_use = 0, peak = 0;
     std::mutex mu;
+    // One LU at a time above kBigLu: its panel kernel's workgroups (up to 128 of 35 KB of
+    // LDS each, or 512 of 70 KB above 8192 rows) must all be resident at once, which
+    // several such panels on the slots' streams, beside their GEMMs, cannot be sure of.
+    // One large factorization fills the GPU anyway.
+    std::mutex big_lu;
+    static constexpr size_t kBigLu = 2048;
     std::condition_variable cv_work, cv_done;
     std::deque<Ticket*> queue;
     std::unordered_map<uint64_t, Ticket*> tickets;
@@ -165,6 +171,8 @@ void bola_hip_server_s::work(int slot) {
                 BOLA_CHECK(hipMallocAsync((void**)&fa, n * n * 8, st));
                 BOLA_CHECK(hipMallocAsync((void**)&ipiv, std::max<size_t>(n, 1) * 4, st));
                 BOLA_CHECK(hipMemcpyAsync(fa, r.a, n * n * 8, hipMemcpyDefault, st));
+                std::unique_lock<std::mutex> big(big_lu, std::defer_lock);
+                if (r.kind == BOLA_HIP_SERVE_LU && n > kBigLu) big.lock();  // released after the sync below
                 int rc = r.kind == BOLA_HIP_SERVE_LU ? bola_hip_dgetrf_dev(h, n, n, fa, n, ipiv, dinfo)
                                                      : bola_hip_dpotrf_dev(h, 'L', n, fa, n, dinfo);
                 if (rc < 0) throw HipFailure{hipErrorUnknown, "factorization", __LINE__};

@@ -181,3 +181,25 @@ def test_serve_reports_a_singular_matrix_and_refuses_bad_requests(B):
         with pytest.raises(B.BolaError) as e:
             s.submit(B.SERVE_LU, n, 1, None, b, x, 99)  # no matrix, nothing resident
         assert e.value.code == -3
+
+
+def test_serve_large_factorizations_in_the_mix(B):
+    # LU requests above 2048 take turns (their persistent panels must all be resident);
+    # the small ones and the Cholesky ones run beside them; every result is the standalone one
+    sizes = [(2600, B.SERVE_LU), (300, B.SERVE_LU), (2100, B.SERVE_LU), (2300, B.SERVE_CHOL), (3000, B.SERVE_LU),
+             (150, B.SERVE_CHOL)]
+    reqs = []
+    rng = np.random.default_rng(77)
+    for i, (n, kind) in enumerate(sizes):
+        nrhs = 2
+        reqs.append(dict(kind=kind, n=n, nrhs=nrhs, a=_matrix(n, kind, 500 + i, B),
+                         b=rng.uniform(-1, 1, n * nrhs), key=0, dev=(i % 2 == 1)))
+    with B.Server(8 << 30, 4) as s:
+        live = []
+        for r in reqs:
+            ops = _operands(r)
+            live.append((r, s.submit(r["kind"], r["n"], r["nrhs"], *ops, 0), ops))
+        for r, t, ops in live:
+            res = s.wait(t)
+            assert res["status"] == 0 and res["info"] == 0, r["n"]
+            assert np.array_equal(_host(ops[2]), _standalone(B, r)), r["n"]
