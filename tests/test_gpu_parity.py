@@ -521,6 +521,34 @@ def test_potrs_bitexact(B, O, uplo):
     same(b, ref, "potrs")
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrs_large_bitexact(B, O, uplo):
+    # the recursive triangular solves (halving down to 64-row base cases, transposed GEMMs in
+    # between) over many columns
+    rng = np.random.default_rng(18 + (uplo == "U"))
+    n, nrhs = 2085, 64
+    f = _spd(rng, n)
+    assert O.potrf(uplo, n, f, n)[0] == 0
+    b = rng.uniform(-1, 1, n * nrhs)
+    ref = b.copy()
+    O.potrs(uplo, n, nrhs, f, n, ref, n)
+    B.potrs(uplo, n, nrhs, f, n, b, n)
+    same(b, ref, f"potrs {uplo} {n}")
+
+
+def test_getrs_large_bitexact(B, O):
+    rng = np.random.default_rng(19)
+    n, nrhs = 2600, 64
+    a = rng.uniform(-1, 1, n * n)
+    ipiv = np.zeros(n, np.int32)
+    assert O.getrf(n, n, a, n, ipiv)[0] == 0
+    b = rng.uniform(-1, 1, n * nrhs)
+    ref = b.copy()
+    O.getrs(n, nrhs, a, n, ipiv, ref, n)
+    B.getrs(n, nrhs, a, n, ipiv, b, n)
+    same(b, ref, f"getrs {n}")
+
+
 # ------------------------------------------------------------------ device-pointer flavour
 
 def test_device_flavour_matches_host(B, O):
