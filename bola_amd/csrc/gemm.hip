@@ -15,8 +15,10 @@
 //     increasing k; a separate kernel does exactly that for those elements.
 //
 // Tiling: 256 threads = 4 wavefronts in a 2x2 grid; each wavefront owns a
-// (BM/2)x(BN/2) block of 16x16 MFMA tiles.  K advances 16 at a time through a
-// double-buffered LDS stage.  LDS layouts put each MFMA operand read on 32 distinct
+// (BM/2)x(BN/2) block of 16x16 MFMA tiles.  K advances BK at a time through a
+// double-buffered LDS stage: 16 for the large tiles, 64 for 32x32 tiles whose slab is a
+// multiple of 64 (the latency-bound narrow products of the triangular solves, where each
+// step is one global-load round trip).  LDS layouts put each MFMA operand read on 32 distinct
 // banks: A as [k][BM] (or [m][BK+1] when op(A) = A^T), B as [n][BK+1] (or [k][BN] when
 // op(B) = B^T).  Block ids are remapped so each XCD's L2 serves a compact C region.
 #include "common.hpp"
@@ -27,10 +29,9 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int BK = 16;
 constexpr int KC = 64;  // reference slab depth
 
-template <int BM, int BN, bool TA, bool TB>
+template <int BM, int BN, bool TA, bool TB, int BK>
 struct GemmCfg {
     static constexpr int WM = BM / 2, WN = BN / 2;  // wavefront tile
     static constexpr int TM = WM / 16, TN = WN / 16;  // MFMA tiles per wavefront
@@ -44,13 +45,13 @@ struct GemmCfg {
 };
 
 // Element (i, kk) of the BMxBK tile of op(A) handled by thread t for load slot s.
-template <int BM, bool TA>
+template <int BM, bool TA, int BK>
 __device__ __forceinline__ void a_coord(int t, int s, int& i, int& kk) {
     int e = t + 256 * s;
     if (!TA) { i = e % BM; kk = e / BM; }       // column-major A: m fastest
     else { kk = e % BK; i = e / BK; }           // A^T: k fastest
 }
-template <int BN, bool TB>
+template <int BN, bool TB, int BK>
 __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
     int e = t + 256 * s;
     if (!TB) { kk = e % BK; j = e / BK; }       // column-major B: k fastest
@@ -59,14 +60,14 @@ __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
 
 // OCC: workgroups per CU the register budget is fitted to.  2 for stand-alone products;
 // 1 (more registers, lighter on the machine) for the updates that run beside a
-// factorization's latency-bound panel kernel.
-template <int BM, int BN, bool TA, bool TB, int OCC>
+// factorization's latency-bound panel kernel.  BK divides the caller's slab.
+template <int BM, int BN, bool TA, bool TB, int OCC, int BK>
 __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
                                                   int edge_m, int edge_n, const int32_t* __restrict__ abort,
                                                   int slab, int mask) {
-    using Cfg = GemmCfg<BM, BN, TA, TB>;
+    using Cfg = GemmCfg<BM, BN, TA, TB, BK>;
     if (abort && *abort != 0) return;
     __shared__ double sA[2][Cfg::A_SIZE];
     __shared__ double sB[2][Cfg::B_SIZE];
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
-            a_coord<BM, TA>(t, s, i, kk);
+            a_coord<BM, TA, BK>(t, s, i, kk);
             int gi = row0 + i, gk = k0 + kk;
             const int ci = min(gi, m - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
             const double v = TA ? A[ck + (size_t)ci * lda] : A[ci + (size_t)ck * lda];
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
-            b_coord<BN, TB>(t, s, kk, j);
+            b_coord<BN, TB, BK>(t, s, kk, j);
             int gj = col0 + j, gk = k0 + kk;
             const int cj = min(gj, n - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
             const double v = TB ? B[cj + (size_t)ck * ldb] : B[ck + (size_t)cj * ldb];
@@ -146,20 +147,20 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
-            a_coord<BM, TA>(t, s, i, kk);
+            a_coord<BM, TA, BK>(t, s, i, kk);
             if (!TA) sA[buf][kk * BM + i] = ra[s];
             else sA[buf][i * (BK + 1) + kk] = ra[s];
         }
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
-            b_coord<BN, TB>(t, s, kk, j);
+            b_coord<BN, TB, BK>(t, s, kk, j);
             if (!TB) sB[buf][j * (BK + 1) + kk] = rb[s];
             else sB[buf][kk * BN + j] = rb[s];
         }
     };
     const int nsteps = (k + BK - 1) / BK;
-    // one BK-deep step from LDS buffer buf, then the slab fold when a 64-deep slab (or k) ends
+    // one BK-deep step from LDS buffer buf, then the slab fold when a slab (or k) ends
     auto compute = [&](int buf, int step) {
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
@@ -268,17 +269,18 @@ __global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double al
     }
 }
 
-template <int BM, int BN, bool TA, bool TB>
+template <int BM, int BN, bool TA, bool TB, int BK = 16>
 void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
                  size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, const int32_t* abort,
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     if (fast)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : 3)>), dim3(tm * tn), dim3(256), 0, st, m, n, k,
-                           alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : 3), BK>), dim3(tm * tn),
+                           dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n,
+                           abort, slab, mask);
     else
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B,
-                           ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A,
+                           lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
     BOLA_LAUNCH_CHECK();
 }
 
@@ -289,9 +291,14 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
     if (edge_m > 0 && edge_n > 0) {
         const long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
         const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
+        // 64-deep steps need fold points on multiples of 64 and at least one full step
+        const bool deep = slab % 64 == 0 && k >= 64;
         if (t128 >= 2L * num_cus)
             launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                          slab, mask, fast);
+        else if (deep && t64 < num_cus)
+            launch_tile<32, 32, TA, TB, 64>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
+                                            slab, mask, fast);
         else if (t64 >= num_cus || m * (long)n <= 64L * 64)
             launch_tile<64, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
                                         mask, fast);

@@ -33,9 +33,14 @@ __device__ __forceinline__ double rdlane(double v, int l) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-template <bool LOWER, int TW>
-__global__ void __launch_bounds__(256) trsm_ln_lanes(int unit, int t, int nvec, const double* __restrict__ A,
-                                                     size_t lda, double* __restrict__ B, size_t ldb,
+// Unit diagonal: each step is straight-line code, the reference's skips (a zero
+// right-hand-side entry) becoming selects, so a wavefront alone on its SIMD runs no branch
+// per step.  Non-unit diagonal: the division sits on the step chain and the skips stay
+// branches (measured faster for that case).  Every element sees exactly the reference's
+// operation or none.
+template <bool LOWER, int TW, bool UNIT>
+__global__ void __launch_bounds__(256) trsm_ln_lanes(int t, int nvec, const double* __restrict__ A, size_t lda,
+                                                     double* __restrict__ B, size_t ldb,
                                                      const int32_t* __restrict__ abort) {
     if (abort && *abort != 0) return;
     constexpr bool TWO = TW == 32;
@@ -54,26 +59,50 @@ __global__ void __launch_bounds__(256) trsm_ln_lanes(int unit, int t, int nvec, 
     }
     const double xb = B[min(i, t - 1) + (size_t)min(v, nvec - 1) * ldb];
     double x = keep_or_zero(xb, act);
+    if constexpr (UNIT) {
+        // steps k >= t need no test: lane k is then inactive, its x is zero, and a zero
+        // entry changes nothing
 #pragma unroll
-    for (int kk = 0; kk < TW; kk++) {
-        const int k = LOWER ? kk : (TW - 1 - kk);
-        if (k < t) {
+        for (int kk = 0; kk < TW; kk++) {
+            const int k = LOWER ? kk : (TW - 1 - kk);
             double bk = rdlane(x, k);
             if (TWO) {
                 const double b1 = rdlane(x, 32 + k);
                 bk = sub ? b1 : bk;
             }
-            if (bk != 0.0) {
-                const double u = unit ? bk : bk / rdlane(arow[k], k);
-                if (i == k) x = u;
-                const double al = -u;
-                if (al != 0.0) {
-                    if (LOWER) {
-                        const int nf = (t - k - 1) & ~3;
-                        if (act && i > k) x = (i - k - 1 < nf) ? fma(al, arow[k], x) : x + al * arow[k];
-                    } else {
-                        const int nf = k & ~3;
-                        if (act && i < k) x = (i < nf) ? fma(al, arow[k], x) : x + al * arow[k];
+            // reference: nothing happens for a zero entry; else x_k = b_k and the rows past
+            // k take -x_k times their coefficient, fused on the 4-aligned part
+            const bool nz = bk != 0.0;
+            const double al = -bk;
+            const double a = arow[k];
+            const int nf = LOWER ? ((t - k - 1) & ~3) : (k & ~3);
+            const bool fused = LOWER ? (i - k - 1 < nf) : (i < nf);
+            const double y = fused ? fma(al, a, x) : x + al * a;
+            const bool upd = nz && act && (LOWER ? i > k : i < k);
+            x = upd ? y : x;
+        }
+    } else {
+#pragma unroll
+        for (int kk = 0; kk < TW; kk++) {
+            const int k = LOWER ? kk : (TW - 1 - kk);
+            if (k < t) {
+                double bk = rdlane(x, k);
+                if (TWO) {
+                    const double b1 = rdlane(x, 32 + k);
+                    bk = sub ? b1 : bk;
+                }
+                if (bk != 0.0) {
+                    const double u = bk / rdlane(arow[k], k);
+                    if (i == k) x = u;
+                    const double al = -u;
+                    if (al != 0.0) {
+                        if (LOWER) {
+                            const int nf = (t - k - 1) & ~3;
+                            if (act && i > k) x = (i - k - 1 < nf) ? fma(al, arow[k], x) : x + al * arow[k];
+                        } else {
+                            const int nf = k & ~3;
+                            if (act && i < k) x = (i < nf) ? fma(al, arow[k], x) : x + al * arow[k];
+                        }
                     }
                 }
             }
@@ -179,12 +208,22 @@ void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size
     if (V == LNL || V == LNU) {
         const size_t waves = t <= 32 ? (nvec + 1) / 2 : nvec;
         const unsigned blocks = (unsigned)((waves + 3) / 4);
-        if (t <= 32)
-            hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 32>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec,
-                               a, lda, b, ldb, abort);
-        else
-            hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 64>), dim3(blocks), dim3(256), 0, st, unit, (int)t, (int)nvec,
-                               a, lda, b, ldb, abort);
+        const int tt = (int)t, nv = (int)nvec;
+        if (t <= 32) {
+            if (unit)
+                hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 32, true>), dim3(blocks), dim3(256), 0, st, tt, nv, a, lda,
+                                   b, ldb, abort);
+            else
+                hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 32, false>), dim3(blocks), dim3(256), 0, st, tt, nv, a,
+                                   lda, b, ldb, abort);
+        } else {
+            if (unit)
+                hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 64, true>), dim3(blocks), dim3(256), 0, st, tt, nv, a, lda,
+                                   b, ldb, abort);
+            else
+                hipLaunchKernelGGL((trsm_ln_lanes<V == LNL, 64, false>), dim3(blocks), dim3(256), 0, st, tt, nv, a,
+                                   lda, b, ldb, abort);
+        }
     } else {
         const unsigned blocks = (unsigned)((nvec + 63) / 64);
         hipLaunchKernelGGL((trsm_dot_regs<V>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda, b,

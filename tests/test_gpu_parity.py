@@ -276,6 +276,29 @@ def test_trsm_bitexact(B, O, side, uplo, trans, diag, size):
     same(b, ref, f"trsm {side}{uplo}{trans}{diag} {m}x{n}")
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsm_sparse_rhs_bitexact(B, O, uplo, diag):
+    # right-hand sides with most entries +0.0 or -0.0: the reference skips every step whose
+    # entry is zero (a -0.0 entry stays -0.0); the lane solves turn those skips into selects
+    m, n = 200, 41
+    rng = np.random.default_rng(5 + "LU".index(uplo) * 2 + "NU".index(diag))
+    lda = m + 1
+    A = rng.uniform(-1, 1, (m, m)) / m
+    np.fill_diagonal(A, rng.uniform(1, 2, m))
+    a = np.zeros(lda * m)
+    a.reshape(m, lda).T[:m] = A
+    b = rng.uniform(-1, 1, m * n)
+    pick = rng.random(m * n)
+    b[pick < 0.6] = 0.0
+    b[pick < 0.2] = -0.0
+    ref = b.copy()
+    O.trsm("L", uplo, "N", diag, m, n, 1.0, a, lda, ref, m)
+    B.trsm("L", uplo, "N", diag, m, n, 1.0, a, lda, b, m)
+    same(b, ref, f"trsm L{uplo}N{diag} sparse")
+    assert np.array_equal(np.signbit(b), np.signbit(ref))
+
+
 # ------------------------------------------------------------------ SYRK
 
 @pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T")])
