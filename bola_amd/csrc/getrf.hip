@@ -1124,8 +1124,10 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
                 const double b1 = rdlane_d(x[m2], 32 + k);
                 bk = sub ? b1 : bk;
                 const double al = -bk;
-                // a zero multiplier (bk == 0, so al == 0) leaves the column as it is
-                if (bk != 0.0 && al != 0.0 && i > k) x[m2] = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
+                // a zero multiplier (bk == 0, so al == 0) leaves the column as it is; a select,
+                // not a branch, since the two half-wavefronts' multipliers differ
+                const double y = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
+                x[m2] = (bk != 0.0 && i > k) ? y : x[m2];
             }
         }
 #pragma unroll
@@ -1181,7 +1183,9 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
                 const double b1 = rdlane_d(x[m2], 32 + k);
                 bk = sub ? b1 : bk;
                 const double al = -bk;
-                if (bk != 0.0 && al != 0.0 && i > k) x[m2] = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
+                // a zero multiplier leaves the column as it is (a select, as above)
+                const double y = (i - k - 1 < nf) ? fma(al, arow[k], x[m2]) : x[m2] + al * arow[k];
+                x[m2] = (bk != 0.0 && i > k) ? y : x[m2];
             }
         }
 #pragma unroll
