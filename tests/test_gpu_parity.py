@@ -559,9 +559,12 @@ def test_potrs_large_bitexact(B, O, uplo):
     same(b, ref, f"potrs {uplo} {n}")
 
 
-def test_getrs_large_bitexact(B, O):
+@pytest.mark.parametrize("nrhs", [64, 37])
+def test_getrs_large_bitexact(B, O, nrhs):
+    # nrhs = 37: the last nrhs % 4 columns of every update GEMM take the reference's unfused
+    # edge path, beside the 64-deep MFMA tiles of the recursive solve
     rng = np.random.default_rng(19)
-    n, nrhs = 2600, 64
+    n = 2600
     a = rng.uniform(-1, 1, n * n)
     ipiv = np.zeros(n, np.int32)
     assert O.getrf(n, n, a, n, ipiv)[0] == 0
@@ -569,7 +572,7 @@ def test_getrs_large_bitexact(B, O):
     ref = b.copy()
     O.getrs(n, nrhs, a, n, ipiv, ref, n)
     B.getrs(n, nrhs, a, n, ipiv, b, n)
-    same(b, ref, f"getrs {n}")
+    same(b, ref, f"getrs {n} nrhs {nrhs}")
 
 
 # ------------------------------------------------------------------ device-pointer flavour
