@@ -296,7 +296,7 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
         if (t128 >= 2L * num_cus)
             launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                          slab, mask, fast);
-        else if (deep && t64 < num_cus)
+        else if (deep && t64 < (fast ? 2L : 1L) * num_cus)  // stand-alone: also below 2 64x64 tiles per CU
             launch_tile<32, 32, TA, TB, 64>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                             slab, mask, fast);
         else if (t64 >= num_cus || m * (long)n <= 64L * 64)
