@@ -326,20 +326,26 @@ def mixed_section(B, torch, n=16384, kappa=1e2):
 
 def serving_section(B, torch, nmax=600):
     """Solve server under Poisson arrivals: orders log-uniform in [32, nmax] (quantised to 16
-    sizes), 70 % LU / 30 % SPD, nrhs uniform in [1, 16], 30 % of requests reuse a resident
-    factorization with a new right-hand side; host matrices (uploaded by the server), 4 slots,
-    16 GiB budget.  Two loads: 200 requests/s for 5 s (latency) and 2000/s for 2 s
+    sizes), 70 % LU (A = U diag(sigma) V^T, kappa log-uniform in [1e2, 1e6]) / 30 % SPD,
+    nrhs uniform in [1, 64], 30 % of requests reuse a resident factorization with a new
+    right-hand side; host matrices (uploaded by the server), 4 slots, 16 GiB budget.  Two loads: 200 requests/s for 5 s (latency) and 2000/s for 2 s
     (throughput).  The full [32, 8192] range of the serving configuration is not run."""
     import math
+    from bola_amd import testmat
     rng = np.random.default_rng(1)
     sizes = np.unique(np.round(np.exp(np.linspace(math.log(32), math.log(nmax), 16))).astype(int))
     pool = []
     for n in sizes:
         for kind in (B.SERVE_LU, B.SERVE_CHOL):
-            G = rng.uniform(-1, 1, (n, n))
-            A = G @ G.T + n * np.eye(n) if kind == B.SERVE_CHOL else G
-            pool.append((int(n), kind, np.asfortranarray(A).ravel(order="F").copy()))
-    out = {"workload": f"Poisson arrivals, n log-uniform [32,{nmax}], 70% LU / 30% SPD, nrhs uniform [1,16], "
+            if kind == B.SERVE_CHOL:
+                G = rng.uniform(-1, 1, (n, n))
+                A = np.asfortranarray(G @ G.T + n * np.eye(n)).ravel(order="F").copy()
+            else:
+                kappa = float(np.exp(rng.uniform(math.log(1e2), math.log(1e6))))
+                A = np.ascontiguousarray(testmat.conditioned(int(n), kappa, int(n)))
+            pool.append((int(n), kind, A))
+    out = {"workload": f"Poisson arrivals, n log-uniform [32,{nmax}], 70% LU (kappa log-uniform [1e2,1e6]) / "
+                       f"30% SPD, nrhs uniform [1,64], "
                        f"30% reuse a resident factorization, FP64, 4 slots, host operands"}
     with B.Server(16 << 30, 4) as s:
         for (n, kind, A) in pool:  # warm-up, one at a time
@@ -353,7 +359,7 @@ def serving_section(B, torch, nmax=600):
                 n = sizes[np.argmin(np.abs(np.log(sizes) - math.log(ln[k])))]
                 kind = B.SERVE_LU if rng.random() < 0.7 else B.SERVE_CHOL
                 idx = [j for j, p in enumerate(pool) if p[0] == n and p[1] == kind][0]
-                nrhs = int(rng.integers(1, 17))
+                nrhs = int(rng.integers(1, 65))
                 reqs.append((arr[k], idx, nrhs, rng.random() < 0.3, rng.uniform(-1, 1, n * nrhs)))
             s.reset_stats()
             live = []
