@@ -74,8 +74,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
-                                   "this build (the factorization shards by column blocks, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; it has no multi-GPU mode (across "
+                                   "GPUs the factorization would shard by block-cyclic column blocks)"}))
         sys.exit(2)
 
     import torch

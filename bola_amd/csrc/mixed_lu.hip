@@ -149,12 +149,12 @@ __device__ __forceinline__ int wg_tournament(const int* set, int base, int rows,
     return steps;
 }
 
-constexpr int GROWS = 1024;  // rows per workgroup in the first round (4 per thread)
+constexpr int GROWS = 1024;  // rows per workgroup in the tournament's first stage (4 per thread)
 constexpr int GFAN = 16;     // tree fan-in: 16 proposals of 32 rows, 2 per thread
 
 // lists: [level][node][LPW] absolute rows and counts; cnt: [level][node] arrival counters
 // (zero on entry; the root clears them for the next panel).  G workgroups of 256 threads:
-// round 1 on GROWS rows each, then a tree with fan-in GFAN where the last workgroup of a
+// the tournament's first stage on GROWS rows each, then a tree with fan-in GFAN where the last workgroup of a
 // group to arrive merges the group's proposals.  Nobody waits for anybody.  The root
 // writes the panel's interchanges ipiv[r0 .. r0+k).
 __global__ void __launch_bounds__(256) calu_wg(int n, int r0, int c0, int w, const float* __restrict__ T, size_t ldt,
