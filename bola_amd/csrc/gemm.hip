@@ -275,9 +275,9 @@ void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     if (fast)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : 3), BK>), dim3(tm * tn),
-                           dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n,
-                           abort, slab, mask);
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : BK == 32 ? 4 : 3), BK>),
+                           dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn,
+                           edge_m, edge_n, abort, slab, mask);
     else
         hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A,
                            lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
@@ -296,7 +296,10 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
         if (t128 >= 2L * num_cus)
             launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                          slab, mask, fast);
-        else if (deep && t64 < (fast ? 2L : 1L) * num_cus)  // stand-alone: also below 2 64x64 tiles per CU
+        else if (deep && fast && t64 < 2L * num_cus)  // stand-alone: below 2 64x64 tiles per CU
+            launch_tile<32, 32, TA, TB, 32>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
+                                            slab, mask, fast);
+        else if (deep && t64 < num_cus)
             launch_tile<32, 32, TA, TB, 64>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                             slab, mask, fast);
         else if (t64 >= num_cus || m * (long)n <= 64L * 64)
