@@ -15,7 +15,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m, size_t n, double* a, size_t lda, 
 void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t lda, int32_t* info,
                  bool recursive_entry);
 void launch_dsyrk(hipStream_t st, char uplo, char trans, size_t n, size_t k, double alpha, const double* a, size_t lda,
-                  double beta, double* c, size_t ldc, const int32_t* abort);
+                  double beta, double* c, size_t ldc, Abort abort);
 
 namespace {
 

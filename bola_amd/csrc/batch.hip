@@ -616,6 +616,12 @@ int check_descs(bola_hip_handle_s* h, size_t count, const Desc* desc) {
     return 0;
 }
 
+bool any_lu(size_t count, const Desc* desc) {
+    for (size_t i = 0; i < count; i++)
+        if (desc[i].kind == BOLA_HIP_BATCH_LU) return true;
+    return false;
+}
+
 std::vector<int> largest_first(size_t count, const Desc* desc) {
     std::vector<int> ord(count);
     std::iota(ord.begin(), ord.end(), 0);
@@ -701,6 +707,10 @@ int bola_hip_batch_solve_dev(bola_hip_handle_t h, int variant, size_t count, con
         return -4;
     }
     if (int r = check_descs(h, count, desc)) return r;
+    if (!piv_arena && any_lu(count, desc)) {
+        set_error(h, "Argument 6 to batch_solve had an illegal value (LU systems need a pivot arena)");
+        return -6;
+    }
     bola_hip_batch_config c{16, 2};
     if (cfg) {
         if (!valid_config(*cfg)) {
@@ -834,6 +844,10 @@ int bola_hip_batch_graph_create(bola_hip_handle_t h, int variant, size_t count, 
         return -3;
     }
     if (int r = check_descs(h, count, desc)) return r;
+    if (!piv_arena && any_lu(count, desc)) {
+        set_error(h, "Argument 6 to batch_graph_create had an illegal value (LU systems need a pivot arena)");
+        return -6;
+    }
     bola_hip_batch_config c{16, 2};
     if (cfg) {
         if (!valid_config(*cfg)) {

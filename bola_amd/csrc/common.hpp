@@ -55,6 +55,56 @@ struct Workspace {
     }
 };
 
+// Where work queued behind a factorization stops when a pivot fails.
+//
+// `word` is the factorization's status word: 0, a global 1-based failing column (a zero
+// pivot found by the panel of 32 columns starting at 32*floor((v-1)/32)), or negative
+// (a device error: everything stops).  An operation names the column `col` of the first
+// panel whose results it applies.  It runs only for panels before the failing one, so
+// that a factorization stopped by a zero pivot leaves the matrix in the reference's
+// state however its work was scheduled (getrf.rs:53-98 returns at the failing panel).
+//   kcols == 0: all or nothing -- the operation applies the panel at `col` only;
+//   kcols == 1: it applies consecutive panels from `col` on (k of a trailing GEMM counts
+//               their columns, a list of row maps holds one per panel) and keeps those
+//               before the failing panel.
+// Converting a plain pointer gives "stop on any failure" (col beyond every matrix),
+// which is what sequential factorizations (Cholesky, the recursive LU) need.
+constexpr int kAbortAny = 1 << 30;
+struct Abort {
+    const int32_t* word = nullptr;
+    int col = kAbortAny;
+    int kcols = 0;
+    Abort() = default;
+    __host__ __device__ Abort(const int32_t* w) : word(w) {}  // NOLINT: implicit on purpose
+    __host__ __device__ static Abort panel(const int32_t* w, int c) {
+        Abort a(w);
+        a.col = c;
+        return a;
+    }
+    __host__ __device__ static Abort span(const int32_t* w, int c) {
+        Abort a(w);
+        a.col = c;
+        a.kcols = 1;
+        return a;
+    }
+};
+// columns completed before the failure (everything when there is none)
+__device__ __forceinline__ long abort_done(const Abort& a) {
+    if (!a.word) return 1L << 62;
+    const int v = __hip_atomic_load(a.word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v == 0) return 1L << 62;
+    if (v < 0) return -(1L << 62);
+    return 32L * ((v - 1) / 32);
+}
+__device__ __forceinline__ bool abort_stop(const Abort& a) { return abort_done(a) <= (long)a.col; }
+// how many of `count` units (k columns, or 32-column panels when `per` = 32) may run
+__device__ __forceinline__ int abort_count(const Abort& a, int count, int per) {
+    const long done = abort_done(a);
+    if (!a.kcols) return done > (long)a.col ? count : 0;
+    const long c = (done - (long)a.col) / per;
+    return (int)(c < 0 ? 0 : (c > count ? count : c));
+}
+
 // A composed row permutation: row `row[s]` receives the old contents of row `src[s]`.
 // Produced by the LASWP compose step (one map per group of up to 64 interchanges) and by
 // the LU panel kernel (one map per panel).
@@ -197,20 +247,20 @@ void launch_scale_tri(hipStream_t st, char uplo, double* a, size_t lda, size_t n
 // laswp.hip
 size_t laswp_ws_bytes(size_t npiv);
 void launch_laswp(hipStream_t st, size_t n, double* a, size_t lda, int32_t k1, int32_t k2, const int32_t* ipiv,
-                  int32_t incx, void* ws, const int32_t* abort = nullptr);
+                  int32_t incx, void* ws, Abort abort = Abort());
 // Applies `nmaps` row maps in order to the columns [0, left) and [skip_to, skip_to + right)
-// of a.  When fix_ipiv is non-null, block 0 also adds fix_off to fix_ipiv[0..fix_cnt) --
-// both only when *abort is zero.
+// of a.  When fix_ipiv is non-null, block 0 also adds fix_off to fix_ipiv[0..fix_cnt).
+// Maps stop as `abort` says (one map per 32-column panel when it counts panels); the
+// pivot fix only when the first map is applied.
 void launch_rowmap_apply(hipStream_t st, double* a, size_t lda, size_t left, size_t skip_to, size_t right,
-                         const RowMap* maps, int nmaps, const int32_t* abort, int32_t* fix_ipiv, int fix_cnt,
-                         int fix_off);
+                         const RowMap* maps, int nmaps, Abort abort, int32_t* fix_ipiv, int fix_cnt, int fix_off);
 
 // gemm.hip
-// `abort`: optional device status word; when it is nonzero at kernel start the launch does nothing
-// (used inside factorizations so that work queued behind a failed pivot is skipped).
+// `abort`: read at kernel start; inside factorizations work queued behind a failed pivot
+// is skipped (all of it, or with Abort::span the k columns of panels from the failing one on).
 void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                  const int32_t* abort = nullptr, bool fast = false);
+                  Abort abort = Abort(), bool fast = false);
 // slab: depth of one accumulate-then-fold step (64 = gemm.rs KC; 32 = the syrk.rs block
 // loop, or one LU panel per step); mask: 0 all of C, 1 only rows in a later 32-block than
 // the column, 2 earlier; edge_m / edge_n: first row / column (relative to C) that takes
@@ -219,11 +269,11 @@ void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, 
 // leaves more of the machine to a factorization's panel running beside it.
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
+                     Abort abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
 void set_num_cus(int n);
 
 // trsm.hip
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
-                  const double* a, size_t lda, double* b, size_t ldb, const int32_t* abort = nullptr);
+                  const double* a, size_t lda, double* b, size_t ldb, Abort abort = Abort());
 
 }  // namespace bola

@@ -65,10 +65,12 @@ template <int BM, int BN, bool TA, bool TB, int OCC, int BK>
 __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
-                                                  int edge_m, int edge_n, const int32_t* __restrict__ abort,
-                                                  int slab, int mask) {
+                                                  int edge_m, int edge_n, Abort abort, int slab, int mask) {
     using Cfg = GemmCfg<BM, BN, TA, TB, BK>;
-    if (abort && *abort != 0) return;
+    if (k > 0) {  // behind a failed pivot: only the k columns of panels before it
+        k = abort_count(abort, k, 1);
+        if (k == 0) return;
+    }
     __shared__ double sA[2][Cfg::A_SIZE];
     __shared__ double sB[2][Cfg::B_SIZE];
 
@@ -239,8 +241,11 @@ template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int edge_m, int edge_n,
-                                                  const int32_t* __restrict__ abort, int mask) {
-    if (abort && *abort != 0) return;
+                                                  Abort abort, int mask) {
+    if (k > 0) {
+        k = abort_count(abort, k, 1);
+        if (k == 0) return;
+    }
     // enumerate edge elements: first the edge rows for all columns, then the edge columns
     // for the interior rows
     const int er = m - edge_m, ec = n - edge_n;
@@ -271,7 +276,7 @@ __global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double al
 
 template <int BM, int BN, bool TA, bool TB, int BK = 16>
 void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
-                 size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, const int32_t* abort,
+                 size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, Abort abort,
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
     if (fast)
@@ -286,7 +291,7 @@ void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double
 
 template <bool TA, bool TB>
 void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
-              size_t ldb, double beta, double* C, size_t ldc, int num_cus, const int32_t* abort, int slab,
+              size_t ldb, double beta, double* C, size_t ldc, int num_cus, Abort abort, int slab,
               int mask, int edge_m, int edge_n, bool fast) {
     if (edge_m > 0 && edge_n > 0) {
         const long t128 = (long)((m + 127) / 128) * ((n + 63) / 64);
@@ -327,13 +332,13 @@ void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
 // Requires m, n, k < 2^31.  alpha == 0 must be handled by the caller (beta-only pass).
 void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                   const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                  const int32_t* abort, bool fast) {
+                  Abort abort, bool fast) {
     launch_dgemm_ex(st, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, abort, KC, 0, -1, -1, fast);
 }
 
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     const int32_t* abort, int slab, int mask, long edge_m, long edge_n, bool fast) {
+                     Abort abort, int slab, int mask, long edge_m, long edge_n, bool fast) {
     bool ta = !(transa == 'N' || transa == 'n');
     bool tb = !(transb == 'N' || transb == 'n');
     int M = (int)m, N = (int)n, K = (int)k;

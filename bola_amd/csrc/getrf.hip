@@ -367,20 +367,24 @@ __global__ void __launch_bounds__(PT) getf2_panel(int m, int w, double* __restri
             int ok = 1;
             for (int q = tid; q < nwg; q += PT) {
                 long sp = 0;
+                // a zero pivot (info > 0) is no reason to stop: the flags workgroup 0 saw
+                // reach every workgroup, which then stops on the same pivot
                 while (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j + 1) {
-                    if (++sp > spin_limit ||
-                        __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    if (++sp > spin_limit) {  // timed out: report it unless a status is set
+                        int zero = 0;
+                        __hip_atomic_compare_exchange_strong(info, &zero, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
                         ok = 0;
                         break;
                     }
                 }
             }
             ok = __syncthreads_and(ok);
-            if (!ok) {
-                if (g == 0 && tid == 0)
-                    __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                return;
-            }
+            if (!ok) return;
             const PanelSlot* base = slots + (size_t)par * nwg * 2;
             if (fast) {
                 // one read round: every candidate (key, multiplier, row) and the row at position j;
@@ -703,6 +707,17 @@ __device__ __forceinline__ void wave_ger_rest(double (&a)[R][WW], double (&al)[R
 
 __device__ __forceinline__ uint64_t join32(uint64_t lo, uint64_t hi) { return (lo & 0xffffffffull) | (hi << 32); }
 
+// A wavefront stops waiting: when its own wait timed out (some lane's `timed`) it reports
+// a device error, unless a status is already set -- a zero pivot's column is never
+// overwritten.  A wavefront that stopped because another one gave up reports nothing.
+__device__ __forceinline__ void give_up(int32_t* info, int timed, int lane) {
+    if (__any(timed) && lane == 0) {
+        int zero = 0;
+        __hip_atomic_compare_exchange_strong(info, &zero, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // slots: [2 parities][nwv][2 candidates][TW] tagged words, cur: [2 parities][TW]; a step's
 // epoch is col_off + j + 1, unique within a factorization whose slots start zeroed.
 template <int R, bool MULTI>
@@ -849,7 +864,7 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
         uint64_t pf = 0;
         long rel;
         if (MULTI) {
-            int ok = 1;
+            int ok = 1, timed = 0;
             const uint64_t* base = slots + (size_t)par * nwv * 2 * TW;
             Cand gv{-1.0, NOKEY}, gt{-1.0, NOKEY};
             int gvs = -1, gts = -1;
@@ -878,11 +893,10 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
 #pragma unroll
                         for (int i = 0; i < 8; i++) good = good && (uint32_t)(t[h][i] >> 32) == ep;
                     if (good) break;
-                    if (++sp > spin_limit ||
-                        __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                        ok = 0;
-                        break;
-                    }
+                    // a zero pivot found by wavefront 0 (info > 0) is no reason to stop: the
+                    // step's words it saw reach every wavefront, which then stops on it too
+                    if (++sp > spin_limit) { ok = 0; timed = 1; break; }
+                    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
                 }
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
@@ -897,7 +911,7 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                 }
             }
             if (!__all(ok)) {
-                if (lane == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                give_up(info, timed, lane);
                 return;
             }
             int lv, lt;
@@ -922,13 +936,11 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
                                   (uint32_t)(x0 >> 32) == ep && (uint32_t)(x1 >> 32) == ep &&
                                   (uint32_t)(f0 >> 32) == ep && (uint32_t)(f1 >> 32) == ep;
                 if (good) break;
-                if (++sp > spin_limit || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = 0;
-                    break;
-                }
+                if (++sp > spin_limit) { ok = 0; timed = 1; break; }
+                if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
             }
             if (!__all(ok)) {
-                if (lane == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                give_up(info, timed, lane);
                 return;
             }
             if (inw) v = __longlong_as_double((long long)join32(r0, r1));
@@ -1044,9 +1056,11 @@ constexpr int UR_W = 8;    // columns per workgroup of urows_fused: one per half
 typedef double ur_d4 __attribute__((ext_vector_type(4)));
 template <int CB>
 __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restrict__ L, size_t lda,
-                                                   double* __restrict__ Bm, int ncols,
-                                                   const int32_t* __restrict__ abort) {
-    if (abort && *abort != 0) return;
+                                                   double* __restrict__ Bm, int ncols, Abort abort) {
+    // behind a zero pivot in panel GA of the super-panel: panels before it are applied in
+    // full, the rows of the later ones only receive those panels' GEMM updates
+    const int GA = abort_count(abort, G, PW);
+    if (GA == 0) return;
     __shared__ double sb[8 * PW][CB + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int cb0 = blockIdx.x * CB;
@@ -1066,8 +1080,9 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
     const int i = lane & 31, sub = lane >> 5;
     for (int q = 0; q < G; q++) {
         const int j0 = q * PW;
+        const int qs = min(q, GA);  // earlier panels whose GEMM these rows receive
         ur_d4 crun = {0.0, 0.0, 0.0, 0.0};
-        if (q > 0 && gemm_wave) {
+        if (qs > 0 && gemm_wave) {
             ur_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int r = 0; r < 4; r++) crun[r] = sb[j0 + wm + lk + 4 * r][gc];
@@ -1078,10 +1093,10 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
             for (int sl = 0; sl < 7; sl++)
 #pragma unroll
                 for (int t = 0; t < PW / 4; t++)
-                    fa[sl][t] = Lr[(size_t)(min(sl, q - 1) * PW + 4 * t + lk) * lda];  // clamped; slabs >= q unused
+                    fa[sl][t] = Lr[(size_t)(min(sl, qs - 1) * PW + 4 * t + lk) * lda];  // clamped; slabs >= qs unused
 #pragma unroll
             for (int sl = 0; sl < 7; sl++) {  // one 32-deep slab per earlier panel
-                if (sl < q) {
+                if (sl < qs) {
 #pragma unroll
                     for (int t = 0; t < PW / 4; t++)
                         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[sl][t], sb[sl * PW + 4 * t + lk][gc], acc, 0,
@@ -1094,13 +1109,14 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
                 }
             }
         }
-        if (q > 0) {
+        if (qs > 0) {
             __syncthreads();  // every wavefront has read the rows it multiplies
             if (gemm_wave && gcol)
 #pragma unroll
                 for (int r = 0; r < 4; r++) sb[j0 + wm + lk + 4 * r][gc] = crun[r];
             __syncthreads();
         }
+        if (q >= GA) continue;  // (uniform over the workgroup)
         // TRSM of rows j0 .. j0+31 with the panel's unit-lower L11: lane i holds row i of
         // L11; each wavefront solves its CB / 4 columns together, CB / 8 per half, whose
         // eliminations are independent and interleave
@@ -1157,8 +1173,8 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
                                                           double* __restrict__ U, const double* __restrict__ L21,
                                                           double* __restrict__ C, size_t lda,
                                                           int* __restrict__ flags, long spin_limit,
-                                                          int32_t* __restrict__ info) {
-    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+                                                          int32_t* __restrict__ info, int col) {
+    if (abort_stop(Abort::panel(info, col))) return;
     const int ct = blockIdx.x / (nrt + PSU_S), rt = blockIdx.x % (nrt + PSU_S);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int c0 = ct * UR_CB;
@@ -1214,21 +1230,23 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
     __syncthreads();
     if (tid < PSU_S) {  // one thread per solver flag
         long sp = 0;
-        int ok = 1;
         while (__hip_atomic_load(flags + ct * PSU_S + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            if (++sp > spin_limit || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                ok = 0;
+            if (++sp > spin_limit) {  // this wait timed out: report it unless a status is set
+                int zero = 0;
+                __hip_atomic_compare_exchange_strong(info, &zero, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok_sh = 0;
+                break;
+            }
+            if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {  // another gave up
+                ok_sh = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
         }
-        if (!ok) ok_sh = 0;
     }
     __syncthreads();
-    if (!ok_sh) {
-        if (tid == 0) __hip_atomic_store(info, BOLA_HIP_ERR_DEVICE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-    }
+    if (!ok_sh) return;
 #pragma unroll
     for (int b = 0; b < 2; b++) {
         const int col = min(c0 + 16 * b + lr, nr - 1);
@@ -1526,7 +1544,7 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
     double* B = a + (size_t)c0 * lda;
     {
         Phase pl(h, "LASWP", 0.0, st);
-        launch_rowmap_apply(st, B, lda, (size_t)ncols, 0, 0, w.maps + p0, G, info, nullptr, 0, 0);
+        launch_rowmap_apply(st, B, lda, (size_t)ncols, 0, 0, w.maps + p0, G, Abort::span(info, s0), nullptr, 0, 0);
     }
     const bool uniform = (sw % PW) == 0;
     if (uniform) {
@@ -1536,7 +1554,7 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
             // one launch, a workgroup per 8 columns
             Phase pt(h, "TRSM", (double)sw * sw * ncols, st);
             hipLaunchKernelGGL(urows_fused<UR_W>, dim3((unsigned)((ncols + UR_W - 1) / UR_W)), dim3(256), 0, st, G,
-                               a + s0 + (size_t)s0 * lda, lda, a + s0 + (size_t)c0 * lda, ncols, info);
+                               a + s0 + (size_t)s0 * lda, lda, a + s0 + (size_t)c0 * lda, ncols, Abort::span(info, s0));
             BOLA_LAUNCH_CHECK();
         } else
         for (int q = 0; q < G; q++) {
@@ -1544,17 +1562,17 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
             if (q > 0) {
                 Phase pg(h, "GEMM", 2.0 * PW * ncols * (q * PW), st);
                 launch_dgemm_ex(st, 'N', 'N', PW, ncols, (size_t)q * PW, -1.0, a + j + (size_t)s0 * lda, lda,
-                                a + s0 + (size_t)c0 * lda, lda, 1.0, a + j + (size_t)c0 * lda, lda, info, PW, 0,
+                                a + s0 + (size_t)c0 * lda, lda, 1.0, a + j + (size_t)c0 * lda, lda, Abort::span(info, s0), PW, 0,
                                 e.row - j, e.col - c0);
             }
             Phase pt(h, "TRSM", (double)PW * PW * ncols, st);
             launch_dtrsm(st, 'L', 'L', 'N', 'U', PW, ncols, 1.0, a + j + (size_t)j * lda, lda, a + j + (size_t)c0 * lda,
-                         lda, info);
+                         lda, Abort::panel(info, j));
         }
         if (s0 + sw < m) {
             Phase pg(h, "GEMM", 2.0 * (double)(m - s0 - sw) * ncols * sw, st);
             launch_dgemm_ex(st, 'N', 'N', (size_t)(m - s0 - sw), ncols, (size_t)sw, -1.0, a + (s0 + sw) + (size_t)s0 * lda,
-                            lda, a + s0 + (size_t)c0 * lda, lda, 1.0, a + (s0 + sw) + (size_t)c0 * lda, lda, info, PW, 0,
+                            lda, a + s0 + (size_t)c0 * lda, lda, 1.0, a + (s0 + sw) + (size_t)c0 * lda, lda, Abort::span(info, s0), PW, 0,
                             e.row - (s0 + sw), e.col - c0);
         }
     } else {
@@ -1564,14 +1582,14 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
             {
                 Phase pt(h, "TRSM", (double)jb * jb * ncols, st);
                 launch_dtrsm(st, 'L', 'L', 'N', 'U', jb, ncols, 1.0, a + j + (size_t)j * lda, lda,
-                             a + j + (size_t)c0 * lda, lda, info);
+                             a + j + (size_t)c0 * lda, lda, Abort::panel(info, j));
             }
             if (j + jb < m) {
                 const Edge e = panel_edge(m, n, j, jb);
                 Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * ncols * jb, st);
                 launch_dgemm_ex(st, 'N', 'N', (size_t)(m - j - jb), ncols, (size_t)jb, -1.0,
                                 a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)c0 * lda, lda, 1.0,
-                                a + (j + jb) + (size_t)c0 * lda, lda, info, PW, 0, e.row - (j + jb), e.col - c0);
+                                a + (j + jb) + (size_t)c0 * lda, lda, Abort::panel(info, j), PW, 0, e.row - (j + jb), e.col - c0);
             }
         }
     }
@@ -1596,7 +1614,7 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
         {
             Phase pl(h, "LASWP", 0.0, st);
             launch_rowmap_apply(st, a + (size_t)s0 * lda, lda, (size_t)(j - s0), (size_t)(j + jb - s0),
-                                (size_t)(s0 + sw - j - jb), w.maps + p, 1, info, ipiv + j, jb, j);
+                                (size_t)(s0 + sw - j - jb), w.maps + p, 1, Abort::panel(info, j), ipiv + j, jb, j);
         }
         const int nr = s0 + sw - j - jb;
         const Edge ef = panel_edge(m, n, j, jb);
@@ -1607,20 +1625,20 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
             hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + PSU_S))), dim3(256), 0, st, mb, nr,
                                nrt, a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda,
                                a + (j + jb) + (size_t)j * lda, a + (j + jb) + (size_t)(j + jb) * lda, lda,
-                               w.uflags + (size_t)p * 32, kSpinLimit, info);
+                               w.uflags + (size_t)p * 32, kSpinLimit, info, j);
             BOLA_LAUNCH_CHECK();
         } else if (nr > 0) {
             {
                 Phase pt(h, "TRSM", (double)jb * jb * nr, st);
                 launch_dtrsm(st, 'L', 'L', 'N', 'U', jb, nr, 1.0, a + j + (size_t)j * lda, lda,
-                             a + j + (size_t)(j + jb) * lda, lda, info);
+                             a + j + (size_t)(j + jb) * lda, lda, Abort::panel(info, j));
             }
             if (j + jb < m) {
                 const Edge e = panel_edge(m, n, j, jb);
                 Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * nr * jb, st);
                 launch_dgemm_ex(st, 'N', 'N', (size_t)(m - j - jb), (size_t)nr, (size_t)jb, -1.0,
                                 a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, 1.0,
-                                a + (j + jb) + (size_t)(j + jb) * lda, lda, info, PW, 0, e.row - (j + jb),
+                                a + (j + jb) + (size_t)(j + jb) * lda, lda, Abort::panel(info, j), PW, 0, e.row - (j + jb),
                                 e.col - (j + jb));
             }
         }
@@ -1682,20 +1700,20 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
             {
                 Phase pl(h, "LASWP", 0.0, st);
                 const size_t right = (j + jb < n) ? (size_t)(n - j - jb) : 0;
-                launch_rowmap_apply(st, a, lda, (size_t)j, (size_t)(j + jb), right, w.maps + p, 1, info, ipiv + j, jb,
+                launch_rowmap_apply(st, a, lda, (size_t)j, (size_t)(j + jb), right, w.maps + p, 1, Abort::panel(info, j), ipiv + j, jb,
                                     j);
             }
             if (j + jb < n) {
                 {
                     Phase pt(h, "TRSM", (double)jb * jb * (n - j - jb), st);
                     launch_dtrsm(st, 'L', 'L', 'N', 'U', (size_t)jb, (size_t)(n - j - jb), 1.0,
-                                 a + j + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, info);
+                                 a + j + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, Abort::panel(info, j));
                 }
                 if (j + jb < m) {
                     Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * (n - j - jb) * jb, st);
                     launch_dgemm(st, 'N', 'N', (size_t)(m - j - jb), (size_t)(n - j - jb), (size_t)jb, -1.0,
                                  a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)(j + jb) * lda, lda, 1.0,
-                                 a + (j + jb) + (size_t)(j + jb) * lda, lda, info);
+                                 a + (j + jb) + (size_t)(j + jb) * lda, lda, Abort::panel(info, j));
                 }
             }
         }
@@ -1723,7 +1741,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
             BOLA_CHECK(hipStreamWaitEvent(s3, fact_done, 0));
             if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(s3, bulk_done, 0));
             Phase pl(h, "LASWP", 0.0, s3);
-            launch_rowmap_apply(s3, a, lda, (size_t)s0, 0, 0, w.maps + s0 / PW, (sw + PW - 1) / PW, info, nullptr, 0, 0);
+            launch_rowmap_apply(s3, a, lda, (size_t)s0, 0, 0, w.maps + s0 / PW, (sw + PW - 1) / PW, Abort::span(info, s0), nullptr, 0, 0);
             left_done = lu_event(h, ev++);
             BOLA_CHECK(hipEventRecord(left_done, s3));
         }

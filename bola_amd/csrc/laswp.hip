@@ -74,9 +74,11 @@ __global__ void __launch_bounds__(64) laswp_compose(int32_t k1, int32_t k2, cons
 // Each wavefront owns whole columns; per group: load all mapped sources, then store.
 __global__ void __launch_bounds__(256) laswp_apply(size_t left, size_t skip_to, size_t right, double* __restrict__ a,
                                                    size_t lda, int ngroups, const GroupMap* __restrict__ maps,
-                                                   const int32_t* __restrict__ abort, int32_t* __restrict__ fix_ipiv,
+                                                   Abort abort, int32_t* __restrict__ fix_ipiv,
                                                    int fix_cnt, int fix_off) {
-    if (abort && *abort != 0) return;
+    // behind a failed pivot: only the maps of panels before it (one map per panel)
+    ngroups = abort_count(abort, ngroups, 32);
+    if (ngroups == 0) return;
     if (fix_ipiv && blockIdx.x == 0 && (int)threadIdx.x < fix_cnt) fix_ipiv[threadIdx.x] += fix_off;
     const int lane = threadIdx.x & 63;
     const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -108,7 +110,7 @@ size_t laswp_ws_bytes(size_t npiv) {
 }
 
 void launch_rowmap_apply(hipStream_t st, double* a, size_t lda, size_t left, size_t skip_to, size_t right,
-                         const RowMap* maps, int nmaps, const int32_t* abort, int32_t* fix_ipiv, int fix_cnt,
+                         const RowMap* maps, int nmaps, Abort abort, int32_t* fix_ipiv, int fix_cnt,
                          int fix_off) {
     size_t waves = left + right;
     unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((waves + 3) / 4, 2048));
@@ -118,7 +120,7 @@ void launch_rowmap_apply(hipStream_t st, double* a, size_t lda, size_t left, siz
 }
 
 void launch_laswp(hipStream_t st, size_t n, double* a, size_t lda, int32_t k1, int32_t k2, const int32_t* ipiv,
-                  int32_t incx, void* ws, const int32_t* abort) {
+                  int32_t incx, void* ws, Abort abort) {
     if (n == 0 || incx == 0 || k1 > k2) return;
     size_t npiv = (size_t)(k2 - k1) + 1;
     int ngroups = (int)((npiv + GROUP - 1) / GROUP);

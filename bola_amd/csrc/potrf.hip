@@ -28,8 +28,8 @@ constexpr int SB = 32;  // syrk.rs:9 NB, potrf.rs:8 NB
 // ---- diagonal blocks of SYRK, trans 'N': C[i,j] += sum_l (alpha*A[j,l]) * A[i,l]
 // Each thread owns one element of one diagonal block and walks l = 0..k-1.
 __global__ void syrk_diag_n(int upper, int n, int k, double alpha, const double* __restrict__ A, size_t lda,
-                            double* __restrict__ C, size_t ldc, const int32_t* __restrict__ abort) {
-    if (abort && *abort != 0) return;
+                            double* __restrict__ C, size_t ldc, Abort abort) {
+    if (abort_stop(abort)) return;
     const int blk = blockIdx.x;
     const int j0 = blk * SB;
     const int nb = min(SB, n - j0);
@@ -77,8 +77,8 @@ __device__ __forceinline__ double bola_dot(int len, const double* x, const doubl
 }
 
 __global__ void syrk_diag_t(int upper, int n, int k, double alpha, const double* __restrict__ A, size_t lda,
-                            double* __restrict__ C, size_t ldc, const int32_t* __restrict__ abort) {
-    if (abort && *abort != 0) return;
+                            double* __restrict__ C, size_t ldc, Abort abort) {
+    if (abort_stop(abort)) return;
     const int blk = blockIdx.x;
     const int j0 = blk * SB;
     const int nb = min(SB, n - j0);
@@ -227,7 +227,7 @@ __global__ void k_zero2(int32_t* p) {
 }  // namespace
 
 void launch_dsyrk(hipStream_t st, char uplo, char trans, size_t n, size_t k, double alpha, const double* a, size_t lda,
-                  double beta, double* c, size_t ldc, const int32_t* abort) {
+                  double beta, double* c, size_t ldc, Abort abort) {
     if (n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return;
     const bool upper = (uplo == 'U');
     if (beta != 1.0) launch_scale_tri(st, upper ? 'U' : 'L', c, ldc, n, beta);

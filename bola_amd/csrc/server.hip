@@ -11,6 +11,7 @@
 // starved.  The slots' work overlaps on the device; each request's result is the same
 // as when it runs alone, since every slot runs the same deterministic kernels.
 #include "common.hpp"
+#include "serve_cache.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -38,17 +39,18 @@ struct Ticket {
     int passed = 0;  // times admission passed over it while it was first in line
 };
 
-struct Factor {
-    int kind = 0;
-    size_t n = 0;
-    double* f = nullptr;
-    int32_t* ipiv = nullptr;
-    size_t bytes = 0;
-    int users = 0;
-    uint64_t last_use = 0;
-};
+// The stream of the worker thread that releases device memory (null on other threads).
+thread_local hipStream_t tl_stream = nullptr;
 
-size_t factor_bytes(size_t n) { return n * n * 8 + ((n * 4 + 255) & ~(size_t)255); }
+void release_device(void* f, void* ipiv) {
+    if (tl_stream) {
+        if (f) (void)hipFreeAsync(f, tl_stream);
+        if (ipiv) (void)hipFreeAsync(ipiv, tl_stream);
+    } else {
+        if (f) (void)hipFree(f);
+        if (ipiv) (void)hipFree(ipiv);
+    }
+}
 
 }  // namespace
 }  // namespace bola
@@ -57,7 +59,6 @@ using namespace bola;
 
 struct bola_hip_server_s {
     int device = 0;
-    size_t budget = 0, in_use = 0, peak = 0;
     std::mutex mu;
     // One LU at a time above kBigLu: its panel kernel's workgroups (up to 128 of 35 KB of
     // LDS each, or 512 of 70 KB above 8192 rows) must all be resident at once, which
@@ -68,53 +69,25 @@ struct bola_hip_server_s {
     std::condition_variable cv_work, cv_done;
     std::deque<Ticket*> queue;
     std::unordered_map<uint64_t, Ticket*> tickets;
-    std::unordered_map<uint64_t, Factor> cache;
+    ServeCache cache;
     std::vector<std::thread> workers;
     bool stop = false;
-    uint64_t next_id = 1, clock = 0;
+    uint64_t next_id = 1;
     int running = 0;
     // statistics
-    uint64_t submitted = 0, completed = 0, rejected = 0, hits = 0, evictions = 0;
+    uint64_t submitted = 0, completed = 0, rejected = 0, hits = 0, evict_base = 0;
     std::vector<double> lat;
     double t_first = -1.0, t_last = 0.0;
 
-    // bytes a request needs beyond what the cache already holds for it
-    size_t need(const bola_hip_request& r, bool& hit) {
-        hit = false;
-        if (r.factor_key) {
-            auto it = cache.find(r.factor_key);
-            if (it != cache.end() && it->second.kind == r.kind && it->second.n == r.n) hit = true;
-        }
-        const size_t rhs = r.n * r.nrhs * 8;
-        return hit ? rhs : factor_bytes(r.n) + rhs;
-    }
-    // evicts idle cached factors, least recently used first, until `bytes` fit
-    bool make_room(size_t bytes, hipStream_t st) {
-        while (in_use + bytes > budget) {
-            auto victim = cache.end();
-            for (auto it = cache.begin(); it != cache.end(); ++it)
-                if (it->second.users == 0 && (victim == cache.end() || it->second.last_use < victim->second.last_use))
-                    victim = it;
-            if (victim == cache.end()) return false;
-            hipFreeAsync(victim->second.f, st);
-            hipFreeAsync(victim->second.ipiv, st);
-            in_use -= victim->second.bytes;
-            evictions++;
-            cache.erase(victim);
-        }
-        return true;
-    }
+    explicit bola_hip_server_s(size_t budget) : cache(budget, release_device) {}
+
     // the next admissible request (the lock is held)
-    Ticket* admit(hipStream_t st, size_t& reserved, bool& hit) {
+    Ticket* admit(size_t& reserved, bool& hit) {
         for (size_t qi = 0; qi < queue.size(); qi++) {
             Ticket* t = queue[qi];
-            const size_t nb = need(t->req, hit);
-            if (in_use + nb <= budget || make_room(nb, st)) {
-                in_use += nb;
-                peak = std::max(peak, in_use);
-                reserved = nb;
+            const bola_hip_request& r = t->req;
+            if (cache.reserve(r.factor_key, r.kind, r.n, r.nrhs, reserved, hit)) {
                 queue.erase(queue.begin() + (long)qi);
-                if (hit) cache[t->req.factor_key].users++;
                 return t;
             }
             // the first in line does not fit yet: others may pass it, a bounded number of times
@@ -126,10 +99,11 @@ struct bola_hip_server_s {
 };
 
 void bola_hip_server_s::work(int slot) {
-    hipSetDevice(device);
+    (void)hipSetDevice(device);
     bola_hip_handle_t h = nullptr;
     if (bola_hip_create(&h) != 0) return;
     hipStream_t st = h->stream;
+    tl_stream = st;  // evictions made by this thread are ordered on its stream
     int32_t* dinfo = nullptr;
     if (hipMalloc(&dinfo, 4 * sizeof(int32_t)) != hipSuccess) {
         bola_hip_destroy(h);
@@ -143,7 +117,7 @@ void bola_hip_server_s::work(int slot) {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] {
                 if (stop) return true;
-                t = admit(st, reserved, hit);
+                t = admit(reserved, hit);
                 return t != nullptr;
             });
             if (!t) break;  // stopping
@@ -163,10 +137,10 @@ void bola_hip_server_s::work(int slot) {
         try {
             if (hit) {
                 std::lock_guard<std::mutex> lk(mu);
-                Factor& f = cache[r.factor_key];
-                fa = f.f;
-                ipiv = f.ipiv;
-                f.last_use = ++clock;
+                ResidentFactor* f = cache.pinned(r.factor_key);
+                if (!f) throw HipFailure{hipErrorInvalidValue, "resident factorization missing", __LINE__};
+                fa = (double*)f->f;
+                ipiv = (int32_t*)f->ipiv;
             } else {
                 BOLA_CHECK(hipMallocAsync((void**)&fa, n * n * 8, st));
                 BOLA_CHECK(hipMallocAsync((void**)&ipiv, std::max<size_t>(n, 1) * 4, st));
@@ -193,48 +167,28 @@ void bola_hip_server_s::work(int slot) {
             }
         } catch (const HipFailure& f) {
             status = BOLA_HIP_ERR_DEVICE;
-            hipStreamSynchronize(st);
-            if (xb) hipFreeAsync(xb, st);
+            (void)hipStreamSynchronize(st);
+            if (xb) (void)hipFreeAsync(xb, st);
             keep = false;
             (void)f;
         }
-        if (!hit && !keep) {
-            if (fa) hipFreeAsync(fa, st);
-            if (ipiv) hipFreeAsync(ipiv, st);
+        if (hit) {
+            fa = nullptr;  // resident: owned by the cache
+            ipiv = nullptr;
         }
-        hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(st);
         const double t_done = now_us();
         {
             std::lock_guard<std::mutex> lk(mu);
-            size_t kept = 0;
-            if (keep) {
-                auto it = cache.find(r.factor_key);
-                if (it != cache.end() && it->second.users == 0) {  // a stale entry under this key
-                    hipFreeAsync(it->second.f, st);
-                    hipFreeAsync(it->second.ipiv, st);
-                    in_use -= it->second.bytes;
-                    cache.erase(it);
-                }
-                Factor f;
-                f.kind = r.kind;
-                f.n = n;
-                f.f = fa;
-                f.ipiv = ipiv;
-                f.bytes = factor_bytes(n);
-                f.last_use = ++clock;
-                if (cache.find(r.factor_key) == cache.end()) {
-                    cache[r.factor_key] = f;
-                    kept = f.bytes;
-                } else {  // the key is busy with another factorization: drop this one
-                    hipFreeAsync(fa, st);
-                    hipFreeAsync(ipiv, st);
-                }
+            const bool stored = cache.finish(reserved, keep, r.factor_key, r.kind, n, fa, ipiv);
+            if (!stored && fa) {  // not kept (or the key is busy with another factorization)
+                (void)hipFreeAsync(fa, st);
+                (void)hipFreeAsync(ipiv, st);
             }
             if (hit) {
-                cache[r.factor_key].users--;
+                cache.unpin(r.factor_key);
                 hits++;
             }
-            in_use -= reserved - kept;
             res.status = status;
             res.info = info;
             res.latency_us = t_done - t->t_submit;
@@ -247,7 +201,8 @@ void bola_hip_server_s::work(int slot) {
         cv_done.notify_all();
         cv_work.notify_all();
     }
-    hipFree(dinfo);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(dinfo);
     bola_hip_destroy(h);
 }
 
@@ -264,9 +219,8 @@ int bola_hip_server_create(size_t hbm_budget, int slots, bola_hip_server_t* out)
         set_error(nullptr, "Argument 1 to server_create had an illegal value (at least 1 MiB)");
         return -1;
     }
-    auto* s = new bola_hip_server_s;
+    auto* s = new bola_hip_server_s(hbm_budget);
     hipGetDevice(&s->device);
-    s->budget = hbm_budget;
     for (int i = 0; i < slots; i++) s->workers.emplace_back([s, i] { s->work(i); });
     *out = s;
     return 0;
@@ -284,8 +238,8 @@ int bola_hip_server_submit(bola_hip_server_t s, const bola_hip_request* r, uint6
     t->t_submit = now_us();
     std::lock_guard<std::mutex> lk(s->mu);
     bool hit = false;
-    const size_t nb = s->need(*r, hit);
-    if (nb > s->budget || (!hit && !r->a)) {
+    const size_t nb = s->cache.need(r->factor_key, r->kind, r->n, r->nrhs, hit);
+    if (nb > s->cache.budget() || (!hit && !r->a)) {
         delete t;
         s->rejected++;
         set_error(nullptr, !r->a ? "server_submit: no matrix and no cached factorization under this key"
@@ -343,13 +297,11 @@ int bola_hip_server_get_stats(bola_hip_server_t s, bola_hip_server_stats* st) {
     st->completed = s->completed;
     st->rejected = s->rejected;
     st->cache_hits = s->hits;
-    st->evictions = s->evictions;
-    st->hbm_in_use = s->in_use;
-    st->hbm_peak = s->peak;
-    st->hbm_budget = s->budget;
-    size_t cached = 0;
-    for (auto& kv : s->cache) cached += kv.second.bytes;
-    st->hbm_cached = cached;
+    st->evictions = s->cache.evictions() - s->evict_base;
+    st->hbm_in_use = s->cache.in_use();
+    st->hbm_peak = s->cache.peak();
+    st->hbm_budget = s->cache.budget();
+    st->hbm_cached = s->cache.cached_bytes();
     if (!s->lat.empty()) {
         std::vector<double> v = s->lat;
         std::sort(v.begin(), v.end());
@@ -371,11 +323,12 @@ int bola_hip_server_get_stats(bola_hip_server_t s, bola_hip_server_stats* st) {
 int bola_hip_server_reset_stats(bola_hip_server_t s) {
     if (!s) return -1;
     std::lock_guard<std::mutex> lk(s->mu);
-    s->submitted = s->completed = s->rejected = s->hits = s->evictions = 0;
+    s->submitted = s->completed = s->rejected = s->hits = 0;
+    s->evict_base = s->cache.evictions();
     s->lat.clear();
     s->t_first = -1.0;
     s->t_last = 0.0;
-    s->peak = s->in_use;
+    s->cache.reset_peak();
     return 0;
 }
 
@@ -395,10 +348,7 @@ int bola_hip_server_destroy(bola_hip_server_t s) {
     s->cv_done.notify_all();
     for (auto& w : s->workers) w.join();
     hipSetDevice(s->device);
-    for (auto& kv : s->cache) {
-        hipFree(kv.second.f);
-        hipFree(kv.second.ipiv);
-    }
+    s->cache.clear();  // the workers have joined: no factorization is in use
     for (auto& kv : s->tickets) delete kv.second;
     delete s;
     return 0;

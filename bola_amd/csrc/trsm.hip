@@ -41,8 +41,8 @@ __device__ __forceinline__ double rdlane(double v, int l) {
 template <bool LOWER, int TW, bool UNIT>
 __global__ void __launch_bounds__(256) trsm_ln_lanes(int t, int nvec, const double* __restrict__ A, size_t lda,
                                                      double* __restrict__ B, size_t ldb,
-                                                     const int32_t* __restrict__ abort) {
-    if (abort && *abort != 0) return;
+                                                     Abort abort) {
+    if (abort_stop(abort)) return;
     constexpr bool TWO = TW == 32;
     const int lane = threadIdx.x & 63;
     const int sub = TWO ? (lane >> 5) : 0;
@@ -117,8 +117,8 @@ __global__ void __launch_bounds__(256) trsm_ln_lanes(int t, int nvec, const doub
 template <int V>
 __global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, const double* __restrict__ A,
                                                     size_t lda, double* __restrict__ B, size_t ldb,
-                                                    const int32_t* __restrict__ abort) {
-    if (abort && *abort != 0) return;
+                                                    Abort abort) {
+    if (abort_stop(abort)) return;
     __shared__ double sA[MAXT * (MAXT + 1)];  // sA[r * (MAXT+1) + c] = A[r, c]
     const int lane = threadIdx.x;
     for (int e = lane; e < t * t; e += 64) {
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, c
 
 template <int V>
 void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size_t lda, double* b, size_t ldb,
-          const int32_t* abort) {
+          Abort abort) {
     if (t == 0 || nvec == 0) return;
     if (V == LNL || V == LNU) {
         const size_t waves = t <= 32 ? (nvec + 1) / 2 : nvec;
@@ -235,7 +235,7 @@ void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size
 constexpr size_t STOP = 64;  // trsm.rs:8
 
 void lnl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (m <= STOP) return base<LNL>(st, unit, m, n, a, lda, b, ldb, abort);
     size_t m1 = m / 2, m2 = m - m1;
     lnl(st, unit, m1, n, a, lda, b, ldb, abort);
@@ -243,7 +243,7 @@ void lnl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     lnl(st, unit, m2, n, a + m1 + m1 * lda, lda, b + m1, ldb, abort);
 }
 void lnu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (m <= STOP) return base<LNU>(st, unit, m, n, a, lda, b, ldb, abort);
     size_t m1 = m / 2, m2 = m - m1;
     lnu(st, unit, m2, n, a + m1 + m1 * lda, lda, b + m1, ldb, abort);
@@ -251,7 +251,7 @@ void lnu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     lnu(st, unit, m1, n, a, lda, b, ldb, abort);
 }
 void ltl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (m <= STOP) return base<LTL>(st, unit, m, n, a, lda, b, ldb, abort);
     size_t m1 = m / 2, m2 = m - m1;
     ltl(st, unit, m2, n, a + m1 + m1 * lda, lda, b + m1, ldb, abort);
@@ -259,7 +259,7 @@ void ltl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     ltl(st, unit, m1, n, a, lda, b, ldb, abort);
 }
 void ltu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (m <= STOP) return base<LTU>(st, unit, m, n, a, lda, b, ldb, abort);
     size_t m1 = m / 2, m2 = m - m1;
     ltu(st, unit, m1, n, a, lda, b, ldb, abort);
@@ -267,7 +267,7 @@ void ltu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     ltu(st, unit, m2, n, a + m1 + m1 * lda, lda, b + m1, ldb, abort);
 }
 void rnl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (n <= STOP) return base<RNL>(st, unit, n, m, a, lda, b, ldb, abort);
     size_t n1 = n / 2, n2 = n - n1;
     rnl(st, unit, m, n2, a + n1 + n1 * lda, lda, b + n1 * ldb, ldb, abort);
@@ -275,7 +275,7 @@ void rnl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     rnl(st, unit, m, n1, a, lda, b, ldb, abort);
 }
 void rnu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (n <= STOP) return base<RNU>(st, unit, n, m, a, lda, b, ldb, abort);
     size_t n1 = n / 2, n2 = n - n1;
     rnu(st, unit, m, n1, a, lda, b, ldb, abort);
@@ -283,7 +283,7 @@ void rnu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     rnu(st, unit, m, n2, a + n1 + n1 * lda, lda, b + n1 * ldb, ldb, abort);
 }
 void rtl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (n <= STOP) return base<RTL>(st, unit, n, m, a, lda, b, ldb, abort);
     size_t n1 = n / 2, n2 = n - n1;
     rtl(st, unit, m, n1, a, lda, b, ldb, abort);
@@ -291,7 +291,7 @@ void rtl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
     rtl(st, unit, m, n2, a + n1 + n1 * lda, lda, b + n1 * ldb, ldb, abort);
 }
 void rtu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
-         const int32_t* abort) {
+         Abort abort) {
     if (n <= STOP) return base<RTU>(st, unit, n, m, a, lda, b, ldb, abort);
     size_t n1 = n / 2, n2 = n - n1;
     rtu(st, unit, m, n2, a + n1 + n1 * lda, lda, b + n1 * ldb, ldb, abort);
@@ -304,7 +304,7 @@ bool eqc(char c, char w) { return c == w || c == (char)(w + 32); }
 }  // namespace
 
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
-                  const double* a, size_t lda, double* b, size_t ldb, const int32_t* abort) {
+                  const double* a, size_t lda, double* b, size_t ldb, Abort abort) {
     if (m == 0 || n == 0) return;
     if (alpha != 1.0) launch_scale(st, b, ldb, m, n, alpha);
     bool lside = eqc(side, 'L'), upper = eqc(uplo, 'U'), notrans = eqc(transa, 'N');

@@ -174,3 +174,24 @@ def test_batch_rejects_bad_descriptors(B):
     with pytest.raises(B.BolaError):
         B.batch_solve(B.BATCH_FP64, d, arena, torch.zeros(ptot, dtype=torch.int32, device="cuda"),
                       torch.zeros(4, dtype=torch.int32, device="cuda"), config=(12, 1))
+
+
+def test_batch_rejects_a_missing_pivot_arena_for_lu_systems(B):
+    # an LU descriptor with no pivot arena is an argument error (no launch), not a device fault
+    import torch
+    d, arena, ptot = _batch(B, 16, 8, 40, 5)
+    assert (d["kind"] == B.BATCH_LU).any()
+    info = torch.zeros(len(d), dtype=torch.int32, device="cuda")
+    with pytest.raises(B.BolaError) as e:
+        B.batch_solve(B.BATCH_FP64, d, arena, None, info)
+    assert e.value.code == -6 and "pivot arena" in str(e.value)
+    with pytest.raises(B.BolaError) as e:
+        B.BatchGraph(B.BATCH_FP64, d, arena, None, info)
+    assert e.value.code == -6
+    # Cholesky-only batches need none
+    dc = d[d["kind"] == B.BATCH_CHOL]
+    if len(dc):
+        infc = torch.zeros(len(dc), dtype=torch.int32, device="cuda")
+        B.batch_solve(B.BATCH_FP64, dc, arena, None, infc)
+        torch.cuda.synchronize()
+        assert int((infc != 0).sum()) == 0

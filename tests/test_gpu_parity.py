@@ -428,6 +428,38 @@ def test_getrf_errors(B, O):
     same(a, ref, "partially factored state")
 
 
+@pytest.mark.parametrize("m,n,zc", [
+    (600, 600, 293),      # 128-column super-panels: the second panel of the third one
+    (2600, 2600, 620),    # 256-column super-panels: the fourth panel of the third one
+    (2600, 2600, 1300),   # the first panel of a super-panel, while the previous one's bulk update runs
+    (2600, 2600, 2590),   # the partial last super-panel
+    (1200, 800, 290),     # rectangular, m > n
+    (2048, 2600, 1800),   # rectangular, m < n, columns beyond the last super-panel
+])
+def test_getrf_zero_pivot_state_bitexact(B, O, m, n, zc):
+    # A zero column stays exactly zero through every update, so the factorization stops at
+    # its pivot.  The reference returns there (getrf2.rs:166-168, getrf.rs:53): the panels
+    # before the failing one are applied to every column, the failing panel is left as far
+    # as it got, ipiv of the failing panel stays panel-local.  The whole matrix and ipiv
+    # must be the reference's, every time.
+    rng = np.random.default_rng(m + n + zc)
+    A = rng.uniform(-1, 1, (m, n))
+    A[:, zc] = 0.0
+    a0 = A.ravel(order="F").copy()
+    ref = a0.copy()
+    rp = np.full(min(m, n), -7, np.int32)
+    code, msg = O.getrf(m, n, ref, m, rp)
+    assert code == zc % 32 + 1
+    for rep in range(2):
+        a = a0.copy()
+        ipiv = np.full(min(m, n), -7, np.int32)
+        with pytest.raises(B.BolaError) as e:
+            B.getrf(m, n, a, m, ipiv)
+        assert e.value.message == msg and e.value.code == zc + 1
+        same(ipiv, rp, f"ipiv after the zero pivot ({rep})")
+        same(a, ref, f"partially factored state ({rep})")
+
+
 def test_getrs_bitexact(B, O):
     rng = np.random.default_rng(14)
     n, nrhs = 300, 64

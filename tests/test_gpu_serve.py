@@ -203,3 +203,29 @@ def test_serve_large_factorizations_in_the_mix(B):
             res = s.wait(t)
             assert res["status"] == 0 and res["info"] == 0, r["n"]
             assert np.array_equal(_host(ops[2]), _standalone(B, r)), r["n"]
+
+
+def test_serve_hit_on_the_least_recently_used_factorization_under_budget_pressure(B):
+    # Two resident factorizations fill the budget.  A hit on the least recently used one
+    # whose right-hand sides need room must evict the other one, never its own.
+    n, nrhs = 256, 40
+    f = n * n * 8 + ((n * 4 + 255) & ~255)
+    budget = 2 * f + n * nrhs * 8 // 2
+    with B.Server(budget, 2) as s:
+        reqs = {}
+        for key in (1, 2):
+            r = dict(kind=B.SERVE_LU, n=n, nrhs=1, a=_matrix(n, B.SERVE_LU, key, B),
+                     b=np.random.default_rng(key).uniform(-1, 1, n), key=key, dev=False)
+            reqs[key] = r
+            assert s.wait(s.submit(B.SERVE_LU, n, 1, *_operands(r), key))["cache_hit"] == 0
+        r = dict(reqs[1])
+        r["nrhs"] = nrhs
+        r["b"] = np.random.default_rng(9).uniform(-1, 1, n * nrhs)
+        ops = _operands(r)
+        res = s.wait(s.submit(B.SERVE_LU, n, nrhs, *ops, 1))
+        assert res["status"] == 0 and res["info"] == 0 and res["cache_hit"] == 1
+        assert np.array_equal(_host(ops[2]), _standalone(B, r))
+        st = s.stats()
+        assert st["evictions"] == 1 and st["hbm_peak"] <= budget
+        # the evicted key is factored again; the kept one still serves
+        assert s.wait(s.submit(B.SERVE_LU, n, 1, *_operands(reqs[2]), 2))["cache_hit"] == 0
