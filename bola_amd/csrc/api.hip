@@ -66,7 +66,7 @@ void collect_profile(bola_hip_handle_s* h) {
     if (h->stream3) BOLA_CHECK(hipStreamSynchronize(h->stream3));
     for (auto& p : h->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, p.start, p.stop);
+        (void)hipEventElapsedTime(&ms, p.start, p.stop);
         PhaseStat& s = h->stats[p.name];
         s.calls += 1;
         s.ms += ms;
@@ -75,6 +75,31 @@ void collect_profile(bola_hip_handle_s* h) {
         h->event_pool.push_back(p.stop);
     }
     h->pending.clear();
+}
+
+void* Workspace::get(size_t need) {
+    if (need <= bytes) return ptr;
+    bola_hip_handle_s* h = owner;
+    hipStream_t st = h->stream;
+    // the free waits for everything queued on the handle's side streams ...
+    hipStream_t side[2] = {h->stream2, h->stream3};
+    if (!h->ws_event) BOLA_CHECK(hipEventCreateWithFlags(&h->ws_event, hipEventDisableTiming));
+    for (hipStream_t s2 : side) {
+        if (!s2) continue;
+        BOLA_CHECK(hipEventRecord(h->ws_event, s2));
+        BOLA_CHECK(hipStreamWaitEvent(st, h->ws_event, 0));
+    }
+    if (ptr) BOLA_CHECK(hipFreeAsync(ptr, st));
+    ptr = nullptr;
+    bytes = 0;
+    const size_t nb = need + (need >> 2);
+    BOLA_CHECK(hipMallocAsync(&ptr, nb, st));
+    bytes = nb;
+    // ... and the side streams' later work waits for the new buffer
+    BOLA_CHECK(hipEventRecord(h->ws_event, st));
+    for (hipStream_t s2 : side)
+        if (s2) BOLA_CHECK(hipStreamWaitEvent(s2, h->ws_event, 0));
+    return ptr;
 }
 
 void set_error(bola_hip_handle_s* h, const std::string& msg) {
@@ -218,12 +243,14 @@ int bola_hip_create(bola_hip_handle_t* out) {
 
 int bola_hip_destroy(bola_hip_handle_t h) {
     if (!h) return 0;
-    hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    if (h->stream3) (void)hipStreamSynchronize(h->stream3);
     for (auto& p : h->pending) {
-        hipEventDestroy(p.start);
-        hipEventDestroy(p.stop);
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
     }
-    for (auto e : h->event_pool) hipEventDestroy(e);
+    for (auto e : h->event_pool) (void)hipEventDestroy(e);
     h->ws.release();
     h->ws_aux.release();
     h->stage.release();
@@ -232,14 +259,9 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     h->bws.release();
     if (h->d_info) (void)hipFree(h->d_info);
     for (auto e : h->sync_events) (void)hipEventDestroy(e);
-    if (h->stream2) {
-        (void)hipStreamSynchronize(h->stream2);
-        (void)hipStreamDestroy(h->stream2);
-    }
-    if (h->stream3) {
-        (void)hipStreamSynchronize(h->stream3);
-        (void)hipStreamDestroy(h->stream3);
-    }
+    if (h->ws_event) (void)hipEventDestroy(h->ws_event);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
+    if (h->stream3) (void)hipStreamDestroy(h->stream3);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     if (t_default == h) t_default = nullptr;
     delete h;

@@ -13,6 +13,8 @@
 
 #include "bola_hip.h"
 
+struct bola_hip_handle_s;
+
 namespace bola {
 
 // Thrown inside the library only; every C-ABI entry point catches it.
@@ -30,26 +32,16 @@ struct HipFailure {
 
 #define BOLA_LAUNCH_CHECK() BOLA_CHECK(hipGetLastError())
 
-// Growable device workspace owned by a handle.
+// Growable device workspace owned by a handle.  Growth is stream-ordered on the handle's
+// stream: the old buffer is freed after all work queued on the handle's streams, the new
+// one is visible to all of them, and no other handle (or server slot) is stalled.
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
-    void* get(size_t need) {
-        if (need > bytes) {
-            // work already queued may still use the old buffer
-            if (ptr) {
-                BOLA_CHECK(hipDeviceSynchronize());
-                BOLA_CHECK(hipFree(ptr));
-            }
-            ptr = nullptr;
-            size_t nb = need + (need >> 2);
-            BOLA_CHECK(hipMalloc(&ptr, nb));
-            bytes = nb;
-        }
-        return ptr;
-    }
+    bola_hip_handle_s* owner = nullptr;
+    void* get(size_t need);  // api.hip
     void release() {
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -152,11 +144,15 @@ struct bola_hip_handle_s {
     hipStream_t stream2 = nullptr;
     hipStream_t stream3 = nullptr;  // a third one: side work that must not wait for the second
     std::vector<hipEvent_t> sync_events;
+    hipEvent_t ws_event = nullptr;  // orders workspace growth against the side streams
     hipStream_t aux_stream() {
         if (!stream2) {
             if (hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking) != hipSuccess) stream2 = nullptr;
         }
         return stream2 ? stream2 : stream;
+    }
+    bola_hip_handle_s() {
+        for (bola::Workspace* w : {&ws, &ws_aux, &stage, &mws, &mws2, &bws}) w->owner = this;
     }
     hipStream_t aux_stream3() {
         if (!stream3) {

@@ -229,3 +229,55 @@ def test_serve_hit_on_the_least_recently_used_factorization_under_budget_pressur
         assert st["evictions"] == 1 and st["hbm_peak"] <= budget
         # the evicted key is factored again; the kept one still serves
         assert s.wait(s.submit(B.SERVE_LU, n, 1, *_operands(reqs[2]), 2))["cache_hit"] == 0
+
+
+def test_serve_config_sizes_match_the_oracle(B):
+    # Orders of the serving configuration up to 8192, none of them a multiple of 64 except
+    # the largest, in exactly sized stream-ordered buffers, LU and Cholesky side by side on
+    # four slots, one repeated right-hand side from a resident factorization: every
+    # solution is the oracle's (the reference's algorithm), bit for bit.
+    import oracle as O
+    O.lib()
+    rng = np.random.default_rng(2026)
+    spec = [(8192, B.SERVE_LU, 3, 11), (5661, B.SERVE_LU, 64, 0), (4097, B.SERVE_CHOL, 17, 12),
+            (3913, B.SERVE_LU, 1, 0), (2705, B.SERVE_CHOL, 40, 0), (1000, B.SERVE_LU, 5, 0),
+            (607, B.SERVE_CHOL, 2, 0), (8192, B.SERVE_LU, 9, 11)]
+    mats = {}
+    reqs = []
+    for n, kind, nrhs, key in spec:
+        if (n, kind) not in mats:
+            mats[(n, kind)] = _matrix(n, kind, n, B)
+        reqs.append(dict(kind=kind, n=n, nrhs=nrhs, a=mats[(n, kind)], b=rng.uniform(-1, 1, n * nrhs), key=key))
+    with B.Server(16 << 30, 4) as s:
+        live = []
+        for i, r in enumerate(reqs):
+            if i == len(reqs) - 1:  # the repeat comes once its factorization is resident
+                s.wait(live[0][1])
+            x = np.zeros(r["n"] * r["nrhs"])
+            live.append((r, s.submit(r["kind"], r["n"], r["nrhs"], r["a"], r["b"], x, r["key"]), x))
+        results = []
+        for r, t, x in live[1:]:
+            results.append((r, s.wait(t), x))
+        st = s.stats()
+    assert results[-1][1]["cache_hit"] == 1
+    factors = {}
+    for r, res, x in [(live[0][0], {"status": 0, "info": 0}, live[0][2])] + results:
+        assert res["status"] == 0 and res["info"] == 0, r["n"]
+        n, kind = r["n"], r["kind"]
+        if (n, kind) not in factors:
+            a = r["a"].copy()
+            if kind == B.SERVE_LU:
+                p = np.zeros(n, np.int32)
+                assert O.getrf(n, n, a, n, p)[0] == 0
+            else:
+                p = None
+                assert O.potrf("L", n, a, n)[0] == 0
+            factors[(n, kind)] = (a, p)
+        a, p = factors[(n, kind)]
+        ref = r["b"].copy()
+        if kind == B.SERVE_LU:
+            O.getrs(n, r["nrhs"], a, n, p, ref, n)
+        else:
+            O.potrs("L", n, r["nrhs"], a, n, ref, n)
+        assert np.array_equal(x, ref), (n, kind)
+    assert st["completed"] == len(reqs) and st["hbm_in_use"] == st["hbm_cached"]
