@@ -324,12 +324,13 @@ def mixed_section(B, torch, n=16384, kappa=1e2):
     return out
 
 
-def serving_section(B, torch, nmax=600):
-    """Solve server under Poisson arrivals: orders log-uniform in [32, nmax] (quantised to 16
-    sizes), 70 % LU (A = U diag(sigma) V^T, kappa log-uniform in [1e2, 1e6]) / 30 % SPD,
-    nrhs uniform in [1, 64], 30 % of requests reuse a resident factorization with a new
-    right-hand side; host matrices (uploaded by the server), 4 slots, 16 GiB budget.  Two loads: 200 requests/s for 5 s (latency) and 2000/s for 2 s
-    (throughput).  The full [32, 8192] range of the serving configuration is not run."""
+def serving_section(B, torch, nmax=8192, loads=(("latency", 10.0, 10.0), ("throughput", 60.0, 5.0))):
+    """Solve server under Poisson arrivals (the serving configuration): orders log-uniform
+    in [32, nmax] (quantised to 16 sizes), 70 % LU (A = U diag(sigma) V^T, kappa
+    log-uniform in [1e2, 1e6]) / 30 % SPD (G G^T + n I, Cholesky), nrhs uniform in [1, 64],
+    30 % of requests reuse a resident factorization with a new right-hand side; host
+    matrices (uploaded by the server), 4 slots, 16 GiB budget.  Two loads: a light one
+    (latency) and one near the server's capacity (throughput)."""
     import math
     from bola_amd import testmat
     rng = np.random.default_rng(1)
@@ -338,19 +339,24 @@ def serving_section(B, torch, nmax=600):
     for n in sizes:
         for kind in (B.SERVE_LU, B.SERVE_CHOL):
             if kind == B.SERVE_CHOL:
-                G = rng.uniform(-1, 1, (n, n))
-                A = np.asfortranarray(G @ G.T + n * np.eye(n)).ravel(order="F").copy()
+                g = torch.Generator(device="cuda")
+                g.manual_seed(int(n))
+                G = torch.rand(int(n), int(n), dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+                A = (G @ G.t() + n * torch.eye(int(n), dtype=torch.float64, device="cuda")).t().contiguous()
+                A = A.reshape(-1).cpu().numpy()
+                del G
             else:
                 kappa = float(np.exp(rng.uniform(math.log(1e2), math.log(1e6))))
-                A = np.ascontiguousarray(testmat.conditioned(int(n), kappa, int(n)))
+                A = testmat.conditioned(int(n), kappa, int(n), device="cuda").cpu().numpy()
             pool.append((int(n), kind, A))
+    torch.cuda.empty_cache()
     out = {"workload": f"Poisson arrivals, n log-uniform [32,{nmax}], 70% LU (kappa log-uniform [1e2,1e6]) / "
                        f"30% SPD, nrhs uniform [1,64], "
-                       f"30% reuse a resident factorization, FP64, 4 slots, host operands"}
+                       f"30% reuse a resident factorization, FP64, 4 slots, 16 GiB budget, host operands"}
     with B.Server(16 << 30, 4) as s:
         for (n, kind, A) in pool:  # warm-up, one at a time
             s.wait(s.submit(kind, n, 1, A, np.ones(n), np.zeros(n), 0))
-        for name, qps, dur in [("latency", 200.0, 5.0), ("throughput", 2000.0, 2.0)]:
+        for name, qps, dur in loads:
             nreq = int(qps * dur)
             arr = np.cumsum(rng.exponential(1.0 / qps, nreq))
             ln = np.exp(rng.uniform(math.log(32), math.log(nmax), nreq))
@@ -378,9 +384,10 @@ def serving_section(B, torch, nmax=600):
             out[name] = {"offered_per_s": qps, "requests": nreq, "p50_ms": round(st["p50_us"] / 1e3, 3),
                          "p99_ms": round(st["p99_us"] / 1e3, 3), "solves_per_s": round(st["solves_per_s"], 1),
                          "gflops": round(flops / span / 1e9, 2), "cache_hits": st["cache_hits"],
+                         "evictions": st["evictions"], "hbm_peak_gib": round(st["hbm_peak"] / 2 ** 30, 2),
                          "failed": sum(1 for r in res if r["status"] != 0 or r["info"] != 0)}
             print(f"# serving {name}: p50 {out[name]['p50_ms']} ms p99 {out[name]['p99_ms']} ms, "
-                  f"{out[name]['solves_per_s']} solves/s", file=sys.stderr, flush=True)
+                  f"{out[name]['solves_per_s']} solves/s, failed {out[name]['failed']}", file=sys.stderr, flush=True)
     return out
 
 
