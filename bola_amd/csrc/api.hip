@@ -257,6 +257,7 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     h->mws.release();
     h->mws2.release();
     h->bws.release();
+    h->cws.release();
     if (h->d_info) (void)hipFree(h->d_info);
     for (auto e : h->sync_events) (void)hipEventDestroy(e);
     if (h->ws_event) (void)hipEventDestroy(h->ws_event);

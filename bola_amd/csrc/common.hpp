@@ -133,6 +133,7 @@ struct bola_hip_handle_s {
     bola::Workspace mws;        // mixed-precision solver: fp32 factors, vectors, GMRES basis
     bola::Workspace mws2;       // its FP64 fallback copy of A
     bola::Workspace bws;        // batched solves: staged descriptors, per-workgroup scratch
+    bola::Workspace cws;        // Cholesky: the saved region of the matrix (restored when a pivot fails)
     int32_t* d_info = nullptr;  // device status word for factorizations
     std::string last_error;
     // profiling: per-phase GPU time measured with events on the handle's stream
@@ -152,7 +153,7 @@ struct bola_hip_handle_s {
         return stream2 ? stream2 : stream;
     }
     bola_hip_handle_s() {
-        for (bola::Workspace* w : {&ws, &ws_aux, &stage, &mws, &mws2, &bws}) w->owner = this;
+        for (bola::Workspace* w : {&ws, &ws_aux, &stage, &mws, &mws2, &bws, &cws}) w->owner = this;
     }
     hipStream_t aux_stream3() {
         if (!stream3) {

@@ -224,6 +224,112 @@ __global__ void k_zero2(int32_t* p) {
     if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
+// ---- the blocked factorization, right-looking with the reference's arithmetic
+//
+// The reference's loop is left-looking: block column j receives, at step j, a SYRK of its
+// diagonal block over all earlier columns (one exact chain per element, l = 0 .. j-1),
+// and a GEMM of the rows below over all earlier columns (64-deep slabs folded in order,
+// the last one 32 deep when j is an odd multiple of 32).  Every one of those terms is
+// known as soon as its earlier block column is final, so here each finished block
+// column c pushes its terms into the later ones at once: 32 more chain steps into every
+// later diagonal block, and the slab folds -- a 32-deep one into block column c + 32
+// when c is an even block (it is that column's last, partial slab), a 64-deep one over
+// columns c - 32 .. c + 31 into every later block column when c is an odd block.  Each
+// element receives the same operations in the same order, so the result is the
+// reference's bit for bit; the folds are large masked GEMMs on the matrix cores and the
+// critical path is one small POTRF2 and TRSM per block column.
+//
+// A failing block leaves later block columns updated, which the reference never does:
+// the part of the matrix the factorization may write is saved first and restored for
+// every block column after the failing one (and below its diagonal block) when a pivot
+// fails.
+
+// the block-triangle region of element (r, col): L: r in or below col's diagonal block;
+// U: r in or above it
+__device__ __forceinline__ bool chol_region(int upper, int r, int col) {
+    return upper ? (r / SB <= col / SB) : (r / SB >= col / SB);
+}
+
+__global__ void chol_save(int upper, int n, const double* __restrict__ A, size_t lda, double* __restrict__ W) {
+    const long total = (long)n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), col = (int)(e / n);
+        if (chol_region(upper, r, col)) W[e] = A[r + (size_t)col * lda];
+    }
+}
+
+// after a failure at block jf: the region of every later block column (L) / block row
+// (U), and of block column / row jf outside its diagonal block, as it was
+__global__ void chol_restore(int upper, int n, double* __restrict__ A, size_t lda, const double* __restrict__ W,
+                             const int32_t* __restrict__ info) {
+    const int v = *info;
+    if (v <= 0) return;
+    const int bf = (v - 1) / SB;
+    const long total = (long)n * n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), col = (int)(e / n);
+        if (!chol_region(upper, r, col)) continue;
+        const int own = upper ? r / SB : col / SB;   // the block step that finalises the element
+        if (own < bf || (r / SB == bf && col / SB == bf)) continue;
+        A[r + (size_t)col * lda] = W[e];
+    }
+}
+
+// The chain steps l = c .. c+31 of every diagonal block after block column c, with the
+// reference's SYRK forms: L (syrk.rs, trans N, daxpy order): c += (alpha a[jj,l]) a[i,l],
+// fused on the 4-aligned prefix of the column segment, skipped when alpha a[jj,l] == 0;
+// U (trans T, dot order per 32-chunk): c = c + alpha * dot(A[c:c+32, i], A[c:c+32, jj]).
+__global__ void __launch_bounds__(1024) chol_diag_steps(int upper, int n, int c, double* __restrict__ A, size_t lda,
+                                                         const int32_t* __restrict__ abort) {
+    if (*abort != 0) return;
+    __shared__ double sl[SB][SB + 1];  // sl[r][l]: rows of the block, columns c .. c+31 (L) / transposed (U)
+    const int j0 = c + SB + (int)blockIdx.x * SB;
+    const int nb = min(SB, n - j0);
+    const int e = threadIdx.x, i = e & 31, jj = e >> 5;
+    // stage the 32 x 32 operand block: L: A[j0 + r, c + l]; U: A[c + l, j0 + r]
+    {
+        const int r = e >> 5, l = e & 31;
+        double v = 0.0;
+        if (r < nb) v = upper ? A[(size_t)(c + l) + (size_t)(j0 + r) * lda] : A[(size_t)(j0 + r) + (size_t)(c + l) * lda];
+        sl[r][l] = v;
+    }
+    __syncthreads();
+    if (i >= nb || jj >= nb) return;
+    if (upper ? (i > jj) : (i < jj)) return;
+    double* cp = A + (size_t)(j0 + i) + (size_t)(j0 + jj) * lda;
+    double v = *cp;
+    if (!upper) {
+        const int len = nb - jj, rel = i - jj;
+        const bool fused = rel < (len & ~3);
+#pragma unroll
+        for (int l = 0; l < SB; l++) {
+            const double temp = -1.0 * sl[jj][l];
+            if (temp != 0.0) {
+                const double x = sl[i][l];
+                v = fused ? fma(temp, x, v) : v + temp * x;
+            }
+        }
+    } else {
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+#pragma unroll
+        for (int t = 0; t < SB / 4; t++) {
+            l0 = fma(sl[i][4 * t + 0], sl[jj][4 * t + 0], l0);
+            l1 = fma(sl[i][4 * t + 1], sl[jj][4 * t + 1], l1);
+            l2 = fma(sl[i][4 * t + 2], sl[jj][4 * t + 2], l2);
+            l3 = fma(sl[i][4 * t + 3], sl[jj][4 * t + 3], l3);
+        }
+        double hs = -0.0;
+        hs = hs + l0;
+        hs = hs + l1;
+        hs = hs + l2;
+        hs = hs + l3;
+        double ts = 0.0;
+        ts = ts + hs;
+        v = v + -1.0 * ts;
+    }
+    *cp = v;
+}
+
 }  // namespace
 
 void launch_dsyrk(hipStream_t st, char uplo, char trans, size_t n, size_t k, double alpha, const double* a, size_t lda,
@@ -299,53 +405,75 @@ void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t ld
         return;
     }
     Phase ph(h, "POTRF", (double)n * n * n / 3.0);
-    for (size_t j = 0; j < n; j += SB) {
-        const size_t jb = std::min((size_t)SB, n - j);
-        double* ajj = a + j + j * lda;
-        if (uplo == 'U') {
-            double* a1j = a + j * lda;
-            if (j > 0) {
-                Phase p(h, "SYRK", (double)jb * jb * j);
-                launch_dsyrk(st, 'U', 'T', jb, j, -1.0, a1j, lda, 1.0, ajj, lda, info);
-            }
-            {
-                Phase p(h, "POTRF2");
-                potrf2_block(st, 'U', jb, ajj, lda, info, (int)j);
-            }
-            if (j + jb < n) {
-                const size_t n_rem = n - j - jb;
-                double* a1jb = a + (j + jb) * lda;
-                double* ajjb = a + j + (j + jb) * lda;
-                if (j > 0) {
-                    Phase p(h, "GEMM", 2.0 * jb * n_rem * j);
-                    launch_dgemm(st, 'T', 'N', jb, n_rem, j, -1.0, a1j, lda, a1jb, lda, 1.0, ajjb, lda, info);
-                }
-                Phase p(h, "TRSM", (double)jb * jb * n_rem);
-                launch_dtrsm(st, 'L', 'U', 'T', 'N', jb, n_rem, 1.0, ajj, lda, ajjb, lda, info);
-            }
+    const int N = (int)n;
+    const int up = uplo == 'U' ? 1 : 0;
+    hipStream_t s2 = h->aux_stream();
+    const unsigned sgrid = (unsigned)std::min<size_t>((n * n + 255) / 256, 4096);
+    double* W = (double*)h->cws.get(n * n * sizeof(double));
+    hipLaunchKernelGGL(chol_save, dim3(sgrid), dim3(256), 0, st, up, N, a, lda, W);
+    BOLA_LAUNCH_CHECK();
+    // the reference's GEMM edges (gemm.rs:177-190) for every block step: L -- the last n % 8
+    // rows (its m is n - j - 32); U -- the last n % 4 columns (its n is n - j - 32)
+    const int edge_r = N - N % 8, edge_c = N - N % 4;
+    auto at = [&](int r, int c) { return a + (size_t)r + (size_t)c * lda; };
+    // fold over columns (L) / rows (U) [k0, k0 + kd) into the block-triangle region whose
+    // diagonal starts at d0, restricted to the nb columns (L) / rows (U) from d0 when nb > 0
+    auto fold = [&](hipStream_t sx, int k0, int kd, int d0, int nb) {
+        const int rest = N - d0;
+        if (rest <= 0) return;
+        const int w = nb > 0 ? std::min(nb, rest) : rest;
+        Phase pg(h, "GEMM", (double)kd * w * (2.0 * rest - w), sx);
+        if (!up)
+            launch_dgemm_ex(sx, 'N', 'T', (size_t)rest, (size_t)w, (size_t)kd, -1.0, at(d0, k0), lda, at(d0, k0), lda,
+                            1.0, at(d0, d0), lda, info, kd, 1, edge_r - d0, (long)w);
+        else
+            launch_dgemm_ex(sx, 'T', 'N', (size_t)w, (size_t)rest, (size_t)kd, -1.0, at(k0, d0), lda, at(k0, d0), lda,
+                            1.0, at(d0, d0), lda, info, kd, 2, (long)w, edge_c - d0);
+    };
+    hipEvent_t bulk_done = nullptr;
+    int ev = 0;
+    for (int c = 0; c < N; c += SB) {
+        const int jb = std::min(SB, N - c);
+        {
+            Phase p(h, "POTRF2", 0.0, st);
+            potrf2_block(st, uplo, (size_t)jb, at(c, c), lda, info, c);
+        }
+        if (c + jb >= N) break;
+        const int n_rem = N - c - jb;
+        {
+            Phase p(h, "TRSM", (double)jb * jb * n_rem, st);
+            if (!up) launch_dtrsm(st, 'R', 'L', 'T', 'N', (size_t)n_rem, (size_t)jb, 1.0, at(c, c), lda, at(c + jb, c), lda, info);
+            else launch_dtrsm(st, 'L', 'U', 'T', 'N', (size_t)jb, (size_t)n_rem, 1.0, at(c, c), lda, at(c, c + jb), lda, info);
+        }
+        {
+            Phase p(h, "SYRK", (double)SB * SB * n_rem, st);
+            hipLaunchKernelGGL(chol_diag_steps, dim3((unsigned)((n_rem + SB - 1) / SB)), dim3(1024), 0, st, up, N, c, a,
+                               lda, info);
+            BOLA_LAUNCH_CHECK();
+        }
+        if ((c / SB) % 2 == 0) {
+            // the partial 32-deep last slab of block column c + 32
+            fold(st, c, SB, c + SB, SB);
         } else {
-            double* aj0 = a + j;
-            if (j > 0) {
-                Phase p(h, "SYRK", (double)jb * jb * j);
-                launch_dsyrk(st, 'L', 'N', jb, j, -1.0, aj0, lda, 1.0, ajj, lda, info);
-            }
-            {
-                Phase p(h, "POTRF2");
-                potrf2_block(st, 'L', jb, ajj, lda, info, (int)j);
-            }
-            if (j + jb < n) {
-                const size_t n_rem = n - j - jb;
-                double* ajb0 = a + j + jb;
-                double* ajbj = a + j + jb + j * lda;
-                if (j > 0) {
-                    Phase p(h, "GEMM", 2.0 * n_rem * jb * j);
-                    launch_dgemm(st, 'N', 'T', n_rem, jb, j, -1.0, ajb0, lda, aj0, lda, 1.0, ajbj, lda, info);
-                }
-                Phase p(h, "TRSM", (double)jb * jb * n_rem);
-                launch_dtrsm(st, 'R', 'L', 'T', 'N', n_rem, jb, 1.0, ajj, lda, ajbj, lda, info);
+            // the 64-deep slab over the two block columns c - 32 .. c + 31: into the next two
+            // block columns here (they come next on this stream), into the rest on stream 2
+            if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
+            fold(st, c - SB, 2 * SB, c + SB, 2 * SB);
+            if (c + 3 * SB < N) {
+                hipEvent_t ready = handle_event(h, ev++);
+                BOLA_CHECK(hipEventRecord(ready, st));
+                BOLA_CHECK(hipStreamWaitEvent(s2, ready, 0));
+                fold(s2, c - SB, 2 * SB, c + 3 * SB, 0);
+                bulk_done = handle_event(h, ev++);
+                BOLA_CHECK(hipEventRecord(bulk_done, s2));
+            } else {
+                bulk_done = nullptr;
             }
         }
     }
+    if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
+    hipLaunchKernelGGL(chol_restore, dim3(sgrid), dim3(256), 0, st, up, N, a, lda, W, info);
+    BOLA_LAUNCH_CHECK();
 }
 
 }  // namespace bola

@@ -534,6 +534,56 @@ def test_potrf_large_bitexact(B, O, uplo, n):
     same(a, ref, f"potrf {uplo} {n}")
 
 
+@pytest.mark.parametrize("uplo,n", [("L", 4096), ("U", 4096), ("U", 8192), ("L", 4133)])
+def test_potrf_config_sizes_bitexact(B, O, uplo, n):
+    # the plain FP64 configuration's Cholesky orders (SPD A = G G^T + n I, G uniform [-1, 1)),
+    # and an order with ragged edges (n % 8 = 5, n % 32 = 5) on the device flavour
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n)
+    G = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+    A = (G @ G.t() + n * torch.eye(n, dtype=torch.float64, device="cuda")).t().contiguous().reshape(-1)
+    ref = A.cpu().numpy().copy()
+    assert O.potrf(uplo, n, ref, n)[0] == 0
+    B.potrf(uplo, n, A, n)
+    torch.cuda.synchronize()
+    same(A.cpu().numpy(), ref, f"potrf {uplo} {n}")
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,k", [(600, 305), (600, 288), (2085, 1000), (2085, 2070)])
+def test_potrf_failure_state_bitexact(B, O, uplo, n, k):
+    # a non-positive leading minor at column k + 1: the reference returns at that block
+    # (potrf.rs:124/144) with the earlier block columns (rows) factored, the failing
+    # diagonal block as far as POTRF2 got, and everything later untouched
+    rng = np.random.default_rng(n + k)
+    A = _spd(rng, n).reshape(n, n, order="F")
+    A[k, k] = -float(n) * n
+    a0 = A.ravel(order="F").copy()
+    ref = a0.copy()
+    code, msg = O.potrf(uplo, n, ref, n)
+    assert code == k % 32 + 1
+    for rep in range(2):
+        a = a0.copy()
+        with pytest.raises(B.BolaError) as e:
+            B.potrf(uplo, n, a, n)
+        assert e.value.message == msg and e.value.code == k + 1
+        same(a, ref, f"potrf {uplo} state after the failure ({rep})")
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrs_config_bitexact(B, O, uplo):
+    n, nrhs = 4096, 64
+    rng = np.random.default_rng(4096)
+    f = _spd(rng, n)
+    assert O.potrf(uplo, n, f, n)[0] == 0
+    b = rng.uniform(-1, 1, n * nrhs)
+    ref = b.copy()
+    O.potrs(uplo, n, nrhs, f, n, ref, n)
+    B.potrs(uplo, n, nrhs, f, n, b, n)
+    same(b, ref, f"potrs {uplo} {n}x{nrhs}")
+
+
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("n", [20, 50, 77])
 def test_potrf2_bitexact(B, O, uplo, n):
