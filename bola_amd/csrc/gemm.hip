@@ -237,8 +237,12 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 
 // The reference's partial micro-tiles: rows >= edge_m or columns >= edge_n.
 // C = C*beta (or 0), then C += (alpha*b_lj)*a_il unfused for l = 0..k-1.
+// The chain is sequential in l; its operands are independent of it, so they are loaded
+// EK steps at a time into one of two register buffers while the other buffer's steps run
+// (an element's loads for steps l+EK .. l+2EK-1 are in flight during steps l .. l+EK-1).
+constexpr int EK = 32;
 template <bool TA, bool TB>
-__global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double alpha, const double* __restrict__ A,
+__global__ void __launch_bounds__(64) dgemm_edge(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int edge_m, int edge_n,
                                                   Abort abort, int mask) {
@@ -262,12 +266,32 @@ __global__ void __launch_bounds__(256) dgemm_edge(int m, int n, int k, double al
             c = C[i + (size_t)j * ldc];
             if (beta != 1.0) c = c * beta;
         }
-        if (alpha != 0.0) {
-            for (int l = 0; l < k; l++) {
-                double bv = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
-                double av = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
-                double temp = alpha * bv;
-                c = c + temp * av;
+        if (alpha != 0.0 && k > 0) {
+            double a0[EK], b0[EK], a1[EK], b1[EK];
+            auto load = [&](int l0, double (&av)[EK], double (&bv)[EK]) {
+#pragma unroll
+                for (int s = 0; s < EK; s++) {
+                    const int l = min(l0 + s, k - 1);  // clamped: every load in flight at once
+                    bv[s] = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+                    av[s] = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
+                }
+            };
+            auto steps = [&](int l0, const double (&av)[EK], const double (&bv)[EK]) {
+#pragma unroll
+                for (int s = 0; s < EK; s++) {
+                    if (l0 + s < k) {
+                        const double temp = alpha * bv[s];
+                        c = c + temp * av[s];
+                    }
+                }
+            };
+            load(0, a0, b0);
+            for (int l0 = 0; l0 < k; l0 += 2 * EK) {
+                if (l0 + EK < k) load(l0 + EK, a1, b1);
+                steps(l0, a0, b0);
+                if (l0 + EK >= k) break;
+                if (l0 + 2 * EK < k) load(l0 + 2 * EK, a0, b0);
+                steps(l0 + EK, a1, b1);
             }
         }
         C[i + (size_t)j * ldc] = c;
@@ -316,8 +340,9 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
     }
     long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
     if (nedge > 0) {
-        unsigned blocks = (unsigned)std::min<long>((nedge + 255) / 256, 4096);
-        hipLaunchKernelGGL((dgemm_edge<TA, TB>), dim3(blocks), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
+        // one wavefront per workgroup, so that a few thousand elements spread over the CUs
+        unsigned blocks = (unsigned)std::min<long>((nedge + 63) / 64, 16384);
+        hipLaunchKernelGGL((dgemm_edge<TA, TB>), dim3(blocks), dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
                            C, ldc, edge_m, edge_n, abort, mask);
         BOLA_LAUNCH_CHECK();
     }

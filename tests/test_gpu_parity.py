@@ -674,3 +674,26 @@ def test_device_flavour_matches_host(B, O):
     same(da.cpu().numpy(), ref, "dev lu")
     x = torch.from_numpy(rng.uniform(-1, 1, 1000)).cuda()
     assert B.idamax(1000, x) == O.idamax(1000, x.cpu().numpy())
+
+
+@pytest.mark.parametrize("n,nrhs", [(4096, 1), (4096, 3), (4096, 64), (8192, 64)])
+def test_getrf_getrs_config_bitexact(B, O, n, nrhs):
+    # the plain FP64 configuration (n = 4096 and 8192, nrhs = 64) and the mixed-precision
+    # configuration's few right-hand sides, whose updates all take the reference's unfused
+    # edge path (nrhs < 4): LU factors, pivots and solutions bit for bit, device flavour
+    import torch
+    rng = np.random.default_rng(n + nrhs)
+    a0 = rng.uniform(-1, 1, n * n)
+    b0 = rng.uniform(-1, 1, n * nrhs)
+    ref, rp, rb = a0.copy(), np.zeros(n, np.int32), b0.copy()
+    assert O.getrf(n, n, ref, n, rp)[0] == 0
+    O.getrs(n, nrhs, ref, n, rp, rb, n)
+    a = torch.from_numpy(a0).cuda()
+    p = torch.zeros(n, dtype=torch.int32, device="cuda")
+    b = torch.from_numpy(b0).cuda()
+    B.getrf(n, n, a, n, p)
+    B.getrs(n, nrhs, a, n, p, b, n)
+    torch.cuda.synchronize()
+    same(p.cpu().numpy(), rp, f"ipiv {n}")
+    same(a.cpu().numpy(), ref, f"lu {n}")
+    same(b.cpu().numpy(), rb, f"x {n}x{nrhs}")
