@@ -102,6 +102,7 @@ _SIGS = [
     ("bola_hip_dsyrk_dev", C.c_int, [_vp, _ch, _ch, _sz, _sz, _d, _vp, _sz, _d, _vp, _sz]),
     ("bola_hip_dgetrf", C.c_int, [_sz, _sz, _vp, _sz, _vp]),
     ("bola_hip_dgetrf_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp]),
+    ("bola_hip_dgetrf_noalloc_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp]),
     ("bola_hip_dgetrf2", C.c_int, [_sz, _sz, _vp, _sz, _vp]),
     ("bola_hip_dgetrf2_dev", C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, _vp]),
     ("bola_hip_dpotrf", C.c_int, [_ch, _sz, _vp, _sz]),
@@ -357,6 +358,14 @@ def getrf(m, n, a, lda, ipiv) -> None:
                                          _ptr(ipiv, np.int32), None), True)
     else:
         _check(lib().bola_hip_dgetrf(m, n, _ptr(a, np.float64), lda, _ptr(ipiv, np.int32)), False)
+
+
+def getrf_noalloc(m, n, a, lda, ipiv, handle=None) -> None:
+    """getrf.rs:37 on device tensors with no device allocation: the library's static
+    device scratch instead of the handle's workspaces (bit-identical to getrf)."""
+    h = handle or default_handle()
+    _check(lib().bola_hip_dgetrf_noalloc_dev(h.ptr, m, n, _ptr(a, np.float64), lda, _ptr(ipiv, np.int32), None),
+           True)
 
 
 def getrf2(m, n, a, lda, ipiv) -> None:

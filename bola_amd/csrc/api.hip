@@ -9,9 +9,7 @@
 
 namespace bola {
 
-// defined in getrf.hip / potrf.hip / trsm.hip
-void lu_factor(bola_hip_handle_s* h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv, int32_t* info,
-               bool recursive_entry);
+// defined in potrf.hip
 void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t lda, int32_t* info,
                  bool recursive_entry);
 void launch_dsyrk(hipStream_t st, char uplo, char trans, size_t n, size_t k, double alpha, const double* a, size_t lda,
@@ -449,6 +447,36 @@ int bola_hip_dgetrf_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size
     }
     int32_t* info = info_dev ? info_dev : h->d_info;
     lu_factor(h, m, n, a, lda, ipiv, info, false);
+    if (info_dev) return 0;
+    return finish_lu(h, info);
+    API_END(h)
+}
+
+int bola_hip_dgetrf_noalloc_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv,
+                                int32_t* info_dev) {
+    if (!h) return -1;
+    API_BEGIN(h)
+    if (lda < std::max<size_t>(1, m)) {
+        char buf[96];
+        snprintf(buf, sizeof buf, "Argument 4 to getrf had an illegal value of %zu", lda);
+        set_error(h, buf);
+        return -4;
+    }
+    size_t have = 0;
+    FixedScratch fx;
+    fx.ptr = noalloc_scratch(h, &have);
+    fx.bytes = have;
+    const size_t need = lu_scratch_bytes(m, n, false);
+    if (need > have) {
+        char buf[160];
+        snprintf(buf, sizeof buf,
+                 "getrf_noalloc: a %zu x %zu factorization needs %zu bytes of scratch, more than the %zu available",
+                 m, n, need, have);
+        set_error(h, buf);
+        return -2;
+    }
+    int32_t* info = info_dev ? info_dev : h->d_info;
+    lu_factor(h, m, n, a, lda, ipiv, info, false, &fx);
     if (info_dev) return 0;
     return finish_lu(h, info);
     API_END(h)

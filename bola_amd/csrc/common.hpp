@@ -97,6 +97,12 @@ __device__ __forceinline__ int abort_count(const Abort& a, int count, int per) {
     return (int)(c < 0 ? 0 : (c > count ? count : c));
 }
 
+// Device scratch handed to a factorization instead of the handle's workspaces.
+struct FixedScratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
 // A composed row permutation: row `row[s]` receives the old contents of row `src[s]`.
 // Produced by the LASWP compose step (one map per group of up to 64 interchanges) and by
 // the LU panel kernel (one map per panel).
@@ -268,6 +274,15 @@ void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t 
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
                      Abort abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
 void set_num_cus(int n);
+
+// getrf.hip: the blocked LU (getrf.rs:37-101) or, recursive_entry, getrf2 (getrf2.rs:39-97)
+// on the handle's streams; `fixed`: scratch to use instead of the handle's workspaces (at
+// least lu_scratch_bytes(m, n, recursive_entry) bytes).
+void lu_factor(bola_hip_handle_s* h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv, int32_t* info,
+               bool recursive_entry, const FixedScratch* fixed = nullptr);
+size_t lu_scratch_bytes(size_t m, size_t n, bool recursive_entry);
+// The no-allocation entry point's device scratch on the current device (bytes: its size).
+void* noalloc_scratch(bola_hip_handle_s* h, size_t* bytes);
 
 // trsm.hip
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,

@@ -27,6 +27,8 @@
 // (partially factored, panel pivots panel-local).
 #include "common.hpp"
 
+#include <mutex>
+
 namespace bola {
 
 namespace {
@@ -1388,10 +1390,12 @@ int wave_count(int m) {
 }
 int panel_slots(int m) { return std::max(panel_nwg(m), wave_count(m)); }
 
-LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
-    LuWs w;
-    w.nwg_max = std::max(1, panel_slots(m));
-    w.npanels = std::max(1, npanels);
+// The factorization's scratch: bytes, and the carve-up into `base` (which must hold them);
+// everything carved here is zeroed on the handle's stream by a kernel.
+size_t lu_ws_layout(int m, int npanels, LuWs* w, char* base) {
+    LuWs l;
+    l.nwg_max = std::max(1, panel_slots(m));
+    l.npanels = std::max(1, npanels);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
@@ -1401,21 +1405,32 @@ LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels) {
     // the slots start zeroed: the wavefront panel's epochs are unique only within one
     // factorization
     const size_t slot_bytes = std::max(sizeof(PanelSlot), sizeof(uint64_t) * TW);
-    size_t o_flags = take(sizeof(int) * (size_t)w.npanels * w.nwg_max);
-    size_t o_uflags = take(sizeof(int) * (size_t)w.npanels * 32);
-    size_t o_maps = take(sizeof(RowMap) * (size_t)w.npanels);
+    size_t o_flags = take(sizeof(int) * (size_t)l.npanels * l.nwg_max);
+    size_t o_uflags = take(sizeof(int) * (size_t)l.npanels * 32);
+    size_t o_maps = take(sizeof(RowMap) * (size_t)l.npanels);
     size_t o_jp = take(sizeof(int32_t) * 4);
-    size_t o_slots = take(slot_bytes * 4 * (size_t)w.nwg_max);
+    size_t o_slots = take(slot_bytes * 4 * (size_t)l.nwg_max);
     size_t o_cur = take(slot_bytes * 2);
-    size_t zero_bytes = off;
-    char* base = (char*)h->ws.get(off);
-    w.flags = (int*)(base + o_flags);
-    w.uflags = (int*)(base + o_uflags);
-    w.maps = (RowMap*)(base + o_maps);
-    w.jp = (int32_t*)(base + o_jp);
-    w.slots = (PanelSlot*)(base + o_slots);
-    w.cur = (PanelSlot*)(base + o_cur);
-    size_t nzero = zero_bytes / 4;
+    if (w) {
+        l.flags = (int*)(base + o_flags);
+        l.uflags = (int*)(base + o_uflags);
+        l.maps = (RowMap*)(base + o_maps);
+        l.jp = (int32_t*)(base + o_jp);
+        l.slots = (PanelSlot*)(base + o_slots);
+        l.cur = (PanelSlot*)(base + o_cur);
+        *w = l;
+    }
+    return off;
+}
+
+// fixed: caller-provided scratch (the no-allocation entry point), else the handle's
+LuWs lu_workspace(bola_hip_handle_s* h, int m, int npanels, const FixedScratch* fixed) {
+    LuWs w;
+    const size_t bytes = lu_ws_layout(m, npanels, nullptr, nullptr);
+    char* base = fixed ? (char*)fixed->ptr : (char*)h->ws.get(bytes);
+    if (fixed && bytes > fixed->bytes) throw HipFailure{hipErrorInvalidValue, "LU scratch too small", __LINE__};
+    lu_ws_layout(m, npanels, &w, base);
+    size_t nzero = bytes / 4;
     hipLaunchKernelGGL(k_zero_i32, dim3(std::min<size_t>((nzero + 255) / 256, 1024)), dim3(256), 0, h->stream,
                        (int32_t*)base, nzero);
     BOLA_LAUNCH_CHECK();
@@ -1668,8 +1683,17 @@ hipEvent_t lu_event(bola_hip_handle_s* h, int i) {
 // super-panel to all columns further right (the bulk GEMM), so the next factorization
 // overlaps the bulk update.  Work on the two streams never touches the same columns at
 // the same time; events order the hand-offs.
+size_t lu_scratch_bytes(size_t m_, size_t n_, bool recursive_entry) {
+    const int m = (int)m_, n = (int)n_;
+    const int min_mn = std::min(m, n);
+    if (m == 0 || n == 0) return 0;
+    if (recursive_entry || PW >= min_mn)
+        return lu_ws_layout(m, 1, nullptr, nullptr) + ((laswp_ws_bytes((size_t)min_mn) + 255) & ~(size_t)255);
+    return lu_ws_layout(m, (min_mn + PW - 1) / PW, nullptr, nullptr);
+}
+
 void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
-               bool recursive_entry) {
+               bool recursive_entry, const FixedScratch* fixed) {
     hipStream_t st = h->stream;
     const int m = (int)m_, n = (int)n_;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (size_t)2);
@@ -1678,14 +1702,21 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     const int min_mn = std::min(m, n);
     if (recursive_entry || PW >= min_mn) {
         Phase ph(h, recursive_entry ? "GETRF2" : "GETRF");
-        void* lws = h->ws_aux.get(laswp_ws_bytes((size_t)min_mn));
-        LuWs w = lu_workspace(h, m, 1);
+        const size_t lbytes = (laswp_ws_bytes((size_t)min_mn) + 255) & ~(size_t)255;
+        void* lws = fixed ? fixed->ptr : h->ws_aux.get(lbytes);
+        FixedScratch rest;  // the LU scratch after the LASWP group maps
+        if (fixed) {
+            if (lbytes > fixed->bytes) throw HipFailure{hipErrorInvalidValue, "LU scratch too small", __LINE__};
+            rest.ptr = (char*)fixed->ptr + lbytes;
+            rest.bytes = fixed->bytes - lbytes;
+        }
+        LuWs w = lu_workspace(h, m, 1, fixed ? &rest : nullptr);
         getrf2_rec(h, m, n, a, lda, ipiv, info, 0, w, lws);
         return;
     }
     Phase ph(h, "GETRF", 2.0 * (double)min_mn * min_mn * (3.0 * std::max(m, n) - min_mn) / 6.0);
     const int npanels = (min_mn + PW - 1) / PW;
-    LuWs w = lu_workspace(h, m, npanels);
+    LuWs w = lu_workspace(h, m, npanels, fixed);
     if (m % 8 != 0) {
         // The reference's GEMM computes the last (m-j-32) % 8 rows of each trailing update
         // unfused; later interchanges move such rows, so interchanges cannot be deferred
@@ -1761,6 +1792,37 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     }
     if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
     if (left_done) BOLA_CHECK(hipStreamWaitEvent(st, left_done, 0));
+}
+
+// ---- scratch of the no-allocation LU entry point
+//
+// A file-scope device array: every GPU the library runs on holds its own copy for the
+// life of the process, and every call on that GPU shares it (calls that overlap on one
+// GPU race on it).  In host code `scratch` names a host-side stand-in, not device memory:
+// the device address comes from hipGetSymbolAddress, once per device, with that device
+// current, and only that address is handed to kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+void* noalloc_scratch(bola_hip_handle_s* h, size_t* bytes) {
+    int dev = -1;
+    BOLA_CHECK(hipGetDevice(&dev));
+    if (dev != h->device) throw HipFailure{hipErrorInvalidDevice, "the handle's device is not current", __LINE__};
+    if (dev < 0 || dev >= kMaxDevices) throw HipFailure{hipErrorInvalidDevice, "device index", __LINE__};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        BOLA_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = p;
+    }
+    *bytes = kNoAllocScratchBytes;
+    return g_scratch_dev[dev];
 }
 
 }  // namespace bola

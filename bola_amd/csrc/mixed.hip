@@ -26,9 +26,6 @@
 
 namespace bola {
 
-// lu.hip / getrf.hip
-void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
-               bool recursive_entry);
 
 namespace {
 

@@ -131,6 +131,16 @@ int bola_hip_dgetrf(size_t m, size_t n, double* a, size_t lda, int32_t* ipiv);
  * it returns at once and the status (0 / +k) is written to *info_dev. */
 int bola_hip_dgetrf_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv,
                         int32_t* info_dev);
+/* No counterpart in the reference: bola_hip_dgetrf_dev for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results (bit for bit); the factorization's
+ * scratch (panel exchange slots, flags, row maps) is a static device array of 4 MiB that
+ * each GPU holds until the process ends, enough for m up to 32768 at any n.  Nothing is
+ * allocated or freed.  Calls that overlap on one GPU race on that array.  -2: the
+ * factorization needs more scratch than the array holds.  info_dev NULL: synchronous,
+ * as bola_hip_dgetrf_dev. */
+int bola_hip_dgetrf_noalloc_dev(bola_hip_handle_t h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv,
+                                int32_t* info_dev);
 
 /* getrf2, getrf2.rs:39 -- the crate's unblocked/recursive panel LU, callable directly. */
 int bola_hip_dgetrf2(size_t m, size_t n, double* a, size_t lda, int32_t* ipiv);
