@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <mutex>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "common.hpp"
 
 namespace bola {
@@ -37,7 +39,11 @@ hipEvent_t take_event(bola_hip_handle_s* h) {
 
 }  // namespace
 
+// Every phase is also a roctx range under its name (the reference's ScopedTimer names,
+// profiling.rs:30-57), so a rocprofv3 --marker-trace shows the host side of each phase
+// beside the kernels it enqueued.
 Phase::Phase(bola_hip_handle_s* h_, const char* n, double fl, hipStream_t s) : h(h_), name(n), flops(fl) {
+    roctxRangePushA(n);
     st = s ? s : (h ? h->stream : nullptr);
     if (h && h->profile) {
         e0 = take_event(h);
@@ -46,6 +52,7 @@ Phase::Phase(bola_hip_handle_s* h_, const char* n, double fl, hipStream_t s) : h
 }
 
 Phase::~Phase() {
+    roctxRangePop();
     if (h && h->profile && e0) {
         hipEvent_t e1 = nullptr;
         try {
