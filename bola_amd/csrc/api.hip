@@ -164,14 +164,17 @@ struct Stager {
     T* at(size_t off) { return (T*)(base + off); }
 };
 
-void h2d_mat(hipStream_t st, double* d, size_t dld, const double* hsrc, size_t ld, size_t rows, size_t cols) {
-    if (rows == 0 || cols == 0) return;
-    BOLA_CHECK(hipMemcpy2DAsync(d, dld * 8, hsrc, ld * 8, rows * 8, cols, hipMemcpyHostToDevice, st));
-}
-void d2h_mat(hipStream_t st, double* hdst, size_t ld, const double* d, size_t dld, size_t rows, size_t cols) {
-    if (rows == 0 || cols == 0) return;
-    BOLA_CHECK(hipMemcpy2DAsync(hdst, ld * 8, d, dld * 8, rows * 8, cols, hipMemcpyDeviceToHost, st));
-}
+// A host-pointer LU: columns uploaded as the factorization reaches them, row strips
+// downloaded as they become final.
+struct HostLuHooks : LuHooks {
+    HostUpload& up;
+    HostDownload& dn;
+    HostLuHooks(HostUpload& u, HostDownload& d) : up(u), dn(d) {}
+    void need_cols(hipStream_t s, int c_end) override { up.need(s, (size_t)c_end); }
+    void rows_done(int r0, int r1, const hipStream_t* after, int nafter) override {
+        dn.strip((size_t)r0, (size_t)r1, after, nafter);
+    }
+};
 
 bool is_ch(char c, char w) { return c == w || c == (char)(w + 32); }
 
@@ -263,6 +266,7 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     h->mws2.release();
     h->bws.release();
     h->cws.release();
+    release_pinned_stage(h);
     if (h->d_info) (void)hipFree(h->d_info);
     for (auto e : h->sync_events) (void)hipEventDestroy(e);
     if (h->ws_event) (void)hipEventDestroy(h->ws_event);
@@ -613,11 +617,11 @@ int bola_hip_dlaswp(size_t n, double* a, size_t lda, int32_t k1, int32_t k2, con
     Stager s{h};
     size_t oa = s.add(rows * n * 8), op = s.add(npiv_span * 4);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), rows, a, lda, rows, n);
+    h2d_mat(h, s.at<double>(oa), rows, a, lda, rows, n);
     BOLA_CHECK(hipMemcpyAsync(s.at<int32_t>(op), ipiv, npiv_span * 4, hipMemcpyHostToDevice, h->stream));
     void* ws = h->ws_aux.get(laswp_ws_bytes((size_t)(k2 - k1) + 1));
     launch_laswp(h->stream, n, s.at<double>(oa), rows, k1, k2, s.at<int32_t>(op), incx, ws, nullptr);
-    d2h_mat(h->stream, a, lda, s.at<double>(oa), rows, rows, n);
+    d2h_mat(h, a, lda, s.at<double>(oa), rows, rows, n);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -636,9 +640,9 @@ int bola_hip_dger(size_t m, size_t n, double alpha, const double* x, const doubl
     s.commit();
     BOLA_CHECK(hipMemcpyAsync(s.at<double>(ox), x, m * 8, hipMemcpyHostToDevice, h->stream));
     BOLA_CHECK(hipMemcpyAsync(s.at<double>(oy), y, span(n, ldy) * 8, hipMemcpyHostToDevice, h->stream));
-    h2d_mat(h->stream, s.at<double>(oa), m, a, lda, m, n);
+    h2d_mat(h, s.at<double>(oa), m, a, lda, m, n);
     launch_dger(h->stream, m, n, alpha, s.at<double>(ox), s.at<double>(oy), ldy, s.at<double>(oa), m);
-    d2h_mat(h->stream, a, lda, s.at<double>(oa), m, m, n);
+    d2h_mat(h, a, lda, s.at<double>(oa), m, m, n);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -712,13 +716,13 @@ int bola_hip_dgemm(char transa, char transb, size_t m, size_t n, size_t k, doubl
     s.commit();
     const bool use_ab = alpha != 0.0 && k > 0;
     if (use_ab) {
-        h2d_mat(h->stream, s.at<double>(oa), ar, a, lda, ar, ac);
-        h2d_mat(h->stream, s.at<double>(ob), br, b, ldb, br, bc);
+        h2d_mat(h, s.at<double>(oa), ar, a, lda, ar, ac);
+        h2d_mat(h, s.at<double>(ob), br, b, ldb, br, bc);
     }
-    h2d_mat(h->stream, s.at<double>(oc), m, c, ldc, m, n);
+    h2d_mat(h, s.at<double>(oc), m, c, ldc, m, n);
     gemm_dev(h, transa, transb, m, n, k, alpha, s.at<double>(oa), std::max<size_t>(ar, 1), s.at<double>(ob),
              std::max<size_t>(br, 1), beta, s.at<double>(oc), m);
-    d2h_mat(h->stream, c, ldc, s.at<double>(oc), m, m, n);
+    d2h_mat(h, c, ldc, s.at<double>(oc), m, m, n);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -734,10 +738,10 @@ int bola_hip_dtrsm(char side, char uplo, char transa, char diag, size_t m, size_
     Stager s{h};
     size_t oa = s.add(na * na * 8), ob = s.add(m * n * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), na, a, lda, na, na);
-    h2d_mat(h->stream, s.at<double>(ob), m, b, ldb, m, n);
+    h2d_mat(h, s.at<double>(oa), na, a, lda, na, na);
+    h2d_mat(h, s.at<double>(ob), m, b, ldb, m, n);
     launch_dtrsm(h->stream, side, uplo, transa, diag, m, n, alpha, s.at<double>(oa), na, s.at<double>(ob), m, nullptr);
-    d2h_mat(h->stream, b, ldb, s.at<double>(ob), m, m, n);
+    d2h_mat(h, b, ldb, s.at<double>(ob), m, m, n);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -754,11 +758,11 @@ int bola_hip_dsyrk(char uplo, char trans, size_t n, size_t k, double alpha, cons
     Stager s{h};
     size_t oa = s.add(std::max<size_t>(ar * ac, 1) * 8), oc = s.add(n * n * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), ar, a, lda, ar, ac);
-    h2d_mat(h->stream, s.at<double>(oc), n, c, ldc, n, n);
+    h2d_mat(h, s.at<double>(oa), ar, a, lda, ar, ac);
+    h2d_mat(h, s.at<double>(oc), n, c, ldc, n, n);
     launch_dsyrk(h->stream, uplo, trans, n, k, alpha, s.at<double>(oa), std::max<size_t>(ar, 1), beta,
                  s.at<double>(oc), n, nullptr);
-    d2h_mat(h->stream, c, ldc, s.at<double>(oc), n, n, n);
+    d2h_mat(h, c, ldc, s.at<double>(oc), n, n, n);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -779,10 +783,12 @@ static int getrf_host(size_t m, size_t n, double* a, size_t lda, int32_t* ipiv, 
     Stager s{h};
     size_t oa = s.add(m * n * 8), op = s.add(mn * 4);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), m, a, lda, m, n);
     BOLA_CHECK(hipMemcpyAsync(s.at<int32_t>(op), ipiv, mn * 4, hipMemcpyHostToDevice, h->stream));
-    lu_factor(h, m, n, s.at<double>(oa), m, s.at<int32_t>(op), h->d_info, rec);
-    d2h_mat(h->stream, a, lda, s.at<double>(oa), m, m, n);
+    HostUpload up(h, s.at<double>(oa), m, a, lda, m, n);
+    HostDownload dn(h, a, lda, s.at<double>(oa), m, m, n, true);
+    HostLuHooks hooks(up, dn);
+    lu_factor(h, m, n, s.at<double>(oa), m, s.at<int32_t>(op), h->d_info, rec, nullptr, &hooks);
+    dn.finish();
     BOLA_CHECK(hipMemcpyAsync(ipiv, s.at<int32_t>(op), mn * 4, hipMemcpyDeviceToHost, h->stream));
     return finish_lu(h, h->d_info);
     API_END(h)
@@ -814,9 +820,9 @@ static int potrf_host(char uplo, size_t n, double* a, size_t lda, bool rec) {
     Stager s{h};
     size_t oa = s.add(n * n * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), n, a, lda, n, n);
+    h2d_mat(h, s.at<double>(oa), n, a, lda, n, n);
     chol_factor(h, uplo, n, s.at<double>(oa), n, h->d_info, rec);
-    d2h_mat(h->stream, a, lda, s.at<double>(oa), n, n, n);
+    d2h_mat(h, a, lda, s.at<double>(oa), n, n, n);
     return finish_chol(h, h->d_info, rec || n <= 32);
     API_END(h)
 }
@@ -837,12 +843,12 @@ int bola_hip_dgetrs(size_t n, size_t nrhs, const double* lu, size_t lda, const i
     Stager s{h};
     size_t oa = s.add(n * n * 8), op = s.add(n * 4), ob = s.add(n * nrhs * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), n, lu, lda, n, n);
+    h2d_mat(h, s.at<double>(oa), n, lu, lda, n, n);
     BOLA_CHECK(hipMemcpyAsync(s.at<int32_t>(op), ipiv, n * 4, hipMemcpyHostToDevice, h->stream));
-    h2d_mat(h->stream, s.at<double>(ob), n, b, ldb, n, nrhs);
+    h2d_mat(h, s.at<double>(ob), n, b, ldb, n, nrhs);
     int r = bola_hip_dgetrs_dev(h, n, nrhs, s.at<double>(oa), n, s.at<int32_t>(op), s.at<double>(ob), n);
     if (r) return r;
-    d2h_mat(h->stream, b, ldb, s.at<double>(ob), n, n, nrhs);
+    d2h_mat(h, b, ldb, s.at<double>(ob), n, n, nrhs);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -856,11 +862,11 @@ int bola_hip_dpotrs(char uplo, size_t n, size_t nrhs, const double* f, size_t ld
     Stager s{h};
     size_t oa = s.add(n * n * 8), ob = s.add(n * nrhs * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), n, f, lda, n, n);
-    h2d_mat(h->stream, s.at<double>(ob), n, b, ldb, n, nrhs);
+    h2d_mat(h, s.at<double>(oa), n, f, lda, n, n);
+    h2d_mat(h, s.at<double>(ob), n, b, ldb, n, nrhs);
     int r = bola_hip_dpotrs_dev(h, uplo, n, nrhs, s.at<double>(oa), n, s.at<double>(ob), n);
     if (r) return r;
-    d2h_mat(h->stream, b, ldb, s.at<double>(ob), n, n, nrhs);
+    d2h_mat(h, b, ldb, s.at<double>(ob), n, n, nrhs);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return 0;
     API_END(h)
@@ -880,12 +886,12 @@ int bola_hip_dgesv_mixed(int precision, size_t n, size_t nrhs, const double* a, 
     Stager s{h};
     size_t oa = s.add(n * n * 8), ob = s.add(n * nrhs * 8), ox = s.add(n * nrhs * 8);
     s.commit();
-    h2d_mat(h->stream, s.at<double>(oa), n, a, lda, n, n);
-    h2d_mat(h->stream, s.at<double>(ob), n, b, ldb, n, nrhs);
+    h2d_mat(h, s.at<double>(oa), n, a, lda, n, n);
+    h2d_mat(h, s.at<double>(ob), n, b, ldb, n, nrhs);
     int r = bola_hip_dgesv_mixed_dev(h, precision, n, nrhs, s.at<double>(oa), n, s.at<double>(ob), n, s.at<double>(ox),
                                      n, stats);
     if (r < 0) return r;
-    d2h_mat(h->stream, x, ldx, s.at<double>(ox), n, n, nrhs);
+    d2h_mat(h, x, ldx, s.at<double>(ox), n, n, nrhs);
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     return r;
     API_END(h)

@@ -16,6 +16,7 @@
 struct bola_hip_handle_s;
 
 namespace bola {
+struct PinnedStage;  // staging.hip
 
 // Thrown inside the library only; every C-ABI entry point catches it.
 struct HipFailure {
@@ -152,6 +153,7 @@ struct bola_hip_handle_s {
     hipStream_t stream3 = nullptr;  // a third one: side work that must not wait for the second
     std::vector<hipEvent_t> sync_events;
     hipEvent_t ws_event = nullptr;  // orders workspace growth against the side streams
+    bola::PinnedStage* pin = nullptr;  // host-pointer calls: pinned staging, copy stream, copy threads
     hipStream_t aux_stream() {
         if (!stream2) {
             if (hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking) != hipSuccess) stream2 = nullptr;
@@ -275,11 +277,59 @@ void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t 
                      Abort abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
 void set_num_cus(int n);
 
+// staging.hip: host <-> device transfers of the host-pointer entry points through pinned
+// memory.  An upload makes a stream wait for the columns it needs (need), so work on the
+// first columns can start while later ones are still in transit; a download copies row
+// strips back once the streams that finalise them have queued their work (strip), and
+// finish() returns when the caller's matrix holds all of them.
+void h2d_mat(bola_hip_handle_s* h, double* d, size_t dld, const double* hsrc, size_t ld, size_t rows, size_t cols);
+void d2h_mat(bola_hip_handle_s* h, double* hdst, size_t ld, const double* d, size_t dld, size_t rows, size_t cols);
+void release_pinned_stage(bola_hip_handle_s* h);
+struct HostUpload {
+    HostUpload(bola_hip_handle_s* h, double* dst, size_t dld, const double* src, size_t ld, size_t rows, size_t cols);
+    void need(hipStream_t s, size_t c_end);  // s waits until columns [0, c_end) are on the device
+    bola_hip_handle_s* h;
+    double* dst;
+    size_t dld;
+    const double* src;
+    size_t ld, rows, cols;
+    bool direct = false;
+    PinnedStage* p = nullptr;
+    size_t chunk = 0, done = 0;
+    hipEvent_t ready = nullptr;
+};
+struct HostDownload {
+    // shares_upload_buffer: the matrix came in through a HostUpload of the same shape
+    HostDownload(bola_hip_handle_s* h, double* dst, size_t ld, const double* src, size_t dld, size_t rows, size_t cols,
+                 bool shares_upload_buffer);
+    void strip(size_t r0, size_t r1, const hipStream_t* after, int nafter);  // rows [r0, r1) are final
+    void finish();
+    struct Strip {
+        size_t r0, r1;
+        hipEvent_t ev;
+    };
+    bola_hip_handle_s* h;
+    double* dst;
+    size_t ld;
+    const double* src;
+    size_t dld, rows, cols;
+    bool direct = false;
+    PinnedStage* p = nullptr;
+    std::vector<Strip> strips;
+};
+// What a factorization tells its caller while it queues work: columns it is about to
+// need, rows that no later work changes.
+struct LuHooks {
+    virtual void need_cols(hipStream_t s, int c_end) = 0;
+    virtual void rows_done(int r0, int r1, const hipStream_t* after, int nafter) = 0;
+    virtual ~LuHooks() = default;
+};
+
 // getrf.hip: the blocked LU (getrf.rs:37-101) or, recursive_entry, getrf2 (getrf2.rs:39-97)
 // on the handle's streams; `fixed`: scratch to use instead of the handle's workspaces (at
 // least lu_scratch_bytes(m, n, recursive_entry) bytes).
 void lu_factor(bola_hip_handle_s* h, size_t m, size_t n, double* a, size_t lda, int32_t* ipiv, int32_t* info,
-               bool recursive_entry, const FixedScratch* fixed = nullptr);
+               bool recursive_entry, const FixedScratch* fixed = nullptr, LuHooks* hooks = nullptr);
 size_t lu_scratch_bytes(size_t m, size_t n, bool recursive_entry);
 // The no-allocation entry point's device scratch on the current device (bytes: its size).
 void* noalloc_scratch(bola_hip_handle_s* h, size_t* bytes);

@@ -1693,7 +1693,7 @@ size_t lu_scratch_bytes(size_t m_, size_t n_, bool recursive_entry) {
 }
 
 void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda, int32_t* ipiv, int32_t* info,
-               bool recursive_entry, const FixedScratch* fixed) {
+               bool recursive_entry, const FixedScratch* fixed, LuHooks* hooks) {
     hipStream_t st = h->stream;
     const int m = (int)m_, n = (int)n_;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, (size_t)2);
@@ -1711,13 +1711,16 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
             rest.bytes = fixed->bytes - lbytes;
         }
         LuWs w = lu_workspace(h, m, 1, fixed ? &rest : nullptr);
+        if (hooks) hooks->need_cols(st, n);
         getrf2_rec(h, m, n, a, lda, ipiv, info, 0, w, lws);
+        if (hooks) hooks->rows_done(0, m, &st, 1);
         return;
     }
     Phase ph(h, "GETRF", 2.0 * (double)min_mn * min_mn * (3.0 * std::max(m, n) - min_mn) / 6.0);
     const int npanels = (min_mn + PW - 1) / PW;
     LuWs w = lu_workspace(h, m, npanels, fixed);
     if (m % 8 != 0) {
+        if (hooks) hooks->need_cols(st, n);
         // The reference's GEMM computes the last (m-j-32) % 8 rows of each trailing update
         // unfused; later interchanges move such rows, so interchanges cannot be deferred
         // past an update.  Keep the reference's own order: panel, interchanges, TRSM, GEMM.
@@ -1748,6 +1751,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
                 }
             }
         }
+        if (hooks) hooks->rows_done(0, m, &st, 1);
         return;
     }
     // Three streams.  Stream 1 (critical): the super-panel's panels, then the next
@@ -1763,6 +1767,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     hipEvent_t left_done = nullptr;  // the last left interchanges (stream 3)
     for (int s0 = 0; s0 < min_mn; s0 += W) {
         const int sw = std::min(W, min_mn - s0);
+        if (hooks) hooks->need_cols(st, s0 + sw);
         factor_super(h, st, m, n, a, lda, ipiv, info, w, s0, sw, nullptr);
         hipEvent_t fact_done = lu_event(h, ev++);
         BOLA_CHECK(hipEventRecord(fact_done, st));
@@ -1780,14 +1785,23 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
         if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
         const int c_lo = s0 + sw;
         const int c_mid = std::min(n, c_lo + W);
+        if (hooks) hooks->need_cols(st, c_mid);
         update_cols(h, st, m, n, a, lda, info, w, s0, sw, c_lo, c_mid);
         if (c_mid < n) {
             BOLA_CHECK(hipStreamWaitEvent(s2, fact_done, 0));
+            if (hooks) hooks->need_cols(s2, n);
             update_cols(h, s2, m, n, a, lda, info, w, s0, sw, c_mid, n);
             bulk_done = lu_event(h, ev++);
             BOLA_CHECK(hipEventRecord(bulk_done, s2));
         } else {
             bulk_done = nullptr;
+        }
+        if (hooks) {
+            // rows [s0, s0 + sw) are final once this super-panel's work on every stream is
+            // done: no later interchange or update touches them (the last strip: to m)
+            hipStream_t after[3] = {st, s2, s3};
+            const bool last = s0 + W >= min_mn;
+            hooks->rows_done(s0, last ? m : s0 + sw, after, 3);
         }
     }
     if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));

@@ -697,3 +697,45 @@ def test_getrf_getrs_config_bitexact(B, O, n, nrhs):
     same(p.cpu().numpy(), rp, f"ipiv {n}")
     same(a.cpu().numpy(), ref, f"lu {n}")
     same(b.cpu().numpy(), rb, f"x {n}x{nrhs}")
+
+
+@pytest.mark.parametrize("m,n,ld", [(8192, 8192, 8192), (3000, 2500, 3011), (2600, 2601, 2600), (700, 700, 705)])
+def test_host_flavour_getrf_matches_the_device_flavour(B, m, n, ld):
+    # the host-pointer entry point stages through pinned memory, uploads columns while the
+    # factorization runs and downloads row strips as they become final: the caller's
+    # matrix (including the padding rows of a larger leading dimension) and ipiv must be
+    # exactly what the device flavour leaves
+    import torch
+    rng = np.random.default_rng(m + n + ld)
+    a = rng.uniform(-1, 1, ld * n)
+    pad = a.reshape(n, ld)[:, m:].copy()
+    dev = torch.from_numpy(a.reshape(n, ld)[:, :m].copy().ravel()).cuda()
+    dp = torch.zeros(min(m, n), dtype=torch.int32, device="cuda")
+    B.getrf(m, n, dev, m, dp)
+    hp = np.zeros(min(m, n), np.int32)
+    B.getrf(m, n, a, ld, hp)
+    same(hp, dp.cpu().numpy(), "ipiv")
+    same(a.reshape(n, ld)[:, :m].ravel(), dev.cpu().numpy(), "factors")
+    same(a.reshape(n, ld)[:, m:], pad, "rows beyond m untouched")
+
+
+def test_host_flavour_large_operands(B, O):
+    # pinned, chunked transfers for every operand of a host-pointer GEMM and GETRS
+    rng = np.random.default_rng(77)
+    m, n, k = 1536, 1100, 1300
+    a = rng.uniform(-1, 1, m * k)
+    b = rng.uniform(-1, 1, k * n)
+    c = rng.uniform(-1, 1, m * n)
+    ref = c.copy()
+    O.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 1.0, ref, m)
+    B.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 1.0, c, m)
+    same(c, ref, "host gemm")
+    n2, nrhs = 1500, 200
+    lu = rng.uniform(-1, 1, n2 * n2)
+    p = np.zeros(n2, np.int32)
+    assert O.getrf(n2, n2, lu, n2, p)[0] == 0
+    x = rng.uniform(-1, 1, n2 * nrhs)
+    rx = x.copy()
+    O.getrs(n2, nrhs, lu, n2, p, rx, n2)
+    B.getrs(n2, nrhs, lu, n2, p, x, n2)
+    same(x, rx, "host getrs")
