@@ -115,6 +115,11 @@ _SIGS = [
     ("bola_hip_dpotrs_dev", C.c_int, [_vp, _ch, _sz, _sz, _vp, _sz, _vp, _sz]),
     ("bola_hip_dgesv_mixed", C.c_int, [C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
     ("bola_hip_dgesv_mixed_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
+    ("bola_hip_factor_create_dev", C.c_int, [_vp, C.c_int, C.c_int, _sz, _vp, _sz, _vp]),
+    ("bola_hip_factor_solve_dev", C.c_int, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, _vp]),
+    ("bola_hip_factor_describe", C.c_int, [_vp, _vp]),
+    ("bola_hip_factor_bytes", C.c_size_t, [C.c_int, C.c_int, _sz]),
+    ("bola_hip_factor_destroy", C.c_int, [_vp]),
     ("bola_hip_sgemm_lp_dev", C.c_int, [_vp, C.c_int, _sz, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _sz]),
     ("bola_hip_batch_solve_dev", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("bola_hip_batch_tune_dev", C.c_int, [_vp, C.c_int, _sz, _vp, _vp, _vp]),
@@ -449,6 +454,76 @@ def dgesv_mixed(n, nrhs, a, lda, b, ldb, x, ldx, precision="fp32", rbt=False) ->
     return st.as_dict()
 
 
+FACTOR_LU, FACTOR_CHOL_L, FACTOR_CHOL_U, FACTOR_MIXED = 0, 1, 2, 3
+
+
+class FactorDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("precision", C.c_int32), ("n", C.c_size_t), ("bytes", C.c_size_t),
+                ("factor", C.c_void_p), ("ld", C.c_size_t), ("ipiv", C.c_void_p), ("row_scale", C.c_void_p),
+                ("col_scale", C.c_void_p), ("matrix", C.c_void_p), ("butterfly", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
+class Factor:
+    """A factorization kept on the device for repeated right-hand sides (the C ABI's
+    bola_hip_factor_*): kind FACTOR_LU / FACTOR_CHOL_L / FACTOR_CHOL_U (the reference's
+    FP64 factorizations, bit-identical) or FACTOR_MIXED (the mixed-precision solver's
+    factorization, `precision` and `rbt` as in dgesv_mixed).  `a`: numpy array or CUDA
+    tensor, not modified."""
+
+    def __init__(self, kind, n, a, lda=None, precision="fp32", rbt=False):
+        p = (PRECISIONS[precision] | (MIX_RBT if rbt else 0)) if kind == FACTOR_MIXED else 0
+        f = C.c_void_p()
+        rc = lib().bola_hip_factor_create_dev(default_handle().ptr, kind, p, n, _ptr(a, np.float64), lda or n,
+                                              C.byref(f))
+        _check(rc, True)
+        self._f = f
+        self.n = n
+        self.kind = kind
+
+    def solve(self, nrhs, b, x, ldb=None, ldx=None) -> dict:
+        """x = A^-1 b on CUDA tensors (x may be b for the FP64 kinds); solver statistics."""
+        st = MixedStats()
+        rc = lib().bola_hip_factor_solve_dev(default_handle().ptr, self._f, nrhs, _ptr(b, np.float64),
+                                             ldb or self.n, _ptr(x, np.float64), ldx or self.n, C.byref(st))
+        _check(rc, True)
+        return st.as_dict()
+
+    def describe(self) -> dict:
+        d = FactorDesc()
+        lib().bola_hip_factor_describe(self._f, C.byref(d))
+        return d.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_f", None):
+            lib().bola_hip_factor_destroy(self._f)
+            self._f = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def factor_bytes(kind, n, precision="fp32", rbt=False) -> int:
+    p = (PRECISIONS[precision] | (MIX_RBT if rbt else 0)) if kind == FACTOR_MIXED else 0
+    return int(lib().bola_hip_factor_bytes(kind, p, n))
+
+
+def serve_mixed(precision="fp32", rbt=False) -> int:
+    """request precision code of a mixed-precision served LU solve"""
+    return 1 + (PRECISIONS[precision] | (MIX_RBT if rbt else 0))
+
+
 def sgemm_lp(m, n, k, a, lda, b, ldb, c, ldc, precision="fp32") -> None:
     """C -= A B on fp32 device tensors, products in `precision` (the mixed LU's update)."""
     _check(lib().bola_hip_sgemm_lp_dev(default_handle().ptr, PRECISIONS[precision], m, n, k, _ptr(a, np.float32),
@@ -538,7 +613,7 @@ ERR_CANCELLED = -1001
 
 
 class _Request(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("n", C.c_size_t), ("nrhs", C.c_size_t),
+    _fields_ = [("kind", C.c_int32), ("precision", C.c_int32), ("n", C.c_size_t), ("nrhs", C.c_size_t),
                 ("a", C.c_void_p), ("b", C.c_void_p), ("x", C.c_void_p), ("factor_key", C.c_uint64)]
 
 
@@ -577,8 +652,9 @@ class Server:
         self._s = s
         self._live = {}
 
-    def submit(self, kind, n, nrhs, a, b, x, key=0) -> int:
-        r = _Request(kind, 0, n, nrhs, _ptr(a, np.float64) if a is not None else None, _ptr(b, np.float64),
+    def submit(self, kind, n, nrhs, a, b, x, key=0, precision=0) -> int:
+        """precision: 0 (FP64, the reference's factorization) or serve_mixed(...) (LU only)."""
+        r = _Request(kind, precision, n, nrhs, _ptr(a, np.float64) if a is not None else None, _ptr(b, np.float64),
                      _ptr(x, np.float64), key)
         t = C.c_uint64()
         rc = lib().bola_hip_server_submit(self._s, C.byref(r), C.byref(t))

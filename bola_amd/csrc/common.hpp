@@ -334,6 +334,12 @@ size_t lu_scratch_bytes(size_t m, size_t n, bool recursive_entry);
 // The no-allocation entry point's device scratch on the current device (bytes: its size).
 void* noalloc_scratch(bola_hip_handle_s* h, size_t* bytes);
 
+// mixed.hip: factorization objects (the C ABI's bola_hip_factor_*); memory from the
+// stream-ordered allocator on the handle's stream when stream_ordered.
+bola_hip_factor_s* factor_build(bola_hip_handle_s* h, int kind, int precision, size_t n, const double* a, size_t lda,
+                                int* info, bool stream_ordered);
+void factor_release(bola_hip_factor_s* f, hipStream_t st);
+
 // trsm.hip
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
                   const double* a, size_t lda, double* b, size_t ldb, Abort abort = Abort());

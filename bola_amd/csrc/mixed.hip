@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <string>
 #include <vector>
 
 namespace bola {
@@ -694,95 +696,128 @@ int gmres_cycle(Solver& S, double* Dout, double tol_rel) {
 
 }  // namespace bola
 
-using namespace bola;
+namespace bola {
 
-extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n_, size_t nrhs_, const double* A,
-                                        size_t lda, const double* B, size_t ldb, double* X, size_t ldx,
-                                        bola_hip_mixed_stats* stats) {
-    BOLA_API_BEGIN
-    if (!h) return -1;
-    const bool want_rbt = (precision & BOLA_HIP_MIX_RBT) != 0;
-    precision &= 0xFF;
-    if (precision > 2) {
-        set_error(h, "Argument 2 to dgesv_mixed had an illegal value");
-        return -2;
-    }
-    if (lda < std::max<size_t>(1, n_)) {
-        set_error(h, "Argument 6 to dgesv_mixed had an illegal value");
-        return -6;
-    }
-    if (nrhs_ > (size_t)NRMAX) {
-        set_error(h, "Argument 4 to dgesv_mixed had an illegal value (at most 16 right-hand sides)");
-        return -4;
-    }
-    if (n_ > (size_t)mixed_lu_max_n()) {
-        set_error(h, "Argument 3 to dgesv_mixed had an illegal value (n above the supported maximum)");
-        return -3;
-    }
-    bola_hip_mixed_stats loc{};
-    bola_hip_mixed_stats& so = stats ? *stats : loc;
-    so = bola_hip_mixed_stats{};
-    if (n_ == 0 || nrhs_ == 0) return 0;
-    const int n = (int)n_, nrhs = (int)nrhs_;
-    Solver S;
-    S.h = h;
-    S.st = h->stream;
-    S.n = n;
-    S.nrhs = nrhs;
-    S.lp = precision;
-    S.A = A;
-    S.lda = lda;
-    S.mgm = 30;
-    S.ldt = ((size_t)n + 63) & ~(size_t)63;
-    S.rbt = want_rbt && n % 4 == 0 && n >= 8;  // two butterfly levels need n divisible by 4
-    // workspace carve-up
+// ---- the mixed-precision factorization as an object: factor once, solve many times
+//
+// Owned by a MixedFactor: the scaled, transposed low-precision factors T (ldt x n), the
+// inverses of their diagonal blocks, pivots (tournament) or butterflies (RBT), the
+// power-of-two scalings, |A| row sums (for |A|_inf); the FP64 matrix the residuals use
+// (the caller's, or a copy); and, once refinement has failed for it, its FP64 LU.
+struct MixedFactor {
+    int n = 0, lp = 0;
+    bool rbt = false;
+    size_t ldt = 0;
+    float* T = nullptr;
+    float* Dinv = nullptr;
+    int32_t* ipiv = nullptr;
+    int* perm = nullptr;
+    double *s = nullptr, *t = nullptr, *rsum = nullptr;
+    double* rl[2] = {};
+    double* rr[2] = {};
+    const double* A = nullptr;
+    size_t lda = 0;
+    double anorm = 0.0;
+    int lu_err = 0;          // a factorization kernel gave up waiting: refinement is not trusted
+    float factor_ms = 0.f;
+    double* F = nullptr;     // FP64 LU of A (ld n) and its pivots, after a failed refinement
+    int32_t* fpiv = nullptr;
+    int finfo = 0;
+    bool fready = false;
+};
+
+namespace {
+
+struct Carve {
     size_t off = 0;
-    auto take = [&](size_t b) {
+    size_t take(size_t b) {
         size_t o = off;
         off += (b + 255) & ~(size_t)255;
         return o;
-    };
-    const int nch = RCH;
-    const size_t oT = take(sizeof(float) * S.ldt * n);
-    const size_t oipiv = take(sizeof(int32_t) * n), operm = take(sizeof(int) * n);
-    const size_t os = take(8 * (size_t)n), ot = take(8 * (size_t)n), orsum = take(8 * (size_t)n);
-    const size_t opmax = take(8 * (size_t)n * nch), opsum = take(8 * (size_t)n * nch);
-    const size_t nv = (size_t)n * NRMAX;
-    const size_t oR = take(8 * nv), oZ = take(8 * nv), oD = take(8 * nv), oW = take(8 * nv);
-    const size_t oP = take(8 * (size_t)n * NRMAX * nch);
-    const size_t onrm = take(8 * 4 * NRMAX), odots = take(8 * (size_t)(S.mgm + 2) * NRMAX * 2);
-    const size_t oVb = take(8 * (size_t)nrhs * (S.mgm + 1) * n);
-    const size_t oerr = take(sizeof(int) * 4);
-    const size_t ozt = take(2 * 8 * nv);
-    const size_t olu = take(mixed_lu_ws_bytes(n, precision));
-    const size_t orb = take(8 * 4 * (size_t)n);
-    const size_t odi = take(sizeof(float) * 2 * (size_t)((n + TB - 1) / TB) * TB * TB);
-    char* base = (char*)h->mws.get(off);
-    S.T = (float*)(base + oT);
-    S.ipiv = (int32_t*)(base + oipiv);
-    S.perm = (int*)(base + operm);
-    S.s = (double*)(base + os);
-    S.t = (double*)(base + ot);
-    S.rsum = (double*)(base + orsum);
-    S.pmax = (double*)(base + opmax);
-    S.psum = (double*)(base + opsum);
-    S.R = (double*)(base + oR);
-    S.Z = (double*)(base + oZ);
-    S.D = (double*)(base + oD);
-    S.W = (double*)(base + oW);
-    S.P = (double*)(base + oP);
-    S.nrm = (double*)(base + onrm);
-    S.dots = (double*)(base + odots);
-    S.Vb = (double*)(base + oVb);
-    S.zt = (uint64_t*)(base + ozt);
-    S.err = (int*)(base + oerr);
-    S.luws = base + olu;
-    S.Dinv = (float*)(base + odi);
-    for (int l = 0; l < 2; l++) {
-        S.rl[l] = (double*)(base + orb) + (size_t)l * n;
-        S.rr[l] = (double*)(base + orb) + (size_t)(2 + l) * n;
     }
-    if (S.rbt) {
+};
+
+// the factor's own buffers, in one block
+size_t factor_layout(MixedFactor& f, char* base) {
+    const int n = f.n;
+    Carve c;
+    const size_t oT = c.take(sizeof(float) * f.ldt * n);
+    const size_t odi = c.take(sizeof(float) * 2 * (size_t)((n + TB - 1) / TB) * TB * TB);
+    const size_t oipiv = c.take(sizeof(int32_t) * n), operm = c.take(sizeof(int) * n);
+    const size_t os = c.take(8 * (size_t)n), ot = c.take(8 * (size_t)n), orsum = c.take(8 * (size_t)n);
+    const size_t orb = c.take(8 * 4 * (size_t)n);
+    if (base) {
+        f.T = (float*)(base + oT);
+        f.Dinv = (float*)(base + odi);
+        f.ipiv = (int32_t*)(base + oipiv);
+        f.perm = (int*)(base + operm);
+        f.s = (double*)(base + os);
+        f.t = (double*)(base + ot);
+        f.rsum = (double*)(base + orsum);
+        for (int l = 0; l < 2; l++) {
+            f.rl[l] = (double*)(base + orb) + (size_t)l * n;
+            f.rr[l] = (double*)(base + orb) + (size_t)(2 + l) * n;
+        }
+    }
+    return c.off;
+}
+
+// scratch of the factorization step
+struct BuildScratch {
+    double *pmax, *psum;
+    void* luws;
+    int* err;
+};
+size_t build_layout(int n, int lp, char* base, BuildScratch* b) {
+    Carve c;
+    const size_t opmax = c.take(8 * (size_t)n * RCH), opsum = c.take(8 * (size_t)n * RCH);
+    const size_t olu = c.take(mixed_lu_ws_bytes(n, lp));
+    const size_t oerr = c.take(sizeof(int) * 4);
+    const size_t onrm = c.take(8 * 4);
+    if (base) {
+        b->pmax = (double*)(base + opmax);
+        b->psum = (double*)(base + opsum);
+        b->luws = base + olu;
+        b->err = (int*)(base + oerr);
+        (void)onrm;
+    }
+    return c.off;
+}
+
+// scratch of one solve (refinement and GMRES-IR vectors)
+size_t solve_layout(Solver& S, char* base) {
+    const int n = S.n;
+    Carve c;
+    const size_t nv = (size_t)n * NRMAX;
+    const size_t oR = c.take(8 * nv), oZ = c.take(8 * nv), oD = c.take(8 * nv), oW = c.take(8 * nv);
+    const size_t oP = c.take(8 * (size_t)n * NRMAX * RCH);
+    const size_t onrm = c.take(8 * 4 * NRMAX), odots = c.take(8 * (size_t)(S.mgm + 2) * NRMAX * 2);
+    const size_t oVb = c.take(8 * (size_t)S.nrhs * (S.mgm + 1) * n);
+    const size_t oerr = c.take(sizeof(int) * 4);
+    const size_t ozt = c.take(2 * 8 * nv);
+    if (base) {
+        S.R = (double*)(base + oR);
+        S.Z = (double*)(base + oZ);
+        S.D = (double*)(base + oD);
+        S.W = (double*)(base + oW);
+        S.P = (double*)(base + oP);
+        S.nrm = (double*)(base + onrm);
+        S.dots = (double*)(base + odots);
+        S.Vb = (double*)(base + oVb);
+        S.err = (int*)(base + oerr);
+        S.zt = (uint64_t*)(base + ozt);
+    }
+    return c.off;
+}
+
+// Equilibration, conversion, (butterflies,) the low-precision LU and the diagonal-block
+// inverses of A into f (whose buffers are carved); f.A / f.lda must be set.
+void mixed_build(bola_hip_handle_s* h, MixedFactor& f, char* scratch) {
+    hipStream_t st = h->stream;
+    const int n = f.n;
+    BuildScratch b;
+    build_layout(n, f.lp, scratch, &b);
+    if (f.rbt) {
         // random factors exp(u / 10), u uniform in [-1/2, 1/2), from a fixed-seed splitmix64
         std::vector<double> rv(4 * (size_t)n);
         uint64_t x = 0x5EEDB077E8F1ull;
@@ -793,62 +828,119 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
             z ^= z >> 31;
             v = std::exp(((double)(z >> 11) * 0x1.0p-53 - 0.5) / 10.0);
         }
-        BOLA_CHECK(hipMemcpyAsync(S.rl[0], rv.data(), sizeof(double) * rv.size(), hipMemcpyHostToDevice, S.st));
-        BOLA_CHECK(hipStreamSynchronize(S.st));
+        BOLA_CHECK(hipMemcpyAsync(f.rl[0], rv.data(), sizeof(double) * rv.size(), hipMemcpyHostToDevice, st));
+        BOLA_CHECK(hipStreamSynchronize(st));
     }
-    hipStream_t st = S.st;
-    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (int)(4 * nv), (int*)S.zt);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, S.err);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, b.err);
     BOLA_LAUNCH_CHECK();
-    hipEvent_t ev[4];
+    hipEvent_t ev[2];
     for (auto& e : ev) BOLA_CHECK(hipEventCreate(&e));
     BOLA_CHECK(hipEventRecord(ev[0], st));
     {
         Phase ph(h, "MIXED_FACTOR", 2.0 * (double)n * n * n / 3.0);
         // 1. equilibration and the fp32 transpose
-        const int cw = (n + nch - 1) / nch;
-        hipLaunchKernelGGL(eq_row_part, dim3(cdiv((size_t)n, 256), (unsigned)nch), dim3(256), 0, st, n, A, lda, S.pmax,
-                           S.psum, cw);
-        hipLaunchKernelGGL(eq_row_fin, g1(n), dim3(256), 0, st, n, nch, S.pmax, S.psum, S.s, S.rsum);
-        hipLaunchKernelGGL(eq_col, dim3((unsigned)n), dim3(256), 0, st, n, A, lda, S.s, S.t);
-        hipLaunchKernelGGL(to_T, dim3(cdiv((size_t)n, 64), cdiv((size_t)n, 64)), dim3(256), 0, st, n, A, lda, S.s, S.t,
-                           S.T, S.ldt);
-        hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, st, n, S.perm);
+        const int cw = (n + RCH - 1) / RCH;
+        hipLaunchKernelGGL(eq_row_part, dim3(cdiv((size_t)n, 256), (unsigned)RCH), dim3(256), 0, st, n, f.A, f.lda,
+                           b.pmax, b.psum, cw);
+        hipLaunchKernelGGL(eq_row_fin, g1(n), dim3(256), 0, st, n, RCH, b.pmax, b.psum, f.s, f.rsum);
+        hipLaunchKernelGGL(eq_col, dim3((unsigned)n), dim3(256), 0, st, n, f.A, f.lda, f.s, f.t);
+        hipLaunchKernelGGL(to_T, dim3(cdiv((size_t)n, 64), cdiv((size_t)n, 64)), dim3(256), 0, st, n, f.A, f.lda, f.s,
+                           f.t, f.T, f.ldt);
+        hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, st, n, f.perm);
         BOLA_LAUNCH_CHECK();
-        if (S.rbt) {  // T <- (W_L^T A_s W_R)^T: left B2^T then B1^T, right B2 then B1
+        if (f.rbt) {  // T <- (W_L^T A_s W_R)^T: left B2^T then B1^T, right B2 then B1
             const dim3 gl(cdiv((size_t)n, 256), (unsigned)std::min(n / 2, 16384));
             const dim3 gr(cdiv((size_t)n / 2, 256), (unsigned)std::min(n, 16384));
-            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, S.T, S.ldt, n / 4, S.rl[1]);
-            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, S.T, S.ldt, n / 2, S.rl[0]);
-            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, S.T, S.ldt, n / 4, S.rr[1]);
-            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, S.T, S.ldt, n / 2, S.rr[0]);
+            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, f.T, f.ldt, n / 4, f.rl[1]);
+            hipLaunchKernelGGL(rbt_T<true>, gl, dim3(256), 0, st, n, f.T, f.ldt, n / 2, f.rl[0]);
+            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, f.T, f.ldt, n / 4, f.rr[1]);
+            hipLaunchKernelGGL(rbt_T<false>, gr, dim3(256), 0, st, n, f.T, f.ldt, n / 2, f.rr[0]);
             BOLA_LAUNCH_CHECK();
         }
         // 2. the low-precision LU
-        mixed_lu_factor(h, precision, !S.rbt, n, S.T, S.ldt, S.ipiv, S.perm, S.luws, S.err);
+        mixed_lu_factor(h, f.lp, !f.rbt, n, f.T, f.ldt, f.ipiv, f.perm, b.luws, b.err);
         const int nbk = (n + TB - 1) / TB;
-        hipLaunchKernelGGL(diag_inv, dim3((unsigned)nbk, 2), dim3(TB), 0, st, n, S.T, S.ldt, S.Dinv, nbk);
+        hipLaunchKernelGGL(diag_inv, dim3((unsigned)nbk, 2), dim3(TB), 0, st, n, f.T, f.ldt, f.Dinv, nbk);
         BOLA_LAUNCH_CHECK();
     }
     BOLA_CHECK(hipEventRecord(ev[1], st));
-    // |A|_inf and |b|_inf
-    hipLaunchKernelGGL(col_amax, dim3(1), dim3(256), 0, st, n, S.rsum, (size_t)n, S.nrm + 2 * NRMAX);
+    // |A|_inf from the row sums, and whether a factorization kernel gave up
+    double* nrm = (double*)((char*)b.err + 256);
+    hipLaunchKernelGGL(col_amax, dim3(1), dim3(256), 0, st, n, f.rsum, (size_t)n, nrm);
     BOLA_LAUNCH_CHECK();
-    const double anorm = fetch(S, S.nrm + 2 * NRMAX, 1)[0];
+    BOLA_CHECK(hipMemcpyAsync(&f.anorm, nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+    BOLA_CHECK(hipMemcpyAsync(&f.lu_err, b.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    BOLA_CHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    BOLA_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    f.factor_ms = ms;
+}
+
+// FP64 LU of f.A into f.F / f.fpiv (which the caller provides), once
+void mixed_fp64_factor(bola_hip_handle_s* h, MixedFactor& f) {
+    hipStream_t st = h->stream;
+    const int n = f.n;
+    Phase ph(h, "MIXED_FALLBACK");
+    BOLA_CHECK(hipMemcpy2DAsync(f.F, sizeof(double) * n, f.A, sizeof(double) * f.lda, sizeof(double) * n, n,
+                                hipMemcpyDeviceToDevice, st));
+    lu_factor(h, n, n, f.F, n, f.fpiv, h->d_info, false);
+    BOLA_CHECK(hipMemcpyAsync(&f.finfo, h->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    BOLA_CHECK(hipStreamSynchronize(st));
+    f.fready = true;
+}
+
+// A X = B from the factorization f: refinement, GMRES-IR, and the FP64 LU as the last
+// resort (made on first need by fp64_mem, which points f.F / f.fpiv at memory).
+int mixed_solve(bola_hip_handle_s* h, MixedFactor& f, int nrhs, const double* B, size_t ldb, double* X, size_t ldx,
+                bola_hip_mixed_stats& so, char* scratch, const std::function<void()>& fp64_mem) {
+    const int n = f.n;
+    hipStream_t st = h->stream;
+    Solver S;
+    S.h = h;
+    S.st = st;
+    S.n = n;
+    S.nrhs = nrhs;
+    S.lp = f.lp;
+    S.A = f.A;
+    S.lda = f.lda;
+    S.mgm = 30;
+    S.ldt = f.ldt;
+    S.rbt = f.rbt;
+    S.T = f.T;
+    S.Dinv = f.Dinv;
+    S.ipiv = f.ipiv;
+    S.perm = f.perm;
+    S.s = f.s;
+    S.t = f.t;
+    S.rsum = f.rsum;
+    for (int l = 0; l < 2; l++) {
+        S.rl[l] = f.rl[l];
+        S.rr[l] = f.rr[l];
+    }
+    solve_layout(S, scratch);
+    so.factor_ms = f.factor_ms;
+    hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (int)(4 * (size_t)n * NRMAX), (int*)S.zt);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 4, S.err);
+    BOLA_LAUNCH_CHECK();
+    hipEvent_t ev[3];
+    for (auto& e : ev) BOLA_CHECK(hipEventCreate(&e));
+    BOLA_CHECK(hipEventRecord(ev[0], st));
     hipLaunchKernelGGL(col_amax, dim3((unsigned)nrhs), dim3(256), 0, st, n, B, ldb, S.nrm + 3 * NRMAX);
     BOLA_LAUNCH_CHECK();
     const std::vector<double> bn = fetch(S, S.nrm + 3 * NRMAX, nrhs);
+    const double anorm = f.anorm;
     const double tol = (double)n * 2.220446049250313e-16;
     so.tolerance = tol;
-    // 3. refinement: X = 0, R = B
-    hipLaunchKernelGGL(mat_zero, g2(n, nrhs), dim3(256), 0, st, n, nrhs, X, ldx);
-    hipLaunchKernelGGL(mat_copy, g2(n, nrhs), dim3(256), 0, st, n, nrhs, B, ldb, S.R, (size_t)n);
-    BOLA_LAUNCH_CHECK();
     std::vector<double> e(nrhs, INFINITY);
-    double prev = INFINITY;
-    int stall = 0;
     bool done = false;
-    {
+    if (!f.fready && !f.lu_err) {
+        // 3. refinement: X = 0, R = B
+        hipLaunchKernelGGL(mat_zero, g2(n, nrhs), dim3(256), 0, st, n, nrhs, X, ldx);
+        hipLaunchKernelGGL(mat_copy, g2(n, nrhs), dim3(256), 0, st, n, nrhs, B, ldb, S.R, (size_t)n);
+        BOLA_LAUNCH_CHECK();
+        double prev = INFINITY;
+        int stall = 0;
         Phase ph(h, "MIXED_REFINE");
         for (int it = 0; it < 30 && !done; it++) {
             apply_minv(S, S.R, (size_t)n, S.D);
@@ -880,34 +972,27 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
                 prev = std::min(prev, m);
             }
         }
-    }
-    BOLA_CHECK(hipEventRecord(ev[2], st));
-    {
         int herr = 0;
         BOLA_CHECK(hipMemcpyAsync(&herr, S.err, sizeof(int), hipMemcpyDeviceToHost, st));
         BOLA_CHECK(hipStreamSynchronize(st));
         if (herr) done = false;  // a triangular solve gave up waiting: trust nothing it produced
     }
+    BOLA_CHECK(hipEventRecord(ev[1], st));
     // 4. fallback: the FP64 reference-exact path
     if (!done) {
-        Phase ph(h, "MIXED_FALLBACK");
         so.fallback = 1;
-        double* Af = (double*)h->mws2.get(sizeof(double) * (size_t)n * n + sizeof(int32_t) * n + 512);
-        int32_t* piv = (int32_t*)(Af + (size_t)n * n);
-        BOLA_CHECK(hipMemcpy2DAsync(Af, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
-                                    hipMemcpyDeviceToDevice, st));
-        lu_factor(h, n, n, Af, n, piv, h->d_info, false);
-        int32_t info = 0;
-        BOLA_CHECK(hipMemcpyAsync(&info, h->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        BOLA_CHECK(hipStreamSynchronize(st));
-        if (info != 0) {
+        if (!f.fready) {
+            fp64_mem();
+            mixed_fp64_factor(h, f);
+        }
+        if (f.finfo != 0) {
             for (auto& x : ev) (void)hipEventDestroy(x);
             set_error(h, "Matrix is singular to working precision (FP64 fallback factorization failed).");
-            return info > 0 ? info : BOLA_HIP_ERR_DEVICE;
+            return f.finfo > 0 ? f.finfo : BOLA_HIP_ERR_DEVICE;
         }
         hipLaunchKernelGGL(mat_copy, g2(n, nrhs), dim3(256), 0, st, n, nrhs, B, ldb, X, ldx);
         BOLA_LAUNCH_CHECK();
-        int r = bola_hip_dgetrs_dev(h, n, nrhs, Af, n, piv, X, ldx);
+        int r = bola_hip_dgetrs_dev(h, n, nrhs, f.F, n, f.fpiv, X, ldx);
         if (r) {
             for (auto& x : ev) (void)hipEventDestroy(x);
             return r;
@@ -915,18 +1000,310 @@ extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size
         residual(S, B, ldb, X, ldx, S.R);
         e = berr(S, X, ldx, bn, anorm);
     }
-    BOLA_CHECK(hipEventRecord(ev[3], st));
-    BOLA_CHECK(hipEventSynchronize(ev[3]));
-    float t01 = 0, t12 = 0, t23 = 0;
+    BOLA_CHECK(hipEventRecord(ev[2], st));
+    BOLA_CHECK(hipEventSynchronize(ev[2]));
+    float t01 = 0, t12 = 0;
     BOLA_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
     BOLA_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
-    BOLA_CHECK(hipEventElapsedTime(&t23, ev[2], ev[3]));
     for (auto& x : ev) (void)hipEventDestroy(x);
-    so.factor_ms = t01;
-    so.refine_ms = t12;
-    so.fallback_ms = t23;
+    so.refine_ms = t01;
+    so.fallback_ms = t12;
     so.final_berr = maxv(e);
     so.converged = so.final_berr <= tol ? 1 : 0;
     return 0;
+}
+
+int mixed_check_args(bola_hip_handle_s* h, int precision, size_t n, size_t lda, const char* who) {
+    precision &= 0xFF;
+    if (precision > 2) {
+        set_error(h, std::string("Argument 2 to ") + who + " had an illegal value");
+        return -2;
+    }
+    if (lda < std::max<size_t>(1, n)) {
+        set_error(h, std::string("Argument 6 to ") + who + " had an illegal value");
+        return -6;
+    }
+    if (n > (size_t)mixed_lu_max_n()) {
+        set_error(h, std::string("Argument 3 to ") + who + " had an illegal value (n above the supported maximum)");
+        return -3;
+    }
+    return 0;
+}
+
+void mixed_init(MixedFactor& f, int precision, int n) {
+    const bool want_rbt = (precision & BOLA_HIP_MIX_RBT) != 0;
+    f.n = n;
+    f.lp = precision & 0xFF;
+    f.ldt = ((size_t)n + 63) & ~(size_t)63;
+    f.rbt = want_rbt && n % 4 == 0 && n >= 8;  // two butterfly levels need n divisible by 4
+}
+
+}  // namespace
+}  // namespace bola
+
+using namespace bola;
+
+extern "C" int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n_, size_t nrhs_, const double* A,
+                                        size_t lda, const double* B, size_t ldb, double* X, size_t ldx,
+                                        bola_hip_mixed_stats* stats) {
+    BOLA_API_BEGIN
+    if (!h) return -1;
+    if (int r = mixed_check_args(h, precision, n_, lda, "dgesv_mixed")) return r;
+    if (nrhs_ > (size_t)NRMAX) {
+        set_error(h, "Argument 4 to dgesv_mixed had an illegal value (at most 16 right-hand sides)");
+        return -4;
+    }
+    bola_hip_mixed_stats loc{};
+    bola_hip_mixed_stats& so = stats ? *stats : loc;
+    so = bola_hip_mixed_stats{};
+    if (n_ == 0 || nrhs_ == 0) return 0;
+    const int n = (int)n_, nrhs = (int)nrhs_;
+    MixedFactor f;
+    mixed_init(f, precision, n);
+    f.A = A;
+    f.lda = lda;
+    // one workspace: the factor's buffers, then the build scratch, then the solve scratch
+    Solver probe;
+    probe.n = n;
+    probe.nrhs = nrhs;
+    probe.mgm = 30;
+    const size_t fb = factor_layout(f, nullptr);
+    const size_t bb = build_layout(n, f.lp, nullptr, nullptr);
+    const size_t sb = solve_layout(probe, nullptr);
+    char* base = (char*)h->mws.get(fb + bb + sb);
+    factor_layout(f, base);
+    mixed_build(h, f, base + fb);
+    return mixed_solve(h, f, nrhs, B, ldb, X, ldx, so, base + fb + bb, [&] {
+        f.F = (double*)h->mws2.get(sizeof(double) * (size_t)n * n + sizeof(int32_t) * n + 512);
+        f.fpiv = (int32_t*)(f.F + (size_t)n * n);
+    });
     BOLA_API_END(h)
+}
+
+
+// ------------------------------------------------------------------ factorization objects
+
+struct bola_hip_factor_s {
+    int kind = 0, precision = 0;
+    size_t n = 0;
+    int device = 0;
+    bool stream_ordered = false;  // memory from hipMallocAsync (the server's), else hipMalloc
+    void* mem = nullptr;      // every device buffer the object owns except the fallback
+    size_t bytes = 0;
+    double* a64 = nullptr;    // FP64 LU / Cholesky factor, or (mixed) the copy of A (ld n)
+    int32_t* ipiv = nullptr;  // LU pivots (1-based, global)
+    bola::MixedFactor mf;
+    void* fb_mem = nullptr;   // mixed: the FP64 LU made on first need
+};
+
+namespace bola {
+
+// Builds a factorization object of A (host or device memory); its device memory comes from
+// the stream-ordered allocator on the handle's stream when `stream_ordered` (serving), else
+// from hipMalloc.  Returns null with *info set when the reference's factorization reports
+// an error (zero pivot, not positive definite; the text is in the handle's error string).
+bola_hip_factor_s* factor_build(bola_hip_handle_s* h, int kind, int precision, size_t n, const double* a, size_t lda,
+                                int* info, bool stream_ordered) {
+    hipStream_t st = h->stream;
+    *info = 0;
+    auto* f = new bola_hip_factor_s;
+    f->kind = kind;
+    f->precision = kind == BOLA_HIP_FACTOR_MIXED ? precision : 0;
+    f->n = n;
+    f->device = h->device;
+    f->stream_ordered = stream_ordered;
+    try {
+        const size_t mat = (sizeof(double) * n * n + 255) & ~(size_t)255;
+        size_t extra = 0;
+        if (kind == BOLA_HIP_FACTOR_LU) extra = (sizeof(int32_t) * n + 255) & ~(size_t)255;
+        if (kind == BOLA_HIP_FACTOR_MIXED) {
+            mixed_init(f->mf, precision, (int)n);
+            extra = factor_layout(f->mf, nullptr);
+        }
+        f->bytes = mat + extra;
+        if (stream_ordered) BOLA_CHECK(hipMallocAsync(&f->mem, f->bytes, st));
+        else BOLA_CHECK(hipMalloc(&f->mem, f->bytes));
+        f->a64 = (double*)f->mem;
+        BOLA_CHECK(hipMemcpy2DAsync(f->a64, sizeof(double) * n, a, sizeof(double) * lda, sizeof(double) * n, n,
+                                    hipMemcpyDefault, st));
+        int32_t rc = 0;
+        if (kind == BOLA_HIP_FACTOR_LU) {
+            f->ipiv = (int32_t*)((char*)f->mem + mat);
+            rc = bola_hip_dgetrf_dev(h, n, n, f->a64, n, f->ipiv, nullptr);
+        } else if (kind == BOLA_HIP_FACTOR_CHOL_L || kind == BOLA_HIP_FACTOR_CHOL_U) {
+            rc = bola_hip_dpotrf_dev(h, kind == BOLA_HIP_FACTOR_CHOL_L ? 'L' : 'U', n, f->a64, n, nullptr);
+        } else {
+            factor_layout(f->mf, (char*)f->mem + mat);
+            f->mf.A = f->a64;
+            f->mf.lda = n;
+            char* scratch = (char*)h->mws.get(build_layout((int)n, f->mf.lp, nullptr, nullptr));
+            mixed_build(h, f->mf, scratch);
+        }
+        if (rc < 0) throw HipFailure{hipErrorUnknown, "factorization", __LINE__};
+        BOLA_CHECK(hipStreamSynchronize(st));
+        if (rc > 0) {
+            *info = rc;
+            factor_release(f, st);
+            return nullptr;
+        }
+        return f;
+    } catch (...) {
+        factor_release(f, st);
+        throw;
+    }
+}
+
+// Frees the object's device memory (stream-ordered on st when it was so allocated).
+void factor_release(bola_hip_factor_s* f, hipStream_t st) {
+    if (!f) return;
+    if (f->stream_ordered) {
+        if (f->mem) (void)hipFreeAsync(f->mem, st);
+        if (f->fb_mem) (void)hipFreeAsync(f->fb_mem, st);
+    } else {
+        if (f->mem) (void)hipFree(f->mem);
+        if (f->fb_mem) (void)hipFree(f->fb_mem);
+    }
+    delete f;
+}
+
+}  // namespace bola
+
+extern "C" int bola_hip_factor_create_dev(bola_hip_handle_t h, int kind, int precision, size_t n, const double* a,
+                                          size_t lda, bola_hip_factor_t* out) {
+    if (!h) h = default_handle();
+    if (!out) return -7;
+    *out = nullptr;
+    BOLA_API_BEGIN
+    if (kind < BOLA_HIP_FACTOR_LU || kind > BOLA_HIP_FACTOR_MIXED) {
+        set_error(h, "Argument 2 to factor_create had an illegal value");
+        return -2;
+    }
+    if (kind == BOLA_HIP_FACTOR_MIXED) {
+        if (int r = mixed_check_args(h, precision, n, lda, "factor_create")) return r;
+    } else if (lda < std::max<size_t>(1, n)) {
+        set_error(h, "Argument 6 to factor_create had an illegal value");
+        return -6;
+    }
+    if (n == 0 || !a) {
+        set_error(h, "factor_create needs n > 0 and a matrix");
+        return -4;
+    }
+    int info = 0;
+    bola_hip_factor_s* f = factor_build(h, kind, precision, n, a, lda, &info, false);
+    if (!f) return info;
+    *out = f;
+    return 0;
+    BOLA_API_END(h)
+}
+
+extern "C" int bola_hip_factor_solve_dev(bola_hip_handle_t h, bola_hip_factor_t f, size_t nrhs, const double* b,
+                                         size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats) {
+    if (!h) h = default_handle();
+    BOLA_API_BEGIN
+    if (!f) {
+        set_error(h, "Argument 2 to factor_solve had an illegal value");
+        return -2;
+    }
+    const size_t n = f->n;
+    if (ldb < std::max<size_t>(1, n) || ldx < std::max<size_t>(1, n)) {
+        set_error(h, "factor_solve: leading dimensions must be at least n");
+        return -5;
+    }
+    bola_hip_mixed_stats loc{};
+    bola_hip_mixed_stats& so = stats ? *stats : loc;
+    so = bola_hip_mixed_stats{};
+    if (nrhs == 0) return 0;
+    hipStream_t st = h->stream;
+    if (f->kind != BOLA_HIP_FACTOR_MIXED) {
+        if (x != b)
+            BOLA_CHECK(hipMemcpy2DAsync(x, sizeof(double) * ldx, b, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                        hipMemcpyDeviceToDevice, st));
+        int r = f->kind == BOLA_HIP_FACTOR_LU
+                    ? bola_hip_dgetrs_dev(h, n, nrhs, f->a64, n, f->ipiv, x, ldx)
+                    : bola_hip_dpotrs_dev(h, f->kind == BOLA_HIP_FACTOR_CHOL_L ? 'L' : 'U', n, nrhs, f->a64, n, x, ldx);
+        if (r) return r;
+        so.converged = 1;
+        return 0;
+    }
+    // mixed: NRMAX right-hand sides at a time
+    const int ni = (int)n;
+    Solver probe;
+    probe.n = ni;
+    probe.nrhs = NRMAX;
+    probe.mgm = 30;
+    char* scratch = (char*)h->mws.get(solve_layout(probe, nullptr));
+    auto fp64_mem = [&] {
+        const size_t fbb = sizeof(double) * n * n + sizeof(int32_t) * n + 512;
+        if (!f->fb_mem) {
+            if (f->stream_ordered) BOLA_CHECK(hipMallocAsync(&f->fb_mem, fbb, st));
+            else BOLA_CHECK(hipMalloc(&f->fb_mem, fbb));
+        }
+        f->mf.F = (double*)f->fb_mem;
+        f->mf.fpiv = (int32_t*)(f->mf.F + n * n);
+    };
+    double worst = 0.0;
+    int conv = 1;
+    for (size_t c0 = 0; c0 < nrhs; c0 += NRMAX) {
+        const int nc = (int)std::min<size_t>(NRMAX, nrhs - c0);
+        bola_hip_mixed_stats part{};
+        int r = mixed_solve(h, f->mf, nc, b + c0 * ldb, ldb, x + c0 * ldx, ldx, part, scratch, fp64_mem);
+        if (r) return r;
+        so.ir_iterations += part.ir_iterations;
+        so.gmres_cycles += part.gmres_cycles;
+        so.gmres_iterations += part.gmres_iterations;
+        so.fallback |= part.fallback;
+        so.refine_ms += part.refine_ms;
+        so.fallback_ms += part.fallback_ms;
+        so.tolerance = part.tolerance;
+        if (c0 == 0) so.first_berr = part.first_berr;
+        worst = std::max(worst, part.final_berr);
+        conv = conv && part.converged;
+    }
+    so.factor_ms = f->mf.factor_ms;
+    so.final_berr = worst;
+    so.converged = conv;
+    return 0;
+    BOLA_API_END(h)
+}
+
+extern "C" int bola_hip_factor_describe(bola_hip_factor_t f, bola_hip_factor_desc* d) {
+    if (!f || !d) return -1;
+    *d = bola_hip_factor_desc{};
+    d->kind = f->kind;
+    d->precision = f->precision;
+    d->n = f->n;
+    d->bytes = f->bytes + (f->fb_mem ? sizeof(double) * f->n * f->n + sizeof(int32_t) * f->n + 512 : 0);
+    if (f->kind == BOLA_HIP_FACTOR_MIXED) {
+        d->factor = f->mf.T;
+        d->ld = f->mf.ldt;
+        d->ipiv = f->mf.rbt ? nullptr : f->mf.ipiv;
+        d->row_scale = f->mf.s;
+        d->col_scale = f->mf.t;
+        d->matrix = f->a64;
+        d->butterfly = f->mf.rbt ? 1 : 0;
+    } else {
+        d->factor = f->a64;
+        d->ld = f->n;
+        d->ipiv = f->ipiv;
+    }
+    return 0;
+}
+
+extern "C" size_t bola_hip_factor_bytes(int kind, int precision, size_t n) {
+    const size_t mat = (sizeof(double) * n * n + 255) & ~(size_t)255;
+    if (kind == BOLA_HIP_FACTOR_LU) return mat + ((sizeof(int32_t) * n + 255) & ~(size_t)255);
+    if (kind == BOLA_HIP_FACTOR_MIXED) {
+        MixedFactor mf;
+        mixed_init(mf, precision, (int)n);
+        return mat + factor_layout(mf, nullptr);
+    }
+    return mat;
+}
+
+extern "C" int bola_hip_factor_destroy(bola_hip_factor_t f) {
+    if (!f) return 0;
+    (void)hipSetDevice(f->device);
+    (void)hipDeviceSynchronize();  // no solve may still read it (callers' streams included)
+    factor_release(f, nullptr);
+    return 0;
 }

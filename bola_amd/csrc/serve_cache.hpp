@@ -5,8 +5,8 @@
 // the same logic is compiled into the library and into a CPU-only test.
 //
 // Rules:
-//   * a request needs its right-hand side, plus a factorization unless a resident one of
-//     the same kind and order is kept under its key (a hit);
+//   * a request needs its right-hand-side buffers, plus a factorization unless a resident
+//     one of the same kind and order is kept under its key (a hit);
 //   * a hit is pinned (users > 0) before anything is evicted for it, so the eviction
 //     that makes room for its right-hand side can never take its own factorization;
 //   * eviction takes idle (users == 0) resident factorizations, least recently used
@@ -25,8 +25,8 @@ namespace bola {
 struct ResidentFactor {
     int kind = 0;
     size_t n = 0;
-    void* f = nullptr;      // the factor (n x n doubles)
-    void* ipiv = nullptr;   // its pivots (LU only; may be null for Cholesky)
+    void* f = nullptr;      // the factorization (the library's factorization object)
+    void* ipiv = nullptr;   // a second buffer released with it (may be null)
     size_t bytes = 0;       // what it counts against the budget
     int users = 0;          // requests being served from it right now
     uint64_t last_use = 0;  // for least-recently-used eviction
@@ -59,19 +59,19 @@ class ServeCache {
         return &it->second;
     }
 
-    // Bytes a request needs beyond what is resident for it.
-    size_t need(uint64_t key, int kind, size_t n, size_t nrhs, bool& hit) {
+    // Bytes a request needs beyond what is resident for it: rhs (its right-hand-side
+    // buffers) and, unless a factorization is resident for it, fbytes.
+    size_t need(uint64_t key, int kind, size_t n, size_t rhs, size_t fbytes, bool& hit) {
         hit = lookup(key, kind, n) != nullptr;
-        const size_t rhs = n * nrhs * 8;
-        return hit ? rhs : resident_factor_bytes(n) + rhs;
+        return hit ? rhs : fbytes + rhs;
     }
 
     // Reserves a request's bytes, evicting idle factorizations if needed.  On success
     // the bytes count against the budget until release(), and on a hit the resident
     // factorization is pinned (unpin() after use).  Returns false, with nothing
     // changed except evictions already made, when the request cannot fit now.
-    bool reserve(uint64_t key, int kind, size_t n, size_t nrhs, size_t& reserved, bool& hit) {
-        const size_t nb = need(key, kind, n, nrhs, hit);
+    bool reserve(uint64_t key, int kind, size_t n, size_t rhs, size_t fbytes, size_t& reserved, bool& hit) {
+        const size_t nb = need(key, kind, n, rhs, fbytes, hit);
         ResidentFactor* own = hit ? lookup(key, kind, n) : nullptr;
         if (own) own->users++;  // pinned before any eviction
         if (in_use_ + nb > budget_ && !make_room(nb)) {
@@ -101,7 +101,7 @@ class ServeCache {
     // A request finished: its reserved bytes are returned, except a factorization it
     // leaves resident under its key.  Returns false (and keeps nothing) when the key
     // is busy with another resident factorization; the caller then frees its own.
-    bool finish(size_t reserved, bool keep, uint64_t key, int kind, size_t n, void* f, void* ipiv) {
+    bool finish(size_t reserved, bool keep, uint64_t key, int kind, size_t n, size_t fbytes, void* f, void* ipiv) {
         size_t kept = 0;
         bool stored = false;
         if (keep && key) {
@@ -118,7 +118,7 @@ class ServeCache {
                 r.n = n;
                 r.f = f;
                 r.ipiv = ipiv;
-                r.bytes = resident_factor_bytes(n);
+                r.bytes = fbytes;
                 r.last_use = ++clock_;
                 map_[key] = r;
                 kept = r.bytes;

@@ -42,14 +42,34 @@ struct Ticket {
 // The stream of the worker thread that releases device memory (null on other threads).
 thread_local hipStream_t tl_stream = nullptr;
 
-void release_device(void* f, void* ipiv) {
-    if (tl_stream) {
-        if (f) (void)hipFreeAsync(f, tl_stream);
-        if (ipiv) (void)hipFreeAsync(ipiv, tl_stream);
+void release_device(void* f, void*) { factor_release((bola_hip_factor_s*)f, tl_stream); }
+
+// the factorization object a request needs, its identity in the cache and its size
+struct FactorKind {
+    int kind, precision, cache_kind;
+    size_t bytes;
+};
+FactorKind factor_kind(const bola_hip_request& r) {
+    FactorKind k;
+    if (r.kind == BOLA_HIP_SERVE_CHOL) {
+        k.kind = BOLA_HIP_FACTOR_CHOL_L;
+        k.precision = 0;
+    } else if (r.precision == BOLA_HIP_SERVE_FP64) {
+        k.kind = BOLA_HIP_FACTOR_LU;
+        k.precision = 0;
     } else {
-        if (f) (void)hipFree(f);
-        if (ipiv) (void)hipFree(ipiv);
+        k.kind = BOLA_HIP_FACTOR_MIXED;
+        k.precision = r.precision - 1;
     }
+    k.cache_kind = k.kind * 4096 + k.precision;
+    k.bytes = bola_hip_factor_bytes(k.kind, k.precision, r.n);
+    // the mixed factorization's FP64 fallback is made on first need: count it from the start
+    if (k.kind == BOLA_HIP_FACTOR_MIXED) k.bytes += r.n * r.n * 8 + r.n * 4 + 512;
+    return k;
+}
+// right-hand-side buffers on the device: b and x (one buffer when the solve is in place)
+size_t rhs_bytes(const bola_hip_request& r, const FactorKind& k) {
+    return (k.kind == BOLA_HIP_FACTOR_MIXED ? 2 : 1) * r.n * r.nrhs * 8;
 }
 
 }  // namespace
@@ -86,7 +106,8 @@ struct bola_hip_server_s {
         for (size_t qi = 0; qi < queue.size(); qi++) {
             Ticket* t = queue[qi];
             const bola_hip_request& r = t->req;
-            if (cache.reserve(r.factor_key, r.kind, r.n, r.nrhs, reserved, hit)) {
+            const FactorKind k = factor_kind(r);
+            if (cache.reserve(r.factor_key, k.cache_kind, r.n, rhs_bytes(r, k), k.bytes, reserved, hit)) {
                 queue.erase(queue.begin() + (long)qi);
                 return t;
             }
@@ -104,11 +125,6 @@ void bola_hip_server_s::work(int slot) {
     if (bola_hip_create(&h) != 0) return;
     hipStream_t st = h->stream;
     tl_stream = st;  // evictions made by this thread are ordered on its stream
-    int32_t* dinfo = nullptr;
-    if (hipMalloc(&dinfo, 4 * sizeof(int32_t)) != hipSuccess) {
-        bola_hip_destroy(h);
-        return;
-    }
     for (;;) {
         Ticket* t = nullptr;
         size_t reserved = 0;
@@ -128,9 +144,10 @@ void bola_hip_server_s::work(int slot) {
         res.slot = slot;
         res.cache_hit = hit ? 1 : 0;
         res.queue_us = now_us() - t->t_submit;
-        double* fa = nullptr;
-        int32_t* ipiv = nullptr;
+        const FactorKind k = factor_kind(r);
+        bola_hip_factor_s* fo = nullptr;
         double* xb = nullptr;
+        double* bb = nullptr;
         bool keep = false;
         int status = 0, info = 0;
         const size_t n = r.n;
@@ -139,52 +156,50 @@ void bola_hip_server_s::work(int slot) {
                 std::lock_guard<std::mutex> lk(mu);
                 ResidentFactor* f = cache.pinned(r.factor_key);
                 if (!f) throw HipFailure{hipErrorInvalidValue, "resident factorization missing", __LINE__};
-                fa = (double*)f->f;
-                ipiv = (int32_t*)f->ipiv;
+                fo = (bola_hip_factor_s*)f->f;
             } else {
-                BOLA_CHECK(hipMallocAsync((void**)&fa, n * n * 8, st));
-                BOLA_CHECK(hipMallocAsync((void**)&ipiv, std::max<size_t>(n, 1) * 4, st));
-                BOLA_CHECK(hipMemcpyAsync(fa, r.a, n * n * 8, hipMemcpyDefault, st));
                 std::unique_lock<std::mutex> big(big_lu, std::defer_lock);
-                if (r.kind == BOLA_HIP_SERVE_LU && n > kBigLu) big.lock();  // released after the sync below
-                int rc = r.kind == BOLA_HIP_SERVE_LU ? bola_hip_dgetrf_dev(h, n, n, fa, n, ipiv, dinfo)
-                                                     : bola_hip_dpotrf_dev(h, 'L', n, fa, n, dinfo);
-                if (rc < 0) throw HipFailure{hipErrorUnknown, "factorization", __LINE__};
-                BOLA_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                BOLA_CHECK(hipStreamSynchronize(st));
+                if (r.kind == BOLA_HIP_SERVE_LU && n > kBigLu) big.lock();  // released once it is factored
+                fo = factor_build(h, k.kind, k.precision, n, r.a, n, &info, true);
             }
             if (info == 0) {
-                BOLA_CHECK(hipMallocAsync((void**)&xb, std::max<size_t>(n * r.nrhs, 1) * 8, st));
+                const size_t rb = std::max<size_t>(n * r.nrhs, 1) * 8;
+                BOLA_CHECK(hipMallocAsync((void**)&xb, rb, st));
                 BOLA_CHECK(hipMemcpyAsync(xb, r.b, n * r.nrhs * 8, hipMemcpyDefault, st));
-                int rc = r.kind == BOLA_HIP_SERVE_LU ? bola_hip_dgetrs_dev(h, n, r.nrhs, fa, n, ipiv, xb, n)
-                                                     : bola_hip_dpotrs_dev(h, 'L', n, r.nrhs, fa, n, xb, n);
+                const double* bsrc = xb;
+                if (k.kind == BOLA_HIP_FACTOR_MIXED) {  // the refinement reads b while it writes x
+                    BOLA_CHECK(hipMallocAsync((void**)&bb, rb, st));
+                    BOLA_CHECK(hipMemcpyAsync(bb, xb, n * r.nrhs * 8, hipMemcpyDeviceToDevice, st));
+                    bsrc = bb;
+                }
+                bola_hip_mixed_stats ms{};
+                int rc = bola_hip_factor_solve_dev(h, fo, r.nrhs, bsrc, n, xb, n, &ms);
                 if (rc < 0) throw HipFailure{hipErrorUnknown, "solve", __LINE__};
-                BOLA_CHECK(hipMemcpyAsync(r.x, xb, n * r.nrhs * 8, hipMemcpyDefault, st));
+                if (rc > 0) info = rc;  // the mixed solver's FP64 fallback met a zero pivot
+                if (k.kind == BOLA_HIP_FACTOR_MIXED && rc == 0 && !ms.converged) status = BOLA_HIP_ERR_DEVICE;
+                if (rc == 0) BOLA_CHECK(hipMemcpyAsync(r.x, xb, n * r.nrhs * 8, hipMemcpyDefault, st));
                 BOLA_CHECK(hipFreeAsync(xb, st));
                 xb = nullptr;
+                if (bb) BOLA_CHECK(hipFreeAsync(bb, st));
+                bb = nullptr;
                 BOLA_CHECK(hipStreamSynchronize(st));
-                keep = !hit && r.factor_key != 0;
+                keep = !hit && r.factor_key != 0 && rc == 0;
             }
         } catch (const HipFailure& f) {
             status = BOLA_HIP_ERR_DEVICE;
             (void)hipStreamSynchronize(st);
             if (xb) (void)hipFreeAsync(xb, st);
+            if (bb) (void)hipFreeAsync(bb, st);
             keep = false;
             (void)f;
         }
-        if (hit) {
-            fa = nullptr;  // resident: owned by the cache
-            ipiv = nullptr;
-        }
+        if (hit) fo = nullptr;  // resident: owned by the cache
         (void)hipStreamSynchronize(st);
         const double t_done = now_us();
         {
             std::lock_guard<std::mutex> lk(mu);
-            const bool stored = cache.finish(reserved, keep, r.factor_key, r.kind, n, fa, ipiv);
-            if (!stored && fa) {  // not kept (or the key is busy with another factorization)
-                (void)hipFreeAsync(fa, st);
-                (void)hipFreeAsync(ipiv, st);
-            }
+            const bool stored = cache.finish(reserved, keep, r.factor_key, k.cache_kind, n, k.bytes, fo, nullptr);
+            if (!stored && fo) factor_release(fo, st);  // not kept (or the key is busy with another)
             if (hit) {
                 cache.unpin(r.factor_key);
                 hits++;
@@ -202,7 +217,6 @@ void bola_hip_server_s::work(int slot) {
         cv_work.notify_all();
     }
     (void)hipStreamSynchronize(st);
-    (void)hipFree(dinfo);
     bola_hip_destroy(h);
 }
 
@@ -238,7 +252,14 @@ int bola_hip_server_submit(bola_hip_server_t s, const bola_hip_request* r, uint6
     t->t_submit = now_us();
     std::lock_guard<std::mutex> lk(s->mu);
     bool hit = false;
-    const size_t nb = s->cache.need(r->factor_key, r->kind, r->n, r->nrhs, hit);
+    if (r->precision != BOLA_HIP_SERVE_FP64 &&
+        (r->kind != BOLA_HIP_SERVE_LU || r->precision < 1 || ((r->precision - 1) & 0xFF) > BOLA_HIP_PREC_BF16 ||
+         ((r->precision - 1) & ~(0xFF | BOLA_HIP_MIX_RBT)) != 0 || r->n > 16384)) {
+        set_error(nullptr, "server_submit: invalid precision (mixed precision for LU requests, n <= 16384)");
+        return -2;
+    }
+    const FactorKind k = factor_kind(*r);
+    const size_t nb = s->cache.need(r->factor_key, k.cache_kind, r->n, rhs_bytes(*r, k), k.bytes, hit);
     if (nb > s->cache.budget() || (!hit && !r->a)) {
         delete t;
         s->rejected++;

@@ -206,6 +206,47 @@ int bola_hip_dgesv_mixed(int precision, size_t n, size_t nrhs, const double* a, 
                          size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats);
 int bola_hip_dgesv_mixed_dev(bola_hip_handle_t h, int precision, size_t n, size_t nrhs, const double* a, size_t lda,
                              const double* b, size_t ldb, double* x, size_t ldx, bola_hip_mixed_stats* stats);
+/* ------------------------------------------------------------------ factorization objects
+ * No crate counterpart (the crate's callers keep an (a, ipiv) pair and call getrs-style
+ * compositions).  A factorization kept in device memory for repeated right-hand sides, in
+ * one format for the plain FP64 path, the mixed-precision solver and the solve server:
+ *   BOLA_HIP_FACTOR_LU      getrf (bit-identical to bola_hip_dgetrf) + getrs
+ *   BOLA_HIP_FACTOR_CHOL_L  potrf('L') + potrs('L'); _CHOL_U likewise
+ *   BOLA_HIP_FACTOR_MIXED   the mixed solver's factorization (precision as in
+ *                           bola_hip_dgesv_mixed): scalings, low-precision factors, pivots or
+ *                           butterflies, and a copy of A for the FP64 residuals; each solve
+ *                           refines to backward error <= n * 2^-52, and the FP64 LU of A is
+ *                           made the first time refinement fails and kept for later solves.
+ * A is read from host or device memory and not modified.  create returns 0, -i for an
+ * illegal argument i, or +k (the reference's Err: zero pivot / not positive definite; no
+ * object is made).  solve: b and x on the device, any nrhs (the mixed solver works 16
+ * columns at a time), x may equal b for the FP64 kinds only. */
+#define BOLA_HIP_FACTOR_LU 0
+#define BOLA_HIP_FACTOR_CHOL_L 1
+#define BOLA_HIP_FACTOR_CHOL_U 2
+#define BOLA_HIP_FACTOR_MIXED 3
+typedef struct bola_hip_factor_s* bola_hip_factor_t;
+typedef struct bola_hip_factor_desc {
+    int32_t kind, precision;
+    size_t n;
+    size_t bytes;              /* device memory held */
+    const void* factor;        /* FP64 LU / L / U (ld n), or the fp32 factors of the scaled A^T (ld) */
+    size_t ld;
+    const int32_t* ipiv;       /* 1-based pivots (LU, mixed with pivoting) or NULL */
+    const double* row_scale;   /* mixed: r (powers of two), A_s = diag(r) A diag(c) */
+    const double* col_scale;   /* mixed: c */
+    const double* matrix;      /* mixed: the FP64 copy of A the residuals use */
+    int32_t butterfly;         /* mixed: 1 when randomised by butterflies (no pivots) */
+    int32_t pad;
+} bola_hip_factor_desc;
+int bola_hip_factor_create_dev(bola_hip_handle_t h, int kind, int precision, size_t n, const double* a, size_t lda,
+                               bola_hip_factor_t* out);
+int bola_hip_factor_solve_dev(bola_hip_handle_t h, bola_hip_factor_t f, size_t nrhs, const double* b, size_t ldb,
+                              double* x, size_t ldx, bola_hip_mixed_stats* stats);
+int bola_hip_factor_describe(bola_hip_factor_t f, bola_hip_factor_desc* d);
+size_t bola_hip_factor_bytes(int kind, int precision, size_t n);
+int bola_hip_factor_destroy(bola_hip_factor_t f);
+
 /* The trailing-update product of that factorization, exposed for tests and measurement:
  * C -= A B with A (m x k), B (k x n), C (m x n) column-major fp32 device arrays, the
  * products in `precision` (BOLA_HIP_PREC_*), accumulated in fp32. */
@@ -270,9 +311,16 @@ int bola_hip_batch_graph_destroy(bola_hip_batch_graph_t g);
  * bit for bit, whatever else is running. */
 #define BOLA_HIP_SERVE_LU 0
 #define BOLA_HIP_SERVE_CHOL 1
+/* request.precision: BOLA_HIP_SERVE_FP64 -- the reference's FP64 factorization (results
+ * bit-identical to the one-shot entry points); BOLA_HIP_SERVE_MIXED(p) with p a
+ * BOLA_HIP_PREC_* value, optionally | BOLA_HIP_MIX_RBT -- LU requests only: the
+ * mixed-precision factorization object, each solve refined to FP64 accuracy
+ * (backward error <= n * 2^-52). */
+#define BOLA_HIP_SERVE_FP64 0
+#define BOLA_HIP_SERVE_MIXED(p) (1 + (p))
 typedef struct bola_hip_request {
     int32_t kind; /* BOLA_HIP_SERVE_LU or BOLA_HIP_SERVE_CHOL (lower triangle used) */
-    int32_t pad;
+    int32_t precision;
     size_t n, nrhs;
     const double* a;     /* n x n column-major, lda = n, host or device memory; may be NULL on a
                             cache hit.  A request with a key must present the same matrix. */

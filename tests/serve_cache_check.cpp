@@ -30,15 +30,16 @@ static void* tag(long k) { return (void*)(0x1000 + 16 * k); }
 // One request's life: reserve, (factor), finish; returns false if it did not fit.
 static bool run(ServeCache& c, uint64_t key, size_t n, size_t nrhs, bool& hit, long id, bool keep = true) {
     size_t reserved = 0;
-    if (!c.reserve(key, 0, n, nrhs, reserved, hit)) return false;
+    if (!c.reserve(key, 0, n, n * nrhs * 8, resident_factor_bytes(n), reserved, hit)) return false;
     if (hit) {
         bola::ResidentFactor* f = c.pinned(key);
         CHECK(f != nullptr);
         if (f) CHECK(freed.count(f->f) == 0);  // the factorization it reads is alive
-        c.finish(reserved, false, key, 0, n, nullptr, nullptr);
+        c.finish(reserved, false, key, 0, n, 0, nullptr, nullptr);
         c.unpin(key);
     } else {
-        const bool stored = c.finish(reserved, keep && key != 0, key, 0, n, tag(2 * id), tag(2 * id + 1));
+        const bool stored =
+            c.finish(reserved, keep && key != 0, key, 0, n, resident_factor_bytes(n), tag(2 * id), tag(2 * id + 1));
         if (!stored) release(tag(2 * id), tag(2 * id + 1));
     }
     return true;
@@ -72,7 +73,7 @@ int main() {
         bool hit = false;
         CHECK(run(c, 7, n, 1, hit, 1) && !hit);
         size_t reserved = 0;
-        CHECK(!c.reserve(7, 0, n, 64, reserved, hit));  // 128 KiB of right-hand side
+        CHECK(!c.reserve(7, 0, n, n * 64 * 8, f, reserved, hit));  // 128 KiB of right-hand side
         CHECK(!hit);
         CHECK(c.lookup(7, 0, n) != nullptr && c.lookup(7, 0, n)->users == 0);
         CHECK(freed.empty());
@@ -87,17 +88,17 @@ int main() {
         CHECK(run(c, 2, n, 1, hit, 2));
         size_t r1 = 0, r2 = 0, r3 = 0;
         bool h1 = false, h2 = false, h3 = false;
-        CHECK(c.reserve(1, 0, n, 1, r1, h1) && h1);
-        CHECK(c.reserve(2, 0, n, 1, r2, h2) && h2);
-        CHECK(!c.reserve(3, 0, n, 1, r3, h3));  // nothing idle to evict
+        CHECK(c.reserve(1, 0, n, n * 8, f, r1, h1) && h1);
+        CHECK(c.reserve(2, 0, n, n * 8, f, r2, h2) && h2);
+        CHECK(!c.reserve(3, 0, n, n * 8, f, r3, h3));  // nothing idle to evict
         CHECK(freed.empty());
-        c.finish(r1, false, 1, 0, n, nullptr, nullptr);
+        c.finish(r1, false, 1, 0, n, 0, nullptr, nullptr);
         c.unpin(1);
-        CHECK(c.reserve(3, 0, n, 1, r3, h3) && !h3);  // now key 1 can go
+        CHECK(c.reserve(3, 0, n, n * 8, f, r3, h3) && !h3);  // now key 1 can go
         CHECK(c.lookup(1, 0, n) == nullptr && c.lookup(2, 0, n) != nullptr);
-        c.finish(r2, false, 2, 0, n, nullptr, nullptr);
+        c.finish(r2, false, 2, 0, n, 0, nullptr, nullptr);
         c.unpin(2);
-        c.finish(r3, false, 3, 0, n, tag(9), tag(10));
+        c.finish(r3, false, 3, 0, n, f, tag(9), tag(10));
         CHECK(c.in_use() == f);
     }
     freed.clear();
@@ -108,8 +109,8 @@ int main() {
         bool hit = false;
         CHECK(run(c, 5, n, 1, hit, 1) && !hit);
         size_t reserved = 0;
-        CHECK(c.reserve(5, 1, n, 1, reserved, hit) && !hit);
-        CHECK(c.finish(reserved, true, 5, 1, n, tag(20), tag(21)));
+        CHECK(c.reserve(5, 1, n, n * 8, f, reserved, hit) && !hit);
+        CHECK(c.finish(reserved, true, 5, 1, n, f, tag(20), tag(21)));
         CHECK(freed.count(tag(2)) == 1);
         CHECK(c.lookup(5, 1, n) != nullptr && c.lookup(5, 0, n) == nullptr);
         CHECK(c.in_use() == f);
