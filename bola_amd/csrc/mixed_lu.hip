@@ -372,6 +372,14 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
         if (FULL || b < w) xr[(size_t)b * ld] = y[b];
 }
 
+// device-scope relaxed float accesses: coherent across XCDs without cache-wide fences
+__device__ __forceinline__ void xstf(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float xldf(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---- the butterfly mode: LU without interchanges, one launch per panel ----------------
 
 constexpr int NPLD = kMixNB + 4;      // LDS row stride (floats) of panel rows
@@ -500,7 +508,7 @@ __device__ __forceinline__ void diag_lu_rows(const float (*a)[NPLD], float (*d)[
 // raises err (the factors are then not to be trusted).
 __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __restrict__ T, size_t ldt,
                                                  float* __restrict__ inv, int* __restrict__ flags, int epoch,
-                                                 int* __restrict__ err, float* __restrict__ nlt) {
+                                                 int* __restrict__ err) {
     __shared__ float a[LPW][NPLD];   // this workgroup's rows, panel columns 0 .. w
     __shared__ float ub[LPW][NPLD];  // the current leaf's rows
     __shared__ float ui[LPW][LPW + 1], li[LPW][LPW + 1], d[LPW][LPW + 1];
@@ -543,16 +551,17 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         }
         __syncthreads();
         if (bail) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         {
+            // the owner's rows and inverses were stored coherently before its flag rose:
+            // read them coherently too (no cache-wide acquire)
             const int cc = min(max(col, cj), w - 1);
             float v[LPW];
 #pragma unroll
-            for (int r = 0; r < LPW; r++) v[r] = P[cc + (size_t)(r0 + cj + min(r, lw - 1)) * ldt];
+            for (int r = 0; r < LPW; r++) v[r] = xldf(&P[cc + (size_t)(r0 + cj + min(r, lw - 1)) * ldt]);
 #pragma unroll
             for (int r = 0; r < LPW; r++) ub[r][col] = (r < lw && col >= cj && col < w) ? v[r] : 0.f;
         }
-        for (int e = tid; e < LPW * LPW; e += 256) ui[e / LPW][e % LPW] = inv[(size_t)j * 2 * LPW * LPW + e];
+        for (int e = tid; e < LPW * LPW; e += 256) ui[e / LPW][e % LPW] = xldf(&inv[(size_t)j * 2 * LPW * LPW + e]);
         __syncthreads();
         rbt_apply_leaf(a, ub, ui, cj, lw, w);
     }
@@ -588,87 +597,153 @@ __global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __
         if (tid < lw)
             for (int r = 0; r < lw; r++) a[r][cj + tid] = d[r][tid];
         __syncthreads();
+        // the leaf's rows and inverses, stored coherently (device scope, past the XCD's L2)
+        // and complete before the flag rises: the readers need no cache-wide fences
         for (int r = 0; r < nr; r++)
-            if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
+            if (col < w) xstf(&P[col + (size_t)(row0 + r) * ldt], a[r][col]);
         float* io = inv + (size_t)wb * 2 * LPW * LPW;
         for (int e = tid; e < LPW * LPW; e += 256) {
-            io[e] = ui[e / LPW][e % LPW];
-            io[LPW * LPW + e] = li[e / LPW][e % LPW];
+            xstf(&io[e], ui[e / LPW][e % LPW]);
+            xstf(&io[LPW * LPW + e], li[e / LPW][e % LPW]);
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(flags + wb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        // Then, off the leaves' chain: row block wb of L11^-1 (the panel's unit lower factor
-        // inverted), for the U12 products.  Block (wb, j) = -Li_wb,wb sum_{m=j}^{wb-1}
-        // L_wb,m Li_m,j, with the rows of earlier owners read back from nlt once their flags
-        // (flags[8 + m]) rise.  nlt holds -(L11^-1)^T column-major, i.e. nlt[c + r w] =
-        // -L11^-1(r, c).
-        float* const nrow = nlt + (size_t)cj * w;  // rows cj .. cj + lw of -L11^-1
-        float tacc[kMixNB / LPW][4];
-#pragma unroll
-        for (int j = 0; j < kMixNB / LPW; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) tacc[j][q] = 0.f;
-        const int er = tid >> 3, ec = (tid & 7) * 4;  // this thread's 4 elements of a 32 x 32 block
-        for (int m = 0; m < wb; m++) {
-            if (tid == 0) {
-                long sp = 0;
-                int ok = 1;
-                while (__hip_atomic_load(flags + kMixNB / LPW + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                       epoch) {
-                    if (++sp > kPanelSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                        ok = 0;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-                if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                bail = !ok;
-            }
-            __syncthreads();
-            if (bail) return;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            // ub = L11^-1 rows of block m, columns 0 .. 32(m+1) (nlt holds the negatives)
-            for (int r = 0; r < LPW; r++)
-                ub[r][col] = col < LPW * (m + 1) ? -nlt[(size_t)col + (size_t)(LPW * m + r) * w] : 0.f;
-            __syncthreads();
-            // tacc[j] += L_wb,m Li_m,j for j <= m
-#pragma unroll
-            for (int j = 0; j < kMixNB / LPW; j++) {
-                if (j > m) break;
-#pragma unroll 4
-                for (int s2 = 0; s2 < LPW; s2++) {
-                    const float l = a[er][LPW * m + s2];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) tacc[j][q] = fmaf(l, ub[s2][LPW * j + ec + q], tacc[j][q]);
-                }
-            }
-            __syncthreads();
-        }
-        // block j < wb: -Li_wb,wb T_j (through LDS d), then the diagonal block and zeros
-        for (int j = 0; j < wb; j++) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) d[er][ec + q] = tacc[j][q];
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                float v = 0.f;
-#pragma unroll 8
-                for (int t = 0; t < LPW; t++) v = fmaf(li[er][t], d[t][ec + q], v);
-                if (er < lw) nrow[(size_t)(LPW * j + ec + q) + (size_t)er * w] = v;  // -(-Li T) stored negated
-            }
-            __syncthreads();
-        }
-        for (int r = 0; r < lw; r++)
-            if (col >= cj && col < w)
-                nrow[(size_t)col + (size_t)r * w] = col < cj + lw ? -li[r][col - cj] : 0.f;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(flags + kMixNB / LPW + wb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_store(flags + wb, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     for (int r = 0; r < nr; r++)
         if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
+}
+
+// -L11^-1 of a factored panel (w x w, unit lower), stored transposed: nlt[c + r w] =
+// -L11^-1(r, c), for the U12 products.  First its diagonal 32 x 32 blocks (the leaves'
+// inverses, inv[j] + 32*32) and zeros above them; then, doubling the block size h = 32,
+// 64, 128, .., the off-diagonal block of every pair of h-blocks:
+//   [A 0; B C]^-1 = [A^-1 0; -C^-1 B A^-1  C^-1].
+// L(r, c) is the panel's entry in T; one workgroup per 32 x 32 tile of -C^-1 B A^-1.
+__global__ void __launch_bounds__(256) linv_diag(int w, const float* __restrict__ inv, float* __restrict__ nlt) {
+    const int r = blockIdx.x;  // row of L11^-1
+    for (int c = threadIdx.x; c < w; c += 256) {
+        const int j = r / LPW;
+        float v = 0.f;
+        if (c / LPW == j) v = -inv[(size_t)j * 2 * LPW * LPW + LPW * LPW + (r % LPW) * LPW + (c % LPW)];
+        if (c / LPW < j) continue;  // filled by the doubling steps
+        nlt[(size_t)c + (size_t)r * w] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) linv_level(int w, int h, const float* __restrict__ T, size_t ldt, int r0,
+                                                  float* __restrict__ nlt) {
+    const int tiles = (h + LPW - 1) / LPW;
+    const int pair = blockIdx.x / (tiles * tiles), tt = blockIdx.x % (tiles * tiles);
+    const int tr = tt / tiles, tc = tt % tiles;  // tile of Y: rows 32 tr.., columns 32 tc..
+    const int i0 = 2 * h * pair;
+    const int hc = min(h, w - i0 - h);           // rows of the C block
+    if (hc <= 0 || 32 * tr >= hc) return;
+    const int tid = threadIdx.x, lane_c = tid & 31, grp = tid >> 5;
+    const int rmax = min(32 * tr + 32, hc);      // X rows needed: t < rmax
+    constexpr int HM = kMixNB / 2;               // largest h
+    // LDS kept under what a CU has left beside one trailing-update workgroup, so that this
+    // launch is not starved by the update running on the other stream
+    __shared__ float Bs[HM][LPW + 1];            // B(t, s) = L(i0 + h + t, i0 + s), t < rmax, one 32-chunk of s
+    __shared__ float Ai[HM][LPW + 1];            // A^-1(s, c) for the tile's columns
+    __shared__ float Xs[HM][LPW + 1];            // X = B A^-1, the tile's columns
+    __shared__ float Ci[LPW][HM + 1];            // C^-1 rows of the tile
+    const float* P = T + r0;
+    auto Lv = [&](int r, int c) { return P[c + (size_t)(r0 + r) * ldt]; };           // L(r, c), r > c
+    auto Li = [&](int r, int c) { return -nlt[(size_t)c + (size_t)r * w]; };       // L11^-1(r, c)
+    // loads in batches of 8 per thread, all in flight together (clamped addresses)
+    constexpr int NB8 = 8;
+    for (int e0 = tid; e0 < h * LPW; e0 += 256 * NB8) {
+        float v[NB8];
+#pragma unroll
+        for (int q = 0; q < NB8; q++) {
+            const int e = min(e0 + 256 * q, h * LPW - 1), s2 = e / LPW, gc = min(32 * tc + e % LPW, h - 1);
+            v[q] = Li(i0 + s2, i0 + gc);
+        }
+#pragma unroll
+        for (int q = 0; q < NB8; q++) {
+            const int e = e0 + 256 * q;
+            if (e >= h * LPW) continue;
+            const int s2 = e / LPW, c = e % LPW, gc = 32 * tc + c;
+            Ai[s2][c] = (gc < h && s2 >= gc) ? v[q] : 0.f;
+        }
+    }
+    for (int e0 = tid; e0 < LPW * hc; e0 += 256 * NB8) {
+        float v[NB8];
+#pragma unroll
+        for (int q = 0; q < NB8; q++) {
+            const int e = min(e0 + 256 * q, LPW * hc - 1), gr = min(32 * tr + e / hc, hc - 1), t = e % hc;
+            v[q] = Li(i0 + h + gr, i0 + h + t);
+        }
+#pragma unroll
+        for (int q = 0; q < NB8; q++) {
+            const int e = e0 + 256 * q;
+            if (e >= LPW * hc) continue;
+            const int r = e / hc, t = e % hc, gr = 32 * tr + r;
+            Ci[r][t] = (gr < hc && t <= gr) ? v[q] : 0.f;
+        }
+    }
+    // X = B A^-1 over 32-column chunks of B; each thread keeps its rows' sums in registers
+    constexpr int RPT = HM / 8;  // rows t per thread: grp, grp + 8, ..
+    float xs[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) xs[q] = 0.f;
+    for (int s0 = 0; s0 < h; s0 += LPW) {
+        __syncthreads();  // the previous chunk has been read
+        for (int e0 = tid; e0 < rmax * LPW; e0 += 256 * NB8) {
+            float v[NB8];
+#pragma unroll
+            for (int q = 0; q < NB8; q++) {
+                const int e = min(e0 + 256 * q, rmax * LPW - 1);
+                v[q] = Lv(i0 + h + e / LPW, i0 + s0 + e % LPW);
+            }
+#pragma unroll
+            for (int q = 0; q < NB8; q++) {
+                const int e = e0 + 256 * q;
+                if (e < rmax * LPW) Bs[e / LPW][e % LPW] = v[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int t = grp + 8 * q;
+            if (t < rmax) {
+                float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+#pragma unroll
+                for (int s2 = 0; s2 < LPW; s2 += 4) {
+                    x0 = fmaf(Bs[t][s2], Ai[s0 + s2][lane_c], x0);
+                    x1 = fmaf(Bs[t][s2 + 1], Ai[s0 + s2 + 1][lane_c], x1);
+                    x2 = fmaf(Bs[t][s2 + 2], Ai[s0 + s2 + 2][lane_c], x2);
+                    x3 = fmaf(Bs[t][s2 + 3], Ai[s0 + s2 + 3][lane_c], x3);
+                }
+                xs[q] += (x0 + x1) + (x2 + x3);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int t = grp + 8 * q;
+        if (t < rmax) Xs[t][lane_c] = xs[q];
+    }
+    __syncthreads();
+    for (int r = grp; r < LPW; r += 8) {
+        const int gr = 32 * tr + r, gc = 32 * tc + lane_c;
+        if (gr >= hc || gc >= h) continue;
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f, y3 = 0.f;
+        int t = 0;
+#pragma unroll 4
+        for (; t + 3 <= gr; t += 4) {
+            y0 = fmaf(Ci[r][t], Xs[t][lane_c], y0);
+            y1 = fmaf(Ci[r][t + 1], Xs[t + 1][lane_c], y1);
+            y2 = fmaf(Ci[r][t + 2], Xs[t + 2][lane_c], y2);
+            y3 = fmaf(Ci[r][t + 3], Xs[t + 3][lane_c], y3);
+        }
+        for (; t <= gr; t++) y0 = fmaf(Ci[r][t], Xs[t][lane_c], y0);
+        const float y = (y0 + y1) + (y2 + y3);
+        // L11^-1(i0 + h + gr, i0 + gc) = -y; nlt holds the negative
+        nlt[(size_t)(i0 + gc) + (size_t)(i0 + h + gr) * w] = y;
+    }
 }
 
 // dst (rows x cols, column-major, ld ldd) = src (ld lds)
@@ -739,6 +814,23 @@ void calu_panel(const LuCtx& c, int r0, int w) {
 
 float* Tp(const LuCtx& c, int col, int row) { return c.T + (size_t)col + (size_t)row * c.ldt; }
 
+// The butterfly mode's panel [k, k+nb): one launch, the leaf owners and every row below.
+void launch_panel_rbt(const LuCtx& c, int k, int nb) {
+    hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(c.n - k), LPW)), dim3(256), 0, c.st, c.n, k, nb, c.T, c.ldt,
+                       c.inv, c.flags, k / kMixNB + 1, c.err);
+    BOLA_LAUNCH_CHECK();
+    // -L11^-1 for the U12 products
+    hipLaunchKernelGGL(linv_diag, dim3((unsigned)nb), dim3(256), 0, c.st, nb, c.inv, c.nlt);
+    BOLA_LAUNCH_CHECK();
+    for (int h = LPW; h < nb; h *= 2) {
+        const int tiles = (h + LPW - 1) / LPW;
+        const int pairs = (nb + 2 * h - 1) / (2 * h);
+        hipLaunchKernelGGL(linv_level, dim3((unsigned)(pairs * tiles * tiles)), dim3(256), 0, c.st, nb, h, c.T, c.ldt,
+                           k, c.nlt);
+        BOLA_LAUNCH_CHECK();
+    }
+}
+
 // X U = B on T blocks: X/B at (xc, r0), nr x w; U at (r0c, r0), w x w unit upper
 void strsm_rupu(const LuCtx& c, int nr, int w, int uc, int ur, int xc) {
     if (nr <= 0 || w <= 0) return;
@@ -801,14 +893,36 @@ void update_cols(const LuCtx& c, int k, int nb, int c_lo, int c_hi) {
     BOLA_LAUNCH_CHECK();
 }
 
+// The butterfly mode's two consecutive panels [k, k+nb) and [k+nb, k+2nb) (nb = kMixNB) on
+// the columns [c_lo, c_hi): the first panel's U rows, its update of the second panel's
+// rows, the second panel's U rows, then the rows below both in one 2nb-deep product --
+// half the passes over the trailing matrix of two single-panel updates.
+void update_pair(const LuCtx& c, int k, const float* nlt_a, const float* nlt_b, int c_lo, int c_hi) {
+    const int ncols = c_hi - c_lo, nb = kMixNB;
+    if (ncols <= 0) return;
+    float* ub = c.ubuf + c_lo;  // columns 0 .. 2nb-1 of U12^T, ld n
+    launch_sgemm_set(c.st, ncols, nb, nb, Tp(c, c_lo, k), c.ldt, nlt_a, (size_t)nb, ub, (size_t)c.n);
+    launch_lp_gemm(c.st, c.lp, ncols, nb, nb, ub, (size_t)c.n, Tp(c, k, k + nb), c.ldt, Tp(c, c_lo, k + nb), c.ldt,
+                   c.gws);
+    launch_sgemm_set(c.st, ncols, nb, nb, Tp(c, c_lo, k + nb), c.ldt, nlt_b, (size_t)nb, ub + (size_t)nb * c.n,
+                     (size_t)c.n);
+    const int below = c.n - k - 2 * nb;
+    if (below > 0)
+        launch_lp_gemm(c.st, c.lp, ncols, below, 2 * nb, ub, (size_t)c.n, Tp(c, k, k + 2 * nb), c.ldt,
+                       Tp(c, c_lo, k + 2 * nb), c.ldt, c.gws);
+    hipLaunchKernelGGL(copy2d, dim3(cdiv((size_t)ncols, 256), (unsigned)std::min(2 * nb, 256)), dim3(256), 0, c.st,
+                       ncols, 2 * nb, ub, (size_t)c.n, Tp(c, c_lo, k), c.ldt);
+    BOLA_LAUNCH_CHECK();
+}
+
 }  // namespace
 
 size_t mixed_lu_ws_bytes(int n, int lp) {
     const size_t G = (size_t)(n + GROWS - 1) / GROWS;
     size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
-    b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 256);
+    b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)2 * kMixNB) + 256);
     b += 2 * kInvPanel * sizeof(float) + 1024;
-    b += (2 * (size_t)kMixNB * kMixNB + (size_t)n * kMixNB) * sizeof(float) + 1024;
+    b += (4 * (size_t)kMixNB * kMixNB + (size_t)n * 2 * kMixNB) * sizeof(float) + 1024;
     return b;
 }
 
@@ -838,12 +952,14 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
     c.cnt = (int*)(b + o);
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
-    const size_t gsz = (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)kMixNB) + 511) & ~(size_t)255;
+    const size_t gsz = (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)2 * kMixNB) + 511) & ~(size_t)255;
     c.gws = b + o;
     float* const invb = (float*)(b + o + 2 * gsz);  // two panels' inverses, by panel parity
     c.flags = (int*)(invb + 2 * kInvPanel);
-    float* const nltb = (float*)((char*)c.flags + 1024);  // two panels' -(L11^-1)^T, by parity
-    c.ubuf = nltb + 2 * (size_t)kMixNB * kMixNB;
+    // four panels' -(L11^-1)^T (panel index mod 4): stream 2 may still read a pair's while
+    // stream 1 factors the next pair
+    float* const nltb = (float*)((char*)c.flags + 1024);
+    c.ubuf = nltb + 4 * (size_t)kMixNB * kMixNB;
     c.err = err;
     LuCtx c2 = c;  // stream 2: its own 16-bit operand buffers
     c2.st = h->aux_stream();
@@ -857,6 +973,45 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
     }
     int ev = 0;
     hipEvent_t bulk_done = nullptr;
+    if (!pivot) {
+        // Butterfly mode, panels in pairs.  Stream 1: panel a, its update of panel b's
+        // columns, panel b, then both panels on the next pair's columns; stream 2 meanwhile
+        // applies the previous pair to every column after that.
+        auto nltp = [&](int k) { return nltb + (size_t)((k / kMixNB) % 4) * kMixNB * kMixNB; };
+        auto panel = [&](int k, int nb) {
+            c.inv = c2.inv = invb + (size_t)((k / kMixNB) & 1) * kInvPanel;
+            c.nlt = nltp(k);
+            launch_panel_rbt(c, k, nb);
+        };
+        c2.ubuf = c.ubuf;
+        for (int k = 0; k < n; k += 2 * kMixNB) {
+            const int nba = std::min(kMixNB, n - k);
+            const int kb = k + nba, nbb = std::min(kMixNB, n - kb);
+            panel(k, nba);
+            if (nbb > 0) {
+                update_cols(c, k, nba, kb, kb + nbb);  // c.nlt: panel a's
+                panel(kb, nbb);
+            }
+            const int lo = kb + std::max(nbb, 0);  // n unless both panels are full
+            if (lo >= n) break;
+            // the next pair's columns were the previous bulk update's until it is done
+            if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
+            const int la_hi = std::min(n, lo + 2 * kMixNB);
+            update_pair(c, k, nltp(k), nltp(kb), lo, la_hi);
+            if (la_hi < n) {
+                hipEvent_t la = handle_event(h, ev++);
+                BOLA_CHECK(hipEventRecord(la, c.st));
+                BOLA_CHECK(hipStreamWaitEvent(c2.st, la, 0));
+                update_pair(c2, k, nltp(k), nltp(kb), la_hi, n);
+                bulk_done = handle_event(h, ev++);
+                BOLA_CHECK(hipEventRecord(bulk_done, c2.st));
+            } else {
+                bulk_done = nullptr;
+            }
+        }
+        if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
+        return;
+    }
     for (int k = 0; k < n; k += kMixNB) {
         const int nb = std::min(kMixNB, n - k);
         // the panel: its own columns take the interchanges at once
@@ -870,9 +1025,7 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
             c.inv = c2.inv = invb + (size_t)par * kInvPanel;
             c.nlt = c2.nlt = nltb + (size_t)par * kMixNB * kMixNB;
             c2.ubuf = c.ubuf;
-            hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(n - k), LPW)), dim3(256), 0, c.st, n, k, nb, T, ldt, c.inv,
-                               c.flags, k / kMixNB + 1, err, c.nlt);
-            BOLA_LAUNCH_CHECK();
+            launch_panel_rbt(c, k, nb);
         }
         // the columns right of the panel are stream 2's until its previous bulk update is done
         if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
