@@ -8,16 +8,22 @@ step).  Throughput counts 2n^3/3 + 2n^2*nrhs flops per solve.  Per-step latencie
 p50/p99.
 
 Beside it, in the same JSON line:
-  mixed_ir  -- A X = b at n = 16384, nrhs = 1, A = U diag(sigma) V^T with kappa = 1e2
-               (seeded Householder products), solved to backward error <= n * 2^-52 by the
-               mixed-precision solver (FP32 or FP16 factors, FP64 refinement / GMRES-IR,
-               FP64 fallback), each mode timed once after a warm-up, next to the plain FP64
-               GETRF + GETRS of the same system; the forward error is stated against the
-               plain FP64 solution (which is bit-identical to the oracle's).
+  dgemm     -- plain DGEMM 'N','N' at 1024^3 (the reference's bench size) and 8192^3.
+  cholesky  -- DPOTRF + DPOTRS ('L' and 'U') at n = 4096 and 8192, nrhs = 64, on
+               A = G G^T + n I, median of 5 solves.
+  mixed_ir  -- the mixed-precision grid: A X = B at n in {8192, 16384}, kappa in {1e2, 1e5,
+               1e8}, nrhs in {1, 16}, A = U diag(sigma) V^T (seeded Householder products),
+               solved to backward error <= n * 2^-52 by the mixed-precision solver in six
+               modes (fp32 / fp16 / bf16 products, tournament pivoting or butterflies;
+               FP64 refinement, GMRES-IR, FP64 fallback), each mode and the plain FP64
+               GETRF + GETRS the median of 3 solves after a warm-up; the forward error is
+               stated against the plain FP64 solution (which is bit-identical to the
+               oracle's).
   batched   -- 20,000 independent systems, orders uniform in [8, 512], half LU and half
                SPD (Cholesky), nrhs uniform in [1, 16], one persistent launch each for the
                FP64 and the FP32 + FP64-refinement variants, with the autotuned launch
                configuration.
+  serving   -- the solve server under Poisson arrivals, orders log-uniform in [32, 8192].
 
 The dominant kernel of the headline step is the LU panel (phase GETRF2_IT, the persistent
 wavefront panel kernel); its algorithmic traffic is the panel read and written once per
@@ -26,9 +32,12 @@ the HBM peak.  The events run in a second pass of as many steps as the timed one
 the timed steps they would add 10 ms or more per step.  The trailing-update DGEMM is
 reported beside it against the FP64 matrix peak.
 
-CPU baseline: the oracle (a C restatement of the reference's own algorithm, OpenMP
-standing in for rayon), timed on this host on one factorization + solve of the same
-size and inputs; its result is also compared bit for bit with the GPU's.
+CPU baseline (a reported baseline, not the target): the oracle (a C restatement of the
+reference's own algorithm, OpenMP standing in for rayon) on this host's cores, one warm-up
+then the median and p10/p90 of 10 runs of the headline solve on the same inputs (its first
+timed result compared bit for bit with the GPU's), plus rows for DGEMM 1024^3, DGETRF +
+DGETRS at 4096 and DPOTRF + DPOTRS 'L' / 'U' at 4096; the CPU model and core counts are
+recorded with it.
 """
 import argparse
 import json
@@ -58,8 +67,11 @@ def main():
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=0, help="size of the CPU baseline sample (default: n)")
+    ap.add_argument("--cpu-runs", type=int, default=10, help="timed CPU runs per row, after one warm-up")
+    ap.add_argument("--no-cpu-rows", action="store_true",
+                    help="time only the headline row on the CPU (not DGEMM 1024, LU 4096, Cholesky L/U 4096)")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
-    ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "serving"], default=None,
+    ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "serving", "cholesky", "cpu"], default=None,
                     help="run one section alone (for profiling) and print it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
@@ -84,7 +96,8 @@ def main():
     if args.only:
         torch.cuda.set_device(0)
         sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section,
-               "serving": serving_section}[args.only](B, torch)
+               "serving": serving_section, "cholesky": chol_section, "cpu": lambda B, torch: dict(cpu_rows(), **cpu_info())}[
+            args.only](B, torch)
         print(json.dumps({args.only: sec}))
         return
 
@@ -170,20 +183,28 @@ def main():
         cA = A0 if cn == n else rng.uniform(-1.0, 1.0, cn * cn)
         cB = B0 if cn == n else rng.uniform(-1.0, 1.0, cn * nrhs)
         a, b, p = cA.copy(), cB.copy(), np.zeros(cn, np.int32)
-        t0 = time.perf_counter()
-        O.getrf(cn, cn, a, cn, p)
-        O.getrs(cn, nrhs, a, cn, p, b, cn)
-        ct = time.perf_counter() - t0
-        same = None
-        if cn == n:
-            same = bool(np.array_equal(p, dp.cpu().numpy()) and np.array_equal(b, dB.cpu().numpy()))
-        cpu = {"value": lu_flops(cn, nrhs) / ct / 1e9, "unit": "GFLOP/s", "cores": O.num_threads(),
-               "kind": "port", "sample": f"one DGETRF+DGETRS n={cn} nrhs={nrhs} of the same seeded inputs, "
-                                         f"{ct:.2f} s; bit-identical to the GPU result: {same}"}
 
-    extra_mixed = extra_batched = extra_dgemm = extra_serving = None
+        def cpu_lu():
+            O.getrf(cn, cn, a, cn, p)
+            O.getrs(cn, nrhs, a, cn, p, b, cn)
+            return bool(cn == n and np.array_equal(p, dp.cpu().numpy()) and np.array_equal(b, dB.cpu().numpy()))
+
+        ts, same = cpu_timed(lambda: (np.copyto(a, cA), np.copyto(b, cB)), cpu_lu, args.cpu_runs)
+        cpu = cpu_stats(ts, lu_flops(cn, nrhs))
+        cpu.update({"value": round(cpu["value"], 2), "unit": "GFLOP/s", "cores": O.num_threads(), "kind": "port",
+                    "sample": f"DGETRF+DGETRS n={cn} nrhs={nrhs} of the same seeded inputs, one warm-up then "
+                              f"{args.cpu_runs} runs (median {cpu['median_s']:.2f} s); first timed run "
+                              f"bit-identical to the GPU result: {same}",
+                    "label": CPU_LABEL})
+        cpu.update(cpu_info())
+        print(f"# cpu headline: median {cpu['median_s']:.3f} s, {cpu['value']} GFLOP/s", file=sys.stderr, flush=True)
+        if not args.no_cpu_rows:
+            cpu["rows"] = cpu_rows(args.cpu_runs)
+
+    extra_mixed = extra_batched = extra_dgemm = extra_serving = extra_chol = None
     if not args.no_extra:
         extra_dgemm = dgemm_section(B, torch)
+        extra_chol = chol_section(B, torch)
         extra_mixed = mixed_section(B, torch)
         extra_batched = batched_section(B, torch)
         extra_serving = serving_section(B, torch)
@@ -221,6 +242,7 @@ def main():
                  "gemm_ms_per_step": round(gemm_ms / args.steps, 3),
                  "gemm_launches_per_step": gemm_calls // max(args.steps, 1)},
         "dgemm": extra_dgemm,
+        "cholesky": extra_chol,
         "mixed_ir": extra_mixed,
         "batched": extra_batched,
         "serving": extra_serving,
@@ -228,6 +250,100 @@ def main():
     }
     print(report, file=sys.stderr)
     print(json.dumps(out))
+
+
+CPU_LABEL = "BOLA CPU algorithm, C restatement (OpenMP for rayon) - reported baseline, not target"
+
+
+def cpu_info():
+    """CPU model, physical cores and logical CPUs of this host, and the CPUs this process may use."""
+    model, cores, cur = None, set(), {}
+    try:
+        for line in list(open("/proc/cpuinfo")) + [""]:
+            if ":" not in line:
+                if cur:
+                    cores.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            if k == "model name" and model is None:
+                model = v
+            if k in ("physical id", "core id"):
+                cur[k] = v
+    except OSError:
+        pass
+    return {"cpu_model": model, "physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_timed(prep, run, runs):
+    """One warm-up, then `runs` timed calls, each after an untimed prep (the inputs copied
+    fresh, as criterion's iter_batched does).  Returns the times in seconds and the result
+    of the first timed call."""
+    prep()
+    run()
+    ts, first = [], None
+    for r in range(runs):
+        prep()
+        t0 = time.perf_counter()
+        res = run()
+        ts.append(time.perf_counter() - t0)
+        if r == 0:
+            first = res
+    return ts, first
+
+
+def cpu_stats(ts, work, unit_scale=1e9):
+    med = float(np.median(ts))
+    return {"value": work / med / unit_scale, "median_s": round(med, 4),
+            "p10_s": round(float(np.percentile(ts, 10)), 4), "p90_s": round(float(np.percentile(ts, 90)), 4),
+            "runs": len(ts)}
+
+
+def cpu_rows(runs=10, n_lu=4096, n_chol=4096, nrhs=64):
+    """The oracle's rows for the plain configurations beside the headline one: DGEMM 'N','N'
+    1024^3 (alpha = beta = 1, uniform [0, 1)), DGETRF + DGETRS at n_lu, DPOTRF + DPOTRS for
+    'L' (single-threaded in the reference) and 'U' at n_chol, each one warm-up then the
+    median and p10/p90 of `runs` calls.  Cholesky at 8192 is left out: its 'L' factorization
+    alone takes more than ten seconds a call on one core."""
+    import oracle as O
+    rng = np.random.default_rng(1)
+    rows = {}
+    n = 1024
+    a, b, c0 = rng.random(n * n), rng.random(n * n), rng.random(n * n)
+    c = c0.copy()
+    ts, _ = cpu_timed(lambda: np.copyto(c, c0), lambda: O.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 1.0, c, n), runs)
+    rows["dgemm_n1024"] = cpu_stats(ts, 2.0 * n ** 3)
+    n = n_lu
+    A0, B0 = rng.uniform(-1, 1, n * n), rng.uniform(-1, 1, n * nrhs)
+    a, x, p = A0.copy(), B0.copy(), np.zeros(n, np.int32)
+
+    def lu():
+        O.getrf(n, n, a, n, p)
+        O.getrs(n, nrhs, a, n, p, x, n)
+
+    ts, _ = cpu_timed(lambda: (np.copyto(a, A0), np.copyto(x, B0)), lu, runs)
+    rows[f"getrf_getrs_n{n}_nrhs{nrhs}"] = cpu_stats(ts, lu_flops(n, nrhs))
+    n = n_chol
+    G = rng.uniform(-1, 1, (n, n))
+    S0 = (G @ G.T + n * np.eye(n)).T.reshape(-1).copy()
+    del G
+    B0 = rng.uniform(-1, 1, n * nrhs)
+    a, x = S0.copy(), B0.copy()
+    for uplo in ("L", "U"):
+        def chol():
+            O.potrf(uplo, n, a, n)
+            O.potrs(uplo, n, nrhs, a, n, x, n)
+
+        ts, _ = cpu_timed(lambda: (np.copyto(a, S0), np.copyto(x, B0)), chol, runs)
+        rows[f"potrf_potrs_{uplo}_n{n}_nrhs{nrhs}"] = cpu_stats(ts, n ** 3 / 3.0 + 2.0 * n * n * nrhs)
+    for k, v in rows.items():
+        v["value"] = round(v["value"], 2)
+        v["unit"] = "GFLOP/s"
+        print(f"# cpu {k}: median {v['median_s'] * 1e3:.1f} ms, {v['value']} GFLOP/s", file=sys.stderr, flush=True)
+    rows["threads"] = O.num_threads()
+    rows["label"] = CPU_LABEL
+    return rows
 
 
 def panel_traffic():
@@ -272,55 +388,136 @@ def dgemm_section(B, torch):
     return out
 
 
-def mixed_section(B, torch, n=16384, kappa=1e2):
-    """Config: mixed-precision IR at n = 16384, nrhs = 1, kappa = 1e2."""
+def chol_section(B, torch, sizes=(4096, 8192), nrhs=64, runs=5):
+    """Plain FP64 Cholesky solve: DPOTRF + DPOTRS ('L' and 'U') of A = G G^T + n I with G
+    uniform [-1, 1) (seeded, built in FP64 on the device), nrhs = 64 right-hand sides
+    uniform [-1, 1); one warm-up, then the median of `runs` solves, each from a fresh HBM
+    copy of A and B made outside its timed interval.  Throughput counts n^3/3 + 2 n^2 nrhs
+    flops per solve."""
+    out = {"workload": f"DPOTRF+DPOTRS, A = G G^T + n I (G uniform [-1,1)), nrhs={nrhs}, column-major f64, "
+                       f"median of {runs}"}
+    for n in sizes:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(n)
+        G = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+        A0 = (G @ G.t() + n * torch.eye(n, dtype=torch.float64, device="cuda")).reshape(-1)
+        del G
+        B0 = torch.rand(n * nrhs, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+        a = torch.empty_like(A0)
+        x = torch.empty_like(B0)
+        flops = n ** 3 / 3.0 + 2.0 * n * n * nrhs
+        for uplo in ("L", "U"):
+            ts = []
+            for r in range(runs + 1):
+                a.copy_(A0)
+                x.copy_(B0)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                B.potrf(uplo, n, a, n)
+                B.potrs(uplo, n, nrhs, a, n, x, n)
+                torch.cuda.synchronize()
+                if r:
+                    ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            # backward error of the last solve, on the device in FP64
+            Am = A0.view(n, n).t()
+            X = x.view(nrhs, n).t()
+            R = B0.view(nrhs, n).t() - Am @ X
+            an = float(Am.abs().sum(dim=1).max())
+            berr = float((R.abs().amax(dim=0) / (an * X.abs().amax(dim=0) + B0.view(nrhs, n).t().abs().amax(dim=0))).max())
+            out[f"n{n}_{uplo}"] = {"ms": round(t * 1e3, 3), "gflops": round(flops / t / 1e9, 1),
+                                   "frac_fp64_peak": round(flops / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+                                   "p10_ms": round(float(np.percentile(ts, 10)) * 1e3, 3),
+                                   "p90_ms": round(float(np.percentile(ts, 90)) * 1e3, 3),
+                                   "backward_error": berr}
+            print(f"# cholesky {n} {uplo}: {t * 1e3:.2f} ms, {flops / t / 1e9:.0f} GFLOP/s, berr {berr:.2e}",
+                  file=sys.stderr, flush=True)
+        del A0, B0, a, x
+        torch.cuda.empty_cache()
+    return out
+
+
+def mixed_section(B, torch, sizes=(8192, 16384), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16), runs=3):
+    """Mixed-precision IR grid: n in {8192, 16384}, kappa in {1e2, 1e5, 1e8}, nrhs in {1, 16},
+    A = U diag(sigma) V^T (seeded Householder products, sigma geometric from 1 to 1/kappa),
+    B uniform [-1, 1).  For every point: the plain FP64 GETRF + GETRS and each mode
+    (fp32 / fp16 / bf16 products, tournament pivoting or random butterflies without
+    interchanges), one warm-up then the median of `runs` timed solves each.  The forward
+    error of each mode is stated against the plain FP64 solution, which equals the oracle's
+    bit for bit (the GPU parity tests check that at these sizes)."""
     from bola_amd import testmat
-    A = testmat.conditioned(n, kappa, 7, device="cuda")
-    b = torch.from_numpy(np.random.default_rng(8).uniform(-1, 1, n)).cuda()
-    flops = lu_flops(n, 1)
-    # plain FP64 on the same system
-    a = A.clone()
-    p = torch.zeros(n, dtype=torch.int32, device="cuda")
-    xp = b.clone()
-    B.getrf(n, n, a, n, p)
-    B.getrs(n, 1, a, n, p, xp, n)
-    torch.cuda.synchronize()
-    a.copy_(A)
-    xp.copy_(b)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    B.getrf(n, n, a, n, p)
-    B.getrs(n, 1, a, n, p, xp, n)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    del a
-    out = {"workload": f"A X = b, n={n} nrhs=1, A = U diag(sigma) V^T, kappa={kappa:g}, b uniform [-1,1)",
-           "plain_fp64": {"ms": round(plain_s * 1e3, 2), "gflops": round(flops / plain_s / 1e9, 1)},
-           "modes": {}}
-    x = torch.zeros(n, dtype=torch.float64, device="cuda")
-    xpn = float(xp.abs().max())
-    for name, prec, rbt in [("fp32_butterfly", "fp32", True), ("fp32_tournament", "fp32", False),
-                            ("fp16_tournament", "fp16", False)]:
-        B.dgesv_mixed(n, 1, A, n, b, n, x, n, prec, rbt=rbt)  # warm-up
+    modes = [(f"{p}_{'butterfly' if r else 'tournament'}", p, r) for p in ("fp32", "fp16", "bf16")
+             for r in (True, False)]
+    out = {"workload": "A X = B, A = U diag(sigma) V^T (Householder products, sigma geometric 1..1/kappa), "
+                       "B uniform [-1,1); stop at backward error <= n * eps64; median of "
+                       f"{runs} solves after a warm-up",
+           "points": {}}
+
+    def timed(f, prep=None):
+        if prep:
+            prep()
+        f()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = B.dgesv_mixed(n, 1, A, n, b, n, x, n, prec, rbt=rbt)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        ferr = float((x - xp).abs().max()) / xpn
-        out["modes"][name] = {"ms": round(dt * 1e3, 2), "gflops": round(flops / dt / 1e9, 1),
-                              "speedup_vs_plain_fp64": round(plain_s / dt, 3),
-                              "final_berr": st["final_berr"], "tolerance": st["tolerance"],
-                              "converged": st["converged"], "ir_iterations": st["ir_iterations"],
-                              "gmres_iterations": st["gmres_iterations"], "fallback": st["fallback"],
-                              "factor_ms": round(st["factor_ms"], 2), "refine_ms": round(st["refine_ms"], 2),
-                              "forward_error_vs_fp64": ferr}
-        print(f"# mixed {name}: {dt * 1e3:.1f} ms, berr {st['final_berr']:.2e}", file=sys.stderr, flush=True)
-    best = max(out["modes"].values(), key=lambda m: m["gflops"] if m["converged"] else 0.0)
-    out["best_gflops"] = best["gflops"]
-    out["best_speedup_vs_plain_fp64"] = best["speedup_vs_plain_fp64"]
-    del A
-    torch.cuda.empty_cache()
+        ts, res = [], None
+        for _ in range(runs):
+            if prep:
+                prep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), res
+
+    for n in sizes:
+        for kappa in kappas:
+            A = testmat.conditioned(n, kappa, 7, device="cuda")
+            a = torch.empty_like(A)
+            p = torch.zeros(n, dtype=torch.int32, device="cuda")
+            for nrhs in nrhss:
+                b = torch.from_numpy(np.random.default_rng(8).uniform(-1, 1, n * nrhs)).cuda()
+                xp = torch.empty_like(b)
+                flops = lu_flops(n, nrhs)
+
+                def restore():
+                    a.copy_(A)
+                    xp.copy_(b)
+
+                def plain():
+                    B.getrf(n, n, a, n, p)
+                    B.getrs(n, nrhs, a, n, p, xp, n)
+
+                plain_s, _ = timed(plain, restore)
+                pt = {"plain_fp64": {"ms": round(plain_s * 1e3, 2), "gflops": round(flops / plain_s / 1e9, 1)},
+                      "modes": {}}
+                xpn = xp.view(nrhs, n).abs().amax(dim=1)
+                x = torch.zeros_like(b)
+                for name, prec, rbt in modes:
+                    dt, st = timed(lambda: B.dgesv_mixed(n, nrhs, A, n, b, n, x, n, prec, rbt=rbt))
+                    ferr = float(((x - xp).view(nrhs, n).abs().amax(dim=1) / xpn).max())
+                    pt["modes"][name] = {
+                        "ms": round(dt * 1e3, 2), "gflops": round(flops / dt / 1e9, 1),
+                        "speedup_vs_plain_fp64": round(plain_s / dt, 3), "final_berr": st["final_berr"],
+                        "tolerance": st["tolerance"], "converged": st["converged"],
+                        "ir_iterations": st["ir_iterations"], "gmres_iterations": st["gmres_iterations"],
+                        "fallback": st["fallback"], "factor_ms": round(st["factor_ms"], 2),
+                        "refine_ms": round(st["refine_ms"], 2), "forward_error_vs_fp64": ferr}
+                ok = {k: v for k, v in pt["modes"].items() if v["converged"] and not v["fallback"]}
+                if ok:
+                    bk = max(ok, key=lambda k: ok[k]["gflops"])
+                    pt["best"] = {"mode": bk, "speedup_vs_plain_fp64": ok[bk]["speedup_vs_plain_fp64"]}
+                key = f"n{n}_kappa{kappa:g}_nrhs{nrhs}"
+                out["points"][key] = pt
+                print(f"# mixed {key}: plain {plain_s * 1e3:.1f} ms; " +
+                      ", ".join(f"{k} {v['ms']:.1f} ms{' (fallback)' if v['fallback'] else ''}"
+                                for k, v in pt["modes"].items()), file=sys.stderr, flush=True)
+                del b, xp, x
+            del A, a
+            torch.cuda.empty_cache()
+    head = out["points"].get("n16384_kappa100_nrhs1")
+    if head and "best" in head:
+        out["best_mode_n16384_kappa1e2_nrhs1"] = head["best"]["mode"]
+        out["best_speedup_n16384_kappa1e2_nrhs1"] = head["best"]["speedup_vs_plain_fp64"]
     return out
 
 
