@@ -224,6 +224,176 @@ __global__ void k_zero2(int32_t* p) {
     if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
+// ---- POTRF2 of a full 32 x 32 block with the recursion unrolled at compile time (no
+// run-time stack), the same arithmetic as potrf2_small: halve; factor the first half;
+// TRSM of the second half's rows (L: multiply by the reciprocal of the diagonal, which
+// the leaf computes once -- the same correctly rounded 1.0 / a as the reference's; U:
+// divide); SYRK into the trailing square (L: daxpy order, fused on the 4-aligned prefix
+// of each column segment, zero multipliers skipped; U: dot order); factor the second
+// half.  All threads of the workgroup run every barrier; a failing leaf records its
+// column and every level returns at once, leaving the block as the reference leaves it.
+template <bool UP, int N>
+__device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* err, int o, int tid) {
+    if constexpr (N == 1) {
+        if (tid == 0) {
+            const double v = s[o][o];
+            if (v > 0.0) {
+                const double r = sqrt(v);
+                s[o][o] = r;
+                rinv[o] = 1.0 / r;
+            } else {
+                *err = o + 1;
+            }
+        }
+        __syncthreads();
+        return *err != 0;
+    } else {
+        constexpr int n1 = N / 2, n2 = N - n1;
+        if (p2_node<UP, n1>(s, rinv, err, o, tid)) return true;
+        if (!UP) {
+            // trsm('R','L','T','N', n2, n1, 1, A11, A21), rows independent (trsm.rs:316-338)
+            if (tid < n2) {
+                const int r = o + n1 + tid;
+#pragma unroll
+                for (int j = 0; j < n1; j++) {
+                    double bj = s[r][o + j];
+#pragma unroll
+                    for (int kk = 0; kk < j; kk++) {
+                        const double a = s[o + j][o + kk];
+                        const double y = bj - a * s[r][o + kk];
+                        bj = a != 0.0 ? y : bj;
+                    }
+                    s[r][o + j] = bj * rinv[o + j];
+                }
+            }
+            __syncthreads();
+            // syrk('L','N', n2, n1, -1, A21, A22): one block, daxpy order (syrk.rs:213-222)
+            if (tid < n2 * n2) {
+                const int i = tid % n2, jj = tid / n2;
+                if (i >= jj) {
+                    const int len = n2 - jj;
+                    const bool fused = (i - jj) < (len & ~3);
+                    double c = s[o + n1 + i][o + n1 + jj];
+#pragma unroll
+                    for (int l = 0; l < n1; l++) {
+                        const double temp = -1.0 * s[o + n1 + jj][o + l];
+                        const double x = s[o + n1 + i][o + l];
+                        const double y = fused ? fma(temp, x, c) : c + temp * x;
+                        c = temp != 0.0 ? y : c;
+                    }
+                    s[o + n1 + i][o + n1 + jj] = c;
+                }
+            }
+        } else {
+            // trsm('L','U','T','N', n1, n2, 1, A11, A12), columns independent (trsm.rs:213-241)
+            if (tid < n2) {
+                const int cc = o + n1 + tid;
+#pragma unroll
+                for (int i = 0; i < n1; i++) {
+                    double temp = s[o + i][cc];
+#pragma unroll
+                    for (int kk = 0; kk < i; kk++) temp = temp - s[o + kk][o + i] * s[o + kk][cc];
+                    s[o + i][cc] = temp / s[o + i][o + i];
+                }
+            }
+            __syncthreads();
+            // syrk('U','T', n2, n1, -1, A12, A22): dot order (syrk.rs:226-235)
+            if (tid < n2 * n2) {
+                const int i = tid % n2, jj = tid / n2;
+                if (i <= jj) {
+                    constexpr int nc = n1 / 4;
+                    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+                    const int ci = o + n1 + i, cj = o + n1 + jj;
+#pragma unroll
+                    for (int t = 0; t < nc; t++) {
+                        l0 = fma(s[o + 4 * t + 0][ci], s[o + 4 * t + 0][cj], l0);
+                        l1 = fma(s[o + 4 * t + 1][ci], s[o + 4 * t + 1][cj], l1);
+                        l2 = fma(s[o + 4 * t + 2][ci], s[o + 4 * t + 2][cj], l2);
+                        l3 = fma(s[o + 4 * t + 3][ci], s[o + 4 * t + 3][cj], l3);
+                    }
+                    double hs = -0.0;
+                    hs = hs + l0;
+                    hs = hs + l1;
+                    hs = hs + l2;
+                    hs = hs + l3;
+                    double ts = 0.0;
+                    ts = ts + hs;
+#pragma unroll
+                    for (int t = nc * 4; t < n1; t++) ts = ts + s[o + t][ci] * s[o + t][cj];
+                    s[o + n1 + i][o + n1 + jj] = s[o + n1 + i][o + n1 + jj] + (-1.0 * ts);
+                }
+            }
+        }
+        __syncthreads();
+        return p2_node<UP, n2>(s, rinv, err, o + n1, tid);
+    }
+}
+
+// ---- one block step of the factorization in one launch: POTRF2 of the 32 x 32 diagonal
+// block at (c, c), then the TRSM of the block column below it (L: trsm('R','L','T','N'),
+// one thread per row) or the block row right of it (U: trsm('L','U','T','N'), one
+// thread per column), each thread running the reference's 32-wide base-case sequence
+// (trsm.rs:252-376 / :213-241).  Every workgroup factors the diagonal block itself (the
+// same bits everywhere), workgroup 0 writes it back, and each workgroup then solves its
+// 256 rows (columns); a failing pivot writes the block as left by the failing step and
+// the error, and no TRSM runs.
+template <bool UP>
+__global__ void __launch_bounds__(256) chol_panel(int n, int c, double* __restrict__ A, size_t lda,
+                                                  int32_t* __restrict__ info) {
+    if (*info != 0) return;
+    __shared__ double s[SB][SB + 1];  // s[r][c]
+    __shared__ double rinv[SB];
+    __shared__ int err;
+    const int tid = threadIdx.x;
+    double* D = A + (size_t)c + (size_t)c * lda;
+    for (int e = tid; e < SB * SB; e += 256) s[e % SB][e / SB] = D[(e % SB) + (size_t)(e / SB) * lda];
+    if (tid == 0) err = 0;
+    __syncthreads();
+    const bool failed = p2_node<UP, SB>(s, rinv, &err, 0, tid);
+    if (blockIdx.x == 0) {
+        for (int e = tid; e < SB * SB; e += 256) D[(e % SB) + (size_t)(e / SB) * lda] = s[e % SB][e / SB];
+        if (tid == 0 && failed) {
+            info[1] = err;
+            __hip_atomic_store(info, c + err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (failed) return;
+    const long v = (long)blockIdx.x * 256 + tid;
+    if (v >= n - c - SB) return;
+    double x[SB];
+    if (!UP) {
+        double* row = A + (size_t)(c + SB + v) + (size_t)c * lda;
+#pragma unroll
+        for (int q = 0; q < SB; q++) x[q] = row[(size_t)q * lda];
+#pragma unroll
+        for (int j = 0; j < SB; j++) {
+            double bj = x[j];
+#pragma unroll
+            for (int k = 0; k < j; k++) {
+                const double a = s[j][k];
+                const double y = bj - a * x[k];
+                bj = a != 0.0 ? y : bj;
+            }
+            x[j] = bj * rinv[j];
+        }
+#pragma unroll
+        for (int q = 0; q < SB; q++) row[(size_t)q * lda] = x[q];
+    } else {
+        double* col = A + (size_t)c + (size_t)(c + SB + v) * lda;
+#pragma unroll
+        for (int q = 0; q < SB; q++) x[q] = col[q];
+#pragma unroll
+        for (int i = 0; i < SB; i++) {
+            double temp = x[i];
+#pragma unroll
+            for (int k = 0; k < i; k++) temp = temp - s[k][i] * x[k];
+            x[i] = temp / s[i][i];
+        }
+#pragma unroll
+        for (int q = 0; q < SB; q++) col[q] = x[q];
+    }
+}
+
 // ---- the blocked factorization, right-looking with the reference's arithmetic
 //
 // The reference's loop is left-looking: block column j receives, at step j, a SYRK of its
@@ -275,59 +445,65 @@ __global__ void chol_restore(int upper, int n, double* __restrict__ A, size_t ld
     }
 }
 
-// The chain steps l = c .. c+31 of every diagonal block after block column c, with the
-// reference's SYRK forms: L (syrk.rs, trans N, daxpy order): c += (alpha a[jj,l]) a[i,l],
-// fused on the 4-aligned prefix of the column segment, skipped when alpha a[jj,l] == 0;
-// U (trans T, dot order per 32-chunk): c = c + alpha * dot(A[c:c+32, i], A[c:c+32, jj]).
-__global__ void __launch_bounds__(1024) chol_diag_steps(int upper, int n, int c, double* __restrict__ A, size_t lda,
+// The chain steps l = l0 .. l0+steps-1 (steps a multiple of 32, taken 32 at a time in
+// order) of the diagonal blocks starting at d0, d0 + 32, ..., with the reference's SYRK
+// forms: L (syrk.rs, trans N, daxpy order): c += (alpha a[jj,l]) a[i,l], fused on the
+// 4-aligned prefix of the column segment, skipped when alpha a[jj,l] == 0; U (trans T,
+// dot order per 32-chunk): c = c + alpha * dot(A[l0:l0+32, i], A[l0:l0+32, jj]).
+__global__ void __launch_bounds__(1024) chol_diag_steps(int upper, int n, int l0, int steps, int d0,
+                                                         double* __restrict__ A, size_t lda,
                                                          const int32_t* __restrict__ abort) {
     if (*abort != 0) return;
-    __shared__ double sl[SB][SB + 1];  // sl[r][l]: rows of the block, columns c .. c+31 (L) / transposed (U)
-    const int j0 = c + SB + (int)blockIdx.x * SB;
+    __shared__ double sl[SB][SB + 1];  // sl[r][l]: rows of the block, columns l .. l+31 (L) / transposed (U)
+    const int j0 = d0 + (int)blockIdx.x * SB;
     const int nb = min(SB, n - j0);
     const int e = threadIdx.x, i = e & 31, jj = e >> 5;
-    // stage the 32 x 32 operand block: L: A[j0 + r, c + l]; U: A[c + l, j0 + r]
-    {
-        const int r = e >> 5, l = e & 31;
-        double v = 0.0;
-        if (r < nb) v = upper ? A[(size_t)(c + l) + (size_t)(j0 + r) * lda] : A[(size_t)(j0 + r) + (size_t)(c + l) * lda];
-        sl[r][l] = v;
-    }
-    __syncthreads();
-    if (i >= nb || jj >= nb) return;
-    if (upper ? (i > jj) : (i < jj)) return;
+    const bool act = i < nb && jj < nb && (upper ? (i <= jj) : (i >= jj));
     double* cp = A + (size_t)(j0 + i) + (size_t)(j0 + jj) * lda;
-    double v = *cp;
-    if (!upper) {
-        const int len = nb - jj, rel = i - jj;
-        const bool fused = rel < (len & ~3);
+    double v = act ? *cp : 0.0;
+    const int len = nb - jj, rel = i - jj;
+    const bool fused = rel < (len & ~3);
+    for (int lc = l0; lc < l0 + steps; lc += SB) {
+        // stage the 32 x 32 operand block: L: A[j0 + r, lc + l]; U: A[lc + l, j0 + r]
+        {
+            const int r = e >> 5, l = e & 31;
+            double w = 0.0;
+            if (r < nb)
+                w = upper ? A[(size_t)(lc + l) + (size_t)(j0 + r) * lda] : A[(size_t)(j0 + r) + (size_t)(lc + l) * lda];
+            sl[r][l] = w;
+        }
+        __syncthreads();
+        if (act) {
+            if (!upper) {
 #pragma unroll
-        for (int l = 0; l < SB; l++) {
-            const double temp = -1.0 * sl[jj][l];
-            if (temp != 0.0) {
-                const double x = sl[i][l];
-                v = fused ? fma(temp, x, v) : v + temp * x;
+                for (int l = 0; l < SB; l++) {
+                    const double temp = -1.0 * sl[jj][l];
+                    const double x = sl[i][l];
+                    const double y = fused ? fma(temp, x, v) : v + temp * x;
+                    v = temp != 0.0 ? y : v;
+                }
+            } else {
+                double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+#pragma unroll
+                for (int t = 0; t < SB / 4; t++) {
+                    q0 = fma(sl[i][4 * t + 0], sl[jj][4 * t + 0], q0);
+                    q1 = fma(sl[i][4 * t + 1], sl[jj][4 * t + 1], q1);
+                    q2 = fma(sl[i][4 * t + 2], sl[jj][4 * t + 2], q2);
+                    q3 = fma(sl[i][4 * t + 3], sl[jj][4 * t + 3], q3);
+                }
+                double hs = -0.0;
+                hs = hs + q0;
+                hs = hs + q1;
+                hs = hs + q2;
+                hs = hs + q3;
+                double ts = 0.0;
+                ts = ts + hs;
+                v = v + -1.0 * ts;
             }
         }
-    } else {
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-#pragma unroll
-        for (int t = 0; t < SB / 4; t++) {
-            l0 = fma(sl[i][4 * t + 0], sl[jj][4 * t + 0], l0);
-            l1 = fma(sl[i][4 * t + 1], sl[jj][4 * t + 1], l1);
-            l2 = fma(sl[i][4 * t + 2], sl[jj][4 * t + 2], l2);
-            l3 = fma(sl[i][4 * t + 3], sl[jj][4 * t + 3], l3);
-        }
-        double hs = -0.0;
-        hs = hs + l0;
-        hs = hs + l1;
-        hs = hs + l2;
-        hs = hs + l3;
-        double ts = 0.0;
-        ts = ts + hs;
-        v = v + -1.0 * ts;
+        __syncthreads();
     }
-    *cp = v;
+    if (act) *cp = v;
 }
 
 }  // namespace
@@ -418,57 +594,74 @@ void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t ld
     auto at = [&](int r, int c) { return a + (size_t)r + (size_t)c * lda; };
     // fold over columns (L) / rows (U) [k0, k0 + kd) into the block-triangle region whose
     // diagonal starts at d0, restricted to the nb columns (L) / rows (U) from d0 when nb > 0
-    auto fold = [&](hipStream_t sx, int k0, int kd, int d0, int nb) {
+    auto fold = [&](hipStream_t sx, int k0, int kd, int slab, int d0, int nb) {
         const int rest = N - d0;
         if (rest <= 0) return;
         const int w = nb > 0 ? std::min(nb, rest) : rest;
         Phase pg(h, "GEMM", (double)kd * w * (2.0 * rest - w), sx);
         if (!up)
             launch_dgemm_ex(sx, 'N', 'T', (size_t)rest, (size_t)w, (size_t)kd, -1.0, at(d0, k0), lda, at(d0, k0), lda,
-                            1.0, at(d0, d0), lda, info, kd, 1, edge_r - d0, (long)w);
+                            1.0, at(d0, d0), lda, info, slab, 1, edge_r - d0, (long)w);
         else
             launch_dgemm_ex(sx, 'T', 'N', (size_t)w, (size_t)rest, (size_t)kd, -1.0, at(k0, d0), lda, at(k0, d0), lda,
-                            1.0, at(d0, d0), lda, info, kd, 2, (long)w, edge_c - d0);
+                            1.0, at(d0, d0), lda, info, slab, 2, (long)w, edge_c - d0);
     };
+    // Super-blocks of kSuper columns (four 64-deep slabs).  Inside one, each block column
+    // pushes its terms into the later block columns of the same super-block only; the
+    // super-block's terms for every column after it are applied once it is done -- the
+    // chain steps of all later diagonal blocks, then the four slab folds in one product
+    // (the next super-block's columns here, the rest on stream 2 while the next
+    // super-block is factored).  Every element still receives its terms in the
+    // reference's order: slabs by increasing k, chain steps by increasing l.
+    constexpr int kSuper = 256;
     hipEvent_t bulk_done = nullptr;
     int ev = 0;
-    for (int c = 0; c < N; c += SB) {
-        const int jb = std::min(SB, N - c);
-        {
-            Phase p(h, "POTRF2", 0.0, st);
-            potrf2_block(st, uplo, (size_t)jb, at(c, c), lda, info, c);
-        }
-        if (c + jb >= N) break;
-        const int n_rem = N - c - jb;
-        {
-            Phase p(h, "TRSM", (double)jb * jb * n_rem, st);
-            if (!up) launch_dtrsm(st, 'R', 'L', 'T', 'N', (size_t)n_rem, (size_t)jb, 1.0, at(c, c), lda, at(c + jb, c), lda, info);
-            else launch_dtrsm(st, 'L', 'U', 'T', 'N', (size_t)jb, (size_t)n_rem, 1.0, at(c, c), lda, at(c, c + jb), lda, info);
-        }
-        {
-            Phase p(h, "SYRK", (double)SB * SB * n_rem, st);
-            hipLaunchKernelGGL(chol_diag_steps, dim3((unsigned)((n_rem + SB - 1) / SB)), dim3(1024), 0, st, up, N, c, a,
-                               lda, info);
-            BOLA_LAUNCH_CHECK();
-        }
-        if ((c / SB) % 2 == 0) {
-            // the partial 32-deep last slab of block column c + 32
-            fold(st, c, SB, c + SB, SB);
-        } else {
-            // the 64-deep slab over the two block columns c - 32 .. c + 31: into the next two
-            // block columns here (they come next on this stream), into the rest on stream 2
-            if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
-            fold(st, c - SB, 2 * SB, c + SB, 2 * SB);
-            if (c + 3 * SB < N) {
-                hipEvent_t ready = handle_event(h, ev++);
-                BOLA_CHECK(hipEventRecord(ready, st));
-                BOLA_CHECK(hipStreamWaitEvent(s2, ready, 0));
-                fold(s2, c - SB, 2 * SB, c + 3 * SB, 0);
-                bulk_done = handle_event(h, ev++);
-                BOLA_CHECK(hipEventRecord(bulk_done, s2));
-            } else {
-                bulk_done = nullptr;
+    auto diag_steps = [&](int l0, int steps, int d0, int d1) {
+        if (d1 <= d0) return;
+        Phase p(h, "SYRK", (double)SB * steps * (d1 - d0), st);
+        hipLaunchKernelGGL(chol_diag_steps, dim3((unsigned)((d1 - d0 + SB - 1) / SB)), dim3(1024), 0, st, up, N, l0,
+                           steps, d0, a, lda, info);
+        BOLA_LAUNCH_CHECK();
+    };
+    for (int s0 = 0; s0 < N; s0 += kSuper) {
+        const int e0 = std::min(N, s0 + kSuper);
+        for (int c = s0; c < e0; c += SB) {
+            const int jb = std::min(SB, N - c);
+            const int n_rem = N - c - jb;
+            if (jb == SB) {
+                Phase p(h, "POTRF2", (double)jb * jb * n_rem, st);  // POTRF2 and the TRSM below it
+                const unsigned g = (unsigned)std::max(1, (n_rem + 255) / 256);
+                if (up) hipLaunchKernelGGL(chol_panel<true>, dim3(g), dim3(256), 0, st, N, c, a, lda, info);
+                else hipLaunchKernelGGL(chol_panel<false>, dim3(g), dim3(256), 0, st, N, c, a, lda, info);
+                BOLA_LAUNCH_CHECK();
+            } else {  // the last, partial block: nothing below it
+                Phase p(h, "POTRF2", 0.0, st);
+                potrf2_block(st, uplo, (size_t)jb, at(c, c), lda, info, c);
             }
+            if (n_rem == 0) break;
+            diag_steps(c, SB, c + SB, e0);
+            if ((c / SB) % 2 == 0) {
+                // the partial 32-deep last slab of block column c + 32
+                if (c + SB < e0) fold(st, c, SB, SB, c + SB, SB);
+            } else if (c + SB < e0) {
+                // the 64-deep slab over block columns c - 32 .. c + 31, into the rest of the super-block
+                fold(st, c - SB, 2 * SB, 2 * SB, c + SB, e0 - c - SB);
+            }
+        }
+        if (e0 >= N) break;
+        diag_steps(s0, e0 - s0, e0, N);
+        // the next super-block's columns were the previous bulk update's until it is done
+        if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));
+        fold(st, s0, e0 - s0, 2 * SB, e0, kSuper);
+        if (e0 + kSuper < N) {
+            hipEvent_t ready = handle_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(ready, st));
+            BOLA_CHECK(hipStreamWaitEvent(s2, ready, 0));
+            fold(s2, s0, e0 - s0, 2 * SB, e0 + kSuper, 0);
+            bulk_done = handle_event(h, ev++);
+            BOLA_CHECK(hipEventRecord(bulk_done, s2));
+        } else {
+            bulk_done = nullptr;
         }
     }
     if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(st, bulk_done, 0));

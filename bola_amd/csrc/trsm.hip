@@ -114,45 +114,57 @@ __global__ void __launch_bounds__(256) trsm_ln_lanes(int t, int nvec, const doub
 // ---- dot-form variants (LTL, LTU and the four right-side ones): one thread per
 // independent vector, the vector in registers (compile-time indices), the triangle in
 // LDS (broadcast reads).  Sequences as in trsm.rs:174-241 (left) and :252-376 (right).
-template <int V>
-__global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, const double* __restrict__ A,
+// TT > 0: the triangle is exactly TT wide (the recursion's usual 64, and the Cholesky
+// block's 32), so every step is straight-line code with no size test; TT == 0: any
+// width up to 64, tested at run time.  The right-side forms multiply by the reciprocal of
+// the diagonal entry; it is computed once per workgroup (the same correctly rounded
+// 1.0 / a as the reference's per-use one).
+template <int V, int TT>
+__global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t_rt, int nvec, const double* __restrict__ A,
                                                     size_t lda, double* __restrict__ B, size_t ldb,
                                                     Abort abort) {
     if (abort_stop(abort)) return;
-    __shared__ double sA[MAXT * (MAXT + 1)];  // sA[r * (MAXT+1) + c] = A[r, c]
+    constexpr int W = TT > 0 ? TT : MAXT;
+    const int t = TT > 0 ? TT : t_rt;
+    __shared__ double sA[W * (W + 1)];  // sA[r * (W+1) + c] = A[r, c]
+    __shared__ double rinv[W];
     const int lane = threadIdx.x;
     for (int e = lane; e < t * t; e += 64) {
         int r = e % t, c = e / t;
-        sA[r * (MAXT + 1) + c] = A[r + (size_t)c * lda];
+        sA[r * (W + 1) + c] = A[r + (size_t)c * lda];
     }
+    constexpr bool LEFT = V <= LTU;
+    if (!LEFT && !unit)
+        for (int q = lane; q < t; q += 64) rinv[q] = 1.0 / A[q + (size_t)q * lda];
     __syncthreads();
     const long v = (long)blockIdx.x * 64 + lane;
     if (v >= nvec) return;
-    constexpr bool LEFT = V <= LTU;
-#define AT(r, c) sA[(r) * (MAXT + 1) + (c)]
-    double x[MAXT];
+#define AT(r, c) sA[(r) * (W + 1) + (c)]
+    double x[W];
 #pragma unroll
-    for (int q = 0; q < MAXT; q++)
+    for (int q = 0; q < W; q++)
         x[q] = LEFT ? B[min(q, t - 1) + (size_t)v * ldb] : B[v + (size_t)min(q, t - 1) * ldb];  // clamped
+    if (TT == 0) {
 #pragma unroll
-    for (int q = 0; q < MAXT; q++) x[q] = keep_or_zero(x[q], q < t);
+        for (int q = 0; q < W; q++) x[q] = keep_or_zero(x[q], q < t);
+    }
     if (V == LTL) {
 #pragma unroll
-        for (int ii = 0; ii < MAXT; ii++) {
-            const int i = MAXT - 1 - ii;
-            if (i < t) {
+        for (int ii = 0; ii < W; ii++) {
+            const int i = W - 1 - ii;
+            if (TT > 0 || i < t) {
                 double temp = x[i];
 #pragma unroll
-                for (int k = i + 1; k < MAXT; k++)
-                    if (k < t) temp = temp - AT(k, i) * x[k];
+                for (int k = i + 1; k < W; k++)
+                    if (TT > 0 || k < t) temp = temp - AT(k, i) * x[k];
                 if (!unit) temp = temp / AT(i, i);
                 x[i] = temp;
             }
         }
     } else if (V == LTU) {
 #pragma unroll
-        for (int i = 0; i < MAXT; i++) {
-            if (i < t) {
+        for (int i = 0; i < W; i++) {
+            if (TT > 0 || i < t) {
                 double temp = x[i];
 #pragma unroll
                 for (int k = 0; k < i; k++) temp = temp - AT(k, i) * x[k];
@@ -162,40 +174,42 @@ __global__ void __launch_bounds__(64) trsm_dot_regs(int unit, int t, int nvec, c
         }
     } else if (V == RNL || V == RTU) {
 #pragma unroll
-        for (int jj = 0; jj < MAXT; jj++) {
-            const int j = MAXT - 1 - jj;
-            if (j < t) {
+        for (int jj = 0; jj < W; jj++) {
+            const int j = W - 1 - jj;
+            if (TT > 0 || j < t) {
                 double bj = x[j];
 #pragma unroll
-                for (int k = j + 1; k < MAXT; k++) {
-                    if (k < t) {
+                for (int k = j + 1; k < W; k++) {
+                    if (TT > 0 || k < t) {
                         const double a = (V == RNL) ? AT(k, j) : AT(j, k);
-                        if (a != 0.0) bj = bj - a * x[k];
+                        const double y = bj - a * x[k];
+                        bj = a != 0.0 ? y : bj;
                     }
                 }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
+                if (!unit) bj = bj * rinv[j];
                 x[j] = bj;
             }
         }
     } else {  // RNU, RTL
 #pragma unroll
-        for (int j = 0; j < MAXT; j++) {
-            if (j < t) {
+        for (int j = 0; j < W; j++) {
+            if (TT > 0 || j < t) {
                 double bj = x[j];
 #pragma unroll
                 for (int k = 0; k < j; k++) {
                     const double a = (V == RNU) ? AT(k, j) : AT(j, k);
-                    if (a != 0.0) bj = bj - a * x[k];
+                    const double y = bj - a * x[k];
+                    bj = a != 0.0 ? y : bj;
                 }
-                if (!unit) bj = bj * (1.0 / AT(j, j));
+                if (!unit) bj = bj * rinv[j];
                 x[j] = bj;
             }
         }
     }
 #undef AT
 #pragma unroll
-    for (int q = 0; q < MAXT; q++)
-        if (q < t) {
+    for (int q = 0; q < W; q++)
+        if (TT > 0 || q < t) {
             if (LEFT) B[q + (size_t)v * ldb] = x[q];
             else B[v + (size_t)q * ldb] = x[q];
         }
@@ -226,8 +240,15 @@ void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size
         }
     } else {
         const unsigned blocks = (unsigned)((nvec + 63) / 64);
-        hipLaunchKernelGGL((trsm_dot_regs<V>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda, b,
-                           ldb, abort);
+        if (t == 64)
+            hipLaunchKernelGGL((trsm_dot_regs<V, 64>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda,
+                               b, ldb, abort);
+        else if (t == 32)
+            hipLaunchKernelGGL((trsm_dot_regs<V, 32>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda,
+                               b, ldb, abort);
+        else
+            hipLaunchKernelGGL((trsm_dot_regs<V, 0>), dim3(blocks), dim3(64), 0, st, unit, (int)t, (int)nvec, a, lda,
+                               b, ldb, abort);
     }
     BOLA_LAUNCH_CHECK();
 }

@@ -109,6 +109,34 @@ def test_batch_fp32_ir_reaches_fp64_accuracy(B, O):
         assert _berr(Af, X, Bm) <= 2 * n * EPS, (i, n, iters[i])
 
 
+@pytest.mark.parametrize("variant", ["fp64", "fp32_ir"])
+def test_batch_full_order_range_against_oracle(B, O, variant):
+    # the batched configuration's whole order range, 8 .. 512, on a sample of 240 systems
+    d, arena, ptot = _batch(B, 240, 8, 512, seed=9)
+    assert int(d["n"].max()) > 480 and int(d["n"].min()) < 40
+    orig = arena.cpu().numpy()
+    v = B.BATCH_FP64 if variant == "fp64" else B.BATCH_FP32_IR
+    out, piv, info, iters = _run(B, v, d, arena, ptot, None)
+    assert np.all(info == 0)
+    for i in range(len(d)):
+        n, r, A, Bm = _sys(d, orig, i)
+        X = out[d["b"][i]:d["b"][i] + n * r].reshape(r, n).T
+        kind = int(d["kind"][i])
+        Af = A if kind == B.BATCH_LU else np.tril(A) + np.tril(A, -1).T
+        xo, po = _oracle_solve(O, B, kind, n, r, Af, Bm)
+        if variant == "fp64":
+            assert iters[i] == 0
+            if kind == B.BATCH_LU:
+                assert np.array_equal(piv[d["piv"][i]:d["piv"][i] + n], po), i
+            assert _berr(Af, X, Bm) <= 10 * n * EPS, i
+        else:
+            assert iters[i] >= 1 or iters[i] == -1, (i, iters[i])
+            assert _berr(Af, X, Bm) <= 2 * n * EPS, (i, n, iters[i])
+        # forward error against the oracle's solution, within the conditioning bound
+        ferr = np.max(np.abs(X - xo)) / np.max(np.abs(xo))
+        assert ferr <= 10 * np.linalg.cond(Af) * n * EPS, (i, n, ferr)
+
+
 def test_batch_graph_replays_equal_eager(B):
     import torch
     d, arena, ptot = _batch(B, 24, 8, 96, seed=7)

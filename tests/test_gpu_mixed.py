@@ -165,3 +165,75 @@ def test_mixed_butterfly_request_falls_back_to_tournament_when_n_is_not_a_multip
     Af, b, x, st = _solve(B, n, nrhs, 1e3, "fp32", seed=21, rbt=True)
     assert st["converged"] == 1 and st["fallback"] == 0, st
     assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
+
+
+# ---- the mixed-precision configuration at full size: n = 8192, kappa in {1e2, 1e5, 1e8},
+# nrhs in {1, 16}, every product precision with and without butterflies
+
+_BIG = {}
+
+
+def _big_case(O, kappa):
+    """A (device and host copies) and the oracle's LU of it, built once per kappa."""
+    if kappa not in _BIG:
+        import torch
+        from bola_amd import testmat
+        n = 8192
+        A = testmat.conditioned(n, kappa, 7, device="cuda")
+        Af = A.cpu().numpy()
+        lu = Af.copy()
+        p = np.zeros(n, np.int32)
+        code, _ = O.getrf(n, n, lu, n, p)
+        assert code == 0
+        _BIG.clear()  # one kappa at a time (half a GiB each)
+        torch.cuda.empty_cache()
+        _BIG[kappa] = (A, Af, lu, p)
+    return _BIG[kappa]
+
+
+@pytest.mark.parametrize("prec,rbt", [("fp32", True), ("fp32", False), ("fp16", True), ("fp16", False),
+                                      ("bf16", True), ("bf16", False)],
+                         ids=["fp32-butterfly", "fp32-tournament", "fp16-butterfly", "fp16-tournament",
+                              "bf16-butterfly", "bf16-tournament"])
+@pytest.mark.parametrize("nrhs", [1, 16])
+@pytest.mark.parametrize("kappa", [1e2, 1e5, 1e8])  # outermost: the matrix is built once per kappa
+def test_mixed_config_n8192(B, O, kappa, nrhs, prec, rbt):
+    import torch
+    n = 8192
+    A, Af, lu, p = _big_case(O, kappa)
+    b = np.random.default_rng(8).uniform(-1, 1, n * nrhs)
+    dx = torch.zeros(n * nrhs, dtype=torch.float64, device="cuda")
+    st = B.dgesv_mixed(n, nrhs, A, n, torch.from_numpy(b).cuda(), n, dx, n, prec, rbt=rbt)
+    torch.cuda.synchronize()
+    x = dx.cpu().numpy()
+    print(f"kappa={kappa:g} nrhs={nrhs} {prec} rbt={rbt}: ir {st['ir_iterations']} gmres "
+          f"{st['gmres_iterations']} fallback {st['fallback']} berr {st['final_berr']:.2e}")
+    assert st["converged"] == 1, st
+    assert st["final_berr"] <= n * EPS64
+    assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
+    if prec == "fp32" and kappa <= 1e5:
+        assert st["fallback"] == 0, st
+    xo = b.copy()
+    O.getrs(n, nrhs, lu, n, p, xo, n)
+    if st["fallback"]:  # the FP64 path is the oracle's
+        assert np.array_equal(x, xo)
+    X, XO = x.reshape(nrhs, n), xo.reshape(nrhs, n)
+    ferr = np.max(np.abs(X - XO), axis=1) / np.max(np.abs(XO), axis=1)
+    assert np.all(ferr <= 10 * kappa * n * EPS64), ferr
+
+
+@pytest.mark.parametrize("prec,rbt", [("fp32", True), ("fp16", False)], ids=["fp32-butterfly", "fp16-tournament"])
+def test_mixed_gmres_ir_converges_without_fallback(B, O, prec, rbt):
+    # at kappa = 1e8 plain refinement from these factors stalls and GMRES-IR finishes the
+    # solve, with no FP64 factorization
+    import torch
+    n, nrhs, kappa = 8192, 1, 1e8
+    A, Af, lu, p = _big_case(O, kappa)
+    b = np.random.default_rng(8).uniform(-1, 1, n)
+    dx = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = B.dgesv_mixed(n, nrhs, A, n, torch.from_numpy(b).cuda(), n, dx, n, prec, rbt=rbt)
+    torch.cuda.synchronize()
+    assert st["converged"] == 1 and st["fallback"] == 0, st
+    assert st["gmres_iterations"] > 0, st
+    assert st["final_berr"] <= n * EPS64
+    assert _berr(Af, b, dx.cpu().numpy(), n, nrhs) <= 2 * n * EPS64
