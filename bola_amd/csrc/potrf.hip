@@ -230,20 +230,122 @@ __global__ void k_zero2(int32_t* p) {
 // the leaf computes once -- the same correctly rounded 1.0 / a as the reference's; U:
 // divide); SYRK into the trailing square (L: daxpy order, fused on the 4-aligned prefix
 // of each column segment, zero multipliers skipped; U: dot order); factor the second
-// half.  All threads of the workgroup run every barrier; a failing leaf records its
-// column and every level returns at once, leaving the block as the reference leaves it.
+// half.  Nodes of 8 or fewer columns run in one thread's registers, straight through
+// (their parallel steps become independent instruction streams of that thread); the 16-
+// and 32-wide levels run their TRSM rows and SYRK elements on separate threads between
+// barriers.  A failing leaf records its column and every level returns at once, leaving
+// the block as the reference leaves it.
+
+// an 8 x 8 diagonal block in registers: m[r][c] (L: r >= c used; U: r <= c used)
+template <bool UP, int N, int O>
+__device__ __forceinline__ bool p2_reg(double (&m)[8][8], double (&ri)[8], int& fail) {
+    if constexpr (N == 1) {
+        const double v = m[O][O];
+        if (!(v > 0.0)) {
+            fail = O + 1;
+            return true;
+        }
+        const double r = sqrt(v);
+        m[O][O] = r;
+        if (!UP) ri[O] = 1.0 / r;
+        return false;
+    } else {
+        constexpr int n1 = N / 2, n2 = N - n1;
+        if (p2_reg<UP, n1, O>(m, ri, fail)) return true;
+        if (!UP) {
+#pragma unroll
+            for (int t = 0; t < n2; t++) {
+                const int r = O + n1 + t;
+#pragma unroll
+                for (int j = 0; j < n1; j++) {
+                    double bj = m[r][O + j];
+#pragma unroll
+                    for (int kk = 0; kk < j; kk++) {
+                        const double a = m[O + j][O + kk];
+                        const double y = bj - a * m[r][O + kk];
+                        bj = a != 0.0 ? y : bj;
+                    }
+                    m[r][O + j] = bj * ri[O + j];
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < n2; jj++)
+#pragma unroll
+                for (int i = jj; i < n2; i++) {
+                    const bool fused = (i - jj) < ((n2 - jj) & ~3);
+                    double c = m[O + n1 + i][O + n1 + jj];
+#pragma unroll
+                    for (int l = 0; l < n1; l++) {
+                        const double temp = -1.0 * m[O + n1 + jj][O + l];
+                        const double x = m[O + n1 + i][O + l];
+                        const double y = fused ? fma(temp, x, c) : c + temp * x;
+                        c = temp != 0.0 ? y : c;
+                    }
+                    m[O + n1 + i][O + n1 + jj] = c;
+                }
+        } else {
+#pragma unroll
+            for (int t = 0; t < n2; t++) {
+                const int cc = O + n1 + t;
+#pragma unroll
+                for (int i = 0; i < n1; i++) {
+                    double temp = m[O + i][cc];
+#pragma unroll
+                    for (int kk = 0; kk < i; kk++) temp = temp - m[O + kk][O + i] * m[O + kk][cc];
+                    m[O + i][cc] = temp / m[O + i][O + i];
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < n2; jj++)
+#pragma unroll
+                for (int i = 0; i <= jj; i++) {
+                    constexpr int nc = n1 / 4;
+                    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+                    const int ci = O + n1 + i, cj = O + n1 + jj;
+#pragma unroll
+                    for (int t = 0; t < nc; t++) {
+                        l0 = fma(m[O + 4 * t + 0][ci], m[O + 4 * t + 0][cj], l0);
+                        l1 = fma(m[O + 4 * t + 1][ci], m[O + 4 * t + 1][cj], l1);
+                        l2 = fma(m[O + 4 * t + 2][ci], m[O + 4 * t + 2][cj], l2);
+                        l3 = fma(m[O + 4 * t + 3][ci], m[O + 4 * t + 3][cj], l3);
+                    }
+                    double hs = -0.0;
+                    hs = hs + l0;
+                    hs = hs + l1;
+                    hs = hs + l2;
+                    hs = hs + l3;
+                    double ts = 0.0;
+                    ts = ts + hs;
+#pragma unroll
+                    for (int t = nc * 4; t < n1; t++) ts = ts + m[O + t][ci] * m[O + t][cj];
+                    m[ci][cj] = m[ci][cj] + (-1.0 * ts);
+                }
+        }
+        return p2_reg<UP, n2, O + n1>(m, ri, fail);
+    }
+}
+
 template <bool UP, int N>
 __device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* err, int o, int tid) {
-    if constexpr (N == 1) {
+    if constexpr (N <= 8) {
         if (tid == 0) {
-            const double v = s[o][o];
-            if (v > 0.0) {
-                const double r = sqrt(v);
-                s[o][o] = r;
-                rinv[o] = 1.0 / r;
-            } else {
-                *err = o + 1;
-            }
+            double m[8][8], ri[8];
+#pragma unroll
+            for (int r = 0; r < N; r++)
+#pragma unroll
+                for (int c = 0; c < N; c++)
+                    if (UP ? r <= c : r >= c) m[r][c] = s[o + r][o + c];
+            int fail = 0;
+            p2_reg<UP, N, 0>(m, ri, fail);
+#pragma unroll
+            for (int r = 0; r < N; r++)
+#pragma unroll
+                for (int c = 0; c < N; c++)
+                    if (UP ? r <= c : r >= c) s[o + r][o + c] = m[r][c];
+            if (!UP)
+#pragma unroll
+                for (int r = 0; r < N; r++) rinv[o + r] = ri[r];
+            if (fail) *err = o + fail;
         }
         __syncthreads();
         return *err != 0;
@@ -254,22 +356,27 @@ __device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* 
             // trsm('R','L','T','N', n2, n1, 1, A11, A21), rows independent (trsm.rs:316-338)
             if (tid < n2) {
                 const int r = o + n1 + tid;
+                double xr[n1];
+#pragma unroll
+                for (int j = 0; j < n1; j++) xr[j] = s[r][o + j];
 #pragma unroll
                 for (int j = 0; j < n1; j++) {
-                    double bj = s[r][o + j];
+                    double bj = xr[j];
 #pragma unroll
                     for (int kk = 0; kk < j; kk++) {
                         const double a = s[o + j][o + kk];
-                        const double y = bj - a * s[r][o + kk];
+                        const double y = bj - a * xr[kk];
                         bj = a != 0.0 ? y : bj;
                     }
-                    s[r][o + j] = bj * rinv[o + j];
+                    xr[j] = bj * rinv[o + j];
                 }
+#pragma unroll
+                for (int j = 0; j < n1; j++) s[r][o + j] = xr[j];
             }
             __syncthreads();
             // syrk('L','N', n2, n1, -1, A21, A22): one block, daxpy order (syrk.rs:213-222)
-            if (tid < n2 * n2) {
-                const int i = tid % n2, jj = tid / n2;
+            for (int e = tid; e < n2 * n2; e += blockDim.x) {
+                const int i = e % n2, jj = e / n2;
                 if (i >= jj) {
                     const int len = n2 - jj;
                     const bool fused = (i - jj) < (len & ~3);
@@ -288,18 +395,23 @@ __device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* 
             // trsm('L','U','T','N', n1, n2, 1, A11, A12), columns independent (trsm.rs:213-241)
             if (tid < n2) {
                 const int cc = o + n1 + tid;
+                double xc[n1];
+#pragma unroll
+                for (int i = 0; i < n1; i++) xc[i] = s[o + i][cc];
 #pragma unroll
                 for (int i = 0; i < n1; i++) {
-                    double temp = s[o + i][cc];
+                    double temp = xc[i];
 #pragma unroll
-                    for (int kk = 0; kk < i; kk++) temp = temp - s[o + kk][o + i] * s[o + kk][cc];
-                    s[o + i][cc] = temp / s[o + i][o + i];
+                    for (int kk = 0; kk < i; kk++) temp = temp - s[o + kk][o + i] * xc[kk];
+                    xc[i] = temp / s[o + i][o + i];
                 }
+#pragma unroll
+                for (int i = 0; i < n1; i++) s[o + i][cc] = xc[i];
             }
             __syncthreads();
             // syrk('U','T', n2, n1, -1, A12, A22): dot order (syrk.rs:226-235)
-            if (tid < n2 * n2) {
-                const int i = tid % n2, jj = tid / n2;
+            for (int e = tid; e < n2 * n2; e += blockDim.x) {
+                const int i = e % n2, jj = e / n2;
                 if (i <= jj) {
                     constexpr int nc = n1 / 4;
                     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
@@ -335,10 +447,10 @@ __device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* 
 // thread per column), each thread running the reference's 32-wide base-case sequence
 // (trsm.rs:252-376 / :213-241).  Every workgroup factors the diagonal block itself (the
 // same bits everywhere), workgroup 0 writes it back, and each workgroup then solves its
-// 256 rows (columns); a failing pivot writes the block as left by the failing step and
+// 64 rows (columns); a failing pivot writes the block as left by the failing step and
 // the error, and no TRSM runs.
 template <bool UP>
-__global__ void __launch_bounds__(256) chol_panel(int n, int c, double* __restrict__ A, size_t lda,
+__global__ void __launch_bounds__(64) chol_panel(int n, int c, double* __restrict__ A, size_t lda,
                                                   int32_t* __restrict__ info) {
     if (*info != 0) return;
     __shared__ double s[SB][SB + 1];  // s[r][c]
@@ -346,19 +458,19 @@ __global__ void __launch_bounds__(256) chol_panel(int n, int c, double* __restri
     __shared__ int err;
     const int tid = threadIdx.x;
     double* D = A + (size_t)c + (size_t)c * lda;
-    for (int e = tid; e < SB * SB; e += 256) s[e % SB][e / SB] = D[(e % SB) + (size_t)(e / SB) * lda];
+    for (int e = tid; e < SB * SB; e += 64) s[e % SB][e / SB] = D[(e % SB) + (size_t)(e / SB) * lda];
     if (tid == 0) err = 0;
     __syncthreads();
     const bool failed = p2_node<UP, SB>(s, rinv, &err, 0, tid);
     if (blockIdx.x == 0) {
-        for (int e = tid; e < SB * SB; e += 256) D[(e % SB) + (size_t)(e / SB) * lda] = s[e % SB][e / SB];
+        for (int e = tid; e < SB * SB; e += 64) D[(e % SB) + (size_t)(e / SB) * lda] = s[e % SB][e / SB];
         if (tid == 0 && failed) {
             info[1] = err;
             __hip_atomic_store(info, c + err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     if (failed) return;
-    const long v = (long)blockIdx.x * 256 + tid;
+    const long v = (long)blockIdx.x * 64 + tid;
     if (v >= n - c - SB) return;
     double x[SB];
     if (!UP) {
@@ -630,9 +742,9 @@ void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t ld
             const int n_rem = N - c - jb;
             if (jb == SB) {
                 Phase p(h, "POTRF2", (double)jb * jb * n_rem, st);  // POTRF2 and the TRSM below it
-                const unsigned g = (unsigned)std::max(1, (n_rem + 255) / 256);
-                if (up) hipLaunchKernelGGL(chol_panel<true>, dim3(g), dim3(256), 0, st, N, c, a, lda, info);
-                else hipLaunchKernelGGL(chol_panel<false>, dim3(g), dim3(256), 0, st, N, c, a, lda, info);
+                const unsigned g = (unsigned)std::max(1, (n_rem + 63) / 64);
+                if (up) hipLaunchKernelGGL(chol_panel<true>, dim3(g), dim3(64), 0, st, N, c, a, lda, info);
+                else hipLaunchKernelGGL(chol_panel<false>, dim3(g), dim3(64), 0, st, N, c, a, lda, info);
                 BOLA_LAUNCH_CHECK();
             } else {  // the last, partial block: nothing below it
                 Phase p(h, "POTRF2", 0.0, st);
