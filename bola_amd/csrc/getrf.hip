@@ -1038,6 +1038,428 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
     }
 }
 
+// ---- the panel with two wavefronts per workgroup
+//
+// The exchange of getf2_wave with the work on a block of 64*R rows split between two
+// wavefronts of one workgroup, which run on two SIMDs of one CU, so that the pivot search
+// no longer waits for the rank-1 updates (a lone wavefront issues one instruction every
+// few cycles, and the updates, the staging and the window shift were most of them):
+//   * the pivot wavefront (wave 0) holds column j of the block: it reduces the candidates
+//     of step j, publishes its best one, polls every workgroup's, reads the winning row,
+//     scales column j, and brings column j+1 -- handed over by the update wavefront --
+//     through step j;
+//   * the update wavefront (wave 1) holds columns j+1 .. w-1: while the pivot wavefront
+//     searches step j it applies step j-1's rank-1 update to them, hands column j+1 over,
+//     stages the rows in LDS and, once the step's candidate is known, publishes the
+//     candidate row and the row at position j, already through step j-1, so that a
+//     receiver applies nothing to them.
+// Two workgroup barriers a step: X_j (the candidate is known, column j+1 and the rows are
+// staged) and Y_j (pivot, multipliers and pivot row of step j are in LDS).  Each element
+// receives the reference's operations in the reference's order and form (getrf2.rs:131-185,
+// ger.rs:28-90); only the wavefront that applies them changes.
+//
+// Pivot candidates form one stream keyed so that the reference's IDAMAX rule (idamax.rs:
+// 23-133) reduces to "largest magnitude, then smallest key": a vector-phase row (offset
+// rel < T from the diagonal) has key (rel mod 4, (rel div 4) mod 4, rel) in bits 34-35,
+// 32-33 and 0-31, a tail row 2^36 + rel, so vector-phase rows win ties against tail rows;
+// an empty candidate is (0, NOKEY).  The winner (v, k) maps to the reference's index as
+// idamax_rule does: nothing -> 0; len < 16 -> rel; else rel when v > 0, 0 otherwise.
+//
+// Global words (zeroed per factorization, each 32-bit half tagged with the step's epoch):
+// keys [2 parities][4][nwv] (candidate x lo, hi, key lo, hi; word-major so a poll reads
+// consecutive words), rows [2][nwv][2 PW] (candidate row at absolute columns), cur [2][2 +
+// 2 PW] (x and row of the row at position j).
+constexpr int DUO_SR = 33;      // staged row stride (doubles)
+constexpr int DUO_CW = 2 + 2 * PW;
+constexpr int kDuoMax = 256;    // workgroups: all must be resident at once
+
+__device__ __forceinline__ uint64_t duo_key(int rel, int T) {
+    const uint64_t r = (uint64_t)(uint32_t)rel;
+    return rel < T ? (((r & 3ull) << 34) | (((r >> 2) & 3ull) << 32) | r) : ((1ull << 36) | r);
+}
+__device__ __forceinline__ double wave_max1(double a) {
+    a = fmax(a, dpp_f64<0xB1>(a));            // quad_perm 1,0,3,2
+    a = fmax(a, dpp_f64<0x4E>(a));            // quad_perm 2,3,0,1
+    a = fmax(a, dpp_f64<0x124>(a));           // row_ror 4
+    a = fmax(a, dpp_f64<0x128>(a));           // row_ror 8
+    a = fmax(a, dpp_f64<0x142, 0xA>(a));      // row_bcast 15
+    a = fmax(a, dpp_f64<0x143, 0xC>(a));      // row_bcast 31
+    return __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(a), 63));
+}
+// best (v, k) over the wavefront into every lane; returns the lane that held it
+__device__ __forceinline__ int wave_best1(double& v, uint64_t& k) {
+    const double mx = wave_max1(v);
+    const int l = best_lane(Cand{v, k}, mx);
+    k = lane_u64(k, l);
+    v = mx;
+    return l;
+}
+
+// Step dj's rank-1 update of the update wavefront's window (slot c = column dj+2+c, rc =
+// c+1) and of the last column; the forms of wave_ger_rest.
+template <int R>
+__device__ __forceinline__ void duo_ger_rest(double (&a)[R][WW], double (&al)[R], const double (&x)[R],
+                                             const bool (&fr)[R], const bool (&up)[R], const double* pr,
+                                             uint64_t zmask, int dj, int np, int nu) {
+    constexpr uint64_t EVEN = 0x5555555555555555ull;
+    const uint64_t z = zmask >> (dj + 1);  // bit rc: pr[dj+1+rc] == 0
+    const uint64_t zpair = z & (((z & EVEN) << 1) | ((z >> 1) & EVEN));
+    const uint64_t paired = (1ull << nu) - 1;
+    const uint64_t skip = (zpair & paired) | (z & ~paired);
+    const uint64_t touch = ((1ull << np) - 1) & ~skip & ~1ull;
+    double tcv[WW - 1];
+#pragma unroll
+    for (int c = 0; c < WW - 1; c++) tcv[c] = -1.0 * pr[dj + 2 + c];
+    const uint64_t want = paired & ~1ull;
+    if ((touch & want) == want) {
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (up[s]) {
+                if (fr[s]) {
+#pragma unroll
+                    for (int c = 0; c < WW - 1; c++) a[s][c] = fma(x[s], tcv[c], a[s][c]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < WW - 1; c++) a[s][c] = a[s][c] + tcv[c] * x[s];
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (up[s]) {
+                if (fr[s]) {
+#pragma unroll
+                    for (int c = 0; c < WW - 1; c++)
+                        if ((touch >> (c + 1)) & 1) a[s][c] = fma(x[s], tcv[c], a[s][c]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < WW - 1; c++)
+                        if ((touch >> (c + 1)) & 1) a[s][c] = a[s][c] + tcv[c] * x[s];
+                }
+            }
+        }
+    }
+    if (np >= 2 && ((touch >> (np - 1)) & 1)) {
+        const double tl = -1.0 * pr[dj + np];  // column w-1
+        const bool lp = !(np & 1);             // the last column is paired
+#pragma unroll
+        for (int s = 0; s < R; s++)
+            if (up[s]) al[s] = (lp && fr[s]) ? fma(x[s], tl, al[s]) : al[s] + tl * x[s];
+    }
+}
+
+// m > 64 R >= w: at least two workgroups, min(m, w) = w.
+template <int R>
+__global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __restrict__ A, size_t lda,
+                                                    int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                                    int col_off, long row_off, uint64_t* __restrict__ slots,
+                                                    uint64_t* __restrict__ cur, RowMap* __restrict__ map, int nwv,
+                                                    long spin_limit) {
+    __shared__ double sret[PW][R * WV];    // retired columns (pivot wavefront)
+    __shared__ double stg[R][WV][DUO_SR];  // columns j+1 .. w-1 of every row (update wavefront)
+    __shared__ double colx[R][WV];         // column j+1 through step j-1: update -> pivot wavefront
+    __shared__ double lbx[R][WV];          // step j's multipliers: pivot -> update wavefront
+    __shared__ int lbf[R][WV];             // their form flags: bit 0 fused row, bit 1 updated row
+    __shared__ double ubuf[2 * PW + 2];    // step j's pivot row at absolute columns, 0 outside [j, w)
+    __shared__ uint64_t uzero;             // its zero mask: bit c set when column c is 0
+    __shared__ int ctl[3];                 // stop; candidate (lane * R + s, or -1); row at j (or -1)
+
+    const int g = blockIdx.x, lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wl = w - 1;
+    uint64_t* keys = slots;                    // [2][4][nwv]
+    uint64_t* rows = slots + (size_t)8 * nwv;  // [2][nwv][2 PW]
+    long rowid[R];
+    bool valid[R];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        rowid[s] = (long)g * WV * R + (long)s * WV + lane;
+        valid[s] = rowid[s] < m;
+    }
+
+    if (wid == 0) {
+        // ---------------- pivot wavefront
+        double xc[R];  // column j of the rows, through step j-1
+        int pos[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            pos[s] = (int)rowid[s];
+            xc[s] = keep_or_zero(A[min(rowid[s], (long)m - 1)], valid[s]);
+        }
+        if (lane == 0) ctl[0] = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 2 : 0;
+        __syncthreads();  // B0
+        if (ctl[0]) return;
+        int jstop = w, stop = 0;
+        for (int j = 0; j < w; j++) {
+            const int len = m - j;
+            const int T = len < 16 ? 0 : (len & ~15);
+            const int par = j & 1;
+            const uint32_t ep = (uint32_t)(col_off + j + 1);
+            const uint64_t tag = (uint64_t)ep << 32;
+            // this block's best candidate and the row at position j
+            double bv = 0.0, bx = 0.0, xcur = 0.0;
+            uint64_t bk = NOKEY;
+            int bs = 0, scur = -1;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (valid[s] && pos[s] >= j) {
+                    const int rel = pos[s] - j;
+                    if (rel == 0) { scur = s; xcur = xc[s]; }
+                    const double v = fabs(xc[s]);
+                    if (v == v) {
+                        const uint64_t k = duo_key(rel, T);
+                        if (better(v, k, bv, bk)) { bv = v; bk = k; bx = xc[s]; bs = s; }
+                    }
+                }
+            }
+            const int lw = wave_best1(bv, bk);
+            bx = rdlane_d(bx, lw);
+            bs = __builtin_amdgcn_readlane(bs, lw);
+            const uint64_t hc = __ballot(scur >= 0);
+            const int lc = hc ? __ffsll((unsigned long long)hc) - 1 : -1;
+            const double xcu = lc >= 0 ? rdlane_d(xcur, lc) : 0.0;
+            const int scu = lc >= 0 ? __builtin_amdgcn_readlane(scur, lc) : 0;
+            if (lane < 4) {
+                const uint64_t src = lane < 2 ? (uint64_t)__double_as_longlong(bx) : bk;
+                xstu(keys + (size_t)(par * 4 + lane) * nwv + g, tag | (uint32_t)(src >> (32 * (lane & 1))));
+            } else if (lane < 6 && lc >= 0) {
+                xstu(cur + (size_t)par * DUO_CW + (lane - 4),
+                     tag | (uint32_t)((uint64_t)__double_as_longlong(xcu) >> (32 * (lane - 4))));
+            }
+            if (lane == 0) {
+                ctl[1] = bk != NOKEY ? lw * R + bs : -1;
+                ctl[2] = lc >= 0 ? lc * R + scu : -1;
+            }
+            __syncthreads();  // X_j
+            double xn[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) xn[s] = colx[s][lane];
+            // every workgroup's candidate, until its words carry the epoch
+            int ok = 1, timed = 0;
+            double gv = 0.0, gx = 0.0;
+            uint64_t gk = NOKEY;
+            int gsl = 0;
+            const uint64_t* kb = keys + (size_t)par * 4 * nwv;
+            for (int q0 = 0; q0 < nwv && ok; q0 += 2 * WV) {
+                uint64_t t[2][4];
+                const bool two = q0 + WV < nwv;
+                const int qa = min(q0 + lane, nwv - 1), qb = min(q0 + WV + lane, nwv - 1);
+                long sp = 0;
+                for (;;) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) t[0][i] = xldu(kb + (size_t)i * nwv + qa);
+                    if (two) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) t[1][i] = xldu(kb + (size_t)i * nwv + qb);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) t[1][i] = t[0][i];
+                    }
+                    bool good = true;
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) good = good && (uint32_t)(t[h][i] >> 32) == ep;
+                    if (good) break;
+                    if (++sp > spin_limit) { ok = 0; timed = 1; break; }
+                    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int q = q0 + h * WV + lane;
+                    if (q < nwv) {
+                        const double x = __longlong_as_double((long long)join32(t[h][0], t[h][1]));
+                        const uint64_t k = join32(t[h][2], t[h][3]);
+                        const double v = fabs(x);
+                        if (better(v, k, gv, gk)) { gv = v; gk = k; gx = x; gsl = q; }
+                    }
+                }
+            }
+            long rel = 0;
+            double piv = 0.0, u = 0.0;
+            if (!__all(ok)) {
+                give_up(info, timed, lane);
+                stop = 2;
+            } else {
+                const int lg = wave_best1(gv, gk);
+                gx = rdlane_d(gx, lg);
+                gsl = __builtin_amdgcn_readlane(gsl, lg);
+                if (gk == NOKEY) rel = 0;
+                else if (len < 16) rel = (long)(gk & 0xffffffffull);
+                else rel = gv > 0.0 ? (long)(gk & 0xffffffffull) : 0;
+                // the pivot row (columns j+1 .. w-1 by lane) and, for the row at j, its x
+                const uint64_t* prw = rel == 0 ? cur + (size_t)par * DUO_CW + 2 : rows + ((size_t)par * nwv + gsl) * (2 * PW);
+                const bool need = lane < 32 ? (lane > j && lane < w) : (lane == 32 && rel == 0);
+                const uint64_t* src = lane < 32 ? prw + 2 * lane : cur + (size_t)par * DUO_CW;
+                uint64_t r0 = 0, r1 = 0;
+                if (need) {
+                    long sp = 0;
+                    for (;;) {
+                        r0 = xldu(src);
+                        r1 = xldu(src + 1);
+                        if ((uint32_t)(r0 >> 32) == ep && (uint32_t)(r1 >> 32) == ep) break;
+                        if (++sp > spin_limit) { ok = 0; timed = 1; break; }
+                        if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
+                    }
+                }
+                if (!__all(ok)) {
+                    give_up(info, timed, lane);
+                    stop = 2;
+                } else {
+                    const double rv = __longlong_as_double((long long)join32(r0, r1));
+                    piv = rel == 0 ? rdlane_d(rv, 32) : gx;
+                    u = (lane > j && lane < w) ? rv : (lane == j ? piv : 0.0);
+                }
+            }
+            if (stop == 0) {
+                if (g == 0 && lane == 0) ipiv[j] = j + (int)rel + 1;
+                if (!(piv != 0.0)) {
+                    if (g == 0 && lane == 0) {
+                        info[1] = j + 1;
+                        __hip_atomic_store(info, col_off + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    stop = 1;
+                }
+            }
+            double l[R];
+            bool fr[R], up[R];
+            if (stop == 0) {
+                const int jp = j + (int)rel;
+                if (jp != j) {
+#pragma unroll
+                    for (int s = 0; s < R; s++) {
+                        if (pos[s] == j) pos[s] = jp;
+                        else if (pos[s] == jp) pos[s] = j;
+                    }
+                }
+                // scale (getrf2.rs:156-164); multipliers and forms of step j's GER
+                const int mf = (m - j - 1) & ~3;
+                const bool do_ger = j < w - 1;
+                const double inv = 1.0 / piv;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    double xj = xc[s];
+                    if (j < m - 1 && valid[s] && pos[s] > j) xj = xj * inv;
+                    up[s] = do_ger && valid[s] && pos[s] > j;
+                    fr[s] = (pos[s] - (j + 1)) < mf;
+                    l[s] = xj;
+                    sret[j][s * WV + lane] = xj;
+                    lbx[s][lane] = xj;
+                    lbf[s][lane] = (fr[s] ? 1 : 0) | (up[s] ? 2 : 0);
+                }
+                ubuf[lane] = u;
+                const uint64_t zm = __ballot(u == 0.0);  // lanes outside [j, w) carry 0.0
+                if (lane == 0) uzero = zm;
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) { l[s] = 0.0; fr[s] = false; up[s] = false; }
+            }
+            if (lane == 0) ctl[0] = stop;
+            __syncthreads();  // Y_j
+            if (stop) {
+                jstop = j;
+                break;
+            }
+            if (j + 1 < w) {
+                // column j+1 through step j (ger.rs:28-90; its partner is column j+2)
+                const int np = w - j - 1, nu = np - (np & 1);
+                const double tc = -1.0 * rdlane_d(u, j + 1);
+                const double tp = -1.0 * rdlane_d(u, j + 2);
+#pragma unroll
+                for (int s = 0; s < R; s++) xc[s] = ger_elem(xn[s], l[s], tc, tp, 0, np, nu, fr[s], up[s]);
+            }
+        }
+        if (stop == 2) return;
+        // rows to their final positions; moved rows form the panel's row map.  After a zero
+        // pivot at step jstop: columns before it retired, column jstop as it stood, the
+        // columns after it as staged for that step (through step jstop-1).
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            if (valid[s]) {
+                for (int c = 0; c < jstop; c++) A[(size_t)pos[s] + (size_t)c * lda] = sret[c][s * WV + lane];
+                if (jstop < w) {
+                    A[(size_t)pos[s] + (size_t)jstop * lda] = xc[s];
+                    for (int c = jstop + 1; c < w; c++)
+                        A[(size_t)pos[s] + (size_t)c * lda] = stg[s][lane][c - jstop - 1];
+                }
+                if (pos[s] != rowid[s]) {
+                    int idx = atomicAdd(&map->count, 1);
+                    if (idx < ROWMAP_MAX) {
+                        map->row[idx] = row_off + pos[s];
+                        map->src[idx] = row_off + rowid[s];
+                    }
+                }
+            }
+        }
+    } else {
+        // ---------------- update wavefront: window slot c = column j+1+c, the last column apart
+        double a[R][WW], al[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            const long rc = min(rowid[s], (long)m - 1);  // clamped addresses: every load in flight at once
+#pragma unroll
+            for (int c = 0; c < WW; c++) {
+                const double v = A[rc + (size_t)min(1 + c, max(wl - 1, 0)) * lda];
+                a[s][c] = keep_or_zero(v, valid[s] && 1 + c < wl);
+            }
+            al[s] = keep_or_zero(A[rc + (size_t)wl * lda], valid[s] && wl >= 1);
+        }
+        if (lane < 2) ubuf[2 * PW + lane] = 0.0;  // read past the panel by the window loop, never written
+        __syncthreads();  // B0
+        if (ctl[0]) return;
+        for (int j = 0; j < w; j++) {
+            const int par = j & 1;
+            const uint64_t tag = (uint64_t)(uint32_t)(col_off + j + 1) << 32;
+            if (j >= 1) {
+                // step j-1's GER on columns j+1 .. w-1
+                double x[R];
+                bool fr[R], up[R];
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    x[s] = lbx[s][lane];
+                    const int f = lbf[s][lane];
+                    fr[s] = (f & 1) != 0;
+                    up[s] = (f & 2) != 0;
+                }
+                const int np = w - j;  // columns right of column j-1
+                duo_ger_rest<R>(a, al, x, fr, up, ubuf, uzero, j - 1, np, np - (np & 1));
+            }
+            if (j + 1 < w) {
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    colx[s][lane] = (j + 1 == wl) ? al[s] : a[s][0];
+                    double* d = stg[s][lane];
+#pragma unroll
+                    for (int c = 0; c < WW; c++) d[c] = a[s][c];  // slots past the panel are never read
+                    d[wl - j - 1] = al[s];
+                }
+            }
+            __syncthreads();  // X_j
+            {
+                // the candidate row (lanes 0-31) and the row at position j (lanes 32-63), by column
+                const int h = lane >> 5, c = lane & 31;
+                const int q = h == 0 ? ctl[1] : ctl[2];
+                if (q >= 0 && c > j && c < w) {
+                    const uint64_t b = (uint64_t)__double_as_longlong(stg[q % R][q / R][c - j - 1]);
+                    uint64_t* dst = h == 0 ? rows + ((size_t)par * nwv + g) * (2 * PW) + 2 * c
+                                           : cur + (size_t)par * DUO_CW + 2 + 2 * c;
+                    xstu(dst, tag | (uint32_t)b);
+                    xstu(dst + 1, tag | (uint32_t)(b >> 32));
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+#pragma unroll
+                for (int c = 0; c < WW - 1; c++) a[s][c] = a[s][c + 1];
+                a[s][WW - 1] = 0.0;
+            }
+            __syncthreads();  // Y_j
+            if (ctl[0]) return;
+        }
+    }
+}
+
 // ---- the U rows of a factored super-panel for one block of 32 columns, in one launch
 //
 // For panels q = 0 .. G-1 of the super-panel (rows j0 = 32 q): first the GEMM of those
@@ -1454,6 +1876,13 @@ void getf2(hipStream_t st, int m, int n, double* a, size_t lda, int32_t* ipiv, i
                                    row_off, wslots, wcur, map, nwv, kSpinLimit);
             else
                 hipLaunchKernelGGL((getf2_wave<2, false>), dim3(1), dim3(WV), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, wslots, wcur, map, nwv, kSpinLimit);
+        } else if (nwv <= kDuoMax) {
+            if (r == 1)
+                hipLaunchKernelGGL((getf2_duo<1>), dim3(nwv), dim3(2 * WV), 0, st, m, n, a, lda, ipiv, info, col_off,
+                                   row_off, wslots, wcur, map, nwv, kSpinLimit);
+            else
+                hipLaunchKernelGGL((getf2_duo<2>), dim3(nwv), dim3(2 * WV), 0, st, m, n, a, lda, ipiv, info, col_off,
                                    row_off, wslots, wcur, map, nwv, kSpinLimit);
         } else {
             if (r == 1)
