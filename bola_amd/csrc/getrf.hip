@@ -1240,13 +1240,18 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
             double gv = 0.0, gx = 0.0;
             uint64_t gk = NOKEY;
             int gsl = 0;
+            uint64_t cx0 = 0, cx1 = 0;
             const uint64_t* kb = keys + (size_t)par * 4 * nwv;
+            const uint64_t* cxw = cur + (size_t)par * DUO_CW;
             for (int q0 = 0; q0 < nwv && ok; q0 += 2 * WV) {
                 uint64_t t[2][4];
                 const bool two = q0 + WV < nwv;
+                const bool withx = q0 == 0 && lane == 0;
                 const int qa = min(q0 + lane, nwv - 1), qb = min(q0 + WV + lane, nwv - 1);
                 long sp = 0;
                 for (;;) {
+                    // one round trip an attempt: the keys, the x of the row at j and the
+                    // status word (another workgroup gave up) are loaded together
 #pragma unroll
                     for (int i = 0; i < 4; i++) t[0][i] = xldu(kb + (size_t)i * nwv + qa);
                     if (two) {
@@ -1256,14 +1261,19 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
 #pragma unroll
                         for (int i = 0; i < 4; i++) t[1][i] = t[0][i];
                     }
-                    bool good = true;
+                    if (withx) {
+                        cx0 = xldu(cxw);
+                        cx1 = xldu(cxw + 1);
+                    }
+                    const int stw = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bool good = !withx || ((uint32_t)(cx0 >> 32) == ep && (uint32_t)(cx1 >> 32) == ep);
 #pragma unroll
                     for (int h = 0; h < 2; h++)
 #pragma unroll
                         for (int i = 0; i < 4; i++) good = good && (uint32_t)(t[h][i] >> 32) == ep;
                     if (good) break;
                     if (++sp > spin_limit) { ok = 0; timed = 1; break; }
-                    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
+                    if (stw < 0) { ok = 0; break; }
                 }
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
@@ -1278,6 +1288,10 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
             }
             long rel = 0;
             double piv = 0.0, u = 0.0;
+            double l[R];
+            bool fr[R], up[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) { l[s] = 0.0; fr[s] = false; up[s] = false; }
             if (!__all(ok)) {
                 give_up(info, timed, lane);
                 stop = 2;
@@ -1288,31 +1302,18 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
                 if (gk == NOKEY) rel = 0;
                 else if (len < 16) rel = (long)(gk & 0xffffffffull);
                 else rel = gv > 0.0 ? (long)(gk & 0xffffffffull) : 0;
-                // the pivot row (columns j+1 .. w-1 by lane) and, for the row at j, its x
-                const uint64_t* prw = rel == 0 ? cur + (size_t)par * DUO_CW + 2 : rows + ((size_t)par * nwv + gsl) * (2 * PW);
-                const bool need = lane < 32 ? (lane > j && lane < w) : (lane == 32 && rel == 0);
-                const uint64_t* src = lane < 32 ? prw + 2 * lane : cur + (size_t)par * DUO_CW;
+                piv = rel == 0 ? __longlong_as_double((long long)lane_u64(join32(cx0, cx1), 0)) : gx;
+                // the pivot row, columns j+1 .. w-1 by lane: the first loads go out now, the
+                // scaling runs while they are in flight
+                const uint64_t* prw =
+                    rel == 0 ? cur + (size_t)par * DUO_CW + 2 : rows + ((size_t)par * nwv + gsl) * (2 * PW);
+                const bool need = lane > j && lane < w;
+                const uint64_t* src = prw + 2 * (need ? lane : 0);
                 uint64_t r0 = 0, r1 = 0;
                 if (need) {
-                    long sp = 0;
-                    for (;;) {
-                        r0 = xldu(src);
-                        r1 = xldu(src + 1);
-                        if ((uint32_t)(r0 >> 32) == ep && (uint32_t)(r1 >> 32) == ep) break;
-                        if (++sp > spin_limit) { ok = 0; timed = 1; break; }
-                        if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) { ok = 0; break; }
-                    }
+                    r0 = xldu(src);
+                    r1 = xldu(src + 1);
                 }
-                if (!__all(ok)) {
-                    give_up(info, timed, lane);
-                    stop = 2;
-                } else {
-                    const double rv = __longlong_as_double((long long)join32(r0, r1));
-                    piv = rel == 0 ? rdlane_d(rv, 32) : gx;
-                    u = (lane > j && lane < w) ? rv : (lane == j ? piv : 0.0);
-                }
-            }
-            if (stop == 0) {
                 if (g == 0 && lane == 0) ipiv[j] = j + (int)rel + 1;
                 if (!(piv != 0.0)) {
                     if (g == 0 && lane == 0) {
@@ -1320,40 +1321,51 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
                         __hip_atomic_store(info, col_off + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     stop = 1;
-                }
-            }
-            double l[R];
-            bool fr[R], up[R];
-            if (stop == 0) {
-                const int jp = j + (int)rel;
-                if (jp != j) {
+                } else {
+                    const int jp = j + (int)rel;
+                    if (jp != j) {
+#pragma unroll
+                        for (int s = 0; s < R; s++) {
+                            if (pos[s] == j) pos[s] = jp;
+                            else if (pos[s] == jp) pos[s] = j;
+                        }
+                    }
+                    // scale (getrf2.rs:156-164); multipliers and forms of step j's GER
+                    const int mf = (m - j - 1) & ~3;
+                    const bool do_ger = j < w - 1;
+                    const double inv = 1.0 / piv;
 #pragma unroll
                     for (int s = 0; s < R; s++) {
-                        if (pos[s] == j) pos[s] = jp;
-                        else if (pos[s] == jp) pos[s] = j;
+                        double xj = xc[s];
+                        if (j < m - 1 && valid[s] && pos[s] > j) xj = xj * inv;
+                        up[s] = do_ger && valid[s] && pos[s] > j;
+                        fr[s] = (pos[s] - (j + 1)) < mf;
+                        l[s] = xj;
+                        sret[j][s * WV + lane] = xj;
+                        lbx[s][lane] = xj;
+                        lbf[s][lane] = (fr[s] ? 1 : 0) | (up[s] ? 2 : 0);
+                    }
+                    if (need) {
+                        long sp = 0;
+                        int stw = 0;
+                        while ((uint32_t)(r0 >> 32) != ep || (uint32_t)(r1 >> 32) != ep) {
+                            if (++sp > spin_limit) { ok = 0; timed = 1; break; }
+                            if (stw < 0) { ok = 0; break; }
+                            r0 = xldu(src);
+                            r1 = xldu(src + 1);
+                            stw = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                    if (!__all(ok)) {
+                        give_up(info, timed, lane);
+                        stop = 2;
+                    } else {
+                        u = need ? __longlong_as_double((long long)join32(r0, r1)) : (lane == j ? piv : 0.0);
+                        ubuf[lane] = u;
+                        const uint64_t zm = __ballot(u == 0.0);  // lanes outside [j, w) carry 0.0
+                        if (lane == 0) uzero = zm;
                     }
                 }
-                // scale (getrf2.rs:156-164); multipliers and forms of step j's GER
-                const int mf = (m - j - 1) & ~3;
-                const bool do_ger = j < w - 1;
-                const double inv = 1.0 / piv;
-#pragma unroll
-                for (int s = 0; s < R; s++) {
-                    double xj = xc[s];
-                    if (j < m - 1 && valid[s] && pos[s] > j) xj = xj * inv;
-                    up[s] = do_ger && valid[s] && pos[s] > j;
-                    fr[s] = (pos[s] - (j + 1)) < mf;
-                    l[s] = xj;
-                    sret[j][s * WV + lane] = xj;
-                    lbx[s][lane] = xj;
-                    lbf[s][lane] = (fr[s] ? 1 : 0) | (up[s] ? 2 : 0);
-                }
-                ubuf[lane] = u;
-                const uint64_t zm = __ballot(u == 0.0);  // lanes outside [j, w) carry 0.0
-                if (lane == 0) uzero = zm;
-            } else {
-#pragma unroll
-                for (int s = 0; s < R; s++) { l[s] = 0.0; fr[s] = false; up[s] = false; }
             }
             if (lane == 0) ctl[0] = stop;
             __syncthreads();  // Y_j
