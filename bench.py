@@ -13,8 +13,8 @@ Beside it, in the same JSON line:
                A = G G^T + n I, median of 5 solves.
   mixed_ir  -- the mixed-precision grid: A X = B at n in {8192, 16384}, kappa in {1e2, 1e5,
                1e8}, nrhs in {1, 16}, A = U diag(sigma) V^T (seeded Householder products),
-               solved to backward error <= n * 2^-52 by the mixed-precision solver in six
-               modes (fp32 / fp16 / bf16 products, tournament pivoting or butterflies;
+               solved to backward error <= n * 2^-52 by the mixed-precision solver in eight
+               modes (fp32 / fp16 / bf16 / split-fp16 products, tournament pivoting or butterflies;
                FP64 refinement, GMRES-IR, FP64 fallback), each mode and the plain FP64
                GETRF + GETRS the median of 3 solves after a warm-up; the forward error is
                stated against the plain FP64 solution (which is bit-identical to the
@@ -446,7 +446,7 @@ def mixed_section(B, torch, sizes=(8192, 16384), kappas=(1e2, 1e5, 1e8), nrhss=(
     error of each mode is stated against the plain FP64 solution, which equals the oracle's
     bit for bit (the GPU parity tests check that at these sizes)."""
     from bola_amd import testmat
-    modes = [(f"{p}_{'butterfly' if r else 'tournament'}", p, r) for p in ("fp32", "fp16", "bf16")
+    modes = [(f"{p}_{'butterfly' if r else 'tournament'}", p, r) for p in ("fp32", "fp16", "bf16", "fp16x3")
              for r in (True, False)]
     out = {"workload": "A X = B, A = U diag(sigma) V^T (Householder products, sigma geometric 1..1/kappa), "
                        "B uniform [-1,1); stop at backward error <= n * eps64; median of "

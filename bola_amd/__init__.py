@@ -420,7 +420,7 @@ def potrs(uplo, n, nrhs, f, lda, b, ldb) -> None:
 # ------------------------------------------------------------------ mixed precision
 # No crate counterpart (the crate solves in FP64 only); see include/bola_hip.h.
 
-PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2, "fp16x3": 3}
 
 
 class MixedStats(C.Structure):

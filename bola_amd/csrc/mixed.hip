@@ -1015,7 +1015,7 @@ int mixed_solve(bola_hip_handle_s* h, MixedFactor& f, int nrhs, const double* B,
 
 int mixed_check_args(bola_hip_handle_s* h, int precision, size_t n, size_t lda, const char* who) {
     precision &= 0xFF;
-    if (precision > 2) {
+    if (precision > LP_F16X3) {
         set_error(h, std::string("Argument 2 to ") + who + " had an illegal value");
         return -2;
     }

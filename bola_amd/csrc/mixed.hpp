@@ -7,7 +7,7 @@
 namespace bola {
 
 // precision of the factorization's trailing-update products
-enum { LP_F32 = 0, LP_F16 = 1, LP_BF16 = 2 };
+enum { LP_F32 = 0, LP_F16 = 1, LP_BF16 = 2, LP_F16X3 = 3 };
 
 // C (fp32, column-major) -= A * B with the products in precision lp; ws holds the 16-bit
 // operand copies (lp_gemm_ws_bytes).

@@ -7,6 +7,12 @@
 //   LP_F32  : A, B fp32, v_mfma_f32_16x16x4_f32 (FP32 matrix peak 157 TF/s)
 //   LP_F16  : A, B rounded to fp16, v_mfma_f32_16x16x32_f16, fp32 accumulation
 //   LP_BF16 : A, B rounded to bf16, v_mfma_f32_16x16x32_bf16, fp32 accumulation
+//   LP_F16X3: A, B split into fp16 pairs, x = h + 2^-11 l with h = fp16(x) and
+//             l = fp16((x - h) 2^11); C -= Ah Bh + 2^-11 (Ah Bl + Al Bh), three
+//             v_mfma_f32_16x16x32_f16 per step into two fp32 accumulators.  The dropped
+//             2^-22 Al Bl term and the rounding of l are both near 2^-22 relative, so the
+//             products are about as accurate as fp32 ones at three sixteenths of the fp32
+//             MFMA cost (fp16 matrix peak 2.5 PF/s).
 //
 // Tiles of 128 x 128 per 256-thread workgroup, 2 x 2 wavefronts of 64 x 64 (4 x 4 MFMA
 // blocks each, 16 accumulators); K steps staged through double-buffered LDS.  LDS rows
@@ -297,6 +303,164 @@ __global__ void cvt_n16(int k, int n, const float* __restrict__ src, size_t lds,
     }
 }
 
+// ---- split fp16 operands (LP_F16X3).  Ath/Atl: A transposed (row i k-contiguous), Bh/Bl:
+// B columns k-contiguous.  BK = 32 elements (one MFMA step) a stage; the four operand
+// tiles are double-buffered in LDS (80 KB).
+__global__ void __launch_bounds__(GT) sgemm_h3(int m, int n, int k, const _Float16* __restrict__ Ath,
+                                               const _Float16* __restrict__ Atl, size_t lda,
+                                               const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                               size_t ldb, float* __restrict__ C, size_t ldc, int tiles_m,
+                                               int tiles_n) {
+    constexpr int BK = 32;
+    constexpr int RW3 = 20;  // LDS row stride in words: 16 data + 4 pad
+    __shared__ uint32_t S[2][4][GBM * RW3];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    const int i0 = tm * GBM, j0 = tn * GBN;
+    const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
+    f4 am[4][4], ac[4][4];  // Ah Bh, and Ah Bl + Al Bh
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            am[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+            ac[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    // a tile is 128 rows x 32 elements = 128 x 4 chunks of 16 bytes; 2 chunks per thread
+    uint4 rr[4][2];
+    const _Float16* src[4] = {Ath, Atl, Bh, Bl};
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int e = tid + q * GT;
+            const int row = e >> 2, ch = (e & 3) * 8;
+            const int gk = k0 + ch;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int gr = (t < 2 ? i0 : j0) + row;
+                const int lim = t < 2 ? m : n;
+                const size_t ld = t < 2 ? lda : ldb;
+                if (gr < lim && gk + 8 <= k) {
+                    rr[t][q] = *(const uint4*)(src[t] + (size_t)gr * ld + gk);
+                } else {
+                    uint16_t tmp[8];
+#pragma unroll
+                    for (int r = 0; r < 8; r++)
+                        tmp[r] = (gr < lim && gk + r < k) ? ((const uint16_t*)src[t])[(size_t)gr * ld + gk + r]
+                                                          : (uint16_t)0;
+                    rr[t][q] = uint4{tmp[0] | ((uint32_t)tmp[1] << 16), tmp[2] | ((uint32_t)tmp[3] << 16),
+                                     tmp[4] | ((uint32_t)tmp[5] << 16), tmp[6] | ((uint32_t)tmp[7] << 16)};
+                }
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int e = tid + q * GT;
+            const int row = e >> 2, w4 = (e & 3) * 4;
+#pragma unroll
+            for (int t = 0; t < 4; t++) *(uint4*)&S[buf][t][row * RW3 + w4] = rr[t][q];
+        }
+    };
+    const int nk = (k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        const int cur = t & 1;
+        if (t + 1 < nk) gload((t + 1) * BK);
+        h8 fah[4], fal[4], fbh[4], fbl[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int o = (wm + 16 * a + (lane & 15)) * RW3 + 4 * (lane >> 4);
+            fah[a] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][0][o]);
+            fal[a] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][1][o]);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int o = (wn + 16 * b + (lane & 15)) * RW3 + 4 * (lane >> 4);
+            fbh[b] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][2][o]);
+            fbl[b] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][3][o]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                am[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[a], fbh[b], am[a][b], 0, 0, 0);
+                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[a], fbl[b], ac[a][b], 0, 0, 0);
+                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal[a], fbh[b], ac[a][b], 0, 0, 0);
+            }
+        if (t + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    // C -= Ah Bh + 2^-11 (Ah Bl + Al Bh), C read from clamped addresses first
+    float cv[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; r++) cv[a][b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+        }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int j = j0 + wn + 16 * b + (lane & 15);
+            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+            if (j < n) {
+                float* cp = C + (size_t)j * ldc;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - (am[a][b][r] + ac[a][b][r] * 0x1p-11f);
+            }
+        }
+}
+
+__device__ __forceinline__ void split16(float x, _Float16& h, _Float16& l) {
+    h = (_Float16)x;
+    l = (_Float16)((x - (float)h) * 2048.0f);
+}
+
+// fp32 -> split fp16 pairs, transposed (as cvt_t16)
+__global__ void __launch_bounds__(256) cvt_t16s(int m, int k, const float* __restrict__ src, size_t lds,
+                                                _Float16* __restrict__ hi, _Float16* __restrict__ lo, size_t ldd) {
+    __shared__ float tile[64][65];
+    const int bi = blockIdx.x * 64, bk = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int i = bi + tx, kk = bk + r;
+        tile[r][tx] = (i < m && kk < k) ? src[(size_t)i + (size_t)kk * lds] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int i = bi + r, kk = bk + tx;
+        if (i < m && kk < k) {
+            _Float16 h, l;
+            split16(tile[tx][r], h, l);
+            hi[(size_t)i * ldd + kk] = h;
+            lo[(size_t)i * ldd + kk] = l;
+        }
+    }
+}
+
+// fp32 -> split fp16 pairs, same layout (as cvt_n16)
+__global__ void cvt_n16s(int k, int n, const float* __restrict__ src, size_t lds, _Float16* __restrict__ hi,
+                         _Float16* __restrict__ lo, size_t ldd) {
+    const size_t total = (size_t)k * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = e / k, kk = e % k;
+        _Float16 h, l;
+        split16(src[kk + j * lds], h, l);
+        hi[kk + j * ldd] = h;
+        lo[kk + j * ldd] = l;
+    }
+}
+
 }  // namespace
 
 void launch_sgemm_set(hipStream_t st, int m, int n, int k, const float* A, size_t lda, const float* B, size_t ldb,
@@ -311,7 +475,7 @@ void launch_sgemm_set(hipStream_t st, int m, int n, int k, const float* A, size_
 size_t lp_gemm_ws_bytes(int lp, size_t m, size_t n, size_t k) {
     if (lp == LP_F32) return 0;
     const size_t kp = (k + 7) & ~(size_t)7;
-    return 2 * (m * kp + n * kp) + 256;
+    return (lp == LP_F16X3 ? 4 : 2) * (m * kp + n * kp) + 256;
 }
 
 void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A, size_t lda, const float* B,
@@ -332,7 +496,17 @@ void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A,
     void* Bh = base + 2 * (size_t)m * kp;
     const dim3 tg((unsigned)((m + 63) / 64), (unsigned)((k + 63) / 64));
     const unsigned cg = (unsigned)std::min<size_t>(((size_t)k * n + 255) / 256, 4096);
-    if (lp == LP_F16) {
+    if (lp == LP_F16X3) {
+        // hi and lo copies of each operand: At_hi, At_lo (m rows of kp), B_hi, B_lo (n columns of kp)
+        _Float16* ath = (_Float16*)base;
+        _Float16* atl = ath + (size_t)m * kp;
+        _Float16* bh = atl + (size_t)m * kp;
+        _Float16* bl = bh + (size_t)n * kp;
+        hipLaunchKernelGGL(cvt_t16s, tg, dim3(256), 0, st, m, k, A, lda, ath, atl, kp);
+        hipLaunchKernelGGL(cvt_n16s, dim3(cg), dim3(256), 0, st, k, n, B, ldb, bh, bl, kp);
+        hipLaunchKernelGGL(sgemm_h3, dim3(grid), dim3(GT), 0, st, m, n, k, ath, atl, kp, bh, bl, kp, C, ldc, tiles_m,
+                           tiles_n);
+    } else if (lp == LP_F16) {
         hipLaunchKernelGGL(cvt_t16<_Float16>, tg, dim3(256), 0, st, m, k, A, lda, (_Float16*)At, kp);
         hipLaunchKernelGGL(cvt_n16<_Float16>, dim3(cg), dim3(256), 0, st, k, n, B, ldb, (_Float16*)Bh, kp);
         hipLaunchKernelGGL((sgemm_h<_Float16, h8>), dim3(grid), dim3(GT), 0, st, m, n, k, (const _Float16*)At, kp,
@@ -354,7 +528,7 @@ extern "C" int bola_hip_sgemm_lp_dev(bola_hip_handle_t h, int precision, size_t 
                                      size_t lda, const float* b, size_t ldb, float* c, size_t ldc) {
     BOLA_API_BEGIN
     if (!h) return -1;
-    if (precision < 0 || precision > 2) return -2;
+    if (precision < 0 || precision > LP_F16X3) return -2;
     void* ws = lp_gemm_ws_bytes(precision, m, n, k) ? h->mws.get(lp_gemm_ws_bytes(precision, m, n, k)) : nullptr;
     launch_lp_gemm(h->stream, precision, (int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, ws);
     return 0;

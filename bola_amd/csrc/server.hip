@@ -253,7 +253,7 @@ int bola_hip_server_submit(bola_hip_server_t s, const bola_hip_request* r, uint6
     std::lock_guard<std::mutex> lk(s->mu);
     bool hit = false;
     if (r->precision != BOLA_HIP_SERVE_FP64 &&
-        (r->kind != BOLA_HIP_SERVE_LU || r->precision < 1 || ((r->precision - 1) & 0xFF) > BOLA_HIP_PREC_BF16 ||
+        (r->kind != BOLA_HIP_SERVE_LU || r->precision < 1 || ((r->precision - 1) & 0xFF) > BOLA_HIP_PREC_FP16X3 ||
          ((r->precision - 1) & ~(0xFF | BOLA_HIP_MIX_RBT)) != 0 || r->n > 16384)) {
         set_error(nullptr, "server_submit: invalid precision (mixed precision for LU requests, n <= 16384)");
         return -2;

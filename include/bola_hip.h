@@ -174,7 +174,7 @@ int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, c
  * right-hand sides, column-major f64) to FP64 accuracy from a low-precision LU:
  * power-of-two row/column equilibration, LU of the scaled matrix in fp32 storage with
  * tournament pivoting (or, with BOLA_HIP_MIX_RBT, butterfly randomisation and no
- * interchanges) and trailing products in fp32, fp16 or bf16 on the MFMA units,
+ * interchanges) and trailing products in fp32, fp16, bf16 or split fp16 on the MFMA units,
  * iterative refinement in FP64 (r = b - A x on the original A), GMRES-IR when plain
  * refinement stagnates, and the FP64 GETRF + GETRS of this library as the last resort.
  * Stops when every column's backward error |r|_inf / (|A|_inf |x|_inf + |b|_inf) is at most
@@ -184,6 +184,10 @@ int bola_hip_dpotrs_dev(bola_hip_handle_t h, char uplo, size_t n, size_t nrhs, c
 #define BOLA_HIP_PREC_FP32 0
 #define BOLA_HIP_PREC_FP16 1
 #define BOLA_HIP_PREC_BF16 2
+/* Split fp16: every fp32 operand x as the pair h = fp16(x), l = fp16((x - h) 2^11); the
+ * product Ah Bh + 2^-11 (Ah Bl + Al Bh) takes three fp16 MFMA steps with fp32
+ * accumulation and is about as accurate as an fp32 product. */
+#define BOLA_HIP_PREC_FP16X3 3
 /* Or-ed into `precision`: randomise A with two-level random butterflies on both sides
  * and factor without interchanges (instead of tournament pivoting); for n divisible by 4,
  * otherwise ignored.  The refinement and the fallback are unchanged. */

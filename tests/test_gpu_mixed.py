@@ -57,6 +57,36 @@ def test_sgemm_lp_matches_fp64_product_of_rounded_operands(B, prec, m, n, k):
     assert np.all(np.abs(got - want) <= bound + 1e-30), float(np.max(np.abs(got - want) - bound))
 
 
+@pytest.mark.parametrize("m,n,k", [(128, 128, 32), (300, 200, 96), (1000, 513, 256), (64, 700, 33), (257, 129, 1),
+                                   (2048, 1024, 512)])
+def test_sgemm_split_fp16_is_as_accurate_as_fp32(B, m, n, k):
+    # each fp32 operand as h + 2^-11 l (h, l fp16): the product, with the 2^-22 l l term
+    # dropped, against the FP64 product of the unrounded fp32 operands.  Per product the
+    # representation errors are near 2^-22 relative; fp32 accumulation adds k eps32.
+    import torch
+    rng = np.random.default_rng(m * 5 + n * 11 + k)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    Bm = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    Cm = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    dA = torch.from_numpy(colmajor(A)).cuda()
+    dB = torch.from_numpy(colmajor(Bm)).cuda()
+    dC = torch.from_numpy(colmajor(Cm)).cuda()
+    B.sgemm_lp(m, n, k, dA, m, dB, k, dC, m, "fp16x3")
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy().reshape(n, m).T.astype(np.float64)
+    A64, B64 = A.astype(np.float64), Bm.astype(np.float64)
+    want = Cm.astype(np.float64) - A64 @ B64
+    bound = ((k + 2) * EPS32 + 8 * 2.0 ** -22) * (np.abs(A64) @ np.abs(B64) + np.abs(Cm))
+    assert np.all(np.abs(got - want) <= bound + 1e-30), float(np.max(np.abs(got - want) - bound))
+    # and clearly better than plain fp16 products of the same operands
+    dC2 = torch.from_numpy(colmajor(Cm)).cuda()
+    B.sgemm_lp(m, n, k, dA, m, dB, k, dC2, m, "fp16")
+    torch.cuda.synchronize()
+    got16 = dC2.cpu().numpy().reshape(n, m).T.astype(np.float64)
+    if k >= 32:
+        assert np.max(np.abs(got - want)) * 64 < np.max(np.abs(got16 - want))
+
+
 def _solve(B, n, nrhs, kappa, prec, seed=7, rbt=False):
     import torch
     from bola_amd import testmat
@@ -81,7 +111,8 @@ def _berr(Af, b, x, n, nrhs):
 
 @pytest.mark.parametrize("rbt", [False, True], ids=["tournament", "butterfly"])
 @pytest.mark.parametrize("prec,kappa", [("fp32", 1e2), ("fp32", 1e5), ("fp32", 1e8), ("fp16", 1e2),
-                                        ("fp16", 1e5), ("bf16", 1e2), ("bf16", 1e5)])
+                                        ("fp16", 1e5), ("bf16", 1e2), ("bf16", 1e5), ("fp16x3", 1e2),
+                                        ("fp16x3", 1e5), ("fp16x3", 1e8)])
 @pytest.mark.parametrize("n,nrhs", [(512, 1), (1024, 16), (1000, 3)])
 def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs, rbt):
     Af, b, x, st = _solve(B, n, nrhs, kappa, prec, rbt=rbt)
@@ -90,7 +121,7 @@ def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs, rbt
     assert st["final_berr"] <= n * EPS64
     # the same criterion recomputed on the host
     assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
-    if kappa <= 1e2:
+    if kappa <= 1e2 or (prec == "fp16x3" and kappa <= 1e5):
         assert st["fallback"] == 0, st
     # forward error against the FP64 oracle's solution, within the conditioning bound
     a = Af.copy()
@@ -192,9 +223,9 @@ def _big_case(O, kappa):
 
 
 @pytest.mark.parametrize("prec,rbt", [("fp32", True), ("fp32", False), ("fp16", True), ("fp16", False),
-                                      ("bf16", True), ("bf16", False)],
+                                      ("bf16", True), ("bf16", False), ("fp16x3", True), ("fp16x3", False)],
                          ids=["fp32-butterfly", "fp32-tournament", "fp16-butterfly", "fp16-tournament",
-                              "bf16-butterfly", "bf16-tournament"])
+                              "bf16-butterfly", "bf16-tournament", "fp16x3-butterfly", "fp16x3-tournament"])
 @pytest.mark.parametrize("nrhs", [1, 16])
 @pytest.mark.parametrize("kappa", [1e2, 1e5, 1e8])  # outermost: the matrix is built once per kappa
 def test_mixed_config_n8192(B, O, kappa, nrhs, prec, rbt):
@@ -211,7 +242,7 @@ def test_mixed_config_n8192(B, O, kappa, nrhs, prec, rbt):
     assert st["converged"] == 1, st
     assert st["final_berr"] <= n * EPS64
     assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
-    if prec == "fp32" and kappa <= 1e5:
+    if prec in ("fp32", "fp16x3") and kappa <= 1e5:
         assert st["fallback"] == 0, st
     xo = b.copy()
     O.getrs(n, nrhs, lu, n, p, xo, n)
