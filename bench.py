@@ -71,7 +71,8 @@ def main():
     ap.add_argument("--no-cpu-rows", action="store_true",
                     help="time only the headline row on the CPU (not DGEMM 1024, LU 4096, Cholesky L/U 4096)")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-IR and batched sections")
-    ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "serving", "cholesky", "cpu"], default=None,
+    ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "lp_gemm", "serving", "cholesky", "cpu"],
+                    default=None,
                     help="run one section alone (for profiling) and print it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
@@ -95,7 +96,7 @@ def main():
 
     if args.only:
         torch.cuda.set_device(0)
-        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section,
+        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section, "lp_gemm": lp_gemm_section,
                "serving": serving_section, "cholesky": chol_section, "cpu": lambda B, torch: dict(cpu_rows(), **cpu_info())}[
             args.only](B, torch)
         print(json.dumps({args.only: sec}))
@@ -201,9 +202,10 @@ def main():
         if not args.no_cpu_rows:
             cpu["rows"] = cpu_rows(args.cpu_runs)
 
-    extra_mixed = extra_batched = extra_dgemm = extra_serving = extra_chol = None
+    extra_mixed = extra_batched = extra_dgemm = extra_serving = extra_chol = extra_lp = None
     if not args.no_extra:
         extra_dgemm = dgemm_section(B, torch)
+        extra_lp = lp_gemm_section(B, torch)
         extra_chol = chol_section(B, torch)
         extra_mixed = mixed_section(B, torch)
         extra_batched = batched_section(B, torch)
@@ -242,6 +244,7 @@ def main():
                  "gemm_ms_per_step": round(gemm_ms / args.steps, 3),
                  "gemm_launches_per_step": gemm_calls // max(args.steps, 1)},
         "dgemm": extra_dgemm,
+        "lp_gemm": extra_lp,
         "cholesky": extra_chol,
         "mixed_ir": extra_mixed,
         "batched": extra_batched,
@@ -384,6 +387,38 @@ def dgemm_section(B, torch):
         out[f"n{n}"] = {"ms": round(ms, 4), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)}
         print(f"# dgemm {n}: {ms:.3f} ms, {tf:.2f} TF/s", file=sys.stderr, flush=True)
         del a, b, c, c0
+    torch.cuda.empty_cache()
+    return out
+
+
+def lp_gemm_section(B, torch, m=16384, n=16384, k=512, reps=5):
+    """The mixed LU's trailing-update product C -= A B (fp32 storage) at the size of its
+    first bulk update (m = n = 16384, k = 512: two 256-column panels) in each product
+    precision, the operand conversions included; rates in fp32-equivalent TFLOP/s (2 m n k
+    flops) and as a fraction of the matrix peak the products run on (fp32 157.3 TF/s;
+    fp16 / bf16 2.5 PF/s, three fp16 products per fp32-equivalent one for the split mode)."""
+    peaks = {"fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0, "fp16x3": 2500.0 / 3.0}
+    rng = torch.Generator(device="cuda")
+    rng.manual_seed(1)
+    a = torch.rand(m * k, dtype=torch.float32, device="cuda", generator=rng) * 2.0 - 1.0
+    b = torch.rand(k * n, dtype=torch.float32, device="cuda", generator=rng) * 2.0 - 1.0
+    c = torch.zeros(m * n, dtype=torch.float32, device="cuda")
+    out = {"workload": f"C -= A B, m={m} n={n} k={k}, fp32 storage, mean of {reps} calls after a warm-up"}
+    for prec in ("fp32", "fp16", "bf16", "fp16x3"):
+        B.sgemm_lp(m, n, k, a, m, b, k, c, m, prec)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            B.sgemm_lp(m, n, k, a, m, b, k, c, m, prec)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * m * n * k / (ms * 1e-3) / 1e12
+        out[prec] = {"ms": round(ms, 4), "tflops_fp32_equiv": round(tf, 1), "frac_peak": round(tf / peaks[prec], 4)}
+        print(f"# lp_gemm {prec}: {ms:.3f} ms, {tf:.1f} TF/s", file=sys.stderr, flush=True)
+    del a, b, c
     torch.cuda.empty_cache()
     return out
 

@@ -304,21 +304,23 @@ __global__ void cvt_n16(int k, int n, const float* __restrict__ src, size_t lds,
 }
 
 // ---- split fp16 operands (LP_F16X3).  Ath/Atl: A transposed (row i k-contiguous), Bh/Bl:
-// B columns k-contiguous.  BK = 32 elements (one MFMA step) a stage; the four operand
-// tiles are double-buffered in LDS (80 KB).
-__global__ void __launch_bounds__(GT) sgemm_h3(int m, int n, int k, const _Float16* __restrict__ Ath,
-                                               const _Float16* __restrict__ Atl, size_t lda,
-                                               const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
-                                               size_t ldb, float* __restrict__ C, size_t ldc, int tiles_m,
-                                               int tiles_n) {
+// B columns k-contiguous.  256 x 128 tiles, 8 wavefronts (two per SIMD, so one hides the
+// other's waits) of 64 x 64; BK = 32 elements (one MFMA step) a stage, the four operand
+// tiles double-buffered in LDS (120 KB).
+constexpr int H3M = 256, H3N = 128, H3T = 512, H3RW = 20;  // LDS row stride in words: 16 data + 4 pad
+__global__ void __launch_bounds__(H3T) sgemm_h3(int m, int n, int k, const _Float16* __restrict__ Ath,
+                                                const _Float16* __restrict__ Atl, size_t lda,
+                                                const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                                size_t ldb, float* __restrict__ C, size_t ldc, int tiles_m,
+                                                int tiles_n) {
     constexpr int BK = 32;
-    constexpr int RW3 = 20;  // LDS row stride in words: 16 data + 4 pad
-    __shared__ uint32_t S[2][4][GBM * RW3];
+    __shared__ uint32_t SA[2][2][H3M * H3RW];
+    __shared__ uint32_t SB[2][2][H3N * H3RW];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
     const int tm = bid % tiles_m, tn = bid / tiles_m;
-    const int i0 = tm * GBM, j0 = tn * GBN;
-    const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
+    const int i0 = tm * H3M, j0 = tn * H3N;
+    const int wm = (wv & 3) * 64, wn = (wv >> 2) * 64;
     f4 am[4][4], ac[4][4];  // Ah Bh, and Ah Bl + Al Bh
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -327,42 +329,40 @@ __global__ void __launch_bounds__(GT) sgemm_h3(int m, int n, int k, const _Float
             am[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             ac[a][b] = f4{0.f, 0.f, 0.f, 0.f};
         }
-    // a tile is 128 rows x 32 elements = 128 x 4 chunks of 16 bytes; 2 chunks per thread
-    uint4 rr[4][2];
-    const _Float16* src[4] = {Ath, Atl, Bh, Bl};
+    // 16-byte chunks: an A tile is 256 rows x 4 chunks (2 a thread), a B tile 128 x 4 (1)
+    uint4 ra[2][2], rb[2];
+    auto ld8 = [&](const _Float16* src, int gr, int lim, size_t ld, int gk) -> uint4 {
+        if (gr < lim && gk + 8 <= k) return *(const uint4*)(src + (size_t)gr * ld + gk);
+        uint16_t tmp[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            tmp[r] = (gr < lim && gk + r < k) ? ((const uint16_t*)src)[(size_t)gr * ld + gk + r] : (uint16_t)0;
+        return uint4{tmp[0] | ((uint32_t)tmp[1] << 16), tmp[2] | ((uint32_t)tmp[3] << 16),
+                     tmp[4] | ((uint32_t)tmp[5] << 16), tmp[6] | ((uint32_t)tmp[7] << 16)};
+    };
     auto gload = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const int e = tid + q * GT;
-            const int row = e >> 2, ch = (e & 3) * 8;
-            const int gk = k0 + ch;
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int gr = (t < 2 ? i0 : j0) + row;
-                const int lim = t < 2 ? m : n;
-                const size_t ld = t < 2 ? lda : ldb;
-                if (gr < lim && gk + 8 <= k) {
-                    rr[t][q] = *(const uint4*)(src[t] + (size_t)gr * ld + gk);
-                } else {
-                    uint16_t tmp[8];
-#pragma unroll
-                    for (int r = 0; r < 8; r++)
-                        tmp[r] = (gr < lim && gk + r < k) ? ((const uint16_t*)src[t])[(size_t)gr * ld + gk + r]
-                                                          : (uint16_t)0;
-                    rr[t][q] = uint4{tmp[0] | ((uint32_t)tmp[1] << 16), tmp[2] | ((uint32_t)tmp[3] << 16),
-                                     tmp[4] | ((uint32_t)tmp[5] << 16), tmp[6] | ((uint32_t)tmp[7] << 16)};
-                }
-            }
+            const int e = tid + q * H3T;
+            const int row = e >> 2, gk = k0 + (e & 3) * 8;
+            ra[0][q] = ld8(Ath, i0 + row, m, lda, gk);
+            ra[1][q] = ld8(Atl, i0 + row, m, lda, gk);
         }
+        const int row = tid >> 2, gk = k0 + (tid & 3) * 8;
+        rb[0] = ld8(Bh, j0 + row, n, ldb, gk);
+        rb[1] = ld8(Bl, j0 + row, n, ldb, gk);
     };
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const int e = tid + q * GT;
-            const int row = e >> 2, w4 = (e & 3) * 4;
-#pragma unroll
-            for (int t = 0; t < 4; t++) *(uint4*)&S[buf][t][row * RW3 + w4] = rr[t][q];
+            const int e = tid + q * H3T;
+            const int o = (e >> 2) * H3RW + (e & 3) * 4;
+            *(uint4*)&SA[buf][0][o] = ra[0][q];
+            *(uint4*)&SA[buf][1][o] = ra[1][q];
         }
+        const int o = (tid >> 2) * H3RW + (tid & 3) * 4;
+        *(uint4*)&SB[buf][0][o] = rb[0];
+        *(uint4*)&SB[buf][1][o] = rb[1];
     };
     const int nk = (k + BK - 1) / BK;
     gload(0);
@@ -371,54 +371,51 @@ __global__ void __launch_bounds__(GT) sgemm_h3(int m, int n, int k, const _Float
     for (int t = 0; t < nk; t++) {
         const int cur = t & 1;
         if (t + 1 < nk) gload((t + 1) * BK);
-        h8 fah[4], fal[4], fbh[4], fbl[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int o = (wm + 16 * a + (lane & 15)) * RW3 + 4 * (lane >> 4);
-            fah[a] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][0][o]);
-            fal[a] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][1][o]);
-        }
+        h8 fbh[4], fbl[4];
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            const int o = (wn + 16 * b + (lane & 15)) * RW3 + 4 * (lane >> 4);
-            fbh[b] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][2][o]);
-            fbl[b] = __builtin_bit_cast(h8, *(const uint4*)&S[cur][3][o]);
+            const int o = (wn + 16 * b + (lane & 15)) * H3RW + 4 * (lane >> 4);
+            fbh[b] = __builtin_bit_cast(h8, *(const uint4*)&SB[cur][0][o]);
+            fbl[b] = __builtin_bit_cast(h8, *(const uint4*)&SB[cur][1][o]);
         }
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int a = 0; a < 4; a++) {
+            const int o = (wm + 16 * a + (lane & 15)) * H3RW + 4 * (lane >> 4);
+            const h8 fah = __builtin_bit_cast(h8, *(const uint4*)&SA[cur][0][o]);
+            const h8 fal = __builtin_bit_cast(h8, *(const uint4*)&SA[cur][1][o]);
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                am[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[a], fbh[b], am[a][b], 0, 0, 0);
-                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[a], fbl[b], ac[a][b], 0, 0, 0);
-                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal[a], fbh[b], ac[a][b], 0, 0, 0);
+                am[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbh[b], am[a][b], 0, 0, 0);
+                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbl[b], ac[a][b], 0, 0, 0);
+                ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal, fbh[b], ac[a][b], 0, 0, 0);
             }
+        }
         if (t + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
-    // C -= Ah Bh + 2^-11 (Ah Bl + Al Bh), C read from clamped addresses first
-    float cv[4][4][4];
+    // C -= Ah Bh + 2^-11 (Ah Bl + Al Bh), a row block at a time, C read from clamped
+    // addresses first
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 4; a++) {
+        float cv[4][4];
+        const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
-            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
 #pragma unroll
-            for (int r = 0; r < 4; r++) cv[a][b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+            for (int r = 0; r < 4; r++) cv[b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
         }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int j = j0 + wn + 16 * b + (lane & 15);
-            const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
             if (j < n) {
                 float* cp = C + (size_t)j * ldc;
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - (am[a][b][r] + ac[a][b][r] * 0x1p-11f);
+                    if (ib + r < m) cp[ib + r] = cv[b][r] - (am[a][b][r] + ac[a][b][r] * 0x1p-11f);
             }
         }
+    }
 }
 
 __device__ __forceinline__ void split16(float x, _Float16& h, _Float16& l) {
@@ -504,8 +501,9 @@ void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A,
         _Float16* bl = bh + (size_t)n * kp;
         hipLaunchKernelGGL(cvt_t16s, tg, dim3(256), 0, st, m, k, A, lda, ath, atl, kp);
         hipLaunchKernelGGL(cvt_n16s, dim3(cg), dim3(256), 0, st, k, n, B, ldb, bh, bl, kp);
-        hipLaunchKernelGGL(sgemm_h3, dim3(grid), dim3(GT), 0, st, m, n, k, ath, atl, kp, bh, bl, kp, C, ldc, tiles_m,
-                           tiles_n);
+        const int t3m = (m + H3M - 1) / H3M, t3n = (n + H3N - 1) / H3N;
+        hipLaunchKernelGGL(sgemm_h3, dim3((unsigned)(t3m * t3n)), dim3(H3T), 0, st, m, n, k, ath, atl, kp, bh, bl, kp,
+                           C, ldc, t3m, t3n);
     } else if (lp == LP_F16) {
         hipLaunchKernelGGL(cvt_t16<_Float16>, tg, dim3(256), 0, st, m, k, A, lda, (_Float16*)At, kp);
         hipLaunchKernelGGL(cvt_n16<_Float16>, dim3(cg), dim3(256), 0, st, k, n, B, ldb, (_Float16*)Bh, kp);
