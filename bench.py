@@ -9,6 +9,8 @@ p50/p99.
 
 Beside it, in the same JSON line:
   dgemm     -- plain DGEMM 'N','N' at 1024^3 (the reference's bench size) and 8192^3.
+  lp_gemm   -- the mixed LU's trailing-update product C -= A B at m = n = 16384, k = 512 in
+               fp32, fp16, bf16 and split-fp16 products.
   cholesky  -- DPOTRF + DPOTRS ('L' and 'U') at n = 4096 and 8192, nrhs = 64, on
                A = G G^T + n I, median of 5 solves.
   mixed_ir  -- the mixed-precision grid: A X = B at n in {8192, 16384}, kappa in {1e2, 1e5,
@@ -26,7 +28,7 @@ Beside it, in the same JSON line:
   serving   -- the solve server under Poisson arrivals, orders log-uniform in [32, 8192].
 
 The dominant kernel of the headline step is the LU panel (phase GETRF2_IT, the persistent
-wavefront panel kernel); its algorithmic traffic is the panel read and written once per
+panel kernel with two wavefronts a workgroup); its algorithmic traffic is the panel read and written once per
 launch (2 * rows * 32 * 8 bytes), measured with HIP events on the library's stream, against
 the HBM peak.  The events run in a second pass of as many steps as the timed ones: inside
 the timed steps they would add 10 ms or more per step.  The trailing-update DGEMM is
@@ -74,6 +76,10 @@ def main():
     ap.add_argument("--only", choices=["mixed", "batched", "dgemm", "lp_gemm", "serving", "cholesky", "cpu"],
                     default=None,
                     help="run one section alone (for profiling) and print it")
+    ap.add_argument("--mixed-n", type=int, default=0, help="mixed section: this order only (default 8192 and 16384)")
+    ap.add_argument("--mixed-kappa", type=float, default=0.0, help="mixed section: this kappa only")
+    ap.add_argument("--mixed-nrhs", type=int, default=0, help="mixed section: this nrhs only")
+    ap.add_argument("--mixed-modes", default="", help="mixed section: these modes only (comma list, e.g. fp16x3_butterfly)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -96,7 +102,16 @@ def main():
 
     if args.only:
         torch.cuda.set_device(0)
-        sec = {"mixed": mixed_section, "batched": batched_section, "dgemm": dgemm_section, "lp_gemm": lp_gemm_section,
+        mixed_kw = {}
+        if args.mixed_n:
+            mixed_kw["sizes"] = (args.mixed_n,)
+        if args.mixed_kappa:
+            mixed_kw["kappas"] = (args.mixed_kappa,)
+        if args.mixed_nrhs:
+            mixed_kw["nrhss"] = (args.mixed_nrhs,)
+        if args.mixed_modes:
+            mixed_kw["only_modes"] = tuple(args.mixed_modes.split(","))
+        sec = {"mixed": lambda B, torch: mixed_section(B, torch, **mixed_kw), "batched": batched_section, "dgemm": dgemm_section, "lp_gemm": lp_gemm_section,
                "serving": serving_section, "cholesky": chol_section, "cpu": lambda B, torch: dict(cpu_rows(), **cpu_info())}[
             args.only](B, torch)
         print(json.dumps({args.only: sec}))
@@ -228,7 +243,7 @@ def main():
         "config": {"workload": f"DGETRF+DGETRS n={n} nrhs={nrhs} column-major f64", "n": n, "nrhs": nrhs},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "backward_error": berr,
-        "roofline": {"bound": "hbm", "kernel": "getf2_wave (LU panel, 32 columns, persistent)",
+        "roofline": {"bound": "hbm", "kernel": "getf2_duo (LU panel, 32 columns, two wavefronts a workgroup, persistent)",
                      "achieved": round(panel_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(panel_gbs / HBM_PEAK_GBS, 6),
                      "traffic": panel_traffic() if n == 8192 else None,
@@ -351,8 +366,8 @@ def cpu_rows(runs=10, n_lu=4096, n_chol=4096, nrhs=64):
 
 def panel_traffic():
     """HBM bytes per panel launch at n = 8192, from the FETCH_SIZE / WRITE_SIZE counter runs
-    (scaled by a plain copy's counts) recorded in profiles/getf2_wave_lu_n8192_hbm_traffic.md."""
-    path = os.path.join(ROOT, "profiles", "getf2_wave_lu_n8192_hbm_traffic.md")
+    (scaled by a plain copy's counts) recorded in profiles/getf2_duo_lu_n8192_hbm_traffic.md."""
+    path = os.path.join(ROOT, "profiles", "getf2_duo_lu_n8192_hbm_traffic.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -472,7 +487,7 @@ def chol_section(B, torch, sizes=(4096, 8192), nrhs=64, runs=5):
     return out
 
 
-def mixed_section(B, torch, sizes=(8192, 16384), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16), runs=3):
+def mixed_section(B, torch, sizes=(8192, 16384), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16), runs=3, only_modes=()):
     """Mixed-precision IR grid: n in {8192, 16384}, kappa in {1e2, 1e5, 1e8}, nrhs in {1, 16},
     A = U diag(sigma) V^T (seeded Householder products, sigma geometric from 1 to 1/kappa),
     B uniform [-1, 1).  For every point: the plain FP64 GETRF + GETRS and each mode
@@ -483,6 +498,8 @@ def mixed_section(B, torch, sizes=(8192, 16384), kappas=(1e2, 1e5, 1e8), nrhss=(
     from bola_amd import testmat
     modes = [(f"{p}_{'butterfly' if r else 'tournament'}", p, r) for p in ("fp32", "fp16", "bf16", "fp16x3")
              for r in (True, False)]
+    if only_modes:
+        modes = [md for md in modes if md[0] in only_modes]
     out = {"workload": "A X = B, A = U diag(sigma) V^T (Householder products, sigma geometric 1..1/kappa), "
                        "B uniform [-1,1); stop at backward error <= n * eps64; median of "
                        f"{runs} solves after a warm-up",
