@@ -11,12 +11,13 @@
 // (syrk.rs:80, :139); those two combinations are computed here with the BLAS meaning,
 // which is what the reference computes whenever n, k <= 32.
 //
-// POTRF follows the reference's left-looking loop over 32-column blocks: SYRK of the
-// diagonal block against all previous columns, POTRF2 of the block, GEMM of the panel
-// against the previous columns, TRSM with the factored block.  POTRF2 of a block of at
-// most 32 runs as one workgroup in LDS, replaying the reference's recursion (halve;
-// factor the first half; TRSM; SYRK; factor the second half; sqrt at the leaves) with
-// its exact base-case arithmetic.
+// POTRF gives every element the operations of the reference's left-looking loop over
+// 32-column blocks (SYRK of the diagonal block against all previous columns, POTRF2 of the
+// block, GEMM of the panel against the previous columns, TRSM with the factored block),
+// scheduled right-looking: each finished block column pushes its terms into the later ones
+// (see chol_factor).  POTRF2 of a block of at most 32 replays the reference's recursion
+// (halve; factor the first half; TRSM; SYRK; factor the second half; sqrt at the leaves)
+// with its exact base-case arithmetic.
 #include "common.hpp"
 
 namespace bola {
