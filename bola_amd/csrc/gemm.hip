@@ -14,8 +14,9 @@
 //     micro-tiles (gemm.rs:177-190), which it computes unfused as C += (alpha*b)*a in
 //     increasing k; a separate kernel does exactly that for those elements.
 //
-// Tiling: 256 threads = 4 wavefronts in a 2x2 grid; each wavefront owns a
-// (BM/2)x(BN/2) block of 16x16 MFMA tiles.  K advances BK at a time through a
+// Tiling: 256 threads = 4 wavefronts in a 2x2 grid (or, for the 128 x 128 tile of
+// short-k updates, 512 threads = 8 wavefronts in a 2x4 grid); each wavefront owns a
+// (BM/2)x(BN/(NT/128)) block of 16x16 MFMA tiles.  K advances BK at a time through a
 // double-buffered LDS stage: 16 for the large tiles, 64 for 32x32 tiles whose slab is a
 // multiple of 64 (the latency-bound narrow products of the triangular solves, where each
 // step is one global-load round trip).  LDS layouts put each MFMA operand read on 32 distinct
@@ -31,29 +32,31 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 64;  // reference slab depth
 
-template <int BM, int BN, bool TA, bool TB, int BK>
+template <int BM, int BN, bool TA, bool TB, int BK, int NT = 256>
 struct GemmCfg {
-    static constexpr int WM = BM / 2, WN = BN / 2;  // wavefront tile
+    // NT / 64 wavefronts: two along m, the rest along n
+    static constexpr int WGN = NT / 128;
+    static constexpr int WM = BM / 2, WN = BN / WGN;  // wavefront tile
     static constexpr int TM = WM / 16, TN = WN / 16;  // MFMA tiles per wavefront
     // LDS strides (doubles)
     static constexpr int A_STRIDE = TA ? (BK + 1) : BM;   // TA: [m][BK+1]; else [k][BM]
     static constexpr int A_SIZE = TA ? BM * (BK + 1) : BK * BM;
     static constexpr int B_STRIDE = TB ? BN : (BK + 1);   // TB: [k][BN]; else [n][BK+1]
     static constexpr int B_SIZE = TB ? BK * BN : BN * (BK + 1);
-    static constexpr int A_LOADS = BM * BK / 256;
-    static constexpr int B_LOADS = BN * BK / 256;
+    static constexpr int A_LOADS = BM * BK / NT;
+    static constexpr int B_LOADS = BN * BK / NT;
 };
 
 // Element (i, kk) of the BMxBK tile of op(A) handled by thread t for load slot s.
-template <int BM, bool TA, int BK>
+template <int BM, bool TA, int BK, int NT>
 __device__ __forceinline__ void a_coord(int t, int s, int& i, int& kk) {
-    int e = t + 256 * s;
+    int e = t + NT * s;
     if (!TA) { i = e % BM; kk = e / BM; }       // column-major A: m fastest
     else { kk = e % BK; i = e / BK; }           // A^T: k fastest
 }
-template <int BN, bool TB, int BK>
+template <int BN, bool TB, int BK, int NT>
 __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
-    int e = t + 256 * s;
+    int e = t + NT * s;
     if (!TB) { kk = e % BK; j = e / BK; }       // column-major B: k fastest
     else { j = e % BN; kk = e / BN; }           // B^T: n fastest
 }
@@ -61,12 +64,12 @@ __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
 // OCC: workgroups per CU the register budget is fitted to.  2 for stand-alone products;
 // 1 (more registers, lighter on the machine) for the updates that run beside a
 // factorization's latency-bound panel kernel.  BK divides the caller's slab.
-template <int BM, int BN, bool TA, bool TB, int OCC, int BK>
-__global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
+template <int BM, int BN, bool TA, bool TB, int OCC, int BK, int NT = 256>
+__global__ void __launch_bounds__(NT, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
                                                   int edge_m, int edge_n, Abort abort, int slab, int mask) {
-    using Cfg = GemmCfg<BM, BN, TA, TB, BK>;
+    using Cfg = GemmCfg<BM, BN, TA, TB, BK, NT>;
     if (k > 0) {  // behind a failed pivot: only the k columns of panels before it
         k = abort_count(abort, k, 1);
         if (k == 0) return;
@@ -129,7 +132,7 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
-            a_coord<BM, TA, BK>(t, s, i, kk);
+            a_coord<BM, TA, BK, NT>(t, s, i, kk);
             int gi = row0 + i, gk = k0 + kk;
             const int ci = min(gi, m - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
             const double v = TA ? A[ck + (size_t)ci * lda] : A[ci + (size_t)ck * lda];
@@ -138,7 +141,7 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
-            b_coord<BN, TB, BK>(t, s, kk, j);
+            b_coord<BN, TB, BK, NT>(t, s, kk, j);
             int gj = col0 + j, gk = k0 + kk;
             const int cj = min(gj, n - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
             const double v = TB ? B[cj + (size_t)ck * ldb] : B[ck + (size_t)cj * ldb];
@@ -149,14 +152,14 @@ __global__ void __launch_bounds__(256, OCC) dgemm_mfma(int m, int n, int k, doub
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
-            a_coord<BM, TA, BK>(t, s, i, kk);
+            a_coord<BM, TA, BK, NT>(t, s, i, kk);
             if (!TA) sA[buf][kk * BM + i] = ra[s];
             else sA[buf][i * (BK + 1) + kk] = ra[s];
         }
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
-            b_coord<BN, TB, BK>(t, s, kk, j);
+            b_coord<BN, TB, BK, NT>(t, s, kk, j);
             if (!TB) sB[buf][j * (BK + 1) + kk] = rb[s];
             else sB[buf][kk * BN + j] = rb[s];
         }
@@ -298,12 +301,15 @@ __global__ void __launch_bounds__(64) dgemm_edge(int m, int n, int k, double alp
     }
 }
 
-template <int BM, int BN, bool TA, bool TB, int BK = 16>
+template <int BM, int BN, bool TA, bool TB, int BK = 16, int NT = 256>
 void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
                  size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, Abort abort,
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
-    if (fast)
+    if (NT == 512)  // one workgroup of 8 wavefronts per CU (two per SIMD)
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK, NT>), dim3(tm * tn), dim3(NT), 0, st, m, n, k, alpha,
+                           A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
+    else if (fast)
         hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : BK == 32 ? 4 : 3), BK>),
                            dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn,
                            edge_m, edge_n, abort, slab, mask);
@@ -322,7 +328,14 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
         const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
         // 64-deep steps need fold points on multiples of 64 and at least one full step
         const bool deep = slab % 64 == 0 && k >= 64;
-        if (t128 >= 2L * num_cus)
+        // short-k updates (the LU's trailing updates, k = 256) with at least four 128 x 128
+        // tiles per CU: 8-wavefront tiles, a third less operand traffic per flop (long-k
+        // products run faster on the 128 x 64 tile at two workgroups per CU)
+        const long t128x128 = (long)((m + 127) / 128) * ((n + 127) / 128);
+        if (mask == 0 && k <= 1024 && t128x128 >= 4L * num_cus)
+            launch_tile<128, 128, TA, TB, 16, 512>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n,
+                                                   abort, slab, mask, fast);
+        else if (t128 >= 2L * num_cus)
             launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                          slab, mask, fast);
         else if (deep && fast && t64 < 2L * num_cus)  // stand-alone: below 2 64x64 tiles per CU

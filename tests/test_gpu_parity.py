@@ -225,6 +225,23 @@ def test_gemm_large_tiles_bitexact(B, O, ta, tb, alpha, beta):
     same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (2.5, -0.3)])
+def test_gemm_128x128_tiles_bitexact(B, O, ta, tb, alpha, beta):
+    # products with at least four 128 x 128 tiles per CU take the 8-wavefront tile; ragged
+    # edges (m % 8, n % 4 nonzero) and k = 150
+    m, n, k = 4096 + 37, 4096 + 9, 150
+    rng = np.random.default_rng(11 + (ta == "T") * 2 + (tb == "T"))
+    a, lda = _rand_op(rng, ta, m, k, 3)
+    b, ldb = _rand_op(rng, tb, k, n, 1)
+    ldc = m + 2
+    c = rng.uniform(-1, 1, ldc * n)
+    ref = c.copy()
+    O.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, ref, ldc)
+    B.gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+    same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
+
+
 def test_gemm_alpha_zero_beta_rules(B, O):
     c = np.array([np.nan, 1.0, 2.0, 3.0])
     B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
@@ -312,6 +329,22 @@ def test_syrk_bitexact(B, O, uplo, trans, n, k, beta):
     ref = c.copy()
     O.syrk(uplo, trans, n, k, -1.0, a, ar, beta, ref, n + 1)
     B.syrk(uplo, trans, n, k, -1.0, a, ar, beta, c, n + 1)
+    same(c, ref, f"syrk {uplo}{trans} {n}x{k}")
+
+
+@pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T")])
+def test_syrk_large_bitexact(B, O, uplo, trans):
+    # off-diagonal blocks on the masked MFMA GEMM at an order with at least four 128 x 128
+    # tiles per CU, ragged (n % 32 = 8, k % 32 = 0 and k % 64 = 32)
+    n, k = 4200, 96
+    rng = np.random.default_rng(n + k + (uplo == "U"))
+    rows = n if trans == "N" else k
+    cols = k if trans == "N" else n
+    a = rng.uniform(-1, 1, rows * cols)
+    c = rng.uniform(-1, 1, n * n)
+    ref = c.copy()
+    O.syrk(uplo, trans, n, k, -1.0, a, rows, 1.0, ref, n)
+    B.syrk(uplo, trans, n, k, -1.0, a, rows, 1.0, c, n)
     same(c, ref, f"syrk {uplo}{trans} {n}x{k}")
 
 
