@@ -328,11 +328,11 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
         const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
         // 64-deep steps need fold points on multiples of 64 and at least one full step
         const bool deep = slab % 64 == 0 && k >= 64;
-        // short-k updates (the LU's trailing updates, k = 256) with at least four 128 x 128
-        // tiles per CU: 8-wavefront tiles, a third less operand traffic per flop (long-k
-        // products run faster on the 128 x 64 tile at two workgroups per CU)
+        // short-k updates (the LU's and the Cholesky's trailing updates, k = 256) with at
+        // least four 128 x 128 tiles per CU: 8-wavefront tiles, a third less operand traffic
+        // per flop (long-k products run faster on the 128 x 64 tile at two workgroups per CU)
         const long t128x128 = (long)((m + 127) / 128) * ((n + 127) / 128);
-        if (mask == 0 && k <= 1024 && t128x128 >= 4L * num_cus)
+        if (k <= 1024 && t128x128 >= 4L * num_cus)
             launch_tile<128, 128, TA, TB, 16, 512>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n,
                                                    abort, slab, mask, fast);
         else if (t128 >= 2L * num_cus)

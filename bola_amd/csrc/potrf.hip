@@ -447,12 +447,16 @@ __device__ __forceinline__ bool p2_node(double (*s)[SB + 1], double* rinv, int* 
 // one thread per row) or the block row right of it (U: trsm('L','U','T','N'), one
 // thread per column), each thread running the reference's 32-wide base-case sequence
 // (trsm.rs:252-376 / :213-241).  Every workgroup factors the diagonal block itself (the
-// same bits everywhere), workgroup 0 writes it back, and each workgroup then solves its
-// 64 rows (columns); a failing pivot writes the block as left by the failing step and
-// the error, and no TRSM runs.
+// same bits everywhere) and then solves its 64 rows (columns); a failing pivot leaves the
+// block as left by the failing step, sets the error, and no TRSM runs.  The factored
+// block goes to `dout` (32 x 32, ld 32) with `flag` = c + 1, never in place: a workgroup
+// that starts after another has finished must still read the unfactored block (on a busy
+// GPU the workgroups of one launch start at different times); chol_diag_put copies it into
+// place after the launch.
 template <bool UP>
 __global__ void __launch_bounds__(64) chol_panel(int n, int c, double* __restrict__ A, size_t lda,
-                                                  int32_t* __restrict__ info) {
+                                                  int32_t* __restrict__ info, double* __restrict__ dout,
+                                                  int* __restrict__ flag) {
     if (*info != 0) return;
     __shared__ double s[SB][SB + 1];  // s[r][c]
     __shared__ double rinv[SB];
@@ -464,10 +468,13 @@ __global__ void __launch_bounds__(64) chol_panel(int n, int c, double* __restric
     __syncthreads();
     const bool failed = p2_node<UP, SB>(s, rinv, &err, 0, tid);
     if (blockIdx.x == 0) {
-        for (int e = tid; e < SB * SB; e += 64) D[(e % SB) + (size_t)(e / SB) * lda] = s[e % SB][e / SB];
-        if (tid == 0 && failed) {
-            info[1] = err;
-            __hip_atomic_store(info, c + err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int e = tid; e < SB * SB; e += 64) dout[e] = s[e % SB][e / SB];
+        if (tid == 0) {
+            *flag = c + 1;
+            if (failed) {
+                info[1] = err;
+                __hip_atomic_store(info, c + err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
     if (failed) return;
@@ -505,6 +512,14 @@ __global__ void __launch_bounds__(64) chol_panel(int n, int c, double* __restric
 #pragma unroll
         for (int q = 0; q < SB; q++) col[q] = x[q];
     }
+}
+
+// the diagonal block chol_panel factored at column c, into place (nothing when that launch
+// returned at once behind an earlier failure: flag is then not c + 1)
+__global__ void __launch_bounds__(256) chol_diag_put(int c, double* __restrict__ A, size_t lda,
+                                                     const double* __restrict__ dout, const int* __restrict__ flag) {
+    if (*flag != c + 1) return;
+    for (int e = threadIdx.x; e < SB * SB; e += 256) A[(size_t)(c + e % SB) + (size_t)(c + e / SB) * lda] = dout[e];
 }
 
 // ---- the blocked factorization, right-looking with the reference's arithmetic
@@ -698,7 +713,13 @@ void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t ld
     const int up = uplo == 'U' ? 1 : 0;
     hipStream_t s2 = h->aux_stream();
     const unsigned sgrid = (unsigned)std::min<size_t>((n * n + 255) / 256, 4096);
-    double* W = (double*)h->cws.get(n * n * sizeof(double));
+    // the save region, then the factored diagonal block of the current step and its flag
+    const size_t wbytes = (n * n * sizeof(double) + 255) & ~(size_t)255;
+    double* W = (double*)h->cws.get(wbytes + SB * SB * sizeof(double) + 256);
+    double* dout = (double*)((char*)W + wbytes);
+    int* dflag = (int*)(dout + SB * SB);
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, st, dflag);
+    BOLA_LAUNCH_CHECK();
     hipLaunchKernelGGL(chol_save, dim3(sgrid), dim3(256), 0, st, up, N, a, lda, W);
     BOLA_LAUNCH_CHECK();
     // the reference's GEMM edges (gemm.rs:177-190) for every block step: L -- the last n % 8
@@ -744,8 +765,10 @@ void chol_factor(bola_hip_handle_s* h, char uplo, size_t n, double* a, size_t ld
             if (jb == SB) {
                 Phase p(h, "POTRF2", (double)jb * jb * n_rem, st);  // POTRF2 and the TRSM below it
                 const unsigned g = (unsigned)std::max(1, (n_rem + 63) / 64);
-                if (up) hipLaunchKernelGGL(chol_panel<true>, dim3(g), dim3(64), 0, st, N, c, a, lda, info);
-                else hipLaunchKernelGGL(chol_panel<false>, dim3(g), dim3(64), 0, st, N, c, a, lda, info);
+                if (up) hipLaunchKernelGGL(chol_panel<true>, dim3(g), dim3(64), 0, st, N, c, a, lda, info, dout, dflag);
+                else hipLaunchKernelGGL(chol_panel<false>, dim3(g), dim3(64), 0, st, N, c, a, lda, info, dout, dflag);
+                BOLA_LAUNCH_CHECK();
+                hipLaunchKernelGGL(chol_diag_put, dim3(1), dim3(256), 0, st, c, a, lda, dout, dflag);
                 BOLA_LAUNCH_CHECK();
             } else {  // the last, partial block: nothing below it
                 Phase p(h, "POTRF2", 0.0, st);
