@@ -22,6 +22,10 @@
 // step is one global-load round trip).  LDS layouts put each MFMA operand read on 32 distinct
 // banks: A as [k][BM] (or [m][BK+1] when op(A) = A^T), B as [n][BK+1] (or [k][BN] when
 // op(B) = B^T).  Block ids are remapped so each XCD's L2 serves a compact C region.
+// Plain products made of whole 128 x 64 (or, short-k, 128 x 128) tiles skip the staging
+// registers: gfx950's global_load_lds_dwordx4 writes each operand stage straight into one of
+// three LDS buffers, so loads run two stages ahead with no registers held for them (the
+// 128 x 64 tile at 8192^3 goes from 60 % to 82 % of the FP64 matrix peak).
 #include "common.hpp"
 
 namespace bola {
@@ -64,7 +68,13 @@ __device__ __forceinline__ void b_coord(int t, int s, int& kk, int& j) {
 // OCC: workgroups per CU the register budget is fitted to.  2 for stand-alone products;
 // 1 (more registers, lighter on the machine) for the updates that run beside a
 // factorization's latency-bound panel kernel.  BK divides the caller's slab.
-template <int BM, int BN, bool TA, bool TB, int OCC, int BK, int NT = 256>
+// DL: operands go straight from global memory into LDS (global_load_lds_dwordx4, a
+// wavefront writes 1 KB of LDS per instruction) through three LDS stages, with no staging
+// registers: A as [k][BM] (a 128-row column of A per instruction), B as [n][BK] with the
+// 16-byte units of each column XOR-swizzled by the column index (unit u of column j at
+// position u ^ (j % 8)), so the operand reads of sixteen consecutive columns fall on
+// distinct banks.  Only whole tiles of op(A) = A, op(B) = B (the launcher checks).
+template <int BM, int BN, bool TA, bool TB, int OCC, int BK, int NT = 256, bool DL = false>
 __global__ void __launch_bounds__(NT, OCC) dgemm_mfma(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                   size_t lda, const double* __restrict__ B, size_t ldb, double beta,
                                                   double* __restrict__ C, size_t ldc, int tiles_m, int tiles_n,
@@ -74,8 +84,10 @@ __global__ void __launch_bounds__(NT, OCC) dgemm_mfma(int m, int n, int k, doubl
         k = abort_count(abort, k, 1);
         if (k == 0) return;
     }
-    __shared__ double sA[2][Cfg::A_SIZE];
-    __shared__ double sB[2][Cfg::B_SIZE];
+    static_assert(!DL || (!TA && !TB && BM == 128 && BK == 16 && ((BN == 64 && NT == 256) || (BN == 128 && NT == 512))),
+                  "direct-load shape");
+    __shared__ __attribute__((aligned(16))) double sA[DL ? 1 : 2][Cfg::A_SIZE];
+    __shared__ __attribute__((aligned(16))) double sB[DL ? 1 : 2][Cfg::B_SIZE];
 
     // XCD-aware block remap: consecutive tile indices land on the same XCD.
     const int ntiles = tiles_m * tiles_n;
@@ -126,59 +138,62 @@ __global__ void __launch_bounds__(NT, OCC) dgemm_mfma(int m, int n, int k, doubl
                 for (int r = 0; r < 4; r++) crun[a][b][r] = crun[a][b][r] * beta;
 
     // Two register stages: the global loads for step t+2 are issued while step t computes,
-    // so each load has two steps of MFMA work to arrive in.
+    // so each load has two steps of MFMA work to arrive in.  The loads are raw (clamped
+    // addresses); out-of-range elements are zeroed only when the stage is stored to LDS,
+    // after the compute step, so that no instruction waits on a load before then.
     double ra0[Cfg::A_LOADS], rb0[Cfg::B_LOADS], ra1[Cfg::A_LOADS], rb1[Cfg::B_LOADS];
     auto gload = [&](int k0, double (&ra)[Cfg::A_LOADS], double (&rb)[Cfg::B_LOADS]) {
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
             a_coord<BM, TA, BK, NT>(t, s, i, kk);
-            int gi = row0 + i, gk = k0 + kk;
-            const int ci = min(gi, m - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
-            const double v = TA ? A[ck + (size_t)ci * lda] : A[ci + (size_t)ck * lda];
-            ra[s] = keep_or_zero(v, gi < m && gk < k);
+            const int ci = min(row0 + i, m - 1), ck = min(k0 + kk, k - 1);  // clamped: loads in flight together
+            ra[s] = TA ? A[ck + (size_t)ci * lda] : A[ci + (size_t)ck * lda];
         }
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
             b_coord<BN, TB, BK, NT>(t, s, kk, j);
-            int gj = col0 + j, gk = k0 + kk;
-            const int cj = min(gj, n - 1), ck = min(gk, k - 1);  // clamped: loads in flight together
-            const double v = TB ? B[cj + (size_t)ck * ldb] : B[ck + (size_t)cj * ldb];
-            rb[s] = keep_or_zero(v, gj < n && gk < k);
+            const int cj = min(col0 + j, n - 1), ck = min(k0 + kk, k - 1);
+            rb[s] = TB ? B[cj + (size_t)ck * ldb] : B[ck + (size_t)cj * ldb];
         }
     };
-    auto lstore = [&](int buf, const double (&ra)[Cfg::A_LOADS], const double (&rb)[Cfg::B_LOADS]) {
+    auto lstore = [&](int buf, int k0, const double (&ra)[Cfg::A_LOADS], const double (&rb)[Cfg::B_LOADS]) {
+        // the whole stage inside the matrices (the common case): no masks
+        const bool full = row0 + BM <= m && col0 + BN <= n && k0 + BK <= k;
 #pragma unroll
         for (int s = 0; s < Cfg::A_LOADS; s++) {
             int i, kk;
             a_coord<BM, TA, BK, NT>(t, s, i, kk);
-            if (!TA) sA[buf][kk * BM + i] = ra[s];
-            else sA[buf][i * (BK + 1) + kk] = ra[s];
+            const double v = full ? ra[s] : keep_or_zero(ra[s], row0 + i < m && k0 + kk < k);
+            if (!TA) sA[buf][kk * BM + i] = v;
+            else sA[buf][i * (BK + 1) + kk] = v;
         }
 #pragma unroll
         for (int s = 0; s < Cfg::B_LOADS; s++) {
             int kk, j;
             b_coord<BN, TB, BK, NT>(t, s, kk, j);
-            if (!TB) sB[buf][j * (BK + 1) + kk] = rb[s];
-            else sB[buf][kk * BN + j] = rb[s];
+            const double v = full ? rb[s] : keep_or_zero(rb[s], col0 + j < n && k0 + kk < k);
+            if (!TB) sB[buf][j * (BK + 1) + kk] = v;
+            else sB[buf][kk * BN + j] = v;
         }
     };
     const int nsteps = (k + BK - 1) / BK;
     // one BK-deep step from LDS buffer buf, then the slab fold when a slab (or k) ends
-    auto compute = [&](int buf, int step) {
+    auto compute = [&](const double* pa, const double* pb, int step) {
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             double fa[Cfg::TM], fb[Cfg::TN];
 #pragma unroll
             for (int a = 0; a < Cfg::TM; a++) {
                 int i = wm + 16 * a + lr, kk = k4 + lk;
-                fa[a] = TA ? sA[buf][i * (BK + 1) + kk] : sA[buf][kk * BM + i];
+                fa[a] = TA ? pa[i * (BK + 1) + kk] : pa[kk * BM + i];
             }
 #pragma unroll
             for (int b = 0; b < Cfg::TN; b++) {
                 int j = wn + 16 * b + lr, kk = k4 + lk;
-                fb[b] = TB ? sB[buf][kk * BN + j] : sB[buf][j * (BK + 1) + kk];
+                if (DL) fb[b] = pb[j * BK + 2 * ((kk >> 1) ^ (j & 7)) + (kk & 1)];
+                else fb[b] = TB ? pb[kk * BN + j] : pb[j * (BK + 1) + kk];
             }
 #pragma unroll
             for (int a = 0; a < Cfg::TM; a++)
@@ -200,26 +215,73 @@ __global__ void __launch_bounds__(NT, OCC) dgemm_mfma(int m, int n, int k, doubl
         }
     };
 
+    if constexpr (DL) {
+        // three stages in separate LDS objects, so that the compiler's bookkeeping of the
+        // direct loads in flight lets a stage be read while the next ones are still loading
+        __shared__ __attribute__((aligned(16))) double dA0[BK * BM], dA1[BK * BM], dA2[BK * BM];
+        __shared__ __attribute__((aligned(16))) double dB0[BN * BK], dB1[BN * BK], dB2[BN * BK];
+        typedef __attribute__((address_space(3))) void* lds_ptr;
+        // a wavefront's share of a stage, in 1 KB direct loads: AQ columns of A (BM = 128
+        // doubles each), BQ groups of 8 columns of B (16 doubles each)
+        constexpr int NW = NT / 64, AQ = BK / NW, BQ = BN / 8 / NW;
+        static_assert(AQ + BQ <= 15, "vmcnt field");
+        // stage <- k0 .. k0+BK: A columns AQ w .. AQ w + AQ-1; B columns 8(BQ w + q) + lane/8,
+        // unit lane % 8 of each holding rows 2u, 2u+1 with u = (lane % 8) ^ (column % 8)
+        auto dissue = [&](double* la, double* lb, int k0) {
+#pragma unroll
+            for (int q = 0; q < AQ; q++) {
+                const int kk = AQ * w + q;
+                __builtin_amdgcn_global_load_lds(A + (size_t)row0 + 2 * lane + (size_t)(k0 + kk) * lda,
+                                                 (lds_ptr)(la + kk * BM), 16, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < BQ; q++) {
+                const int jb = 8 * (BQ * w + q), j = jb + (lane >> 3), u = (lane & 7) ^ (j & 7);
+                __builtin_amdgcn_global_load_lds(B + (size_t)(k0 + 2 * u) + (size_t)(col0 + j) * ldb,
+                                                 (lds_ptr)(lb + jb * BK), 16, 0, 0);
+            }
+        };
+        // one step: its stage (ca, cb) has landed when at most the next stage's AQ + BQ loads
+        // per lane are pending; after the barrier every wavefront's part is in and the stage
+        // read in the previous step (ia, ib) can take step + 2
+        auto body = [&](int st, const double* ca, const double* cb, double* ia, double* ib) {
+            if (st + 1 < nsteps) __builtin_amdgcn_s_waitcnt(0x0F70 | (AQ + BQ));  // vmcnt(AQ + BQ)
+            else __builtin_amdgcn_s_waitcnt(0x0F70);                               // vmcnt(0)
+            __builtin_amdgcn_s_barrier();
+            if (st + 2 < nsteps) dissue(ia, ib, (st + 2) * BK);
+            compute(ca, cb, st);
+        };
+        if (nsteps > 0) dissue(dA0, dB0, 0);
+        if (nsteps > 1) dissue(dA1, dB1, BK);
+        for (int step = 0; step < nsteps; step += 3) {
+            body(step, dA0, dB0, dA2, dB2);
+            if (step + 1 >= nsteps) break;
+            body(step + 1, dA1, dB1, dA0, dB0);
+            if (step + 2 >= nsteps) break;
+            body(step + 2, dA2, dB2, dA1, dB1);
+        }
+    } else {
     if (nsteps > 0) {  // k == 0: C = beta C, and A, B are not read
         gload(0, ra0, rb0);
-        lstore(0, ra0, rb0);
+        lstore(0, 0, ra0, rb0);
         if (nsteps > 1) gload(BK, ra1, rb1);
     }
     __syncthreads();
     for (int step = 0; step < nsteps; step += 2) {
         // even step: LDS 0 holds step, registers 1 hold step + 1
         if (step + 2 < nsteps) gload((step + 2) * BK, ra0, rb0);
-        compute(0, step);
+        compute(sA[0], sB[0], step);
         if (step + 1 >= nsteps) break;
-        lstore(1, ra1, rb1);
+        lstore(1, (step + 1) * BK, ra1, rb1);
         __syncthreads();
         // odd step: LDS 1 holds step + 1, registers 0 hold step + 2
         if (step + 3 < nsteps) gload((step + 3) * BK, ra1, rb1);
-        compute(1, step + 1);
+        compute(sA[1], sB[1], step + 1);
         if (step + 2 < nsteps) {
-            lstore(0, ra0, rb0);
+            lstore(0, (step + 2) * BK, ra0, rb0);
             __syncthreads();
         }
+    }
     }
 
 #pragma unroll
@@ -301,22 +363,36 @@ __global__ void __launch_bounds__(64) dgemm_edge(int m, int n, int k, double alp
     }
 }
 
-template <int BM, int BN, bool TA, bool TB, int BK = 16, int NT = 256>
+template <int BM, int BN, bool TA, bool TB, int BK = 16, int NT = 256, bool DL = false>
 void launch_tile(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
                  size_t ldb, double beta, double* C, size_t ldc, int edge_m, int edge_n, Abort abort,
                  int slab, int mask, bool fast) {
     int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
-    if (NT == 512)  // one workgroup of 8 wavefronts per CU (two per SIMD)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK, NT>), dim3(tm * tn), dim3(NT), 0, st, m, n, k, alpha,
+    if constexpr (NT == 512)  // one workgroup of 8 wavefronts per CU (two per SIMD)
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK, NT, DL>), dim3(tm * tn), dim3(NT), 0, st, m, n, k, alpha,
                            A, lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
     else if (fast)
-        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : BK == 32 ? 4 : 3), BK>),
+        hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, (BM == 128 ? 2 : BK == 64 ? 2 : BK == 32 ? 4 : 3), BK, 256, DL>),
                            dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn,
                            edge_m, edge_n, abort, slab, mask);
     else
         hipLaunchKernelGGL((dgemm_mfma<BM, BN, TA, TB, 1, BK>), dim3(tm * tn), dim3(256), 0, st, m, n, k, alpha, A,
                            lda, B, ldb, beta, C, ldc, tm, tn, edge_m, edge_n, abort, slab, mask);
     BOLA_LAUNCH_CHECK();
+}
+
+// the reference's partial micro-tiles of a product (rows >= edge_m, columns >= edge_n)
+template <bool TA, bool TB>
+void launch_edge(hipStream_t st, int m, int n, int k, double alpha, const double* A, size_t lda, const double* B,
+                 size_t ldb, double beta, double* C, size_t ldc, Abort abort, int mask, int edge_m, int edge_n) {
+    long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
+    if (nedge > 0) {
+        // one wavefront per workgroup, so that a few thousand elements spread over the CUs
+        unsigned blocks = (unsigned)std::min<long>((nedge + 63) / 64, 16384);
+        hipLaunchKernelGGL((dgemm_edge<TA, TB>), dim3(blocks), dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, edge_m, edge_n, abort, mask);
+        BOLA_LAUNCH_CHECK();
+    }
 }
 
 template <bool TA, bool TB>
@@ -332,12 +408,36 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
         // least four 128 x 128 tiles per CU: 8-wavefront tiles, a third less operand traffic
         // per flop (long-k products run faster on the 128 x 64 tile at two workgroups per CU)
         const long t128x128 = (long)((m + 127) / 128) * ((n + 127) / 128);
-        if (k <= 1024 && t128x128 >= 4L * num_cus)
-            launch_tile<128, 128, TA, TB, 16, 512>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n,
-                                                   abort, slab, mask, fast);
-        else if (t128 >= 2L * num_cus)
-            launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
-                                         slab, mask, fast);
+        // whole tiles of A B (m % 128, n % BN, k % 16 zero) on 16-byte aligned operands take
+        // the direct-to-LDS loads
+        const bool dl_ok = m % 128 == 0 && k % 16 == 0 && slab % 16 == 0 && lda % 2 == 0 && ldb % 2 == 0 &&
+                           ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+        if (k <= 1024 && t128x128 >= 4L * num_cus) {
+            bool done = false;
+            if constexpr (!TA && !TB) {
+                if (dl_ok && n % 128 == 0) {
+                    launch_tile<128, 128, TA, TB, 16, 512, true>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                                 edge_m, edge_n, abort, slab, mask, fast);
+                    done = true;
+                }
+            }
+            if (!done)
+                launch_tile<128, 128, TA, TB, 16, 512>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m,
+                                                       edge_n, abort, slab, mask, fast);
+        } else if (t128 >= 2L * num_cus) {
+            const bool dl = fast && dl_ok && n % 64 == 0;
+            bool done = false;
+            if constexpr (!TA && !TB) {
+                if (dl) {
+                    launch_tile<128, 64, TA, TB, 16, 256, true>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                                edge_m, edge_n, abort, slab, mask, fast);
+                    done = true;
+                }
+            }
+            if (!done)
+                launch_tile<128, 64, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
+                                             slab, mask, fast);
+        }
         else if (deep && fast && t64 < 2L * num_cus)  // stand-alone: below 2 64x64 tiles per CU
             launch_tile<32, 32, TA, TB, 32>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort,
                                             slab, mask, fast);
@@ -351,14 +451,7 @@ void launch_t(hipStream_t st, int m, int n, int k, double alpha, const double* A
             launch_tile<32, 32, TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, edge_m, edge_n, abort, slab,
                                         mask, fast);
     }
-    long nedge = (long)(m - edge_m) * n + (long)edge_m * (n - edge_n);
-    if (nedge > 0) {
-        // one wavefront per workgroup, so that a few thousand elements spread over the CUs
-        unsigned blocks = (unsigned)std::min<long>((nedge + 63) / 64, 16384);
-        hipLaunchKernelGGL((dgemm_edge<TA, TB>), dim3(blocks), dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
-                           C, ldc, edge_m, edge_n, abort, mask);
-        BOLA_LAUNCH_CHECK();
-    }
+    launch_edge<TA, TB>(st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort, mask, edge_m, edge_n);
 }
 
 int g_num_cus = 256;

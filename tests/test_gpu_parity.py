@@ -242,6 +242,25 @@ def test_gemm_128x128_tiles_bitexact(B, O, ta, tb, alpha, beta):
     same(c, ref, f"gemm {ta}{tb} {m}x{n}x{k}")
 
 
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (2.5, -0.3), (0.7, 0.0)])
+@pytest.mark.parametrize("m,n,k", [(2048, 2048, 208), (2048, 2048, 1040), (4096, 4096, 256)])
+def test_gemm_direct_load_tiles_bitexact(B, O, alpha, beta, m, n, k):
+    # whole tiles of a plain product with even leading dimensions and k % 16 == 0 take the
+    # direct-to-LDS kernels (three LDS stages, swizzled B columns): 128 x 64 tiles for the
+    # 2048^2 products (k = 208 ends in a partial 64-deep slab, k = 1040 runs many stages
+    # round the three buffers), 128 x 128 tiles on 8 wavefronts for the short-k 4096^2
+    # product (the shape of the LU's trailing updates)
+    lda, ldb, ldc = m + 2, k + 4, m + 6
+    rng = np.random.default_rng(k + int(10 * alpha))
+    a = rng.uniform(-1, 1, lda * k)
+    b = rng.uniform(-1, 1, ldb * n)
+    c = rng.uniform(-1, 1, ldc * n)
+    ref = c.copy()
+    O.gemm("N", "N", m, n, k, alpha, a, lda, b, ldb, beta, ref, ldc)
+    B.gemm("N", "N", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+    same(c, ref, f"gemm NN {m}x{n}x{k} direct loads")
+
+
 def test_gemm_alpha_zero_beta_rules(B, O):
     c = np.array([np.nan, 1.0, 2.0, 3.0])
     B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
