@@ -40,6 +40,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return (bid % 8) * per + bid / 8;
 }
 
+// A lane's four accumulator entries of an MFMA 16 x 16 tile are rows ib .. ib+3 of one column
+// j of C: 16-byte accesses when they lie inside the matrix and C's columns are 16-byte
+// aligned (c16), element by element (rows clamped on load, masked on store) otherwise.
+__device__ __forceinline__ f4 ld_c4(const float* __restrict__ C, size_t ldc, int ib, int j, int m, bool c16) {
+    if (c16 && ib + 3 < m) return *(const f4*)(C + (size_t)ib + (size_t)j * ldc);
+    f4 v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+    return v;
+}
+__device__ __forceinline__ void st_c4(float* __restrict__ C, size_t ldc, int ib, int j, int m, bool c16, const f4& v) {
+    float* cp = C + (size_t)ib + (size_t)j * ldc;
+    if (c16 && ib + 3 < m) {
+        *(f4*)cp = v;
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (ib + r < m) cp[r] = v[r];
+}
+
 // ---- fp32 operands.  A column-major (lda), B column-major (ldb).  BK = 32 floats.
 __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float* __restrict__ A, size_t lda,
                                                 const float* __restrict__ B, size_t ldb, float* __restrict__ C,
@@ -54,6 +75,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
     const int wm = (wv & 1) * 64, wn = (wv >> 1) * 64;
     // The accumulators start from C (clamped addresses; zero with overwrite) and A enters
     // negated, so they end as C - A B.  D layout: reg r of lane l -> (row 4*(l/16)+r, col l%16).
+    const bool c16 = (ldc & 3) == 0 && ((uintptr_t)C & 15) == 0;
     f4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -61,9 +83,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
         for (int b = 0; b < 4; b++) {
             const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
             const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                acc[a][b][r] = overwrite ? 0.f : C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+            acc[a][b] = overwrite ? f4{0.f, 0.f, 0.f, 0.f} : ld_c4(C, ldc, ib, j, m, c16);
         }
 
     // loaders: A tile 128 x 32 as 4 rows x 1 k per thread-chunk; B tile 32 x 128 as 4 k x 1 col.
@@ -148,12 +168,7 @@ __global__ void __launch_bounds__(GT) sgemm_f32(int m, int n, int k, const float
         for (int b = 0; b < 4; b++) {
             const int j = j0 + wn + 16 * b + (lane & 15);
             const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
-            if (j < n) {
-                float* cp = C + (size_t)j * ldc;
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] = acc[a][b][r];
-            }
+            if (j < n) st_c4(C, ldc, ib, j, m, c16, acc[a][b]);
         }
 }
 
@@ -250,15 +265,15 @@ __global__ void __launch_bounds__(GT) sgemm_h(int m, int n, int k, const T* __re
         __syncthreads();
     }
     // C -= acc, C read from clamped addresses first (every load in flight at once)
-    float cv[4][4][4];
+    const bool c16 = (ldc & 3) == 0 && ((uintptr_t)C & 15) == 0;
+    f4 cv[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
             const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
-#pragma unroll
-            for (int r = 0; r < 4; r++) cv[a][b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
+            cv[a][b] = ld_c4(C, ldc, ib, j, m, c16);
         }
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -266,12 +281,7 @@ __global__ void __launch_bounds__(GT) sgemm_h(int m, int n, int k, const T* __re
         for (int b = 0; b < 4; b++) {
             const int j = j0 + wn + 16 * b + (lane & 15);
             const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
-            if (j < n) {
-                float* cp = C + (size_t)j * ldc;
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] = cv[a][b][r] - acc[a][b][r];
-            }
+            if (j < n) st_c4(C, ldc, ib, j, m, c16, cv[a][b] - acc[a][b]);
         }
 }
 
@@ -305,23 +315,33 @@ __global__ void cvt_n16(int k, int n, const float* __restrict__ src, size_t lds,
 
 // ---- split fp16 operands (LP_F16X3).  Ath/Atl: A transposed (row i k-contiguous), Bh/Bl:
 // B columns k-contiguous.  256 x 128 tiles, 8 wavefronts (two per SIMD, so one hides the
-// other's waits) of 64 x 64; BK = 32 elements (one MFMA step) a stage, the four operand
-// tiles double-buffered in LDS (120 KB).
+// other's waits) of 64 x 64; BK = 32 elements (one MFMA step) a stage.
+//   DL = false: the four operand tiles double-buffered in LDS (120 KB) through registers,
+//               rows padded to 20 words;
+//   DL = true:  (k % 32 == 0) operands k-chunked in memory ([k / 32][rows][32], so a stage of
+//               16 rows is 1 KB of contiguous memory), three LDS stages of 48 KB filled by
+//               global_load_lds_dwordx4 straight from memory, two stages ahead; a row's four
+//               16-byte chunks are stored at chunk c ^ ((row / 4) % 4), so the sixteen rows an
+//               MFMA operand read touches fall on distinct banks without padding.
 constexpr int H3M = 256, H3N = 128, H3T = 512, H3RW = 20;  // LDS row stride in words: 16 data + 4 pad
+template <bool DL>
 __global__ void __launch_bounds__(H3T) sgemm_h3(int m, int n, int k, const _Float16* __restrict__ Ath,
                                                 const _Float16* __restrict__ Atl, size_t lda,
                                                 const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                 size_t ldb, float* __restrict__ C, size_t ldc, int tiles_m,
                                                 int tiles_n) {
     constexpr int BK = 32;
-    __shared__ uint32_t SA[2][2][H3M * H3RW];
-    __shared__ uint32_t SB[2][2][H3N * H3RW];
+    __shared__ uint32_t SA[DL ? 1 : 2][2][DL ? 4 : H3M * H3RW];
+    __shared__ uint32_t SB[DL ? 1 : 2][2][DL ? 4 : H3N * H3RW];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
     const int tm = bid % tiles_m, tn = bid / tiles_m;
     const int i0 = tm * H3M, j0 = tn * H3N;
     const int wm = (wv & 3) * 64, wn = (wv >> 2) * 64;
-    f4 am[4][4], ac[4][4];  // Ah Bh, and Ah Bl + Al Bh
+    // am = Ah Bh and ac = Ah Bl + Al Bh from zero; C enters once, at the end (accumulating
+    // into C instead would round every step at C's magnitude, which in a trailing update can
+    // exceed the product's by far)
+    f4 am[4][4], ac[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -365,24 +385,25 @@ __global__ void __launch_bounds__(H3T) sgemm_h3(int m, int n, int k, const _Floa
         *(uint4*)&SB[buf][1][o] = rb[1];
     };
     const int nk = (k + BK - 1) / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int t = 0; t < nk; t++) {
-        const int cur = t & 1;
-        if (t + 1 < nk) gload((t + 1) * BK);
+    // one BK step from operand tiles (ah, al: 256 rows; bh, bl: 128 rows) with row stride rw
+    // words; swz: chunks stored at c ^ ((row / 4) % 4)
+    auto step = [&](const uint32_t* pah, const uint32_t* pal, const uint32_t* pbh, const uint32_t* pbl, int rw,
+                    bool swz) {
         h8 fbh[4], fbl[4];
+        const int c = lane >> 4;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            const int o = (wn + 16 * b + (lane & 15)) * H3RW + 4 * (lane >> 4);
-            fbh[b] = __builtin_bit_cast(h8, *(const uint4*)&SB[cur][0][o]);
-            fbl[b] = __builtin_bit_cast(h8, *(const uint4*)&SB[cur][1][o]);
+            const int r = wn + 16 * b + (lane & 15);
+            const int o = r * rw + 4 * (swz ? (c ^ ((r >> 2) & 3)) : c);
+            fbh[b] = __builtin_bit_cast(h8, *(const uint4*)&pbh[o]);
+            fbl[b] = __builtin_bit_cast(h8, *(const uint4*)&pbl[o]);
         }
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            const int o = (wm + 16 * a + (lane & 15)) * H3RW + 4 * (lane >> 4);
-            const h8 fah = __builtin_bit_cast(h8, *(const uint4*)&SA[cur][0][o]);
-            const h8 fal = __builtin_bit_cast(h8, *(const uint4*)&SA[cur][1][o]);
+            const int r = wm + 16 * a + (lane & 15);
+            const int o = r * rw + 4 * (swz ? (c ^ ((r >> 2) & 3)) : c);
+            const h8 fah = __builtin_bit_cast(h8, *(const uint4*)&pah[o]);
+            const h8 fal = __builtin_bit_cast(h8, *(const uint4*)&pal[o]);
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 am[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbh[b], am[a][b], 0, 0, 0);
@@ -390,30 +411,68 @@ __global__ void __launch_bounds__(H3T) sgemm_h3(int m, int n, int k, const _Floa
                 ac[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal, fbh[b], ac[a][b], 0, 0, 0);
             }
         }
-        if (t + 1 < nk) sstore(cur ^ 1);
+    };
+    if constexpr (DL) {
+        // three stages, each A hi | A lo | B hi | B lo; the barrier is a bare s_barrier (LDS
+        // needs no cache fence), so no wait on the loads in flight is implied (offsets in words)
+        constexpr int OAH = 0, OAL = H3M * 16, OBH = 2 * H3M * 16, OBL = 2 * H3M * 16 + H3N * 16;
+        constexpr int STW = 2 * (H3M + H3N) * 16;
+        __shared__ __attribute__((aligned(16))) uint32_t stg[3][STW];
+        typedef __attribute__((address_space(3))) void* lds_ptr;
+        // a direct load moves 16 rows (64 lanes x one 16-byte chunk): lane L takes row L / 4,
+        // physical chunk L % 4, i.e. logical chunk (L % 4) ^ ((L / 16) % 4)
+        const int lr = lane >> 2, lc = (lane & 3) ^ ((lane >> 4) & 3);
+        // operands k-chunked ([k / 32][rows][32]): a direct load reads 1 KB of contiguous memory
+        auto dl16 = [&](const _Float16* src, int row, int lim, int k0, uint32_t* dst) {
+            __builtin_amdgcn_global_load_lds(src + ((((size_t)(k0 >> 5)) * lim + min(row, lim - 1)) << 5) + 8 * lc,
+                                             (lds_ptr)dst, 16, 0, 0);
+        };
+        auto dissue = [&](uint32_t* sg, int k0) {
+            uint32_t *ah = sg + OAH, *al = sg + OAL, *bh = sg + OBH, *bl = sg + OBL;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {  // A: rows 32 wv + 16 q ..
+                const int rb = 32 * wv + 16 * q;
+                dl16(Ath, i0 + rb + lr, m, k0, ah + rb * 16);
+                dl16(Atl, i0 + rb + lr, m, k0, al + rb * 16);
+            }
+            const int rb = 16 * wv;  // B: rows 16 wv ..
+            dl16(Bh, j0 + rb + lr, n, k0, bh + rb * 16);
+            dl16(Bl, j0 + rb + lr, n, k0, bl + rb * 16);
+        };
+        auto body = [&](int t, const uint32_t* sg, uint32_t* nsg) {
+            if (t + 1 < nk) __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt(6): the next stage may still load
+            else __builtin_amdgcn_s_waitcnt(0x0F70);
+            __builtin_amdgcn_s_barrier();
+            if (t + 2 < nk) dissue(nsg, (t + 2) * BK);
+            step(sg + OAH, sg + OAL, sg + OBH, sg + OBL, 16, true);
+        };
+        if (nk > 0) dissue(stg[0], 0);
+        if (nk > 1) dissue(stg[1], BK);
+        for (int t = 0; t < nk; t++) body(t, stg[t % 3], stg[(t + 2) % 3]);
+    } else {
+        gload(0);
+        sstore(0);
         __syncthreads();
+        for (int t = 0; t < nk; t++) {
+            const int cur = t & 1;
+            if (t + 1 < nk) gload((t + 1) * BK);
+            step(SA[cur][0], SA[cur][1], SB[cur][0], SB[cur][1], H3RW, false);
+            if (t + 1 < nk) sstore(cur ^ 1);
+            __syncthreads();
+        }
     }
-    // C -= Ah Bh + 2^-11 (Ah Bl + Al Bh), a row block at a time, C read from clamped
-    // addresses first
+    // C -= am + 2^-11 ac, a row block at a time, C read first
+    const bool c16 = (ldc & 3) == 0 && ((uintptr_t)C & 15) == 0;
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-        float cv[4][4];
         const int ib = i0 + wm + 16 * a + 4 * (lane >> 4);
+        f4 cv[4];
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int j = min(j0 + wn + 16 * b + (lane & 15), n - 1);
-#pragma unroll
-            for (int r = 0; r < 4; r++) cv[b][r] = C[(size_t)min(ib + r, m - 1) + (size_t)j * ldc];
-        }
+        for (int b = 0; b < 4; b++) cv[b] = ld_c4(C, ldc, ib, min(j0 + wn + 16 * b + (lane & 15), n - 1), m, c16);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int j = j0 + wn + 16 * b + (lane & 15);
-            if (j < n) {
-                float* cp = C + (size_t)j * ldc;
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (ib + r < m) cp[ib + r] = cv[b][r] - (am[a][b][r] + ac[a][b][r] * 0x1p-11f);
-            }
+            if (j < n) st_c4(C, ldc, ib, j, m, c16, cv[b] - (am[a][b] + ac[a][b] * 0x1p-11f));
         }
     }
 }
@@ -424,8 +483,10 @@ __device__ __forceinline__ void split16(float x, _Float16& h, _Float16& l) {
 }
 
 // fp32 -> split fp16 pairs, transposed (as cvt_t16)
+// chunked: destination [k / 32][m][32] instead of rows of ldd
 __global__ void __launch_bounds__(256) cvt_t16s(int m, int k, const float* __restrict__ src, size_t lds,
-                                                _Float16* __restrict__ hi, _Float16* __restrict__ lo, size_t ldd) {
+                                                _Float16* __restrict__ hi, _Float16* __restrict__ lo, size_t ldd,
+                                                bool chunked) {
     __shared__ float tile[64][65];
     const int bi = blockIdx.x * 64, bk = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -439,22 +500,24 @@ __global__ void __launch_bounds__(256) cvt_t16s(int m, int k, const float* __res
         if (i < m && kk < k) {
             _Float16 h, l;
             split16(tile[tx][r], h, l);
-            hi[(size_t)i * ldd + kk] = h;
-            lo[(size_t)i * ldd + kk] = l;
+            const size_t o = chunked ? ((((size_t)(kk >> 5)) * m + i) << 5) + (kk & 31) : (size_t)i * ldd + kk;
+            hi[o] = h;
+            lo[o] = l;
         }
     }
 }
 
 // fp32 -> split fp16 pairs, same layout (as cvt_n16)
 __global__ void cvt_n16s(int k, int n, const float* __restrict__ src, size_t lds, _Float16* __restrict__ hi,
-                         _Float16* __restrict__ lo, size_t ldd) {
+                         _Float16* __restrict__ lo, size_t ldd, bool chunked) {
     const size_t total = (size_t)k * n;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
         const size_t j = e / k, kk = e % k;
         _Float16 h, l;
         split16(src[kk + j * lds], h, l);
-        hi[kk + j * ldd] = h;
-        lo[kk + j * ldd] = l;
+        const size_t o = chunked ? (((kk >> 5) * n + j) << 5) + (kk & 31) : kk + j * ldd;
+        hi[o] = h;
+        lo[o] = l;
     }
 }
 
@@ -499,11 +562,18 @@ void launch_lp_gemm(hipStream_t st, int lp, int m, int n, int k, const float* A,
         _Float16* atl = ath + (size_t)m * kp;
         _Float16* bh = atl + (size_t)m * kp;
         _Float16* bl = bh + (size_t)n * kp;
-        hipLaunchKernelGGL(cvt_t16s, tg, dim3(256), 0, st, m, k, A, lda, ath, atl, kp);
-        hipLaunchKernelGGL(cvt_n16s, dim3(cg), dim3(256), 0, st, k, n, B, ldb, bh, bl, kp);
+        // whole 32-deep stages (the workspace is 256-byte aligned): k-chunked copies and
+        // direct-to-LDS operand loads
+        const bool dl = k % 32 == 0 && ((uintptr_t)ws & 15) == 0;
+        hipLaunchKernelGGL(cvt_t16s, tg, dim3(256), 0, st, m, k, A, lda, ath, atl, kp, dl);
+        hipLaunchKernelGGL(cvt_n16s, dim3(cg), dim3(256), 0, st, k, n, B, ldb, bh, bl, kp, dl);
         const int t3m = (m + H3M - 1) / H3M, t3n = (n + H3N - 1) / H3N;
-        hipLaunchKernelGGL(sgemm_h3, dim3((unsigned)(t3m * t3n)), dim3(H3T), 0, st, m, n, k, ath, atl, kp, bh, bl, kp,
-                           C, ldc, t3m, t3n);
+        if (dl)
+            hipLaunchKernelGGL(sgemm_h3<true>, dim3((unsigned)(t3m * t3n)), dim3(H3T), 0, st, m, n, k, ath, atl, kp,
+                               bh, bl, kp, C, ldc, t3m, t3n);
+        else
+            hipLaunchKernelGGL(sgemm_h3<false>, dim3((unsigned)(t3m * t3n)), dim3(H3T), 0, st, m, n, k, ath, atl, kp,
+                               bh, bl, kp, C, ldc, t3m, t3n);
     } else if (lp == LP_F16) {
         hipLaunchKernelGGL(cvt_t16<_Float16>, tg, dim3(256), 0, st, m, k, A, lda, (_Float16*)At, kp);
         hipLaunchKernelGGL(cvt_n16<_Float16>, dim3(cg), dim3(256), 0, st, k, n, B, ldb, (_Float16*)Bh, kp);
