@@ -261,6 +261,53 @@ def test_gemm_direct_load_tiles_bitexact(B, O, alpha, beta, m, n, k):
     same(c, ref, f"gemm NN {m}x{n}x{k} direct loads")
 
 
+def _fma_exact(a, b, c):
+    """fma(a, b, c) rounded once: exact rational arithmetic, then float() (round to nearest even)"""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def test_mfma_f64_rounds_as_four_chained_fmas(B, O):
+    # The bit-exactness of every product rests on v_mfma_f64_16x16x4f64 rounding like
+    # fma(a3, b3, fma(a2, b2, fma(a1, b1, fma(a0, b0, acc)))), the reference's _mm256_fmadd_pd
+    # chain.  A 16 x 16 x 4 product with alpha = 1, beta = 0 is one MFMA per element (16 x 16: no
+    # edge elements), C = fma(1, acc, 0) = acc.  Terms spread over 2^+-40 with mixed signs make
+    # the chained result differ from a once-rounded sum of the four products in many elements,
+    # so the comparison separates the two.
+    rng = np.random.default_rng(2024)
+    m = n = 16
+    k = 4
+    mag = lambda size: rng.choice([-1.0, 1.0], size) * np.ldexp(rng.uniform(1, 2, size), rng.integers(-20, 21, size))
+    a = mag(m * k)
+    b = mag(k * n)
+    c = np.full(m * n, np.nan)  # beta = 0: C is not read
+    B.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    from fractions import Fraction
+    chained = np.empty(m * n)
+    backward = np.empty(m * n)
+    once = np.empty(m * n)
+    for j in range(n):
+        for i in range(m):
+            terms = [(a[i + q * m], b[q + j * k]) for q in range(k)]
+            acc = 0.0
+            for x, y in terms:
+                acc = _fma_exact(x, y, acc)
+            chained[i + j * m] = acc
+            acc = 0.0
+            for x, y in reversed(terms):
+                acc = _fma_exact(x, y, acc)
+            backward[i + j * m] = acc
+            once[i + j * m] = float(sum(Fraction(x) * Fraction(y) for x, y in terms))
+    same(c, chained, "MFMA vs four chained FMAs in k order")
+    # the data separates the chain from a once-rounded sum and from the chain in reverse k order
+    assert np.count_nonzero(chained != once) >= 20
+    assert np.count_nonzero(chained != backward) >= 20
+    # and the oracle (the reference's chain in C) agrees
+    ref = np.full(m * n, np.nan)
+    O.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, ref, m)
+    same(c, ref, "oracle")
+
+
 def test_gemm_alpha_zero_beta_rules(B, O):
     c = np.array([np.nan, 1.0, 2.0, 3.0])
     B.gemm("N", "N", 2, 2, 1, 0.0, np.ones(2), 2, np.ones(2), 1, 0.0, c, 2)
