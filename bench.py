@@ -366,15 +366,26 @@ def cpu_rows(runs=10, n_lu=4096, n_chol=4096, nrhs=64):
 
 def panel_traffic():
     """HBM bytes per panel launch at n = 8192, from the FETCH_SIZE / WRITE_SIZE counter runs
-    (scaled by a plain copy's counts) recorded in profiles/getf2_duo_lu_n8192_hbm_traffic.md."""
+    (scaled by a plain copy's counts) recorded in profiles/getf2_duo_lu_n8192_hbm_traffic.md.
+    The record names the SHA-256 of the panel source it was measured on; when getrf.hip has
+    changed since, the figure no longer describes the kernel that runs and None is returned."""
+    import hashlib
     path = os.path.join(ROOT, "profiles", "getf2_duo_lu_n8192_hbm_traffic.md")
+    src = os.path.join(ROOT, "bola_amd", "csrc", "getrf.hip")
+    fields = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+            key, sep, val = line.partition(":")
+            if sep and key in ("traffic_bytes_per_launch", "panel_source_sha256"):
+                fields[key] = val.strip()
+        with open(src, "rb") as f:
+            digest = hashlib.sha256(f.read()).hexdigest()
     except OSError:
-        pass
-    return None
+        return None
+    if fields.get("panel_source_sha256") != digest:
+        print("# panel traffic record is from another panel source: roofline.traffic null", file=sys.stderr)
+        return None
+    return float(fields["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in fields else None
 
 
 def dgemm_section(B, torch):
