@@ -1067,8 +1067,9 @@ __global__ void __launch_bounds__(WV) getf2_wave(int m, int w, double* __restric
 //
 // Global words (zeroed per factorization, each 32-bit half tagged with the step's epoch):
 // keys [2 parities][4][nwv] (candidate x lo, hi, key lo, hi; word-major so a poll reads
-// consecutive words), rows [2][nwv][2 PW] (candidate row at absolute columns), cur [2][2 +
-// 2 PW] (x and row of the row at position j).
+// consecutive words), rows [2][nwv][2][PW] (candidate row at absolute columns: the low
+// halves, then the high halves), cur [2][2 + 2 PW] (x, then the row at position j laid out
+// as a candidate row).
 constexpr int DUO_SR = 33;      // staged row stride (doubles)
 constexpr int DUO_CW = 2 + 2 * PW;
 constexpr int kDuoMax = 256;    // workgroups: all must be resident at once
@@ -1308,11 +1309,11 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
                 const uint64_t* prw =
                     rel == 0 ? cur + (size_t)par * DUO_CW + 2 : rows + ((size_t)par * nwv + gsl) * (2 * PW);
                 const bool need = lane > j && lane < w;
-                const uint64_t* src = prw + 2 * (need ? lane : 0);
+                const uint64_t* src = prw + (need ? lane : 0);
                 uint64_t r0 = 0, r1 = 0;
                 if (need) {
                     r0 = xldu(src);
-                    r1 = xldu(src + 1);
+                    r1 = xldu(src + PW);
                 }
                 if (g == 0 && lane == 0) ipiv[j] = j + (int)rel + 1;
                 if (!(piv != 0.0)) {
@@ -1352,7 +1353,7 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
                             if (++sp > spin_limit) { ok = 0; timed = 1; break; }
                             if (stw < 0) { ok = 0; break; }
                             r0 = xldu(src);
-                            r1 = xldu(src + 1);
+                            r1 = xldu(src + PW);
                             stw = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         }
                     }
@@ -1449,15 +1450,17 @@ __global__ void __launch_bounds__(2 * WV) getf2_duo(int m, int w, double* __rest
             }
             __syncthreads();  // X_j
             {
-                // the candidate row (lanes 0-31) and the row at position j (lanes 32-63), by column
+                // the candidate row (lanes 0-31) and the row at position j (lanes 32-63), by
+                // column: the low halves of the row's words, then its high halves, so each store
+                // instruction fills whole lines
                 const int h = lane >> 5, c = lane & 31;
                 const int q = h == 0 ? ctl[1] : ctl[2];
                 if (q >= 0 && c > j && c < w) {
                     const uint64_t b = (uint64_t)__double_as_longlong(stg[q % R][q / R][c - j - 1]);
-                    uint64_t* dst = h == 0 ? rows + ((size_t)par * nwv + g) * (2 * PW) + 2 * c
-                                           : cur + (size_t)par * DUO_CW + 2 + 2 * c;
+                    uint64_t* dst = h == 0 ? rows + ((size_t)par * nwv + g) * (2 * PW) + c
+                                           : cur + (size_t)par * DUO_CW + 2 + c;
                     xstu(dst, tag | (uint32_t)b);
-                    xstu(dst + 1, tag | (uint32_t)(b >> 32));
+                    xstu(dst + PW, tag | (uint32_t)(b >> 32));
                 }
             }
 #pragma unroll
