@@ -540,15 +540,22 @@ int bola_hip_dgetrs_dev(bola_hip_handle_t h, size_t n, size_t nrhs, const double
     API_BEGIN(h)
     if (n == 0 || nrhs == 0) return 0;
     Phase ph(h, "GETRS", 2.0 * (double)n * n * nrhs);
+    const bool pers = h->persistent_solves && ln_persistent_fits(n, nrhs);
+    const size_t fo = (laswp_ws_bytes(n) + 255) & ~(size_t)255;
+    char* ws = (char*)h->ws_aux.get(fo + (pers ? ln_persistent_ws_bytes(n, nrhs) : 0));
     {
         Phase pl(h, "LASWP");
-        void* ws = h->ws_aux.get(laswp_ws_bytes(n));
         launch_laswp(h->stream, nrhs, b, ldb, 1, (int32_t)n, ipiv, 1, ws, nullptr);
     }
     {
         Phase pt(h, "TRSM", (double)n * n * nrhs);
-        launch_dtrsm(h->stream, 'L', 'L', 'N', 'U', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);
-        launch_dtrsm(h->stream, 'L', 'U', 'N', 'N', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);
+        if (pers) {
+            launch_dtrsm_ln_persistent(h->stream, false, 1, n, nrhs, lu, lda, b, ldb, (int*)(ws + fo));
+            launch_dtrsm_ln_persistent(h->stream, true, 0, n, nrhs, lu, lda, b, ldb, (int*)(ws + fo));
+        } else {
+            launch_dtrsm(h->stream, 'L', 'L', 'N', 'U', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);
+            launch_dtrsm(h->stream, 'L', 'U', 'N', 'N', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);
+        }
     }
     return 0;
     API_END(h)

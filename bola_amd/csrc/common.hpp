@@ -141,6 +141,7 @@ struct bola_hip_handle_s {
     bola::Workspace mws2;       // its FP64 fallback copy of A
     bola::Workspace bws;        // batched solves: staged descriptors, per-workgroup scratch
     bola::Workspace cws;        // Cholesky: the saved region of the matrix (restored when a pivot fails)
+    bool persistent_solves = true;  // one-launch triangular solves (off where other work shares the device)
     int32_t* d_info = nullptr;  // device status word for factorizations
     std::string last_error;
     // profiling: per-phase GPU time measured with events on the handle's stream
@@ -343,5 +344,13 @@ void factor_release(bola_hip_factor_s* f, hipStream_t st);
 // trsm.hip
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
                   const double* a, size_t lda, double* b, size_t ldb, Abort abort = Abort());
+// One-launch left/no-transpose solve, alpha = 1 (trsm.rs:95-170, same results as
+// launch_dtrsm): m = 64 * 2^k with 2 to 256 leaves, n % 16 == 0, n <= 4096.  Its
+// workgroups wait for one another, so it runs only where nothing else fills the device;
+// flags: ln_persistent_ws_bytes(m, n) of scratch.
+bool ln_persistent_fits(size_t m, size_t n);
+size_t ln_persistent_ws_bytes(size_t m, size_t n);
+void launch_dtrsm_ln_persistent(hipStream_t st, bool upper, int unit, size_t m, size_t n, const double* a, size_t lda,
+                                double* b, size_t ldb, int* flags);
 
 }  // namespace bola

@@ -253,6 +253,191 @@ void base(hipStream_t st, int unit, size_t t, size_t nvec, const double* a, size
     BOLA_LAUNCH_CHECK();
 }
 
+// ---- persistent left/no-transpose solve (one launch for the whole recursion)
+//
+// For B of m x n with m = 64 * 2^k (2 to 256 leaves) and n % 16 == 0, the reference's
+// recursion (trsm.rs:95-170) reduces, element by element, to a fixed sequence: every gemm
+// of the recursion has k a multiple of 64 starting on a leaf boundary, m a multiple of 8
+// and n a multiple of 4, so a row of leaf b receives one fold per 64-row slab it depends
+// on -- acc = sum of the slab's 64 products by an FMA chain from zero, C = fma(-1, acc, C)
+// (gemm.rs:232-270) -- and then leaf b's base-case solve.  The lower solve folds slabs
+// 0 .. b-1 in increasing order; the upper solve folds the recursion's right siblings from
+// the outermost in, each sibling's slabs in increasing order.  One workgroup per (leaf,
+// 16-column group) walks that list, waiting for each slab's leaf to be published, folds it
+// with v_mfma_f64_16x16x4f64 (which rounds like four chained FMAs in k order), solves its
+// leaf and publishes it.  A workgroup waits only for leaves that come earlier in the
+// solve, and workgroups are numbered in that order, so the one that is waited for has
+// always been dispatched.
+constexpr int PLN_COLS = 16;  // right-hand-side columns per workgroup
+typedef double pln_d4 __attribute__((ext_vector_type(4)));
+
+template <int K>
+__device__ __forceinline__ double row_bcast(double x) {  // lane K of each row of 16, to the row
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x150 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x150 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Leaf of 64 rows for the 4 columns of one wavefront: lane (r = lane & 15, cq = lane >> 4)
+// holds rows 16 q + r of its column in x[q].  The leaf is solved in 16-row blocks: block p's
+// 16 steps (the wavefront's rows of block p exchange the solved entry with a row broadcast),
+// then the rows of the other blocks that the steps update take them, in step order, from
+// the multipliers each lane kept -- the same operation on each element, in the same order,
+// as the reference's step-by-step daxpy form (trsm.rs:95-170, utilities.rs:107-132).
+
+// step S of block P's own 16 steps (step k = S for the lower solve, 15 - S for the upper);
+// ad: this lane's row of the block's diagonal part, dg: the diagonal, both in registers so
+// the step chain waits on nothing but the row broadcast (and the division)
+template <bool LOWER, bool UNIT, int P, int S>
+__device__ __forceinline__ void pln_diag(double& xp, double (&al)[16], bool (&act)[16], const double (&ad)[16],
+                                         const double (&dg)[16], int r) {
+    if constexpr (S < 16) {
+        constexpr int k = LOWER ? S : 15 - S;
+        constexpr int kk = 16 * P + k;
+        const int i = 16 * P + r;
+        const double bk = row_bcast<k>(xp);
+        const double u = UNIT ? bk : bk / dg[k];
+        const bool nz = bk != 0.0;
+        xp = (nz && r == k) ? u : xp;
+        al[k] = -u;
+        act[k] = nz && al[k] != 0.0;
+        const bool fused = LOWER ? (i - kk - 1 < ((63 - kk) & ~3)) : (i < (kk & ~3));
+        const double y = fused ? fma(al[k], ad[k], xp) : xp + al[k] * ad[k];
+        xp = (act[k] && (LOWER ? r > k : r < k)) ? y : xp;
+        pln_diag<LOWER, UNIT, P, S + 1>(xp, al, act, ad, dg, r);
+    }
+}
+
+template <bool LOWER, bool UNIT, int P>
+__device__ __forceinline__ void pln_block(double (&x)[4], const double (*la)[65], int r) {
+    double al[16], ad[16], dg[16];
+    bool act[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        ad[k] = la[16 * P + r][16 * P + k];
+        dg[k] = UNIT ? 1.0 : la[16 * P + k][16 * P + k];
+    }
+    pln_diag<LOWER, UNIT, P, 0>(x[P], al, act, ad, dg, r);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (LOWER ? q > P : q < P) {
+            const int i = 16 * q + r;
+            double aq[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) aq[k] = la[i][16 * P + k];
+            double xq = x[q];
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int k = LOWER ? s : 15 - s;
+                const int kk = 16 * P + k;
+                const bool fused = LOWER ? (i - kk - 1 < ((63 - kk) & ~3)) : (i < (kk & ~3));
+                const double y = fused ? fma(al[k], aq[k], xq) : xq + al[k] * aq[k];
+                xq = act[k] ? y : xq;
+            }
+            x[q] = xq;
+        }
+    }
+}
+
+template <bool LOWER, bool UNIT>
+__global__ void __launch_bounds__(256) trsm_ln_persistent(int nb, int ncg, const double* __restrict__ A, size_t lda,
+                                                          double* __restrict__ B, size_t ldb, int* __restrict__ flags,
+                                                          long spin_limit) {
+    __shared__ double la[64][65];             // the leaf's diagonal block of the triangle, [row][col]
+    __shared__ double tb[PLN_COLS][65];       // the folded block, [col][row]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int pos = blockIdx.x / ncg, cg = blockIdx.x - pos * ncg;
+    const int b = LOWER ? pos : nb - 1 - pos;
+    const size_t r0 = (size_t)b * 64, c0 = (size_t)cg * PLN_COLS;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int rr = e & 63, cc = e >> 6;
+        const bool need = LOWER ? cc <= rr : cc >= rr;
+        la[rr][cc] = need ? A[r0 + rr + (r0 + cc) * lda] : 0.0;
+    }
+    // this wavefront's 16 x 16 tile of the block: rows 16 w + lk + 4 i, column lr
+    const int lr = lane & 15, lk = lane >> 4;
+    double crun[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) crun[i] = B[r0 + 16 * w + lk + 4 * i + (c0 + lr) * ldb];
+    int bad = 0;
+    auto fold = [&](int s) {
+        // the triangle's part first: it does not wait for leaf s
+        const size_t k0 = (size_t)s * 64;
+        double fa[16], fb[16];
+        const double* ap = A + r0 + 16 * w + lr + (k0 + lk) * lda;
+        const double* bp = B + k0 + lk + (c0 + lr) * ldb;
+#pragma unroll
+        for (int t = 0; t < 16; t++) fa[t] = ap[(size_t)4 * t * lda];
+        const int* f = flags + (size_t)s * ncg + cg;
+        long sp = 0;
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            if (++sp > spin_limit) { bad = 1; break; }
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int t = 0; t < 16; t++) fb[t] = __hip_atomic_load(bp + 4 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pln_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 16; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], fb[t], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) crun[i] = fma(-1.0, acc[i], crun[i]);
+    };
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (LOWER) {
+            if (b >= mid) {
+                for (int s = lo; s < mid; s++) fold(s);
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        } else {
+            if (b < mid) {
+                for (int s = mid; s < hi; s++) fold(s);
+                hi = mid;
+            } else {
+                lo = mid;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) tb[lr][16 * w + lk + 4 * i] = crun[i];
+    __syncthreads();
+    // the leaf, in the layout of pln_block
+    const int r = lane & 15, c = 4 * w + (lane >> 4);
+    double x[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[q] = tb[c][16 * q + r];
+    if (LOWER) {
+        pln_block<true, UNIT, 0>(x, la, r);
+        pln_block<true, UNIT, 1>(x, la, r);
+        pln_block<true, UNIT, 2>(x, la, r);
+        pln_block<true, UNIT, 3>(x, la, r);
+    } else {
+        pln_block<false, UNIT, 3>(x, la, r);
+        pln_block<false, UNIT, 2>(x, la, r);
+        pln_block<false, UNIT, 1>(x, la, r);
+        pln_block<false, UNIT, 0>(x, la, r);
+    }
+    // a wait that gave up leaves NaN rather than a wrong solution
+    if (__any(bad)) bad = 1;
+    bad = __syncthreads_or(bad);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double v = bad ? __builtin_nan("") : x[q];
+        __hip_atomic_store(B + r0 + 16 * q + r + (c0 + c) * ldb, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // every wavefront's stores have completed
+    if (tid == 0) __hip_atomic_store(flags + (size_t)b * ncg + cg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void pln_zero(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 constexpr size_t STOP = 64;  // trsm.rs:8
 
 void lnl(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t lda, double* b, size_t ldb,
@@ -323,6 +508,37 @@ void rtu(hipStream_t st, int unit, size_t m, size_t n, const double* a, size_t l
 bool eqc(char c, char w) { return c == w || c == (char)(w + 32); }
 
 }  // namespace
+
+bool ln_persistent_fits(size_t m, size_t n) {
+    const size_t nb = m / 64;
+    return n > 0 && n % PLN_COLS == 0 && n <= 4096 && m % 64 == 0 && nb >= 2 && nb <= 256 && (nb & (nb - 1)) == 0;
+}
+
+size_t ln_persistent_ws_bytes(size_t m, size_t n) { return sizeof(int) * (m / 64) * (n / PLN_COLS); }
+
+void launch_dtrsm_ln_persistent(hipStream_t st, bool upper, int unit, size_t m, size_t n, const double* a, size_t lda,
+                                double* b, size_t ldb, int* flags) {
+    const int nb = (int)(m / 64), ncg = (int)(n / PLN_COLS), nf = nb * ncg;
+    hipLaunchKernelGGL(pln_zero, dim3((nf + 255) / 256), dim3(256), 0, st, flags, nf);
+    BOLA_LAUNCH_CHECK();
+    const long spin = 1L << 22;
+    if (upper) {
+        if (unit)
+            hipLaunchKernelGGL((trsm_ln_persistent<false, true>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
+                               flags, spin);
+        else
+            hipLaunchKernelGGL((trsm_ln_persistent<false, false>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
+                               flags, spin);
+    } else {
+        if (unit)
+            hipLaunchKernelGGL((trsm_ln_persistent<true, true>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
+                               flags, spin);
+        else
+            hipLaunchKernelGGL((trsm_ln_persistent<true, false>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
+                               flags, spin);
+    }
+    BOLA_LAUNCH_CHECK();
+}
 
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
                   const double* a, size_t lda, double* b, size_t ldb, Abort abort) {

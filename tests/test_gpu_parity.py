@@ -798,6 +798,41 @@ def test_getrf_getrs_config_bitexact(B, O, n, nrhs):
     same(b.cpu().numpy(), rb, f"x {n}x{nrhs}")
 
 
+@pytest.mark.parametrize("n,nrhs,ldb,special", [(128, 16, 128, False), (256, 48, 260, False), (1024, 64, 1024, True),
+                                                (2048, 128, 2051, True), (16384, 16, 16384, False)])
+def test_getrs_one_launch_solves_bitexact(B, O, n, nrhs, ldb, special):
+    # device GETRS on shapes the one-launch triangular solves take (n = 64 * 2^k, nrhs a
+    # multiple of 16; lda and ldb padded): against the oracle's recursive solves.  The
+    # factor is arbitrary data (the solves read its unit-lower and upper triangles), with a
+    # dominant diagonal; 'special' plants exact zeros in B (the base case's skipped steps),
+    # a zero on U's diagonal (division by zero, infinities and NaN through the rest) and
+    # columns of L that are zero
+    import torch
+    rng = np.random.default_rng(n + nrhs)
+    lda = n + 3
+    lu = rng.uniform(-1, 1, lda * n)
+    for j in range(n):
+        lu[j + j * lda] += 4.0 if j % 3 else -4.0
+    piv = np.array([rng.integers(j, n) + 1 for j in range(n)], np.int32)
+    b0 = rng.uniform(-1, 1, ldb * nrhs)
+    if special:
+        b0[rng.integers(0, ldb * nrhs, ldb * nrhs // 8)] = 0.0
+        lu[(n // 2 + 5) * (lda + 1)] = 0.0
+        for j in (3, 70, n - 40):
+            lu[j * lda + j + 1: j * lda + n] = 0.0
+    rb = b0.copy()
+    O.getrs(n, nrhs, lu, lda, piv, rb, ldb)
+    a = torch.from_numpy(lu).cuda()
+    p = torch.from_numpy(piv).cuda()
+    b = torch.from_numpy(b0.copy()).cuda()
+    B.getrs(n, nrhs, a, lda, p, b, ldb)
+    torch.cuda.synchronize()
+    got = b.cpu().numpy().reshape(nrhs, ldb)[:, :n]
+    same(got, rb.reshape(nrhs, ldb)[:, :n], f"getrs {n}x{nrhs}")
+    if special:
+        assert np.isnan(got).any() or np.isinf(got).any()
+
+
 @pytest.mark.parametrize("m,n,ld", [(8192, 8192, 8192), (3000, 2500, 3011), (2600, 2601, 2600), (700, 700, 705)])
 def test_host_flavour_getrf_matches_the_device_flavour(B, m, n, ld):
     # the host-pointer entry point stages through pinned memory, uploads columns while the
