@@ -550,8 +550,8 @@ int bola_hip_dgetrs_dev(bola_hip_handle_t h, size_t n, size_t nrhs, const double
     {
         Phase pt(h, "TRSM", (double)n * n * nrhs);
         if (pers) {
-            launch_dtrsm_ln_persistent(h->stream, false, 1, n, nrhs, lu, lda, b, ldb, (int*)(ws + fo));
-            launch_dtrsm_ln_persistent(h->stream, true, 0, n, nrhs, lu, lda, b, ldb, (int*)(ws + fo));
+            launch_dtrsm_ln_persistent(h->stream, false, 1, n, nrhs, lu, lda, b, ldb, ws + fo);
+            launch_dtrsm_ln_persistent(h->stream, true, 0, n, nrhs, lu, lda, b, ldb, ws + fo);
         } else {
             launch_dtrsm(h->stream, 'L', 'L', 'N', 'U', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);
             launch_dtrsm(h->stream, 'L', 'U', 'N', 'N', n, nrhs, 1.0, lu, lda, b, ldb, nullptr);

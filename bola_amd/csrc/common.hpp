@@ -345,12 +345,12 @@ void factor_release(bola_hip_factor_s* f, hipStream_t st);
 void launch_dtrsm(hipStream_t st, char side, char uplo, char transa, char diag, size_t m, size_t n, double alpha,
                   const double* a, size_t lda, double* b, size_t ldb, Abort abort = Abort());
 // One-launch left/no-transpose solve, alpha = 1 (trsm.rs:95-170, same results as
-// launch_dtrsm): m = 64 * 2^k with 2 to 256 leaves, n % 16 == 0, n <= 4096.  Its
-// workgroups wait for one another, so it runs only where nothing else fills the device;
-// flags: ln_persistent_ws_bytes(m, n) of scratch.
+// launch_dtrsm): m = 64 * 2^k with 2 to 256 leaves, n % 16 == 0, scratch at most 1 GiB.
+// Its workgroups wait for one another, so it runs only where nothing else fills the
+// device; ws: ln_persistent_ws_bytes(m, n) of scratch, 256-byte aligned.
 bool ln_persistent_fits(size_t m, size_t n);
 size_t ln_persistent_ws_bytes(size_t m, size_t n);
 void launch_dtrsm_ln_persistent(hipStream_t st, bool upper, int unit, size_t m, size_t n, const double* a, size_t lda,
-                                double* b, size_t ldb, int* flags);
+                                double* b, size_t ldb, void* ws);
 
 }  // namespace bola

@@ -273,8 +273,8 @@ typedef double pln_d4 __attribute__((ext_vector_type(4)));
 
 template <int K>
 __device__ __forceinline__ double row_bcast(double x) {  // lane K of each row of 16, to the row
-    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), 0x150 + K, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), 0x150 + K, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x150 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x150 + K, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -298,7 +298,7 @@ __device__ __forceinline__ void pln_diag(double& xp, double (&al)[16], bool (&ac
         const double bk = row_bcast<k>(xp);
         const double u = UNIT ? bk : bk / dg[k];
         const bool nz = bk != 0.0;
-        xp = (nz && r == k) ? u : xp;
+        if constexpr (!UNIT) xp = (nz && r == k) ? u : xp;  // with a unit diagonal u is lane k's own x
         al[k] = -u;
         act[k] = nz && al[k] != 0.0;
         const bool fused = LOWER ? (i - kk - 1 < ((63 - kk) & ~3)) : (i < (kk & ~3));
@@ -339,10 +339,67 @@ __device__ __forceinline__ void pln_block(double (&x)[4], const double (*la)[65]
     }
 }
 
+// Upper solve: the products leaf b keeps at most at once (its first sibling group's size
+// less one; 0 for the last leaf), and the number kept by the leaves before it.  Leaves with
+// j leading one bits (of log2 nb) form the range [nb - nb/2^j, nb - nb/2^(j+1)) and keep
+// nb/2^(j+1) - 1 each.
+__host__ __device__ inline int pln_lead_ones(int nb, int b) {
+    int j = 0;
+    for (int h = nb >> 1; h >= 1 && (b & h); h >>= 1) j++;
+    return j;
+}
+__host__ __device__ inline int pln_kept_slots(int nb, int b) {
+    const int g = nb >> (pln_lead_ones(nb, b) + 1);
+    return g > 0 ? g - 1 : 0;
+}
+__host__ __device__ inline long pln_kept_offset(int nb, int b) {
+    const int j = pln_lead_ones(nb, b);
+    long off = 0;
+    for (int q = 0; q < j; q++) {
+        const long g = nb >> (q + 1);
+        off += g * (g - 1);  // range q: g leaves keeping g - 1 each
+    }
+    const long start = nb - (nb >> j);
+    return off + (b - start) * (long)pln_kept_slots(nb, b);
+}
+
+// The slabs leaf b folds, in the recursion's order: the ranges of the siblings on its path
+// from the root (left siblings for the lower solve, right ones for the upper), outermost first.
+struct PlnWalk {
+    int lo, hi;  // the node whose sibling ranges are still to come
+    int s, e;    // the current range
+};
+template <bool LOWER>
+__device__ __forceinline__ bool pln_next(PlnWalk& wk, int b, int& out) {
+    while (wk.s >= wk.e) {
+        if (wk.hi - wk.lo <= 1) return false;
+        const int mid = (wk.lo + wk.hi) >> 1;
+        if (LOWER) {
+            if (b >= mid) {
+                wk.s = wk.lo;
+                wk.e = mid;
+                wk.lo = mid;
+            } else {
+                wk.hi = mid;
+            }
+        } else {
+            if (b < mid) {
+                wk.s = mid;
+                wk.e = wk.hi;
+                wk.hi = mid;
+            } else {
+                wk.lo = mid;
+            }
+        }
+    }
+    out = wk.s++;
+    return true;
+}
+
 template <bool LOWER, bool UNIT>
 __global__ void __launch_bounds__(256) trsm_ln_persistent(int nb, int ncg, const double* __restrict__ A, size_t lda,
                                                           double* __restrict__ B, size_t ldb, int* __restrict__ flags,
-                                                          long spin_limit) {
+                                                          double* __restrict__ kept, long spin_limit) {
     __shared__ double la[64][65];             // the leaf's diagonal block of the triangle, [row][col]
     __shared__ double tb[PLN_COLS][65];       // the folded block, [col][row]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -360,41 +417,109 @@ __global__ void __launch_bounds__(256) trsm_ln_persistent(int nb, int ncg, const
 #pragma unroll
     for (int i = 0; i < 4; i++) crun[i] = B[r0 + 16 * w + lk + 4 * i + (c0 + lr) * ldb];
     int bad = 0;
-    auto fold = [&](int s) {
-        // the triangle's part first: it does not wait for leaf s
-        const size_t k0 = (size_t)s * 64;
-        double fa[16], fb[16];
-        const double* ap = A + r0 + 16 * w + lr + (k0 + lk) * lda;
-        const double* bp = B + k0 + lk + (c0 + lr) * ldb;
+    // operands of slab s: this tile's rows of the triangle (no wait) and the slab's solved rows
+    auto load_a = [&](int s, double (&fa)[16]) {
+        const double* ap = A + r0 + 16 * w + lr + ((size_t)s * 64 + lk) * lda;
 #pragma unroll
         for (int t = 0; t < 16; t++) fa[t] = ap[(size_t)4 * t * lda];
-        const int* f = flags + (size_t)s * ncg + cg;
+    };
+    auto ready = [&](int s) {
+        return __hip_atomic_load(flags + (size_t)s * ncg + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    };
+    auto wait = [&](int s) {
         long sp = 0;
-        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        while (!ready(s)) {
             if (++sp > spin_limit) { bad = 1; break; }
             __builtin_amdgcn_s_sleep(1);
         }
+    };
+    auto load_b = [&](int s, double (&fb)[16]) {
+        const double* bp = B + (size_t)s * 64 + lk + (c0 + lr) * ldb;
 #pragma unroll
         for (int t = 0; t < 16; t++) fb[t] = __hip_atomic_load(bp + 4 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto product = [&](int s) {
+        double fa[16], fb[16];
+        load_a(s, fa);
+        wait(s);
+        load_b(s, fb);
         pln_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < 16; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], fb[t], acc, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) crun[i] = fma(-1.0, acc[i], crun[i]);
+        return acc;
     };
-    int lo = 0, hi = nb;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (LOWER) {
-            if (b >= mid) {
-                for (int s = lo; s < mid; s++) fold(s);
-                lo = mid;
-            } else {
-                hi = mid;
+    if constexpr (LOWER) {
+        // the slabs arrive in the order they are folded: each is folded as it comes, the next
+        // slab's operands loaded while this one's products run
+        PlnWalk wk{0, nb, 0, 0};
+        int s = 0;
+        bool have = pln_next<LOWER>(wk, b, s);
+        double fa[16], fb[16];
+        if (have) {
+            load_a(s, fa);
+            wait(s);
+            load_b(s, fb);
+        }
+        while (have) {
+            int sn = 0;
+            const bool hn = pln_next<LOWER>(wk, b, sn);
+            double na[16], nv[16];
+            bool got = false;
+            if (hn) {
+                load_a(sn, na);
+                if (ready(sn)) {
+                    load_b(sn, nv);
+                    got = true;
+                }
             }
-        } else {
+            pln_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < 16; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t], fb[t], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) crun[i] = fma(-1.0, acc[i], crun[i]);
+            if (hn) {
+                if (!got) {
+                    wait(sn);
+                    load_b(sn, nv);
+                }
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    fa[t] = na[t];
+                    fb[t] = nv[t];
+                }
+            }
+            have = hn;
+        }
+    } else {
+        // a right sibling's slabs arrive last-to-first, but are folded first-to-last: the
+        // products of all but its first slab are formed as the slabs arrive and kept (this
+        // lane's 4 entries of each, in global scratch), and once the first slab is there
+        // they are folded in order, each with one fma
+        double* kp = kept + ((size_t)pln_kept_offset(nb, b) * ncg + (size_t)cg * pln_kept_slots(nb, b)) * 1024 +
+                     (size_t)tid * 4;
+        int lo = 0, hi = nb;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
             if (b < mid) {
-                for (int s = mid; s < hi; s++) fold(s);
+                const int glo = mid, ghi = hi;
+                for (int s = ghi - 1; s > glo; s--) {
+                    const pln_d4 acc = product(s);
+                    *reinterpret_cast<pln_d4*>(kp + (size_t)(s - glo - 1) * 1024) = acc;
+                }
+                const pln_d4 acc0 = product(glo);
+#pragma unroll
+                for (int i = 0; i < 4; i++) crun[i] = fma(-1.0, acc0[i], crun[i]);
+                for (int s0 = glo + 1; s0 < ghi; s0 += 8) {
+                    pln_d4 kv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (s0 + u < ghi) kv[u] = *reinterpret_cast<const pln_d4*>(kp + (size_t)(s0 + u - glo - 1) * 1024);
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (s0 + u < ghi)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) crun[i] = fma(-1.0, kv[u][i], crun[i]);
+                }
                 hi = mid;
             } else {
                 lo = mid;
@@ -509,33 +634,45 @@ bool eqc(char c, char w) { return c == w || c == (char)(w + 32); }
 
 }  // namespace
 
-bool ln_persistent_fits(size_t m, size_t n) {
-    const size_t nb = m / 64;
-    return n > 0 && n % PLN_COLS == 0 && n <= 4096 && m % 64 == 0 && nb >= 2 && nb <= 256 && (nb & (nb - 1)) == 0;
+// scratch: the flags, then (upper solve) the kept products, 8 KB per product and group
+static size_t pln_flag_bytes(size_t m, size_t n) {
+    return (sizeof(int) * (m / 64) * (n / PLN_COLS) + 255) & ~(size_t)255;
+}
+static size_t pln_kept_bytes(size_t m, size_t n) {
+    const int nb = (int)(m / 64);
+    return (size_t)pln_kept_offset(nb, nb - 1) * (n / PLN_COLS) * 1024 * sizeof(double);
 }
 
-size_t ln_persistent_ws_bytes(size_t m, size_t n) { return sizeof(int) * (m / 64) * (n / PLN_COLS); }
+bool ln_persistent_fits(size_t m, size_t n) {
+    const size_t nb = m / 64;
+    if (!(n > 0 && n % PLN_COLS == 0 && m % 64 == 0 && nb >= 2 && nb <= 256 && (nb & (nb - 1)) == 0)) return false;
+    return pln_flag_bytes(m, n) + pln_kept_bytes(m, n) <= ((size_t)1 << 30);
+}
+
+size_t ln_persistent_ws_bytes(size_t m, size_t n) { return pln_flag_bytes(m, n) + pln_kept_bytes(m, n); }
 
 void launch_dtrsm_ln_persistent(hipStream_t st, bool upper, int unit, size_t m, size_t n, const double* a, size_t lda,
-                                double* b, size_t ldb, int* flags) {
+                                double* b, size_t ldb, void* ws) {
     const int nb = (int)(m / 64), ncg = (int)(n / PLN_COLS), nf = nb * ncg;
+    int* flags = (int*)ws;
+    double* kept = (double*)((char*)ws + pln_flag_bytes(m, n));
     hipLaunchKernelGGL(pln_zero, dim3((nf + 255) / 256), dim3(256), 0, st, flags, nf);
     BOLA_LAUNCH_CHECK();
     const long spin = 1L << 22;
     if (upper) {
         if (unit)
             hipLaunchKernelGGL((trsm_ln_persistent<false, true>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
-                               flags, spin);
+                               flags, kept, spin);
         else
             hipLaunchKernelGGL((trsm_ln_persistent<false, false>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
-                               flags, spin);
+                               flags, kept, spin);
     } else {
         if (unit)
             hipLaunchKernelGGL((trsm_ln_persistent<true, true>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
-                               flags, spin);
+                               flags, kept, spin);
         else
             hipLaunchKernelGGL((trsm_ln_persistent<true, false>), dim3(nf), dim3(256), 0, st, nb, ncg, a, lda, b, ldb,
-                               flags, spin);
+                               flags, kept, spin);
     }
     BOLA_LAUNCH_CHECK();
 }
