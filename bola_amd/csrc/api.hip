@@ -69,6 +69,8 @@ void collect_profile(bola_hip_handle_s* h) {
     BOLA_CHECK(hipStreamSynchronize(h->stream));
     if (h->stream2) BOLA_CHECK(hipStreamSynchronize(h->stream2));
     if (h->stream3) BOLA_CHECK(hipStreamSynchronize(h->stream3));
+    for (hipStream_t ls : {h->lu_crit, h->lu_bulk, h->lu_left})
+        if (ls) BOLA_CHECK(hipStreamSynchronize(ls));
     for (auto& p : h->pending) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, p.start, p.stop);
@@ -87,7 +89,7 @@ void* Workspace::get(size_t need) {
     bola_hip_handle_s* h = owner;
     hipStream_t st = h->stream;
     // the free waits for everything queued on the handle's side streams ...
-    hipStream_t side[2] = {h->stream2, h->stream3};
+    hipStream_t side[5] = {h->stream2, h->stream3, h->lu_crit, h->lu_bulk, h->lu_left};
     if (!h->ws_event) BOLA_CHECK(hipEventCreateWithFlags(&h->ws_event, hipEventDisableTiming));
     for (hipStream_t s2 : side) {
         if (!s2) continue;
@@ -254,6 +256,8 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     if (h->stream3) (void)hipStreamSynchronize(h->stream3);
+    for (hipStream_t ls : {h->lu_crit, h->lu_bulk, h->lu_left})
+        if (ls) (void)hipStreamSynchronize(ls);
     for (auto& p : h->pending) {
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
@@ -272,6 +276,8 @@ int bola_hip_destroy(bola_hip_handle_t h) {
     if (h->ws_event) (void)hipEventDestroy(h->ws_event);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream3) (void)hipStreamDestroy(h->stream3);
+    for (hipStream_t ls : {h->lu_crit, h->lu_bulk, h->lu_left})
+        if (ls) (void)hipStreamDestroy(ls);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     if (t_default == h) t_default = nullptr;
     delete h;
@@ -540,7 +546,7 @@ int bola_hip_dgetrs_dev(bola_hip_handle_t h, size_t n, size_t nrhs, const double
     API_BEGIN(h)
     if (n == 0 || nrhs == 0) return 0;
     Phase ph(h, "GETRS", 2.0 * (double)n * n * nrhs);
-    const bool pers = h->persistent_solves && ln_persistent_fits(n, nrhs);
+    const bool pers = h->exclusive_device && ln_persistent_fits(n, nrhs);
     const size_t fo = (laswp_ws_bytes(n) + 255) & ~(size_t)255;
     char* ws = (char*)h->ws_aux.get(fo + (pers ? ln_persistent_ws_bytes(n, nrhs) : 0));
     {

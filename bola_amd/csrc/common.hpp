@@ -141,7 +141,9 @@ struct bola_hip_handle_s {
     bola::Workspace mws2;       // its FP64 fallback copy of A
     bola::Workspace bws;        // batched solves: staged descriptors, per-workgroup scratch
     bola::Workspace cws;        // Cholesky: the saved region of the matrix (restored when a pivot fails)
-    bool persistent_solves = true;  // one-launch triangular solves (off where other work shares the device)
+    // the handle's work has the device to itself (off for the server's slots): one-launch
+    // triangular solves, the LU's CU-partitioned streams
+    bool exclusive_device = true;
     int32_t* d_info = nullptr;  // device status word for factorizations
     std::string last_error;
     // profiling: per-phase GPU time measured with events on the handle's stream
@@ -152,6 +154,11 @@ struct bola_hip_handle_s {
     // second stream for work that overlaps the critical path, and ordering events
     hipStream_t stream2 = nullptr;
     hipStream_t stream3 = nullptr;  // a third one: side work that must not wait for the second
+    // the LU's streams on disjoint sets of CUs: its critical path, and its bulk update and
+    // left interchanges (lu_split: CUs i with (i / 8) % lu_split == 0 serve the critical
+    // path; 0: no split, -1: not yet created)
+    hipStream_t lu_crit = nullptr, lu_bulk = nullptr, lu_left = nullptr;
+    int lu_split = -1;
     std::vector<hipEvent_t> sync_events;
     hipEvent_t ws_event = nullptr;  // orders workspace growth against the side streams
     bola::PinnedStage* pin = nullptr;  // host-pointer calls: pinned staging, copy stream, copy threads
