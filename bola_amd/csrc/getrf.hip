@@ -2156,9 +2156,12 @@ hipStream_t lu_masked_stream(int ncu, int k, bool crit) {
 
 // The LU's critical path on a quarter of the CUs, its bulk update and left interchanges
 // on the other three quarters: sharing CUs with the bulk GEMM slowed the latency-bound
-// critical kernels by 30-270 % (n = 8192: 39.2 -> 37.2 ms per DGETRF + DGETRS).  Only
-// for a handle that has the device to itself.
+// critical kernels by 30-270 % (n = 8192: 39.2 -> 37.6 ms per DGETRF + DGETRS).  From
+// about 12K on, the bulk GEMM on three quarters of the CUs becomes the longer path (n =
+// 16384: 136 -> 144 ms), so larger matrices keep undivided streams.  Only for a handle
+// that has the device to itself.
 constexpr int kLuCuSplit = 4;
+constexpr int kLuCuSplitMax = 12288;  // largest m and n that take the split
 
 bool lu_split_streams(bola_hip_handle_s* h) {
     if (!h->exclusive_device) return false;
@@ -2244,7 +2247,7 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     hipStream_t s3 = h->aux_stream3();
     int ev = 0;
     hipStream_t user = st;
-    if (lu_split_streams(h)) {
+    if (m <= kLuCuSplitMax && n <= kLuCuSplitMax && lu_split_streams(h)) {
         // the critical path on its own CUs, the bulk update and left interchanges on the rest
         hipEvent_t e_in = lu_event(h, ev++);
         BOLA_CHECK(hipEventRecord(e_in, user));
