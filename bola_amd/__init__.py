@@ -74,6 +74,7 @@ _SIGS = [
     ("bola_hip_create", C.c_int, [C.POINTER(_vp)]),
     ("bola_hip_destroy", C.c_int, [_vp]),
     ("bola_hip_set_stream", C.c_int, [_vp, _vp]),
+    ("bola_hip_set_exclusive", C.c_int, [_vp, C.c_int]),
     ("bola_hip_get_stream", _vp, [_vp]),
     ("bola_hip_synchronize", C.c_int, [_vp]),
     ("bola_hip_error_string", C.c_char_p, [_vp]),
@@ -207,6 +208,11 @@ class Handle:
 
     def error(self) -> str:
         return lib().bola_hip_error_string(self._h).decode()
+
+    def set_exclusive(self, on: bool):
+        """False: this handle's work shares the GPU (no CU-partitioned LU streams, no one-launch
+        triangular solves)."""
+        lib().bola_hip_set_exclusive(self._h, 1 if on else 0)
 
     def synchronize(self):
         lib().bola_hip_synchronize(self.ptr)

@@ -297,6 +297,12 @@ int bola_hip_set_stream(bola_hip_handle_t h, void* stream) {
 
 void* bola_hip_get_stream(bola_hip_handle_t h) { return h ? (void*)h->stream : nullptr; }
 
+int bola_hip_set_exclusive(bola_hip_handle_t h, int exclusive) {
+    if (!h) return -1;
+    h->exclusive_device = exclusive != 0;
+    return 0;
+}
+
 int bola_hip_synchronize(bola_hip_handle_t h) {
     API_BEGIN(h)
     BOLA_CHECK(hipStreamSynchronize(h->stream));

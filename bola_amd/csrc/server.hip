@@ -125,7 +125,7 @@ void bola_hip_server_s::work(int slot) {
     if (bola_hip_create(&h) != 0) return;
     // the slots' work shares the device: no solve whose workgroups wait for one another
     // beyond the factorization's own panel, no CU-partitioned LU streams
-    h->exclusive_device = false;
+    bola_hip_set_exclusive(h, 0);
     hipStream_t st = h->stream;
     tl_stream = st;  // evictions made by this thread are ordered on its stream
     for (;;) {

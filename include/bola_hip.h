@@ -53,6 +53,12 @@ int bola_hip_set_stream(bola_hip_handle_t h, void* stream);
 void* bola_hip_get_stream(bola_hip_handle_t h);
 /* Waits for all work enqueued on the handle's stream. */
 int bola_hip_synchronize(bola_hip_handle_t h);
+/* No crate counterpart (a device-placement setting): exclusive != 0 (the default) lets the
+ * handle assume its work has the device to itself -- an LU of up to 12288 rows and columns
+ * then runs its critical path on streams restricted to a quarter of the CUs, and GETRS may
+ * run each triangular solve as one launch whose workgroups wait for one another.  Set 0 for
+ * a handle whose work shares the GPU with other heavy kernels (the server's slots do). */
+int bola_hip_set_exclusive(bola_hip_handle_t h, int exclusive);
 /* Text of the last failing call on h / on this thread's host-flavour calls. */
 const char* bola_hip_error_string(bola_hip_handle_t h);
 const char* bola_hip_last_error(void);

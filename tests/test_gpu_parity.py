@@ -833,6 +833,31 @@ def test_getrs_one_launch_solves_bitexact(B, O, n, nrhs, ldb, special):
         assert np.isnan(got).any() or np.isinf(got).any()
 
 
+def test_shared_device_handle_same_bits(B, O):
+    # a handle told that its work shares the GPU takes the undivided LU streams and the
+    # recursive triangular solves: the same bits as the oracle (and as the default handle)
+    import torch
+    n, nrhs = 2048, 64
+    rng = np.random.default_rng(77)
+    a0 = rng.uniform(-1, 1, n * n)
+    b0 = rng.uniform(-1, 1, n * nrhs)
+    ref, rp, rb = a0.copy(), np.zeros(n, np.int32), b0.copy()
+    assert O.getrf(n, n, ref, n, rp)[0] == 0
+    O.getrs(n, nrhs, ref, n, rp, rb, n)
+    h = B.Handle()
+    h.set_exclusive(False)
+    a = torch.from_numpy(a0).cuda()
+    p = torch.zeros(n, dtype=torch.int32, device="cuda")
+    b = torch.from_numpy(b0).cuda()
+    assert B.lib().bola_hip_dgetrf_dev(h.ptr, n, n, B._ptr(a, np.float64), n, B._ptr(p, np.int32), None) == 0
+    assert B.lib().bola_hip_dgetrs_dev(h.ptr, n, nrhs, B._ptr(a, np.float64), n, B._ptr(p, np.int32),
+                                       B._ptr(b, np.float64), n) == 0
+    h.synchronize()
+    same(p.cpu().numpy(), rp, "ipiv")
+    same(a.cpu().numpy(), ref, "lu")
+    same(b.cpu().numpy(), rb, "x")
+
+
 @pytest.mark.parametrize("m,n,ld", [(8192, 8192, 8192), (3000, 2500, 3011), (2600, 2601, 2600), (700, 700, 705)])
 def test_host_flavour_getrf_matches_the_device_flavour(B, m, n, ld):
     # the host-pointer entry point stages through pinned memory, uploads columns while the
