@@ -282,7 +282,8 @@ void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, 
 // leaves more of the machine to a factorization's panel running beside it.
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     Abort abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false);
+                     Abort abort, int slab, int mask, long edge_m = -1, long edge_n = -1, bool fast = false,
+                     int ncu = 0);  // ncu: CUs the stream may use (0: the device's)
 void set_num_cus(int n);
 
 // staging.hip: host <-> device transfers of the host-pointer entry points through pinned

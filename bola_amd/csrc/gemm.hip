@@ -469,7 +469,8 @@ void launch_dgemm(hipStream_t st, char transa, char transb, size_t m, size_t n, 
 
 void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t n, size_t k, double alpha,
                      const double* a, size_t lda, const double* b, size_t ldb, double beta, double* c, size_t ldc,
-                     Abort abort, int slab, int mask, long edge_m, long edge_n, bool fast) {
+                     Abort abort, int slab, int mask, long edge_m, long edge_n, bool fast, int ncu) {
+    const int cus = ncu > 0 ? ncu : g_num_cus;
     bool ta = !(transa == 'N' || transa == 'n');
     bool tb = !(transb == 'N' || transb == 'n');
     int M = (int)m, N = (int)n, K = (int)k;
@@ -477,10 +478,10 @@ void launch_dgemm_ex(hipStream_t st, char transa, char transb, size_t m, size_t 
     // by default the last m % 8 rows and n % 4 columns of this call (gemm.rs:177-190)
     const int em = edge_m < 0 ? M - (M % 8) : (int)std::min<long>(std::max<long>(edge_m, 0), M);
     const int en = edge_n < 0 ? N - (N % 4) : (int)std::min<long>(std::max<long>(edge_n, 0), N);
-    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
-    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
-    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
-    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, g_num_cus, abort, slab, mask, em, en, fast);
+    if (!ta && !tb) launch_t<false, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, cus, abort, slab, mask, em, en, fast);
+    else if (!ta && tb) launch_t<false, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, cus, abort, slab, mask, em, en, fast);
+    else if (ta && !tb) launch_t<true, false>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, cus, abort, slab, mask, em, en, fast);
+    else launch_t<true, true>(st, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, cus, abort, slab, mask, em, en, fast);
 }
 
 }  // namespace bola

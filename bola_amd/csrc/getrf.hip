@@ -1999,8 +1999,10 @@ inline Edge panel_edge(int m, int n, int j, int jb) {
 // earlier panels of the super-panel, TRSM with its L11), then one GEMM for all rows
 // below the super-panel with one 32-deep fold per panel.  Every element receives the
 // reference's operations in the reference's order (getrf.rs:53-98).
+// ncu, fast: the CUs the stream may use (0: all) and whether its GEMMs have them to
+// themselves (two workgroups per CU)
 void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, size_t lda, int32_t* info,
-                 const LuWs& w, int s0, int sw, int c0, int c1) {
+                 const LuWs& w, int s0, int sw, int c0, int c1, int ncu = 0, bool fast = false) {
     if (c1 <= c0) return;
     const int ncols = c1 - c0;
     const int p0 = s0 / PW, G = (sw + PW - 1) / PW;
@@ -2026,7 +2028,7 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
                 Phase pg(h, "GEMM", 2.0 * PW * ncols * (q * PW), st);
                 launch_dgemm_ex(st, 'N', 'N', PW, ncols, (size_t)q * PW, -1.0, a + j + (size_t)s0 * lda, lda,
                                 a + s0 + (size_t)c0 * lda, lda, 1.0, a + j + (size_t)c0 * lda, lda, Abort::span(info, s0), PW, 0,
-                                e.row - j, e.col - c0);
+                                e.row - j, e.col - c0, fast, ncu);
             }
             Phase pt(h, "TRSM", (double)PW * PW * ncols, st);
             launch_dtrsm(st, 'L', 'L', 'N', 'U', PW, ncols, 1.0, a + j + (size_t)j * lda, lda, a + j + (size_t)c0 * lda,
@@ -2036,7 +2038,7 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
             Phase pg(h, "GEMM", 2.0 * (double)(m - s0 - sw) * ncols * sw, st);
             launch_dgemm_ex(st, 'N', 'N', (size_t)(m - s0 - sw), ncols, (size_t)sw, -1.0, a + (s0 + sw) + (size_t)s0 * lda,
                             lda, a + s0 + (size_t)c0 * lda, lda, 1.0, a + (s0 + sw) + (size_t)c0 * lda, lda, Abort::span(info, s0), PW, 0,
-                            e.row - (s0 + sw), e.col - c0);
+                            e.row - (s0 + sw), e.col - c0, fast, ncu);
         }
     } else {
         for (int q = 0; q < G; q++) {
@@ -2052,7 +2054,8 @@ void update_cols(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a, 
                 Phase pg(h, "GEMM", 2.0 * (double)(m - j - jb) * ncols * jb, st);
                 launch_dgemm_ex(st, 'N', 'N', (size_t)(m - j - jb), ncols, (size_t)jb, -1.0,
                                 a + (j + jb) + (size_t)j * lda, lda, a + j + (size_t)c0 * lda, lda, 1.0,
-                                a + (j + jb) + (size_t)c0 * lda, lda, Abort::panel(info, j), PW, 0, e.row - (j + jb), e.col - c0);
+                                a + (j + jb) + (size_t)c0 * lda, lda, Abort::panel(info, j), PW, 0, e.row - (j + jb), e.col - c0,
+                                fast, ncu);
             }
         }
     }
@@ -2247,7 +2250,10 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
     hipStream_t s3 = h->aux_stream3();
     int ev = 0;
     hipStream_t user = st;
-    if (m <= kLuCuSplitMax && n <= kLuCuSplitMax && lu_split_streams(h)) {
+    const bool split = m <= kLuCuSplitMax && n <= kLuCuSplitMax && lu_split_streams(h);
+    // CUs of the critical stream's partition (runs of 8, one in kLuCuSplit)
+    const int crit_cus = split ? 8 * ((h->num_cus / 8 + kLuCuSplit - 1) / kLuCuSplit) : 0;
+    if (split) {
         // the critical path on its own CUs, the bulk update and left interchanges on the rest
         hipEvent_t e_in = lu_event(h, ev++);
         BOLA_CHECK(hipEventRecord(e_in, user));
@@ -2279,11 +2285,11 @@ void lu_factor(bola_hip_handle_s* h, size_t m_, size_t n_, double* a, size_t lda
         const int c_lo = s0 + sw;
         const int c_mid = std::min(n, c_lo + W);
         if (hooks) hooks->need_cols(st, c_mid);
-        update_cols(h, st, m, n, a, lda, info, w, s0, sw, c_lo, c_mid);
+        update_cols(h, st, m, n, a, lda, info, w, s0, sw, c_lo, c_mid, crit_cus, split);
         if (c_mid < n) {
             BOLA_CHECK(hipStreamWaitEvent(s2, fact_done, 0));
             if (hooks) hooks->need_cols(s2, n);
-            update_cols(h, s2, m, n, a, lda, info, w, s0, sw, c_mid, n);
+            update_cols(h, s2, m, n, a, lda, info, w, s0, sw, c_mid, n, split ? h->num_cus - crit_cus : 0, split);
             bulk_done = lu_event(h, ev++);
             BOLA_CHECK(hipEventRecord(bulk_done, s2));
         } else {
