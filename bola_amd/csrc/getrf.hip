@@ -1611,17 +1611,59 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
 // their flag; the block's other workgroups each update 64 rows of its columns once all
 // PSU_S flags are up.  A block's solvers precede its updaters in dispatch order, so a
 // waiting workgroup never holds back the ones it waits for.
+//
+// The panel's row interchanges (its row map, absolute rows) are applied here too, in place
+// of a separate launch before this one: rows below the panel's 32 only ever receive rows
+// that started among those 32, and the rows of the 32 receive rows from anywhere.  So each
+// solver column gathers its 32 rows through the map and first moves the displaced rows of
+// the 32 to their rows below (all read before its solved rows overwrite the 32, and stored
+// before its flag rises); the updaters read their rows ahead of the flag and read again,
+// after it, the few that received a displaced row.  The nmain-th workgroup on apply the map
+// to the `left` columns at Aleft (a wavefront a column), and workgroup 0 turns the panel's
+// pivots (fix, 1-based within the panel) into global ones.
 constexpr int PSU_S = UR_CB / 8;  // solver workgroups per column block
 __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nrt, const double* __restrict__ L11,
                                                           double* __restrict__ U, const double* __restrict__ L21,
                                                           double* __restrict__ C, size_t lda,
                                                           int* __restrict__ flags, long spin_limit,
-                                                          int32_t* __restrict__ info, int col) {
+                                                          int32_t* __restrict__ info, int col,
+                                                          const RowMap* __restrict__ map, double* __restrict__ Aleft,
+                                                          int left, int32_t* __restrict__ fix, int nmain) {
     if (abort_stop(Abort::panel(info, col))) return;
-    const int ct = blockIdx.x / (nrt + PSU_S), rt = blockIdx.x % (nrt + PSU_S);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int cnt = min(map->count, ROWMAP_MAX);
+    if ((int)blockIdx.x >= nmain) {
+        const int cc = ((int)blockIdx.x - nmain) * 4 + w;
+        if (cc >= left) return;
+        double* c = Aleft + (size_t)cc * lda;
+        double v0 = 0.0, v1 = 0.0;
+        int64_t d0 = -1, d1 = -1;
+        if (lane < cnt) { d0 = map->row[lane]; v0 = c[map->src[lane]]; }
+        if (lane + 64 < cnt) { d1 = map->row[lane + 64]; v1 = c[map->src[lane + 64]]; }
+        if (d0 >= 0) c[d0] = v0;
+        if (d1 >= 0) c[d1] = v1;
+        return;
+    }
+    if (blockIdx.x == 0 && tid < PW) fix[tid] += col;
+    const int ct = blockIdx.x / (nrt + PSU_S), rt = blockIdx.x % (nrt + PSU_S);
     const int c0 = ct * UR_CB;
+    const int64_t top = col;  // U's first row (absolute)
     if (rt < PSU_S) {
+        __shared__ int64_t tsrc[PW], brow[PW], bsrc[PW];
+        __shared__ int nbelow;
+        if (tid < PW) tsrc[tid] = top + tid;
+        if (tid == 0) nbelow = 0;
+        __syncthreads();
+        if (tid < cnt) {
+            const int64_t r = map->row[tid], sr = map->src[tid];
+            if (r < top + PW) {
+                tsrc[r - top] = sr;
+            } else {
+                const int k = atomicAdd(&nbelow, 1);
+                if (k < PW) { brow[k] = r; bsrc[k] = sr; }
+            }
+        }
+        __syncthreads();
         const int i = lane & 31, sub = lane >> 5;
         double arow[PW];
 #pragma unroll
@@ -1631,8 +1673,15 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
         }
         constexpr int NC = 1;
         const int cbase = rt * 8 + w * 2 + sub;  // this lane's column within the block
+        const bool cval = c0 + cbase < nr;
+        const size_t cof = (size_t)min(c0 + cbase, nr - 1) * lda;
         double x[NC];
-        x[0] = keep_or_zero(U[(size_t)i + (size_t)min(c0 + cbase, nr - 1) * lda], c0 + cbase < nr);
+        x[0] = keep_or_zero(U[(size_t)(tsrc[i] - top) + cof], cval);
+        if (cval && i < min(nbelow, PW)) {
+            const double v = U[(size_t)(bsrc[i] - top) + cof];
+            __builtin_amdgcn_s_waitcnt(0);  // this column's gathered rows are in before any store
+            xst(U + (size_t)(brow[i] - top) + cof, v);
+        }
 #pragma unroll
         for (int k = 0; k < PW; k++) {
             const int nf = (PW - k - 1) & ~3;
@@ -1669,8 +1718,17 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
         for (int r = 0; r < 4; r++)
             crun[b][r] = C[(size_t)min(r0 + lk + 4 * r, mb - 1) + (size_t)min(c0 + 16 * b + lr, nr - 1) * lda];
     __shared__ int ok_sh;
-    if (tid == 0) ok_sh = 1;
+    __shared__ unsigned long long rmask;  // which of the workgroup's 64 rows receive a displaced row
+    if (tid == 0) {
+        ok_sh = 1;
+        rmask = 0;
+    }
     __syncthreads();
+    const int64_t tb = top + PW + (int64_t)(rt - PSU_S) * 64;  // the workgroup's first row (absolute)
+    if (tid < cnt) {
+        const int64_t r = map->row[tid];
+        if (r >= tb && r < tb + 64) atomicOr(&rmask, 1ull << (r - tb));
+    }
     if (tid < PSU_S) {  // one thread per solver flag
         long sp = 0;
         while (__hip_atomic_load(flags + ct * PSU_S + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -1690,6 +1748,15 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
     }
     __syncthreads();
     if (!ok_sh) return;
+    if (const unsigned long long rm = rmask) {
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if ((rm >> (16 * w + lk + 4 * r)) & 1)
+                    crun[b][r] = xld(C + (size_t)min(r0 + lk + 4 * r, mb - 1) +
+                                     (size_t)min(c0 + 16 * b + lr, nr - 1) * lda);
+    }
 #pragma unroll
     for (int b = 0; b < 2; b++) {
         const int col = min(c0 + 16 * b + lr, nr - 1);
@@ -2077,21 +2144,25 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
             getf2(st, m - j, jb, a + j + (size_t)j * lda, lda, ipiv + j, info, j, j, w, p);
         }
         if (q == 0 && after_first) BOLA_CHECK(hipStreamWaitEvent(st, after_first, 0));
-        {
+        const int nr = s0 + sw - j - jb;
+        const Edge ef = panel_edge(m, n, j, jb);
+        const bool fused = nr > 0 && jb == PW && j + jb < m && nr <= 8 * UR_CB && ef.row >= m && ef.col >= j + jb + nr;
+        if (!fused) {
             Phase pl(h, "LASWP", 0.0, st);
             launch_rowmap_apply(st, a + (size_t)s0 * lda, lda, (size_t)(j - s0), (size_t)(j + jb - s0),
                                 (size_t)(s0 + sw - j - jb), w.maps + p, 1, Abort::panel(info, j), ipiv + j, jb, j);
         }
-        const int nr = s0 + sw - j - jb;
-        const Edge ef = panel_edge(m, n, j, jb);
-        if (nr > 0 && jb == PW && j + jb < m && nr <= 8 * UR_CB && ef.row >= m && ef.col >= j + jb + nr) {
-            // no edge element among the rows below and these columns: both in one launch
+        if (fused) {
+            // no edge element among the rows below and these columns: the interchanges, the
+            // TRSM and the GEMM in one launch
             const int mb = m - j - jb, nrt = (mb + 63) / 64, ncb = (nr + UR_CB - 1) / UR_CB;
+            const int nmain = ncb * (nrt + PSU_S), left = j - s0;
             Phase pt(h, "TRSM", (double)jb * jb * nr + 2.0 * (double)mb * nr * jb, st);
-            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + PSU_S))), dim3(256), 0, st, mb, nr,
+            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(nmain + (left + 3) / 4)), dim3(256), 0, st, mb, nr,
                                nrt, a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda,
                                a + (j + jb) + (size_t)j * lda, a + (j + jb) + (size_t)(j + jb) * lda, lda,
-                               w.uflags + (size_t)p * 32, kSpinLimit, info, j);
+                               w.uflags + (size_t)p * 32, kSpinLimit, info, j, w.maps + p, a + (size_t)s0 * lda, left,
+                               ipiv + j, nmain);
             BOLA_LAUNCH_CHECK();
         } else if (nr > 0) {
             {
