@@ -1618,9 +1618,9 @@ __global__ void __launch_bounds__(256) urows_fused(int G, const double* __restri
 // solver column gathers its 32 rows through the map and first moves the displaced rows of
 // the 32 to their rows below (all read before its solved rows overwrite the 32, and stored
 // before its flag rises); the updaters read their rows ahead of the flag and read again,
-// after it, the few that received a displaced row.  The nmain-th workgroup on apply the map
-// to the `left` columns at Aleft (a wavefront a column), and workgroup 0 turns the panel's
-// pivots (fix, 1-based within the panel) into global ones.
+// after it, the few that received a displaced row.  The first nlwg workgroups (they wait for
+// nothing) apply the map to the `left` columns at Aleft, a wavefront a column, and the next
+// one turns the panel's pivots (fix, 1-based within the panel) into global ones.
 constexpr int PSU_S = UR_CB / 8;  // solver workgroups per column block
 __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nrt, const double* __restrict__ L11,
                                                           double* __restrict__ U, const double* __restrict__ L21,
@@ -1628,12 +1628,14 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
                                                           int* __restrict__ flags, long spin_limit,
                                                           int32_t* __restrict__ info, int col,
                                                           const RowMap* __restrict__ map, double* __restrict__ Aleft,
-                                                          int left, int32_t* __restrict__ fix, int nmain) {
+                                                          int left, int32_t* __restrict__ fix, int nlwg) {
     if (abort_stop(Abort::panel(info, col))) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int cnt = min(map->count, ROWMAP_MAX);
-    if ((int)blockIdx.x >= nmain) {
-        const int cc = ((int)blockIdx.x - nmain) * 4 + w;
+    // loaded beside the count, used only below it
+    const int64_t mrow = map->row[tid & (ROWMAP_MAX - 1)], msrc = map->src[tid & (ROWMAP_MAX - 1)];
+    if ((int)blockIdx.x < nlwg) {
+        const int cc = (int)blockIdx.x * 4 + w;
         if (cc >= left) return;
         double* c = Aleft + (size_t)cc * lda;
         double v0 = 0.0, v1 = 0.0;
@@ -1644,8 +1646,9 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
         if (d1 >= 0) c[d1] = v1;
         return;
     }
-    if (blockIdx.x == 0 && tid < PW) fix[tid] += col;
-    const int ct = blockIdx.x / (nrt + PSU_S), rt = blockIdx.x % (nrt + PSU_S);
+    const int bid = (int)blockIdx.x - nlwg;
+    if (bid == 0 && tid < PW) fix[tid] += col;
+    const int ct = bid / (nrt + PSU_S), rt = bid % (nrt + PSU_S);
     const int c0 = ct * UR_CB;
     const int64_t top = col;  // U's first row (absolute)
     if (rt < PSU_S) {
@@ -1655,7 +1658,7 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
         if (tid == 0) nbelow = 0;
         __syncthreads();
         if (tid < cnt) {
-            const int64_t r = map->row[tid], sr = map->src[tid];
+            const int64_t r = mrow, sr = msrc;
             if (r < top + PW) {
                 tsrc[r - top] = sr;
             } else {
@@ -1726,7 +1729,7 @@ __global__ void __launch_bounds__(256) panel_solve_update(int mb, int nr, int nr
     __syncthreads();
     const int64_t tb = top + PW + (int64_t)(rt - PSU_S) * 64;  // the workgroup's first row (absolute)
     if (tid < cnt) {
-        const int64_t r = map->row[tid];
+        const int64_t r = mrow;
         if (r >= tb && r < tb + 64) atomicOr(&rmask, 1ull << (r - tb));
     }
     if (tid < PSU_S) {  // one thread per solver flag
@@ -2156,13 +2159,13 @@ void factor_super(bola_hip_handle_s* h, hipStream_t st, int m, int n, double* a,
             // no edge element among the rows below and these columns: the interchanges, the
             // TRSM and the GEMM in one launch
             const int mb = m - j - jb, nrt = (mb + 63) / 64, ncb = (nr + UR_CB - 1) / UR_CB;
-            const int nmain = ncb * (nrt + PSU_S), left = j - s0;
+            const int left = j - s0, nlwg = (left + 3) / 4;
             Phase pt(h, "TRSM", (double)jb * jb * nr + 2.0 * (double)mb * nr * jb, st);
-            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(nmain + (left + 3) / 4)), dim3(256), 0, st, mb, nr,
+            hipLaunchKernelGGL(panel_solve_update, dim3((unsigned)(ncb * (nrt + PSU_S) + nlwg)), dim3(256), 0, st, mb, nr,
                                nrt, a + j + (size_t)j * lda, a + j + (size_t)(j + jb) * lda,
                                a + (j + jb) + (size_t)j * lda, a + (j + jb) + (size_t)(j + jb) * lda, lda,
                                w.uflags + (size_t)p * 32, kSpinLimit, info, j, w.maps + p, a + (size_t)s0 * lda, left,
-                               ipiv + j, nmain);
+                               ipiv + j, nlwg);
             BOLA_LAUNCH_CHECK();
         } else if (nr > 0) {
             {
